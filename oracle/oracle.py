@@ -1,0 +1,400 @@
+"""CPU oracle for the voxel-morphology path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this module, and only as the checker.  The product
+(``pmmoto_amd``) never imports it.
+
+Layers:
+
+* ctypes wrappers over ``liboracle.so`` (``pmx_oracle.c``): the float32
+  restatement of the reference per-line kernels and drivers, the exact
+  integer EDT, the 26-connected labelling and the sphere rasteriser;
+* optional access to ``_ref/libpmmoto_ref.so``, the reference's own
+  ``_distance.hpp`` / ``_voxels.hpp`` compiled by ``oracle/Makefile``;
+* numpy restatements of the Python-level reference drivers:
+  morphology (``filters/morphological_operators.py:103-228``), inlet
+  connectivity (``filters/connected_components.py:254-296``,
+  ``core/voxels.py:400-425``) and porosimetry
+  (``filters/porosimetry.py:84-227``).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libpmmoto_ref.so")
+
+BC_END, BC_WALL, BC_PERIODIC = 0, 1, 2
+D2_INF = np.iinfo(np.int32).max
+FLT_MAX = np.float32(np.finfo(np.float32).max)
+
+
+class Hull(C.Structure):
+    _fields_ = [("vertex", C.c_int), ("height", C.c_float), ("range", C.c_float)]
+
+
+class LineOps(C.Structure):
+    _fields_ = [
+        ("sq_edt_1d", C.c_void_p),
+        ("parabolic", C.c_void_p),
+        ("boundary_hull", C.c_void_p),
+        ("nearest_index", C.c_void_p),
+    ]
+
+
+_lib = None
+_ref = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.orc_ref_sq_edt_1d.argtypes = [P, P, C.c_int, C.c_long, C.c_float, C.c_float, C.c_float]
+        L.orc_ref_sq_edt_1d_parabolic.argtypes = [P, C.c_int, C.c_float, C.c_long, P, C.c_int, P, C.c_int]
+        L.orc_ref_boundary_hull_1d.argtypes = [P, C.c_long, C.c_int, C.c_float, C.c_int, C.c_int, P]
+        L.orc_ref_boundary_hull_1d.restype = C.c_int
+        L.orc_ref_nearest_index_1d.argtypes = [P, C.c_int, C.c_uint8, C.c_int, C.c_int, C.c_int]
+        L.orc_ref_nearest_index_1d.restype = C.c_long
+        L.orc_ref_edt3d_sq.argtypes = [P, C.c_long, C.c_long, C.c_long, P, P, P, P]
+        L.orc_ref_edt2d_sq.argtypes = [P, C.c_long, C.c_long, P, P, P, P]
+        L.orc_exact_edt_sq.argtypes = [P, C.c_int, P, P, P]
+        L.orc_exact_ccl26.argtypes = [P, C.c_long, C.c_long, C.c_long, P, P]
+        L.orc_exact_ccl26.restype = C.c_int64
+        L.orc_rasterize_spheres.argtypes = [P, C.c_long, C.c_long, C.c_long, P, P, C.c_long, C.c_double, P]
+        L.orc_rasterize_spheres.restype = C.c_long
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's own per-line kernels, or None when not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        R = C.CDLL(REF_PATH)
+        P = C.c_void_p
+        R.ref_sq_edt_1d.argtypes = [P, P, C.c_int, C.c_long, C.c_float, C.c_float, C.c_float]
+        R.ref_parabolic.argtypes = [P, C.c_int, C.c_float, C.c_long, P, C.c_int, P, C.c_int]
+        R.ref_boundary_hull.argtypes = [P, C.c_int, C.c_float, C.c_long, C.c_int, C.c_int, C.c_int, P]
+        R.ref_boundary_hull.restype = C.c_int
+        R.ref_nearest_index.argtypes = [P, C.c_int, C.c_long, C.c_uint8, C.c_int, C.c_int, C.c_int]
+        R.ref_nearest_index.restype = C.c_long
+        _ref = R
+    return _ref
+
+
+def ref_ops():
+    """An orc_line_ops table pointing at the reference's kernels (or None)."""
+    R = ref_lib()
+    if R is None:
+        return None
+    ops = LineOps()
+    for name, fn in (("sq_edt_1d", R.ref_sq_edt_1d), ("parabolic", R.ref_parabolic),
+                     ("boundary_hull", R.ref_boundary_hull), ("nearest_index", R.ref_nearest_index)):
+        setattr(ops, name, C.cast(fn, C.c_void_p).value)
+    return ops
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# --------------------------------------------------------------------------
+# Per-line kernels (restated), used by the known-answer tests
+# --------------------------------------------------------------------------
+
+def sq_edt_1d(img, lower_corrector=np.inf, upper_corrector=np.inf, resolution=1.0, use_ref=False):
+    """squared_edt_1d on a contiguous uint8 line (_distance.hpp:135-182)."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    out = np.zeros(img.shape[0], dtype=np.float32)
+    fn = ref_lib().ref_sq_edt_1d if use_ref else lib().orc_ref_sq_edt_1d
+    fn(_p(img), _p(out), img.shape[0], 1, resolution, lower_corrector, upper_corrector)
+    return out
+
+
+def parabolic_1d(f, lower=(), upper=(), resolution=1.0, use_ref=False):
+    """squared_edt_1d_parabolic on a float32 line (_distance.hpp:184-272)."""
+    f = np.array(f, dtype=np.float32)
+    lo = (Hull * max(1, len(lower)))(*[Hull(*h) for h in lower])
+    hi = (Hull * max(1, len(upper)))(*[Hull(*h) for h in upper])
+    fn = ref_lib().ref_parabolic if use_ref else lib().orc_ref_sq_edt_1d_parabolic
+    fn(_p(f), f.shape[0], resolution, 1, C.cast(lo, C.c_void_p), len(lower), C.cast(hi, C.c_void_p), len(upper))
+    return f
+
+
+def boundary_hull_1d(img, start, end, resolution, num_hull, forward):
+    """get_boundary_hull_1d (_distance.pyx:679-702) -> list of dicts.
+    Like the reference, reads img[start:start+end]; the buffer is zero-padded
+    so a window running past the array (as one reference test does) reads
+    zeros instead of foreign memory."""
+    img = np.concatenate([np.array(img, dtype=np.float32), np.zeros(start + end, np.float32)])
+    out = (Hull * (num_hull + 2))()
+    n = lib().orc_ref_boundary_hull_1d(_p(img), start, end, resolution, num_hull, 1 if forward else 0,
+                                       C.cast(out, C.c_void_p))
+    return [dict(vertex=out[i].vertex, height=out[i].height, range=out[i].range) for i in range(n)]
+
+
+def ref_boundary_hull_line(f, resolution, num_hull, index_corrector, forward):
+    """The reference's return_boundary_hull on a contiguous float32 line."""
+    f = np.array(f, dtype=np.float32)
+    out = (Hull * (num_hull + 2))()
+    n = ref_lib().ref_boundary_hull(_p(f), f.shape[0], resolution, 1, num_hull, index_corrector,
+                                    1 if forward else 0, C.cast(out, C.c_void_p))
+    return [(out[i].vertex, out[i].height, out[i].range) for i in range(n)]
+
+
+def nearest_index_1d(img, label, lower_skip=0, upper_skip=0, forward=True):
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    return lib().orc_ref_nearest_index_1d(_p(img), img.shape[0], label, lower_skip, upper_skip,
+                                          1 if forward else 0)
+
+
+# --------------------------------------------------------------------------
+# Whole-volume reference EDT (float32, reference pass order)
+# --------------------------------------------------------------------------
+
+def ref_edt_sq(img, periodic=None, resolution=None, use_ref=False):
+    """Squared float32 EDT as edt3d/edt2d compute it before the sqrt
+    (euclidean_distance_transform.py:104-303)."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    nd = img.ndim
+    periodic = periodic or (False,) * nd
+    resolution = resolution or (1.0,) * nd
+    per = np.array([1 if p else 0 for p in periodic], dtype=np.int32)
+    res = np.array(resolution, dtype=np.float32)
+    out = np.empty(img.shape, dtype=np.float32)
+    ops = ref_ops() if use_ref else None
+    if use_ref and ops is None:
+        raise RuntimeError("reference kernels not built (make -C oracle ref)")
+    opp = C.byref(ops) if ops is not None else None
+    if nd == 3:
+        lib().orc_ref_edt3d_sq(_p(img), *img.shape, _p(per), _p(res), _p(out), opp)
+    elif nd == 2:
+        lib().orc_ref_edt2d_sq(_p(img), *img.shape, _p(per), _p(res), _p(out), opp)
+    else:
+        raise ValueError(f"Wrong img dimension {img.shape}")
+    return out
+
+
+def ref_edt(img, periodic=None, resolution=None, use_ref=False):
+    """edt3d/edt2d output: np.sqrt of the float32 squared EDT."""
+    return np.sqrt(ref_edt_sq(img, periodic, resolution, use_ref))
+
+
+# --------------------------------------------------------------------------
+# Exact integer EDT and labelling
+# --------------------------------------------------------------------------
+
+def exact_edt_sq(img, bc=None):
+    """Exact squared EDT (int32; D2_INF where no solid is reachable).
+    bc: per-face boundary types, sequence of 2*ndim ints (END/WALL/PERIODIC)."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    nd = img.ndim
+    bc = np.array(bc if bc is not None else [BC_END] * (2 * nd), dtype=np.int32)
+    shape = np.array(img.shape, dtype=np.int64)
+    out = np.empty(img.shape, dtype=np.int32)
+    lib().orc_exact_edt_sq(_p(img), nd, _p(shape), _p(bc), _p(out))
+    return out
+
+
+def d2_to_f32(d2):
+    """Map exact integer squared distances to the reference's float32
+    squared values (D2_INF -> FLT_MAX, the to_finite sentinel)."""
+    out = d2.astype(np.float32)
+    out[d2 == D2_INF] = FLT_MAX
+    return out
+
+
+def ccl26(img, periodic=(False, False, False)):
+    """26-connected multi-label labelling, canonical numbering (C-order of
+    first voxel).  Returns (int64 labels, count)."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    per = np.array([1 if p else 0 for p in periodic], dtype=np.int32)
+    labels = np.empty(img.shape, dtype=np.int64)
+    n = lib().orc_exact_ccl26(_p(img), *img.shape, _p(per), _p(labels))
+    return labels, int(n)
+
+
+def canonical_relabel(labels):
+    """Renumber labels 1..N by C-order of first occurrence (0 stays 0)."""
+    flat = labels.ravel()
+    nz = flat != 0
+    vals, first = np.unique(flat[nz], return_index=True)
+    order = np.argsort(first)
+    remap = np.zeros(vals.shape[0], dtype=np.int64)
+    remap[order] = np.arange(1, vals.shape[0] + 1)
+    out = np.zeros(flat.shape, dtype=np.int64)
+    out[nz] = remap[np.searchsorted(vals, flat[nz])]
+    return out.reshape(labels.shape)
+
+
+def rasterize_spheres(shape, periodic, spheres, target):
+    """Sequential overlapping-sphere rasteriser: returns (img, n_used)."""
+    img = np.ones(shape, dtype=np.uint8)
+    per = np.array([1 if p else 0 for p in periodic], dtype=np.int32)
+    sp = np.ascontiguousarray(spheres, dtype=np.float64)
+    pores = np.array([img.size], dtype=np.int64)
+    n = lib().orc_rasterize_spheres(_p(img), *shape, _p(per), _p(sp), sp.shape[0], float(target), _p(pores))
+    return img, int(n)
+
+
+# --------------------------------------------------------------------------
+# Padded-subdomain model and the Python-level reference drivers
+# --------------------------------------------------------------------------
+
+@dataclass
+class Box:
+    """What the reference drivers read from a PaddedSubdomain: per-face
+    boundary types, pad widths, inlet/outlet flags and resolution."""
+
+    bc: tuple  # ((lo, hi),) * 3 of BC_*
+    pad: tuple  # ((lo, hi),) * 3
+    inlet: tuple = ((False, False),) * 3
+    outlet: tuple = ((False, False),) * 3
+    resolution: tuple = (1.0, 1.0, 1.0)
+
+
+def _own_slices(box: Box, shape):
+    return tuple(slice(p[0], s - p[1]) for p, s in zip(box.pad, shape))
+
+
+def halo_image(box: Box, img, halo):
+    """update_extended_buffer (communication.py:121-157) for one rank:
+    periodic faces receive the periodic extension of the own region, END
+    faces and 1-wide WALL pads receive no halo (subdomain_padded.py:116-167)."""
+    out = img
+    pads = []
+    for ax in range(3):
+        lo_bc, hi_bc = box.bc[ax]
+        plo, phi = box.pad[ax]
+        elo = 0 if lo_bc in (BC_END,) or (lo_bc == BC_WALL and plo == 1) else halo[ax]
+        ehi = 0 if hi_bc in (BC_END,) or (hi_bc == BC_WALL and phi == 1) else halo[ax]
+        if elo or ehi:
+            n = out.shape[ax]
+            own = np.take(out, np.arange(plo, n - phi), axis=ax)
+            ext = np.pad(own, [(0, 0)] * ax + [(plo + elo, phi + ehi)] + [(0, 0)] * (2 - ax), mode="wrap")
+            # keep the caller's pad voxels, as buffer_unpack writes only the halo
+            idx = [slice(None)] * 3
+            idx[ax] = slice(elo, elo + n)
+            ext[tuple(idx)] = out
+            out = ext
+        pads.append((elo, ehi))
+    return out, tuple(pads)
+
+
+def unpad(img, pads):
+    return img[tuple(slice(p[0], img.shape[i] - p[1]) for i, p in enumerate(pads))]
+
+
+def set_wall_bcs(box: Box, img):
+    """subdomain.py:273-287: outermost layer of WALL faces forced to 0."""
+    img = img.copy()
+    for ax in range(3):
+        for side in (0, 1):
+            if box.bc[ax][side] == BC_WALL:
+                idx = [slice(None)] * 3
+                idx[ax] = 0 if side == 0 else -1
+                img[tuple(idx)] = 0
+    return img
+
+
+def struct_ratio(resolution, radius):
+    """gen_struct_ratio (morphological_operators.py:35-63)."""
+    if len(resolution) not in (2, 3):
+        raise ValueError("Resolution must be a list of length 2 or 3")
+    if radius < np.min(resolution):
+        raise ValueError("radius too small")
+    return tuple(math.ceil(radius / r) for r in resolution)
+
+
+def erode(box: Box, img, radius):
+    """subtraction, EDT mode (morphological_operators.py:175-228)."""
+    halo, pads = halo_image(box, img, struct_ratio(box.resolution, radius))
+    d = ref_edt(halo.astype(np.uint8), resolution=box.resolution)
+    out = np.where(d <= radius, 0, 1).astype(np.uint8)
+    return set_wall_bcs(box, unpad(out, pads))
+
+
+def dilate(box: Box, img, radius):
+    """addition, EDT mode (morphological_operators.py:103-149)."""
+    halo, pads = halo_image(box, img, struct_ratio(box.resolution, radius))
+    d = ref_edt(np.logical_not(halo).astype(np.uint8), resolution=box.resolution)
+    out = np.where(d <= radius, 1, 0).astype(np.uint8)
+    return set_wall_bcs(box, unpad(out, pads))
+
+
+def label_padded(box: Box, img):
+    """connect_components for one rank (connected_components.py:32-101):
+    periodic axes label the own region with wrap and copy labels into the
+    pads; other axes label the padded extent as data."""
+    per = tuple(box.bc[a][0] == BC_PERIODIC for a in range(3))
+    sl = tuple(slice(box.pad[a][0], img.shape[a] - box.pad[a][1]) if per[a] else slice(None) for a in range(3))
+    core = np.ascontiguousarray(img[sl])
+    lab, n = ccl26(core, per)
+    pads = [(box.pad[a] if per[a] else (0, 0)) for a in range(3)]
+    full = np.pad(lab, pads, mode="wrap") if any(per) else lab
+    return full, n
+
+
+def inlet_labels(box: Box, labels):
+    """gen_inlet_label_map (core/voxels.py:400-425) with the face own slabs
+    of core/features.py:76-139 (1 voxel thick on END faces)."""
+    own = _own_slices(box, labels.shape)
+    found = set()
+    for ax in range(3):
+        for side in (0, 1):
+            if box.inlet[ax][side] and box.bc[ax][side] == BC_END:
+                idx = list(own)
+                p = box.pad[ax][side]
+                if side == 0:
+                    idx[ax] = slice(p, 2 * p) if p else slice(0, 1)
+                else:
+                    n = labels.shape[ax]
+                    idx[ax] = slice(n - 2 * p, n - p) if p else slice(n - 1, n)
+                found |= set(np.unique(labels[tuple(idx)]).tolist())
+    found.discard(0)
+    return found
+
+
+def inlet_connected(box: Box, img, phase=None):
+    """inlet_connected_img (connected_components.py:254-296): voxels whose
+    component touches the inlet keep their phase value, others become 0."""
+    lab, _ = label_padded(box, img)
+    keep = np.zeros(int(lab.max()) + 1, dtype=bool)
+    for l in inlet_labels(box, lab):
+        keep[l] = True
+    out = np.where(keep[lab], img, 0).astype(np.uint8)
+    if phase is not None:
+        out = np.where(out == phase, out, 0).astype(np.uint8)
+    return out
+
+
+def porosimetry(box: Box, pm_img, radius, inlet=False):
+    """porosimetry, mode="distance" (porosimetry.py:84-160), no multiphase."""
+    res = erode(box, pm_img, radius)
+    if inlet:
+        res = inlet_connected(box, res)
+    if np.count_nonzero(res == 1) > 0:  # utils.phase_exists (utils.py:127-145)
+        res = dilate(box, res, radius)
+    return res.astype(np.uint8)
+
+
+def pore_size_distribution(box: Box, pm_img, radii, inlet=False):
+    """pore_size_distribution, mode="distance" (porosimetry.py:163-227)."""
+    radii = np.sort(np.asarray(radii))[::-1]
+    out = np.zeros(pm_img.shape, dtype=np.float64)
+    for r in radii:
+        t = porosimetry(box, pm_img, r, inlet)
+        if np.any(t):
+            out = np.where((pm_img == 1) & (out == 0) & (t == 1), r, out)
+    return out

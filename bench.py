@@ -1,0 +1,159 @@
+"""Benchmark: exact EDT throughput at 1024^3 on one MI355X.
+
+Workload: the 1024^3 seeded polydisperse sphere pack (lognormal radii, mean
+10 voxels, sigma 0.3, porosity 0.35, seed 3) with x END faces (inlet x=0)
+and WALL faces in y and z -- the domain of the 32-radius drainage workload.
+One step = one exact squared EDT of the whole domain (int32 output), inputs
+resident in HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per voxel of each EDT kernel (u8 in, int32 working array)
+BYTES_PER_VOXEL = {"edt_pass_z": 1 + 4, "edt_pass_y": 4 + 4, "edt_pass_x": 4 + 4}
+EDT_BYTES_PER_VOXEL = 21
+
+
+def cpu_baseline(img_t, edge: int):
+    """The reference EDT on the host, single thread, on an edge^3 crop of
+    the same image: the reference's own per-line kernels (oracle/_ref) when
+    built, else the oracle's restatement of them."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    crop = np.ascontiguousarray(img_t[:edge, :edge, :edge].cpu().numpy())
+    use_ref = O.ref_lib() is not None
+    t0 = time.perf_counter()
+    O.ref_edt_sq(crop, use_ref=use_ref)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(crop.size / dt / 1e9, 6),
+        "unit": "Gvoxel/s",
+        "cores": 1,
+        "kind": "reference" if use_ref else "port",
+        "sample": f"squared EDT (reference pass order, float32) of the {edge}^3 corner crop of the "
+                  f"same pack, non-periodic, {dt:.2f} s on one host core",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--cpu-edge", type=int, default=320)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
+              "the multi-GPU sharding design is in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from pmmoto_amd import _lib
+    from pmmoto_amd.domain_generation import pack_radii, sphere_pack
+    from pmmoto_amd.filters.distance import euclidean_distance_transform as E
+
+    torch.cuda.set_device(0)
+    n = args.size
+    t0 = time.perf_counter()
+    img, spheres = sphere_pack((n, n, n), 0.35, pack_radii("lognormal", mean=10.0, sigma=0.3), seed=3)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    print(f"[bench] generated {n}^3 pack: {spheres.shape[0]} spheres in {gen_s:.1f} s", file=sys.stderr,
+          flush=True)
+
+    bc = [_lib.BC_END, _lib.BC_END, _lib.BC_WALL, _lib.BC_WALL, _lib.BC_WALL, _lib.BC_WALL]
+    origin, ext, pads = [0, 0, 0], [n, n, n], [0] * 6
+    d2 = torch.empty((n, n, n), dtype=torch.int32, device="cuda")
+    import ctypes as C
+
+    from pmmoto_amd import _device
+
+    g = E._geom((n, n, n), origin, ext, bc)
+    nbytes = C.c_size_t()
+    _lib.check(_lib.lib().pmx_edt_workspace_size(C.byref(g), C.byref(nbytes)))
+    ws = _device.workspace(nbytes.value)
+
+    def step():
+        _lib.check(_lib.lib().pmx_edt(_device.ptr(img), C.byref(g), _device.ptr(d2), None, None, None,
+                                      _device.ptr(ws), nbytes.value, _device.stream()))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    _lib.profile_begin()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(args.steps):
+        step()
+    ev1.record()
+    torch.cuda.synchronize()
+    prof = _lib.profile_end()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    vox = float(n) ** 3
+    gvox = vox / (ms * 1e-3) / 1e9
+    per_kernel = {k: {"avg_ms": round(t / c, 4), "launches": c} for k, (t, c) in prof.items()}
+    dom = max((k for k in prof if k in BYTES_PER_VOXEL), key=lambda k: prof[k][0])
+    dom_ms = prof[dom][0] / prof[dom][1]
+    achieved = BYTES_PER_VOXEL[dom] * vox / (dom_ms * 1e-3) / 1e9
+    print(f"[bench] {args.steps} steps: {ms:.3f} ms/step; kernels {per_kernel}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(img, min(args.cpu_edge, n))
+
+    line = {
+        "metric": "EDT Gvoxel/s @1024³ (% HBM peak); porosimetry curves/s surrogate vs exact",
+        "value": round(gvox, 3),
+        "unit": "Gvoxel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"exact squared EDT (int32 out) of a {n}^3 seeded lognormal sphere pack "
+                        "(mean r 10, sigma 0.3, porosity 0.35, seed 3), x END / y,z WALL",
+            "voxels": int(vox),
+            "edt_hbm_frac_21B": round(EDT_BYTES_PER_VOXEL * vox / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "kernels": per_kernel,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

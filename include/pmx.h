@@ -1,0 +1,138 @@
+/*
+ * pmx.h -- C ABI of libpmx, the MI355X voxel-morphology library.
+ *
+ * All pointers are device pointers unless stated otherwise.  The caller
+ * owns every buffer; scratch memory comes from a caller-provided workspace
+ * whose size the *_workspace_size functions report.  Every call is
+ * stream-ordered on the given hipStream_t (0 = the null stream) and
+ * returns PMX_OK or a negative status; pmx_last_error() then describes
+ * the failure (thread-local).
+ *
+ * Arrays are C-order [x][y][z] with z contiguous, as PMMoTo's numpy
+ * arrays are (src/pmmoto/core/_voxels.pyx:62-74 derives element strides
+ * from numpy strides the same way).
+ *
+ * Entry points and the reference interface each one replaces:
+ *
+ *   pmx_edt           the three per-line passes of edt3d/corrected_edt
+ *                     (src/pmmoto/filters/distance/euclidean_distance_transform.py:56-101,
+ *                     :208-303) driven line by line through
+ *                     _distance.get_initial_envelope (_distance.pyx:308-353) and
+ *                     _distance.get_parabolic_envelope (_distance.pyx:86-127),
+ *                     whose C++ kernels are squared_edt_1d / squared_edt_1d_parabolic
+ *                     (_distance.hpp:135-272); the periodic correctors and hull
+ *                     exchange (euclidean_distance_transform.py:395-528) become
+ *                     in-kernel wrap handling.
+ *   pmx_morph         the EDT-mode dilation / erosion bodies of
+ *                     addition / subtraction (src/pmmoto/filters/morphological_operators.py:103-228):
+ *                     EDT of the (inverted) halo image, threshold, set_wall_bcs.
+ *   pmx_ccl26         cc3d.connected_components(img, return_N=True) at
+ *                     src/pmmoto/filters/connected_components.py:52-54 plus the periodic
+ *                     merge of connect_subdomain_boundaries (:65-101).
+ *   pmx_face_labels   gen_inlet_label_map / gen_outlet_label_map
+ *                     (src/pmmoto/core/voxels.py:400-454).
+ *   pmx_keep_labels   the label->phase map + renumber_img of inlet_connected_img and
+ *                     its siblings (connected_components.py:254-392, _voxels.pyx:453-494).
+ *   pmx_fill_periodic_pads  update_buffer's self-neighbour copy for periodic faces
+ *                     (src/pmmoto/core/communication.py:101-118, :278-285).
+ *   pmx_rasterize_spheres   gen_pm_spheres_domain's centroid-in-sphere rasterisation
+ *                     (src/pmmoto/domain_generation/domain_generation.py:125-155,
+ *                     particles/spheres.hpp:42-48), with a porosity stopping rule.
+ */
+#ifndef PMX_H
+#define PMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMX_OK 0
+#define PMX_ERR_INVALID -1   /* bad argument (maps to ValueError)   */
+#define PMX_ERR_HIP -2       /* a HIP runtime call failed           */
+#define PMX_ERR_WORKSPACE -3 /* workspace smaller than required      */
+
+/* Boundary type of one face (src/pmmoto/core/boundary_types.py:11-17). */
+#define PMX_BC_END 0      /* nothing beyond the face                  */
+#define PMX_BC_WALL 1     /* a solid layer just beyond the face       */
+#define PMX_BC_PERIODIC 2 /* wrap to the opposite face (both faces)   */
+
+/* Squared distance where no solid voxel is reachable.  The reference's
+ * float32 EDT holds FLT_MAX there (to_finite, _distance.hpp:22-32). */
+#define PMX_D2_INF INT32_MAX
+
+/* The box the kernels compute on, and how the caller's arrays map onto it.
+ * Element (x, y, z) of the box lives at x*sx + y*sy + z of a view. */
+typedef struct {
+    int64_t n[3];    /* box extents (x, y, z); each in [1, 16384]          */
+    int64_t in_s[2]; /* element strides (x, y) of the input view           */
+    int64_t out_s[2];/* element strides (x, y) of every output view        */
+    int32_t bc[6];   /* PMX_BC_* for faces x-, x+, y-, y+, z-, z+           */
+} pmx_geom;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int pmx_version(void);
+
+/* Description of the last failure on this thread ("" if none). */
+const char *pmx_last_error(void);
+
+/* Per-kernel timing for benchmarks: after pmx_profile_begin every library
+ * kernel launch is bracketed by HIP events on its own stream.
+ * pmx_profile_end waits for them, stops recording and reports, per kernel
+ * name (newline-separated in names), the summed milliseconds and launch
+ * count (host buffers; up to max_kernels entries; *n_kernels set). */
+int pmx_profile_begin(void);
+int pmx_profile_end(char *names, int names_len, double *ms, int64_t *calls, int max_kernels,
+                    int *n_kernels);
+
+/* Workspace bytes pmx_edt / pmx_morph need for geometry g. */
+int pmx_edt_workspace_size(const pmx_geom *g, size_t *bytes);
+
+/* Exact squared Euclidean distance from every voxel to the nearest voxel
+ * whose value is 0, with per-face boundary handling.  Outputs (any may be
+ * NULL, at least one must not be): d2 (int32, PMX_D2_INF where no solid is
+ * reachable), f32sq (float(d2), FLT_MAX for PMX_D2_INF: the reference's
+ * squared float32 output), f32 (correctly rounded sqrtf of f32sq: the
+ * reference's np.sqrt of it), f64 (correctly rounded sqrt of double(d2),
+ * sqrt(double(FLT_MAX)) for PMX_D2_INF). */
+int pmx_edt(const uint8_t *img, const pmx_geom *g, int32_t *d2, float *f32sq, float *f32,
+            double *f64, void *workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* Morphology by EDT threshold.  With invert == 0 the distance is taken to
+ * voxels equal to 0 (erosion: subtraction), with invert != 0 to voxels not
+ * equal to 0 (dilation: addition of np.logical_not(img)).  out = d2 <= t ?
+ * v_le : v_gt, then the outermost layer of every face whose bit is set in
+ * wall_mask (bit 2*axis+side) is forced to 0 (set_wall_bcs). */
+int pmx_morph(const uint8_t *img, const pmx_geom *g, int invert, int64_t t, uint8_t v_le,
+              uint8_t v_gt, uint32_t wall_mask, uint8_t *out, void *workspace,
+              size_t workspace_bytes, hipStream_t stream);
+
+/* Copy own voxels into the pads of periodic axes of a padded C-order array
+ * of `elem` bytes per element (1, 4 or 8): shape[3] is the padded extent,
+ * pad[6] the pad widths (x-, x+, y-, y+, z-, z+), periodic[3] flags. */
+int pmx_fill_periodic_pads(void *arr, int elem, const int64_t *shape, const int64_t *pad,
+                           const int32_t *periodic, hipStream_t stream);
+
+/* Sphere packs.  spheres: n_spheres x (cx, cy, cz, r) float64 in voxel
+ * units (voxel (i,j,k) has its centroid at (i+.5, j+.5, k+.5)); a voxel is
+ * covered by a sphere when dx*dx+dy*dy (+) dz*dz <= r*r in float64, periodic
+ * axes using the minimum image when a sphere spans the whole period.
+ * pmx_sphere_first_cover writes first[v] = index of the first sphere that
+ * covers voxel v (INT32_MAX if none) and counts[s] = number of voxels whose
+ * first cover is s (n_spheres <= 2^24).  After k spheres the pore voxels
+ * are those with first >= k; pmx_sphere_image writes img = first >= k. */
+int pmx_sphere_first_cover(const double *spheres, int64_t n_spheres, const int64_t *shape,
+                           const int32_t *periodic, int32_t *first, int64_t *counts,
+                           hipStream_t stream);
+int pmx_sphere_image(const int32_t *first, const int64_t *shape, int32_t k, uint8_t *img,
+                     hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PMX_H */

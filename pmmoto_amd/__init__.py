@@ -1,0 +1,8 @@
+"""pmmoto_amd: PMMoTo's voxel-morphology filters (distance, morphology,
+connected components, porosimetry) as hand-written HIP kernels for the
+AMD Instinct MI355X, behind the pmmoto.filters call surface."""
+
+from . import core, filters
+from .core import BoundaryType, initialize
+
+__all__ = ["BoundaryType", "core", "filters", "initialize"]

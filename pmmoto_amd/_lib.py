@@ -1,0 +1,94 @@
+"""ctypes binding of libpmx.so (the HIP kernels behind include/pmx.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or ``make -C
+pmmoto_amd/csrc``).  There is no CPU fallback: every filter in this package
+runs on the GPU through this binding and raises if the library or a HIP
+device is missing.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpmx.so")
+
+OK, ERR_INVALID, ERR_HIP, ERR_WORKSPACE = 0, -1, -2, -3
+BC_END, BC_WALL, BC_PERIODIC = 0, 1, 2
+D2_INF = np.iinfo(np.int32).max
+
+
+class Geom(C.Structure):
+    """pmx_geom: the box the kernels compute on and the views' strides."""
+
+    _fields_ = [
+        ("n", C.c_int64 * 3),
+        ("in_s", C.c_int64 * 2),
+        ("out_s", C.c_int64 * 2),
+        ("bc", C.c_int32 * 6),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/pmx.h declares
+P, I32, I64, U8, U32, SZ = C.c_void_p, C.c_int32, C.c_int64, C.c_uint8, C.c_uint32, C.c_size_t
+SIGNATURES = {
+    "pmx_version": (C.c_int, []),
+    "pmx_last_error": (C.c_char_p, []),
+    "pmx_profile_begin": (C.c_int, []),
+    "pmx_profile_end": (C.c_int, [P, C.c_int, P, P, C.c_int, P]),
+    "pmx_edt_workspace_size": (C.c_int, [P, P]),
+    "pmx_edt": (C.c_int, [P, P, P, P, P, P, P, SZ, P]),
+    "pmx_morph": (C.c_int, [P, P, C.c_int, I64, U8, U8, U32, P, P, SZ, P]),
+    "pmx_fill_periodic_pads": (C.c_int, [P, C.c_int, P, P, P, P]),
+    "pmx_sphere_first_cover": (C.c_int, [P, I64, P, P, P, P, P]),
+    "pmx_sphere_image": (C.c_int, [P, P, I32, P, P]),
+}
+
+_lib = None
+
+
+class PmxError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """Load libpmx.so once; raise loudly if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PmxError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "or `make -C pmmoto_amd/csrc` (there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    """Map a pmx status to the exception type the reference raises."""
+    if rc == OK:
+        return
+    msg = lib().pmx_last_error().decode()
+    if rc == ERR_INVALID:
+        raise ValueError(msg)
+    raise PmxError(f"libpmx status {rc}: {msg}")
+
+
+def profile_begin() -> None:
+    check(lib().pmx_profile_begin())
+
+
+def profile_end() -> dict:
+    """{kernel name: (total ms, launches)} recorded since profile_begin()."""
+    names = C.create_string_buffer(8192)
+    ms = (C.c_double * 64)()
+    calls = (C.c_int64 * 64)()
+    n = C.c_int()
+    check(lib().pmx_profile_end(names, 8192, ms, calls, 64, C.byref(n)))
+    keys = names.value.decode().split("\n")[: n.value]
+    return {k: (ms[i], int(calls[i])) for i, k in enumerate(keys)}

@@ -1,0 +1,95 @@
+"""Seeded synthetic porous media, rasterised on the GPU.
+
+``gen_img_random_binary`` mirrors the reference generator
+(src/pmmoto/domain_generation/domain_generation.py:40-60) exactly (same
+numpy calls, so the same image for the same seed).  ``sphere_pack`` grows
+an overlapping-sphere pack in voxel units until the porosity reaches a
+target: centres are uniform in the box, radii come from a sampler, voxel
+(i,j,k) is solid when its centroid lies in a sphere (the reference's
+Sphere::contains, particles/spheres.hpp:42-48, on voxel centroids,
+domain_discretization.py:70-90).  The stopping rule -- keep the first
+sphere after which porosity <= target -- is evaluated exactly from
+libpmx's first-cover counts (pmx_sphere_first_cover).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from .. import _device, _lib
+
+__all__ = ["gen_img_random_binary", "sphere_pack", "pack_radii"]
+
+
+def gen_img_random_binary(shape, p_zero=0.5, seed=None):
+    """domain_generation.py:40-60 (numpy; pore = 1)."""
+    if not (0 <= p_zero <= 1):
+        raise ValueError("Probability p_zero must be between 0 and 1.")
+    rng = np.random.default_rng(seed)
+    return rng.choice([0, 1], size=shape, p=[p_zero, 1 - p_zero]).astype(np.uint8)
+
+
+def pack_radii(kind: str, **kw):
+    """Radius samplers: ("fixed", r), ("uniform", lo, hi),
+    ("lognormal", mean, sigma) with E[r] = mean."""
+    if kind == "fixed":
+        r = float(kw["r"])
+        return (lambda rng, n: np.full(n, r)), r ** 3
+    if kind == "uniform":
+        lo, hi = float(kw["lo"]), float(kw["hi"])
+        return (lambda rng, n: rng.uniform(lo, hi, n)), (hi ** 4 - lo ** 4) / (4 * (hi - lo))
+    if kind == "lognormal":
+        mean, sigma = float(kw["mean"]), float(kw["sigma"])
+        mu = math.log(mean) - sigma * sigma / 2
+        return (lambda rng, n: rng.lognormal(mu, sigma, n)), math.exp(3 * mu + 4.5 * sigma * sigma)
+    raise ValueError(f"unknown radius distribution {kind}")
+
+
+def sphere_pack(shape, porosity, radii, seed, periodic=(False, False, False), device_out=True):
+    """Overlapping-sphere pack: returns (uint8 image, pore = 1, as a device
+    tensor or numpy array; float64 (n, 4) array of the spheres used).
+
+    Spheres are drawn in batches from ``numpy.random.default_rng(seed)``:
+    each batch draws centres ``rng.uniform(0, shape, (m, 3))`` then radii;
+    batches are appended until the target is reached."""
+    sampler, mean_r3 = radii
+    shape = tuple(int(s) for s in shape)
+    vol = float(np.prod(shape))
+    if not (0.0 < porosity < 1.0):
+        raise ValueError("porosity must be in (0, 1)")
+    rng = np.random.default_rng(seed)
+    est = int(math.ceil(-math.log(porosity) * vol / (4.0 / 3.0 * math.pi * mean_r3) * 1.25)) + 64
+    batches = []
+    dev = _device.require_gpu()
+    first = torch.empty(shape, dtype=torch.int32, device=dev)
+    L = _lib.lib()
+    sh = (C.c_int64 * 3)(*shape)
+    per = (C.c_int32 * 3)(*[1 if p else 0 for p in periodic])
+    while True:
+        m = est if not batches else max(64, est // 4)
+        centres = rng.uniform(0.0, 1.0, (m, 3)) * np.array(shape, dtype=np.float64)
+        rad = np.asarray(sampler(rng, m), dtype=np.float64)
+        batches.append(np.column_stack([centres, rad]))
+        spheres = np.ascontiguousarray(np.concatenate(batches))
+        if not np.all(np.isfinite(spheres)) or np.any(spheres[:, 3] < 0):
+            raise ValueError("sphere parameters must be finite with r >= 0")
+        sp = torch.from_numpy(spheres).to(dev)
+        counts = torch.empty(spheres.shape[0], dtype=torch.int64, device=dev)
+        _lib.check(L.pmx_sphere_first_cover(_device.ptr(sp), spheres.shape[0], sh, per,
+                                            _device.ptr(first), _device.ptr(counts), _device.stream()))
+        solid_after = np.cumsum(counts.cpu().numpy())  # solid voxels after k+1 spheres
+        pores_after = vol - solid_after
+        hit = np.nonzero(pores_after / vol <= porosity)[0]
+        if hit.size:
+            k = int(hit[0]) + 1
+            break
+        if spheres.shape[0] > (1 << 24) - est:
+            raise ValueError("porosity target not reachable with this radius distribution")
+    img = torch.empty(shape, dtype=torch.uint8, device=dev)
+    _lib.check(L.pmx_sphere_image(_device.ptr(first), sh, k, _device.ptr(img), _device.stream()))
+    del first
+    return (img if device_out else img.cpu().numpy()), spheres[:k]

@@ -1,0 +1,72 @@
+"""CPU checks of the drop-in boundary: libpmx.so loads without a GPU and
+exports every symbol include/pmx.h declares; the Python surface mirrors the
+reference's names and argument errors."""
+
+import os
+import re
+
+import numpy as np
+import pytest
+
+from pmmoto_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "pmx.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(pmx_\w+)\s*\(", src, flags=re.M)))
+
+
+def test_header_lists_symbols():
+    syms = declared_symbols()
+    assert "pmx_edt" in syms and "pmx_morph" in syms and len(syms) >= 6
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_version_and_error_string():
+    L = _lib.lib()
+    assert L.pmx_version() >= 100
+    assert isinstance(L.pmx_last_error(), bytes)
+
+
+def test_invalid_geometry_is_rejected_without_gpu():
+    import ctypes as C
+    L = _lib.lib()
+    g = _lib.Geom()
+    g.n[0], g.n[1], g.n[2] = 0, 4, 4
+    nbytes = C.c_size_t()
+    rc = L.pmx_edt_workspace_size(C.byref(g), C.byref(nbytes))
+    assert rc == _lib.ERR_INVALID
+    assert b"extent" in L.pmx_last_error()
+    with pytest.raises(ValueError):
+        _lib.check(rc)
+    g.n[0] = 4
+    g.in_s[0], g.in_s[1], g.out_s[0], g.out_s[1] = 16, 4, 16, 4
+    g.bc[0], g.bc[1] = _lib.BC_PERIODIC, _lib.BC_END
+    assert L.pmx_edt_workspace_size(C.byref(g), C.byref(nbytes)) == _lib.ERR_INVALID
+    g.bc[1] = _lib.BC_PERIODIC
+    assert L.pmx_edt_workspace_size(C.byref(g), C.byref(nbytes)) == _lib.OK
+    assert nbytes.value > 4 * 64
+
+
+def test_filters_fail_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from pmmoto_amd.filters import distance
+    with pytest.raises(_lib.PmxError):
+        distance.edt(np.ones((4, 4, 4), np.uint8))
+
+
+def test_edt_dimension_errors():
+    from pmmoto_amd.filters import distance
+    with pytest.raises(ValueError):
+        distance.edt(np.ones(4))  # test_edt.py:369-373

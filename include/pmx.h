@@ -102,6 +102,35 @@ int pmx_edt_workspace_size(const pmx_geom *g, size_t *bytes);
 int pmx_edt(const uint8_t *img, const pmx_geom *g, int32_t *d2, float *f32sq, float *f32,
             double *f64, void *workspace, size_t workspace_bytes, hipStream_t stream);
 
+/* The reference's float32 EDT arithmetic, bit for bit, at any resolution
+ * (edt3d / edt2d / corrected_edt, euclidean_distance_transform.py:104-303,
+ * per-line kernels _distance.hpp:135-356).  img: uint8 view (x, y strides
+ * in_s[2]) of the box n[3]; d: float32 view (strides out_s[2]) receiving the
+ * squared distance, or its float32 sqrt with take_sqrt.  first_axis gets the
+ * 1-D label-aware sweep, then the other axes in ascending order get the
+ * parabolic envelope (3-D: first_axis 0; 2-D image as a (1, X, Y) box:
+ * first_axis 2).  periodic[3] selects the reference's corrector and
+ * five-member boundary-hull handling on that axis; resolution[3] are the
+ * float32 voxel sizes.  Voxels with no reachable solid hold FLT_MAX. */
+int pmx_edt_ref_workspace_size(const int64_t *n, size_t *bytes);
+int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float *d,
+                const int64_t *out_s, int first_axis, const int32_t *periodic,
+                const float *resolution, int take_sqrt, void *workspace, size_t workspace_bytes,
+                hipStream_t stream);
+
+/* The reference's halo-image morphology at any resolution
+ * (morphological_operators.py:103-228): the padded uint8 array (shape[3],
+ * pad widths pad[6]) is extended by halo_width[3] voxels per side (periodic
+ * copies of the own region; 0 on axes without a halo), its float32 EDT is
+ * taken with pmx_edt_ref (invert: of np.logical_not), and out (shape[3],
+ * contiguous) = (d <= radius ? v_le : v_gt) compared in float64 when
+ * compare_f64, else against (float)radius; WALL layers (wall_mask) -> 0. */
+int pmx_halo_morph_workspace_size(const int64_t *shape, const int64_t *halo_width, size_t *bytes);
+int pmx_halo_morph(const uint8_t *img, const int64_t *shape, const int64_t *pad,
+                   const int64_t *halo_width, const float *resolution, int invert, double radius,
+                   int compare_f64, uint8_t v_le, uint8_t v_gt, uint32_t wall_mask, uint8_t *out,
+                   void *workspace, size_t workspace_bytes, hipStream_t stream);
+
 /* Morphology by EDT threshold.  With invert == 0 the distance is taken to
  * voxels equal to 0 (erosion: subtraction), with invert != 0 to voxels not
  * equal to 0 (dilation: addition of np.logical_not(img)).  out = d2 <= t ?
@@ -116,6 +145,48 @@ int pmx_morph(const uint8_t *img, const pmx_geom *g, int invert, int64_t t, uint
  * pad[6] the pad widths (x-, x+, y-, y+, z-, z+), periodic[3] flags. */
 int pmx_fill_periodic_pads(void *arr, int elem, const int64_t *shape, const int64_t *pad,
                            const int32_t *periodic, hipStream_t stream);
+
+/* Workspace bytes pmx_ccl26 needs for geometry g. */
+int pmx_ccl_workspace_size(const pmx_geom *g, size_t *bytes);
+
+/* 26-connected labelling of voxels != 0; neighbours join when their values
+ * are equal (multi-label, 0 is background), wrapping on PERIODIC faces
+ * (WALL is rejected: wall pads are ordinary zero voxels, label them as END).
+ * Components are numbered 1..N in C-order of their first voxel within the
+ * box; exactly one of labels32 / labels64 receives them (output view);
+ * *count (device int64) receives N. */
+int pmx_ccl26(const uint8_t *img, const pmx_geom *g, int32_t *labels32, uint64_t *labels64,
+              int64_t *count, void *workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* flags[l] = 1 for every non-zero label l in the sub-box lo[3] + [0,
+ * extent[3]) of a C-order label array (label_bytes 4 or 8, element
+ * strides[2] for x and y).  flags must hold max label + 1 bytes. */
+int pmx_mark_labels(const void *labels, int label_bytes, const int64_t *strides, const int64_t *lo,
+                    const int64_t *extent, uint8_t *flags, hipStream_t stream);
+
+/* out[i] = keep ? img[i] : 0 over n contiguous voxels, keep decided from
+ * the voxel's label l != 0: mode 0 flags_a[l]; mode 1 flags_a[l] &&
+ * flags_b[l]; mode 2 !flags_a[l] && !flags_b[l]; and, if phase >= 0,
+ * img[i] == phase. */
+int pmx_keep_labels(const uint8_t *img, const void *labels, int label_bytes, int64_t n,
+                    const uint8_t *flags_a, const uint8_t *flags_b, int mode, int phase,
+                    uint8_t *out, hipStream_t stream);
+
+/* Porosimetry steps over n contiguous voxels.  pmx_psd_update: map[i] =
+ * code where pm[i] == 1 && temp[i] == 1 && map[i] == 0 (the first-invasion
+ * rule of pore_size_distribution, porosimetry.py:220-225, with radius codes
+ * index+1 instead of radius values).  pmx_codes_to_values: out[i] =
+ * values[map[i]-1] (0.0 for code 0).  pmx_code_histogram: counts[c] =
+ * number of voxels with pm == 1 and map == c inside the sub-box lo +
+ * [0, extent) of a C-order array of `shape` (c in 0..n_codes <= 4096).
+ * pmx_both_one: out = (a == 1 && b == 1). */
+int pmx_psd_update(const uint8_t *pm, const uint8_t *temp, int64_t n, int32_t code, int32_t *map,
+                   hipStream_t stream);
+int pmx_codes_to_values(const int32_t *map, int64_t n, const double *values, int32_t n_values,
+                        double *out, hipStream_t stream);
+int pmx_code_histogram(const uint8_t *pm, const int32_t *map, const int64_t *shape, const int64_t *lo,
+                       const int64_t *extent, int32_t n_codes, uint64_t *counts, hipStream_t stream);
+int pmx_both_one(const uint8_t *a, const uint8_t *b, int64_t n, uint8_t *out, hipStream_t stream);
 
 /* Sphere packs.  spheres: n_spheres x (cx, cy, cz, r) float64 in voxel
  * units (voxel (i,j,k) has its centroid at (i+.5, j+.5, k+.5)); a voxel is

@@ -40,8 +40,20 @@ SIGNATURES = {
     "pmx_profile_end": (C.c_int, [P, C.c_int, P, P, C.c_int, P]),
     "pmx_edt_workspace_size": (C.c_int, [P, P]),
     "pmx_edt": (C.c_int, [P, P, P, P, P, P, P, SZ, P]),
+    "pmx_edt_ref_workspace_size": (C.c_int, [P, P]),
+    "pmx_edt_ref": (C.c_int, [P, P, P, P, P, C.c_int, P, P, C.c_int, P, SZ, P]),
+    "pmx_halo_morph_workspace_size": (C.c_int, [P, P, P]),
+    "pmx_halo_morph": (C.c_int, [P, P, P, P, P, C.c_int, C.c_double, C.c_int, U8, U8, U32, P, P, SZ, P]),
     "pmx_morph": (C.c_int, [P, P, C.c_int, I64, U8, U8, U32, P, P, SZ, P]),
     "pmx_fill_periodic_pads": (C.c_int, [P, C.c_int, P, P, P, P]),
+    "pmx_ccl_workspace_size": (C.c_int, [P, P]),
+    "pmx_ccl26": (C.c_int, [P, P, P, P, P, P, SZ, P]),
+    "pmx_mark_labels": (C.c_int, [P, C.c_int, P, P, P, P, P]),
+    "pmx_keep_labels": (C.c_int, [P, P, C.c_int, I64, P, P, C.c_int, C.c_int, P, P]),
+    "pmx_psd_update": (C.c_int, [P, P, I64, I32, P, P]),
+    "pmx_codes_to_values": (C.c_int, [P, I64, P, I32, P, P]),
+    "pmx_code_histogram": (C.c_int, [P, P, P, P, P, I32, P, P]),
+    "pmx_both_one": (C.c_int, [P, P, I64, P, P]),
     "pmx_sphere_first_cover": (C.c_int, [P, I64, P, P, P, P, P]),
     "pmx_sphere_image": (C.c_int, [P, P, I32, P, P]),
 }

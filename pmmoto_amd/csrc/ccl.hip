@@ -1,0 +1,394 @@
+// ccl.hip -- 26-connected multi-label connected components.
+//
+// Replaces cc3d.connected_components(img, return_N=True) used at
+// src/pmmoto/filters/connected_components.py:52-54 and, for periodic
+// faces, the boundary-label matching of connect_subdomain_boundaries
+// (:65-101, core/voxels.py:167-397): the wrap is part of the neighbourhood.
+//
+//   init      parent[i] = i for voxels != 0, background marked
+//   merge     each voxel unions with its 13 backward 26-neighbours of the
+//             same value (wrapping on periodic axes); lock-free union by
+//             atomicCAS hooking the larger root under the smaller, so every
+//             component's root is its smallest C-order index
+//   compress  parent[i] = root(i)
+//   rank      roots counted per 4096-voxel tile, tile counts scanned, each
+//             root's rank written over its parent entry as -(rank+1)
+//   label     label[i] = rank of root(i) + 1: components numbered 1..N in
+//             C-order of their first voxel (the canonical numbering)
+#include "common.hpp"
+
+namespace pmx {
+namespace {
+
+constexpr int kBg = INT32_MIN;
+constexpr int kTile = 4096;  // voxels per rank tile: 256 lanes x 16
+constexpr int kPer = 16;
+
+struct Box {
+    int n0, n1, n2;
+    int64_t is0, is1;  // input strides
+    int per0, per1, per2;
+};
+
+__device__ __forceinline__ int ld(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+__device__ __forceinline__ int find_root(int *parent, int x)
+{
+    int p = ld(parent + x);
+    while (p != x) {
+        const int gp = ld(parent + p);
+        if (gp != p) parent[x] = gp;  // path halving; stays within the tree
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+__device__ __forceinline__ void unite(int *parent, int a, int b)
+{
+    while (true) {
+        a = find_root(parent, a);
+        b = find_root(parent, b);
+        if (a == b) return;
+        const int hi = max(a, b), lo = min(a, b);
+        const int old = atomicCAS(parent + hi, hi, lo);
+        if (old == hi) return;
+        a = old;
+        b = lo;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_init(const uint8_t *__restrict__ img, Box b,
+                                              int *__restrict__ parent)
+{
+    const int64_t n = (int64_t)b.n0 * b.n1 * b.n2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t z = i % b.n2, t = i / b.n2, y = t % b.n1, x = t / b.n1;
+        parent[i] = img[x * b.is0 + y * b.is1 + z] ? (int)i : kBg;
+    }
+}
+
+// one thread per voxel; z fastest for coalesced image reads
+__global__ __launch_bounds__(256) void k_merge(const uint8_t *__restrict__ img, Box b,
+                                               int *__restrict__ parent)
+{
+    const int64_t n = (int64_t)b.n0 * b.n1 * b.n2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(i % b.n2);
+        const int64_t t = i / b.n2;
+        const int y = (int)(t % b.n1), x = (int)(t / b.n1);
+        const uint8_t v = img[x * b.is0 + y * b.is1 + z];
+        if (!v) continue;
+        // 13 backward neighbours: dx=-1 (9), dx=0,dy=-1 (3), dx=0,dy=0,dz=-1 (1)
+#pragma unroll
+        for (int k = 0; k < 13; ++k) {
+            int dx, dy, dz;
+            if (k < 9) {
+                dx = -1;
+                dy = k / 3 - 1;
+                dz = k % 3 - 1;
+            } else if (k < 12) {
+                dx = 0;
+                dy = -1;
+                dz = k - 10;
+            } else {
+                dx = 0;
+                dy = 0;
+                dz = -1;
+            }
+            int nx = x + dx, ny = y + dy, nz = z + dz;
+            if (nx < 0) {
+                if (!b.per0) continue;
+                nx += b.n0;
+            }
+            if (ny < 0 || ny >= b.n1) {
+                if (!b.per1) continue;
+                ny = ny < 0 ? ny + b.n1 : ny - b.n1;
+            }
+            if (nz < 0 || nz >= b.n2) {
+                if (!b.per2) continue;
+                nz = nz < 0 ? nz + b.n2 : nz - b.n2;
+            }
+            if (img[(int64_t)nx * b.is0 + (int64_t)ny * b.is1 + nz] != v) continue;
+            const int j = (int)(((int64_t)nx * b.n1 + ny) * b.n2 + nz);
+            unite(parent, (int)i, j);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_compress(int *__restrict__ parent, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int p = parent[i];
+        if (p == kBg || p == (int)i) continue;
+        parent[i] = find_root(parent, (int)i);
+    }
+}
+
+__device__ __forceinline__ int block_excl_scan(int v, int *sh, int *total)
+{
+    // 256 lanes: wave inclusive scans then wave offsets
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) sh[w] = inc;
+    __syncthreads();
+    int off = 0;
+    for (int k = 0; k < w; ++k) off += sh[k];
+    *total = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(256) void k_count_roots(const int *__restrict__ parent, int64_t n,
+                                                     int *__restrict__ tile_counts)
+{
+    __shared__ int sh[4];
+    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kPer;
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int64_t i = base + k;
+        if (i < n && parent[i] == (int)i) ++c;
+    }
+    int total;
+    block_excl_scan(c, sh, &total);
+    if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
+}
+
+// single block: exclusive scan of tile counts in place, total to *count
+__global__ __launch_bounds__(1024) void k_scan_tiles(int *__restrict__ tile_counts, int64_t ntiles,
+                                                     int64_t *__restrict__ count)
+{
+    __shared__ int64_t sh[1024];
+    const int64_t per = (ntiles + 1023) / 1024;
+    const int64_t b0 = threadIdx.x * per;
+    int64_t s = 0;
+    for (int64_t k = 0; k < per; ++k)
+        if (b0 + k < ntiles) s += tile_counts[b0 + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int64_t t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int64_t run = sh[threadIdx.x] - s;
+    for (int64_t k = 0; k < per; ++k)
+        if (b0 + k < ntiles) {
+            const int c = tile_counts[b0 + k];
+            tile_counts[b0 + k] = (int)run;
+            run += c;
+        }
+    if (threadIdx.x == 1023) *count = sh[1023];
+}
+
+__global__ __launch_bounds__(256) void k_rank_roots(int *__restrict__ parent, int64_t n,
+                                                    const int *__restrict__ tile_offsets)
+{
+    __shared__ int sh[4];
+    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kPer;
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int64_t i = base + k;
+        if (i < n && parent[i] == (int)i) ++c;
+    }
+    int total;
+    int r = block_excl_scan(c, sh, &total) + tile_offsets[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int64_t i = base + k;
+        if (i < n && parent[i] == (int)i) parent[i] = -(++r);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_labels(const int *__restrict__ parent, Box b, int64_t os0,
+                                                int64_t os1, T *__restrict__ out)
+{
+    const int64_t n = (int64_t)b.n0 * b.n1 * b.n2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int p = parent[i];
+        T lab = 0;
+        if (p != kBg) lab = (T)(p < 0 ? -(int64_t)p : -(int64_t)parent[p]);
+        const int64_t z = i % b.n2, t = i / b.n2, y = t % b.n1, x = t / b.n1;
+        out[x * os0 + y * os1 + z] = lab;
+    }
+}
+
+// flags[label] = 1 for every label inside the sub-box [lo, hi)
+template <class T>
+__global__ __launch_bounds__(256) void k_mark(const T *__restrict__ labels, int64_t s0, int64_t s1,
+                                              int64_t lo0, int64_t lo1, int64_t lo2, int64_t e0,
+                                              int64_t e1, int64_t e2, uint8_t *__restrict__ flags)
+{
+    const int64_t n = e0 * e1 * e2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t z = i % e2, t = i / e2, y = t % e1, x = t / e1;
+        const T l = labels[(lo0 + x) * s0 + (lo1 + y) * s1 + lo2 + z];
+        if (l) flags[l] = 1;
+    }
+}
+
+// out = keep(label) ? img : 0 over the whole array
+template <class T>
+__global__ __launch_bounds__(256) void k_keep(const uint8_t *__restrict__ img, const T *__restrict__ labels,
+                                              int64_t n, const uint8_t *__restrict__ fa,
+                                              const uint8_t *__restrict__ fb, int mode, int phase,
+                                              uint8_t *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const T l = labels[i];
+        const uint8_t v = img[i];
+        bool keep = false;
+        if (l) {
+            const bool a = fa[l] != 0, bb = fb ? fb[l] != 0 : false;
+            keep = mode == 0 ? a : (mode == 1 ? (a && bb) : (!a && !bb));
+            if (phase >= 0 && v != phase) keep = false;
+        }
+        out[i] = keep ? v : 0;
+    }
+}
+
+struct CclPlan {
+    int *parent = nullptr, *tiles = nullptr;
+    int64_t ntiles = 0;
+};
+
+size_t plan_ccl(const pmx_geom *g, void *ws, CclPlan *p)
+{
+    Carver c(ws, SIZE_MAX);
+    const int64_t n = g->n[0] * g->n[1] * g->n[2];
+    p->parent = c.take<int>((size_t)n);
+    p->ntiles = (n + kTile - 1) / kTile;
+    p->tiles = c.take<int>((size_t)p->ntiles);
+    return c.need();
+}
+
+unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 65536); }
+
+}  // namespace
+}  // namespace pmx
+
+extern "C" {
+
+int pmx_ccl_workspace_size(const pmx_geom *g, size_t *bytes)
+{
+    pmx::clear_error();
+    if (int e = pmx::validate_geom(g)) return e;
+    if (!bytes) return pmx::fail(PMX_ERR_INVALID, "bytes is NULL");
+    pmx::CclPlan p;
+    *bytes = pmx::plan_ccl(g, nullptr, &p);
+    return PMX_OK;
+}
+
+int pmx_ccl26(const uint8_t *img, const pmx_geom *g, int32_t *labels32, uint64_t *labels64,
+              int64_t *count, void *workspace, size_t workspace_bytes, hipStream_t s)
+{
+    using namespace pmx;
+    clear_error();
+    if (int e = validate_geom(g)) return e;
+    if (!img || !count || (!labels32 == !labels64))
+        return fail(PMX_ERR_INVALID, "need img, count and exactly one of labels32/labels64");
+    const int64_t n = g->n[0] * g->n[1] * g->n[2];
+    if (n >= INT32_MAX) return fail(PMX_ERR_INVALID, "box has %lld voxels (limit 2^31-1)", (long long)n);
+    for (int a = 0; a < 3; ++a)
+        if (g->bc[2 * a] == PMX_BC_WALL || g->bc[2 * a + 1] == PMX_BC_WALL)
+            return fail(PMX_ERR_INVALID, "labelling treats faces as END or PERIODIC only");
+    CclPlan p;
+    const size_t need = plan_ccl(g, nullptr, &p);
+    if (!workspace || workspace_bytes < need)
+        return fail(PMX_ERR_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, need);
+    plan_ccl(g, workspace, &p);
+    Box b{(int)g->n[0], (int)g->n[1], (int)g->n[2], g->in_s[0], g->in_s[1],
+          g->bc[0] == PMX_BC_PERIODIC, g->bc[2] == PMX_BC_PERIODIC, g->bc[4] == PMX_BC_PERIODIC};
+    const unsigned gr = grid_for(n);
+    {
+        ProfScope ps("ccl_init", s);
+        hipLaunchKernelGGL(k_init, dim3(gr), dim3(256), 0, s, img, b, p.parent);
+    }
+    if (int e = check_launch("ccl init")) return e;
+    {
+        ProfScope ps("ccl_merge", s);
+        hipLaunchKernelGGL(k_merge, dim3(gr), dim3(256), 0, s, img, b, p.parent);
+    }
+    if (int e = check_launch("ccl merge")) return e;
+    {
+        ProfScope ps("ccl_compress", s);
+        hipLaunchKernelGGL(k_compress, dim3(gr), dim3(256), 0, s, p.parent, n);
+    }
+    if (int e = check_launch("ccl compress")) return e;
+    {
+        ProfScope ps("ccl_rank", s);
+        hipLaunchKernelGGL(k_count_roots, dim3((unsigned)p.ntiles), dim3(256), 0, s, p.parent, n, p.tiles);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, p.tiles, p.ntiles, count);
+        hipLaunchKernelGGL(k_rank_roots, dim3((unsigned)p.ntiles), dim3(256), 0, s, p.parent, n, p.tiles);
+    }
+    if (int e = check_launch("ccl rank")) return e;
+    {
+        ProfScope ps("ccl_labels", s);
+        if (labels32)
+            hipLaunchKernelGGL(k_labels<int32_t>, dim3(gr), dim3(256), 0, s, p.parent, b, g->out_s[0],
+                               g->out_s[1], labels32);
+        else
+            hipLaunchKernelGGL(k_labels<uint64_t>, dim3(gr), dim3(256), 0, s, p.parent, b, g->out_s[0],
+                               g->out_s[1], labels64);
+    }
+    return check_launch("ccl labels");
+}
+
+int pmx_mark_labels(const void *labels, int label_bytes, const int64_t *strides, const int64_t *lo,
+                    const int64_t *extent, uint8_t *flags, hipStream_t s)
+{
+    using namespace pmx;
+    clear_error();
+    if (!labels || !strides || !lo || !extent || !flags) return fail(PMX_ERR_INVALID, "NULL argument");
+    const int64_t n = extent[0] * extent[1] * extent[2];
+    if (n <= 0) return PMX_OK;
+    if (label_bytes == 4)
+        hipLaunchKernelGGL(k_mark<int32_t>, dim3(grid_for(n)), dim3(256), 0, s,
+                           static_cast<const int32_t *>(labels), strides[0], strides[1], lo[0], lo[1],
+                           lo[2], extent[0], extent[1], extent[2], flags);
+    else if (label_bytes == 8)
+        hipLaunchKernelGGL(k_mark<uint64_t>, dim3(grid_for(n)), dim3(256), 0, s,
+                           static_cast<const uint64_t *>(labels), strides[0], strides[1], lo[0], lo[1],
+                           lo[2], extent[0], extent[1], extent[2], flags);
+    else
+        return fail(PMX_ERR_INVALID, "label_bytes must be 4 or 8");
+    return check_launch("mark labels");
+}
+
+int pmx_keep_labels(const uint8_t *img, const void *labels, int label_bytes, int64_t n,
+                    const uint8_t *flags_a, const uint8_t *flags_b, int mode, int phase, uint8_t *out,
+                    hipStream_t s)
+{
+    using namespace pmx;
+    clear_error();
+    if (!img || !labels || !flags_a || !out || n < 0) return fail(PMX_ERR_INVALID, "NULL argument");
+    if (mode < 0 || mode > 2 || (mode > 0 && !flags_b)) return fail(PMX_ERR_INVALID, "bad mode");
+    if (phase > 255) return fail(PMX_ERR_INVALID, "phase must be < 256 (or < 0 for any)");
+    if (n == 0) return PMX_OK;
+    if (label_bytes == 4)
+        hipLaunchKernelGGL(k_keep<int32_t>, dim3(grid_for(n)), dim3(256), 0, s, img,
+                           static_cast<const int32_t *>(labels), n, flags_a, flags_b, mode, phase, out);
+    else if (label_bytes == 8)
+        hipLaunchKernelGGL(k_keep<uint64_t>, dim3(grid_for(n)), dim3(256), 0, s, img,
+                           static_cast<const uint64_t *>(labels), n, flags_a, flags_b, mode, phase, out);
+    else
+        return fail(PMX_ERR_INVALID, "label_bytes must be 4 or 8");
+    return check_launch("keep labels");
+}
+
+}  // extern "C"

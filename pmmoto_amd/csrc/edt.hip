@@ -173,9 +173,40 @@ struct Stack {
     int *v, *h;
     int64_t step;
     int cap;
+    int *buf;  // kChunk staging slots of this lane (stride `step`)
     __device__ __forceinline__ int &V(int k) const { return v[k * step]; }
     __device__ __forceinline__ int &H(int k) const { return h[k * step]; }
 };
+
+// Streams positions p0..p1 (unrolled coordinates; the value at p is
+// load(p mod n)) through body(p, f) in chunks of kChunk: the next chunk's
+// loads are issued into registers before the current chunk, parked in the
+// lane's LDS slots `buf` (stride `bstep`), is processed, so every lane
+// keeps kChunk loads in flight.  Stops early if body returns false.
+constexpr int kChunk = 16;
+
+template <class Load, class Body>
+__device__ __forceinline__ bool stream_line(int p0, int p1, int n, Load load, Body body, int *buf,
+                                            int64_t bstep)
+{
+    auto wrap = [n](int p) { return p < 0 ? p + n : (p >= n ? p - n : p); };
+    int nxt[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) nxt[j] = (p0 + j <= p1) ? load(wrap(p0 + j)) : 0;
+    for (int base = p0; base <= p1; base += kChunk) {
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) buf[j * bstep] = nxt[j];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            const int q = base + kChunk + j;
+            nxt[j] = (q <= p1) ? load(wrap(q)) : 0;
+        }
+        const int m = min(kChunk, p1 - base + 1);
+        for (int j = 0; j < m; ++j)
+            if (!body(base + j, buf[j * bstep])) return false;
+    }
+    return true;
+}
 
 // Envelope of one line.  `load(i)` returns the input at position i in
 // [0, n), `emit(i, d2)` receives the output at position i.  Returns false
@@ -227,22 +258,25 @@ __device__ bool envelope_line(int n, int bc_lo, int bc_hi, const Stack &st, bool
             emit(x >= n ? x - n : x, best);
         }
     };
+    int done = -1;
+    // a parabola per finite value; at a solid evaluate everything behind it
+    // and restart the stack from that solid
+    auto body = [&](int p, int f) -> bool {
+        if (f == kInf) return true;
+        if (!push(p, f)) return false;
+        if (f == 0) {
+            flush(done + 1, p);
+            done = p;
+            top = 0;
+            st.V(0) = p;
+            st.H(0) = 0;
+        }
+        return true;
+    };
 
     if (bc_lo != PMX_BC_PERIODIC) {
-        int done = -1;
         if (bc_lo == PMX_BC_WALL) push(-1, 0);
-        for (int p = 0; p < n; ++p) {
-            const int f = load(p);
-            if (f == kInf) continue;
-            if (!push(p, f)) return false;
-            if (f == 0) {
-                flush(done + 1, p);
-                done = p;
-                top = 0;
-                st.V(0) = p;
-                st.H(0) = 0;
-            }
-        }
+        if (!stream_line(0, n - 1, n, load, body, st.buf, st.step)) return false;
         if (bc_hi == PMX_BC_WALL && !push(n, 0)) return false;
         if (done < n - 1) flush(done + 1, n - 1);
         return true;
@@ -251,37 +285,26 @@ __device__ bool envelope_line(int n, int bc_lo, int bc_hi, const Stack &st, bool
     // periodic: rotate the line to start at its first solid b and run one
     // full period to the same solid at b + n
     int b = -1;
-    for (int p = 0; p < n; ++p)
-        if (load(p) == 0) {
+    stream_line(0, n - 1, n, load, [&](int p, int f) -> bool {
+        if (f == 0) {
             b = p;
-            break;
+            return false;
         }
+        return true;
+    }, st.buf, st.step);
     if (b >= 0) {
         top = 0;
         st.V(0) = b;
         st.H(0) = 0;
-        int done = b - 1;
-        for (int p = b + 1; p <= b + n; ++p) {
-            const int f = load(p < n ? p : p - n);
-            if (f == kInf) continue;
-            if (!push(p, f)) return false;
-            if (f == 0) {
-                flush(done + 1, p);
-                done = p;
-                top = 0;
-                st.V(0) = p;
-                st.H(0) = 0;
-            }
-        }
-        return true;
+        done = b - 1;
+        return stream_line(b + 1, b + n, n, load, body, st.buf, st.step);
     }
     if (!allow_triple) return false;
     // no solid on the line: parabolas of the previous, own and next period
-    for (int p = -n; p < 2 * n; ++p) {
-        const int f = load(p < 0 ? p + n : (p < n ? p : p - n));
-        if (f == kInf) continue;
-        if (!push(p, f)) return false;
-    }
+    if (!stream_line(-n, 2 * n - 1, n, load, [&](int p, int f) -> bool {
+            return f == kInf ? true : push(p, f);
+        }, st.buf, st.step))
+        return false;
     flush(0, n - 1);
     return true;
 }
@@ -306,12 +329,12 @@ __global__ __launch_bounds__(kBlock) void k_pass_y(const int32_t *__restrict__ i
                                                    uint32_t *__restrict__ ovf_count,
                                                    uint32_t *__restrict__ ovf_list)
 {
-    __shared__ int sv[K * kBlock], sh[K * kBlock];
+    __shared__ int sv[K * kBlock], sh[K * kBlock], sb[kChunk * kBlock];
     const int64_t line = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (line >= (int64_t)nx * nz) return;
     const int64_t x = line / nz, z = line - x * nz;
     const int64_t base = x * (int64_t)ny * nz + z;
-    Stack st{sv + threadIdx.x, sh + threadIdx.x, kBlock, K};
+    Stack st{sv + threadIdx.x, sh + threadIdx.x, kBlock, K, sb + threadIdx.x};
     const int32_t *src = in + base;
     int32_t *dst = out + base;
     const bool ok = envelope_line(
@@ -347,13 +370,13 @@ __global__ __launch_bounds__(kBlock) void k_pass_x(const int32_t *__restrict__ i
                                                    uint32_t *__restrict__ ovf_count,
                                                    uint32_t *__restrict__ ovf_list)
 {
-    __shared__ int sv[K * kBlock], sh[K * kBlock];
+    __shared__ int sv[K * kBlock], sh[K * kBlock], sb[kChunk * kBlock];
     const int64_t line = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (line >= (int64_t)ny * nz) return;
     const int64_t y = line / nz, z = line - y * nz;
     const int64_t pstride = (int64_t)ny * nz;
     const int32_t *src = in + line;
-    Stack st{sv + threadIdx.x, sh + threadIdx.x, kBlock, K};
+    Stack st{sv + threadIdx.x, sh + threadIdx.x, kBlock, K, sb + threadIdx.x};
     const bool ok = envelope_line(
         nx, bc_lo, bc_hi, st, false, [&](int i) { return src[(int64_t)i * pstride]; },
         [&](int i, int v) { emit_x(o, i, y, z, nx, ny, nz, v); });
@@ -371,7 +394,8 @@ __global__ __launch_bounds__(256) void k_fallback_y(const int32_t *__restrict__ 
     const uint32_t count = *ovf_count;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
-    Stack st{stk + tid, stk + (int64_t)cap * nthreads + tid, nthreads, cap};
+    Stack st{stk + tid, stk + (int64_t)cap * nthreads + tid, nthreads, cap,
+             stk + (int64_t)2 * cap * nthreads + tid};
     for (int64_t i = tid; i < (int64_t)count; i += nthreads) {
         const int64_t line = ovf_list[i];
         const int64_t x = line / nz, z = line - x * nz;
@@ -394,7 +418,8 @@ __global__ __launch_bounds__(256) void k_fallback_x(const int32_t *__restrict__ 
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
     const int64_t pstride = (int64_t)ny * nz;
-    Stack st{stk + tid, stk + (int64_t)cap * nthreads + tid, nthreads, cap};
+    Stack st{stk + tid, stk + (int64_t)cap * nthreads + tid, nthreads, cap,
+             stk + (int64_t)2 * cap * nthreads + tid};
     for (int64_t i = tid; i < (int64_t)count; i += nthreads) {
         const int64_t line = ovf_list[i];
         const int64_t y = line / nz, z = line - y * nz;
@@ -424,7 +449,7 @@ size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p)
     const size_t lines = (size_t)std::max(g->n[0] * g->n[2], g->n[1] * g->n[2]);
     p->ovf_list = c.take<uint32_t>(lines);
     p->cap = (int)(3 * std::max(g->n[0], g->n[1]) + 4);
-    p->stk = c.take<int>((size_t)2 * p->cap * kFallbackThreads);
+    p->stk = c.take<int>((size_t)(2 * p->cap + kChunk) * kFallbackThreads);
     return c.need();
 }
 

@@ -1,0 +1,134 @@
+// porosimetry.hip -- element-wise steps of the porosimetry / drainage sweeps.
+//
+// pore_size_distribution (src/pmmoto/filters/porosimetry.py:163-227) keeps,
+// per voxel, the first (largest) radius whose porosimetry image covers it:
+//   img_results = where(pm.img == 1 & img_results == 0 & img_temp == 1, r, img_results)
+// Here the map holds radius codes (index + 1 into the descending radius
+// list, 0 = never invaded) so the sweep stays integer; pmx_codes_to_values
+// writes the float64 radius map at the end, and pmx_code_histogram counts
+// own pore voxels per code for saturation curves.
+#include "common.hpp"
+
+namespace pmx {
+namespace {
+
+unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 65536); }
+
+__global__ __launch_bounds__(256) void k_psd_update(const uint8_t *__restrict__ pm,
+                                                    const uint8_t *__restrict__ temp, int64_t n,
+                                                    int32_t code, int32_t *__restrict__ map)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (pm[i] == 1 && temp[i] == 1 && map[i] == 0) map[i] = code;
+}
+
+__global__ __launch_bounds__(256) void k_codes_to_values(const int32_t *__restrict__ map, int64_t n,
+                                                         const double *__restrict__ values,
+                                                         int32_t n_values, double *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t c = map[i];
+        out[i] = (c > 0 && c <= n_values) ? values[c - 1] : 0.0;
+    }
+}
+
+// counts[c] += number of own voxels with pm == 1 and map == c
+__global__ __launch_bounds__(256) void k_code_hist(const uint8_t *__restrict__ pm,
+                                                   const int32_t *__restrict__ map, int64_t n1,
+                                                   int64_t n2, int64_t lo0, int64_t lo1, int64_t lo2,
+                                                   int64_t e0, int64_t e1, int64_t e2, int32_t n_codes,
+                                                   unsigned long long *__restrict__ counts)
+{
+    extern __shared__ unsigned long long sh[];
+    for (int c = threadIdx.x; c <= n_codes; c += blockDim.x) sh[c] = 0;
+    __syncthreads();
+    const int64_t n = e0 * e1 * e2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t z = i % e2, t = i / e2, y = t % e1, x = t / e1;
+        const int64_t v = ((lo0 + x) * n1 + lo1 + y) * n2 + lo2 + z;
+        if (pm[v] == 1) {
+            int32_t c = map[v];
+            if (c < 0 || c > n_codes) c = 0;
+            atomicAdd(&sh[c], 1ull);
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c <= n_codes; c += blockDim.x)
+        if (sh[c]) atomicAdd(&counts[c], sh[c]);
+}
+
+__global__ __launch_bounds__(256) void k_both_one(const uint8_t *__restrict__ a,
+                                                  const uint8_t *__restrict__ b, int64_t n,
+                                                  uint8_t *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (a[i] == 1 && b[i] == 1) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_fill_u64(unsigned long long *a, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = 0;
+}
+
+}  // namespace
+}  // namespace pmx
+
+extern "C" {
+
+int pmx_psd_update(const uint8_t *pm, const uint8_t *temp, int64_t n, int32_t code, int32_t *map,
+                   hipStream_t s)
+{
+    pmx::clear_error();
+    if (!pm || !temp || !map || n < 0 || code <= 0) return pmx::fail(PMX_ERR_INVALID, "bad argument");
+    if (n == 0) return PMX_OK;
+    hipLaunchKernelGGL(pmx::k_psd_update, dim3(pmx::grid_for(n)), dim3(256), 0, s, pm, temp, n, code, map);
+    return pmx::check_launch("psd update");
+}
+
+int pmx_codes_to_values(const int32_t *map, int64_t n, const double *values, int32_t n_values,
+                        double *out, hipStream_t s)
+{
+    pmx::clear_error();
+    if (!map || !values || !out || n < 0 || n_values < 0) return pmx::fail(PMX_ERR_INVALID, "bad argument");
+    if (n == 0) return PMX_OK;
+    hipLaunchKernelGGL(pmx::k_codes_to_values, dim3(pmx::grid_for(n)), dim3(256), 0, s, map, n, values,
+                       n_values, out);
+    return pmx::check_launch("codes to values");
+}
+
+int pmx_code_histogram(const uint8_t *pm, const int32_t *map, const int64_t *shape, const int64_t *lo,
+                       const int64_t *extent, int32_t n_codes, uint64_t *counts, hipStream_t s)
+{
+    pmx::clear_error();
+    if (!pm || !map || !shape || !lo || !extent || !counts || n_codes < 0 || n_codes > 4096)
+        return pmx::fail(PMX_ERR_INVALID, "bad argument");
+    for (int a = 0; a < 3; ++a)
+        if (lo[a] < 0 || extent[a] < 0 || lo[a] + extent[a] > shape[a])
+            return pmx::fail(PMX_ERR_INVALID, "sub-box outside the array on axis %d", a);
+    auto *c = reinterpret_cast<unsigned long long *>(counts);
+    hipLaunchKernelGGL(pmx::k_fill_u64, dim3(1), dim3(256), 0, s, c, (int64_t)n_codes + 1);
+    const int64_t n = extent[0] * extent[1] * extent[2];
+    if (n == 0) return pmx::check_launch("histogram clear");
+    const size_t shm = sizeof(unsigned long long) * (size_t)(n_codes + 1);
+    hipLaunchKernelGGL(pmx::k_code_hist, dim3(std::min<unsigned>(pmx::grid_for(n), 2048)), dim3(256), shm, s,
+                       pm, map, shape[1], shape[2], lo[0], lo[1], lo[2], extent[0], extent[1], extent[2],
+                       n_codes, c);
+    return pmx::check_launch("code histogram");
+}
+
+int pmx_both_one(const uint8_t *a, const uint8_t *b, int64_t n, uint8_t *out, hipStream_t s)
+{
+    pmx::clear_error();
+    if (!a || !b || !out || n < 0) return pmx::fail(PMX_ERR_INVALID, "bad argument");
+    if (n == 0) return PMX_OK;
+    hipLaunchKernelGGL(pmx::k_both_one, dim3(pmx::grid_for(n)), dim3(256), 0, s, a, b, n, out);
+    return pmx::check_launch("both one");
+}
+
+}  // extern "C"

@@ -1,3 +1,3 @@
-from . import distance
+from . import connected_components, distance, morphological_operators, porosimetry
 
-__all__ = ["distance"]
+__all__ = ["connected_components", "distance", "morphological_operators", "porosimetry"]

@@ -32,10 +32,25 @@ def _unit(resolution) -> bool:
     return all(float(r) == 1.0 for r in resolution)
 
 
-def _nonunit_error(resolution):
-    return NotImplementedError(
-        f"resolution {tuple(resolution)}: the GPU EDT is exact in voxel units; "
-        "use box=((0, Nx), (0, Ny), (0, Nz)) so that resolution is 1.0")
+def run_edt_ref(img_t: torch.Tensor, origin, n, first_axis, periodic, resolution, take_sqrt, out=None):
+    """pmx_edt_ref: the reference's float32 arithmetic (any resolution) on
+    the box `origin`+`n` of a contiguous 3-D uint8 tensor; returns a
+    float32 tensor of the full shape (box region written)."""
+    shape = tuple(img_t.shape)
+    off = (origin[0] * shape[1] + origin[1]) * shape[2] + origin[2]
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=img_t.device)
+    nn = (C.c_int64 * 3)(*[int(v) for v in n])
+    st = (C.c_int64 * 2)(shape[1] * shape[2], shape[2])
+    per = (C.c_int32 * 3)(*[1 if p else 0 for p in periodic])
+    res = (C.c_float * 3)(*[float(r) for r in resolution])
+    L = _lib.lib()
+    nbytes = C.c_size_t()
+    _lib.check(L.pmx_edt_ref_workspace_size(nn, C.byref(nbytes)))
+    ws = _device.workspace(nbytes.value)
+    _lib.check(L.pmx_edt_ref(_device.ptr(img_t, off), nn, st, _device.ptr(out, off), st, int(first_axis), per, res,
+                             1 if take_sqrt else 0, _device.ptr(ws), nbytes.value, _device.stream()))
+    return out
 
 
 def compute_box(shape, subdomain=None, periodic=None):
@@ -118,27 +133,34 @@ def _finish(t, squeeze, as_numpy):
     return _device.to_host(t, as_numpy)
 
 
-def edt3d(img, periodic=(False, False, False), resolution=(1.0, 1.0, 1.0), squared=False):
+def edt3d(img, periodic=(False, False, False), resolution=(1.0, 1.0, 1.0), squared=False, mode="auto"):
     """edt3d (euclidean_distance_transform.py:208-303): float32 distance (or
-    squared distance with ``squared=True``) to the nearest zero voxel."""
-    if not _unit(resolution):
-        raise _nonunit_error(resolution)
+    squared distance with ``squared=True``) to the nearest zero voxel.
+    mode "auto": exact integer kernels at unit resolution, the reference's
+    float32 arithmetic (pmx_edt_ref) otherwise; "reference" forces the
+    latter (bitwise the reference's periodic five-parabola hull handling)."""
     t, as_np = _device.as_device_u8(img)
     if t.dim() != 3:
         raise ValueError(f"edt3d needs a 3-D image, got {tuple(t.shape)}")
+    if mode == "reference" or not _unit(resolution):
+        out = run_edt_ref(t, (0, 0, 0), t.shape, 0, periodic, resolution, not squared)
+        return _device.to_host(out, as_np)
     origin, n, bc, pads = compute_box(tuple(t.shape), periodic=periodic)
     key = "f32sq" if squared else "f32"
     return _finish(run_edt(t, origin, n, bc, pads, (key,))[key], False, as_np)
 
 
-def edt2d(img, periodic=(False, False), resolution=(1.0, 1.0)):
+def edt2d(img, periodic=(False, False), resolution=(1.0, 1.0), mode="auto"):
     """edt2d (euclidean_distance_transform.py:104-205)."""
-    if not _unit(resolution):
-        raise _nonunit_error(resolution)
     t, as_np = _device.as_device_u8(img)
     if t.dim() != 2:
         raise ValueError(f"edt2d needs a 2-D image, got {tuple(t.shape)}")
     t3, _ = _as3d(t)
+    if mode == "reference" or not _unit(resolution):
+        # the reference sweeps axis 1 first, then axis 0: box (1, X, Y), axis 2 first
+        out = run_edt_ref(t3, (0, 0, 0), t3.shape, 2, (False,) + tuple(periodic),
+                          (1.0,) + tuple(resolution), True)
+        return _finish(out, True, as_np)
     origin, n, bc, pads = compute_box(tuple(t3.shape), periodic=(False,) + tuple(periodic))
     return _finish(run_edt(t3, origin, n, bc, pads, ("f32",))["f32"], True, as_np)
 
@@ -155,12 +177,18 @@ def edt(img, subdomain=None):
         if nd == 2:
             return edt2d(img)
         raise ValueError(f"Wrong img dimension {np.shape(img)}")
-    if not _unit(subdomain.domain.resolution):
-        raise _nonunit_error(subdomain.domain.resolution)
     t, as_np = _device.as_device_u8(img)
     if t.dim() != 3:
         raise ValueError(f"Wrong img dimension {tuple(t.shape)}")
     origin, n, bc, pads = compute_box(tuple(t.shape), subdomain=subdomain)
+    res = subdomain.domain.resolution
+    if not _unit(res):
+        # non-periodic: edt3d of the padded image, as the reference; periodic:
+        # the reference arithmetic on the own box with wrap, pads copied
+        per = [bc[2 * a] == _lib.BC_PERIODIC for a in range(3)]
+        out = run_edt_ref(t, origin, n, 0, per, res, True)
+        fill_periodic_pads([out], tuple(t.shape), pads)
+        return _device.to_host(out, as_np)
     return _device.to_host(run_edt(t, origin, n, bc, pads, ("f32",))["f32"], as_np)
 
 

@@ -1,0 +1,156 @@
+"""Morphological porosimetry on the GPU.
+
+Same call surface as src/pmmoto/filters/porosimetry.py:33-227:
+``get_sizes``, ``porosimetry``, ``pore_size_distribution``.  Every step of
+one porosimetry call is a libpmx launch on the padded device image:
+
+  erosion       pmx_morph (EDT threshold, invert = 0)
+  inlet         pmx_ccl26 + pmx_mark_labels on inlet face slabs + pmx_keep_labels
+  multiphase    the same on the multiphase image, then pmx_both_one
+  dilation      pmx_morph (invert = 1)
+
+``mode`` follows the reference: "morph" uses the FFT semantics for both
+steps, "hybrid" for the dilation only, "distance" none (the FFT semantics
+are an integer threshold of the exact EDT, see morphological_operators).  The sweep keeps integer
+radius codes on the device and converts to the float64 radius map once.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import logging
+
+import numpy as np
+import torch
+
+from .. import _device, _lib
+from . import connected_components as cc
+from . import morphological_operators as morph
+
+__all__ = ["get_sizes", "porosimetry", "pore_size_distribution", "psd_sweep", "saturation_curve"]
+
+
+def get_sizes(min_value, max_value, num_values, spacing="linear"):
+    """porosimetry.py:33-81."""
+    if min_value >= max_value:
+        raise ValueError(f"Error: min_value {min_value} must be greater than max value {max_value}")
+    if num_values <= 0:
+        raise ValueError(f"Error: num_values {num_values} must be greater than 0")
+    if spacing == "linear":
+        return np.linspace(min_value, max_value, num_values)[::-1]
+    if spacing == "log":
+        if min_value < 1:
+            raise ValueError(f"Error: min_value {min_value} must be greater than or equal to 1")
+        return np.logspace(np.log10(min_value), np.log10(max_value), num_values)[::-1]
+    raise ValueError(f"spacing {spacing} can only be 'linear' or 'log'")
+
+
+def _split_radius(radius):
+    if isinstance(radius, (int, float)):
+        return radius, radius
+    if isinstance(radius, list) and len(radius) == 2:
+        return radius[0], radius[1]
+    raise ValueError(f"Radius {radius} must either be a number or a list of length 2")
+
+
+def porosimetry_device(subdomain, pm_t: torch.Tensor, erosion_radius, dilation_radius, inlet=False,
+                       mp_t: torch.Tensor | None = None, mode="hybrid") -> torch.Tensor:
+    """One porosimetry step (porosimetry.py:117-160) on device tensors.
+    mode: "morph" FFT erosion and dilation, "hybrid" EDT erosion and FFT
+    dilation, "distance" EDT both (:120-158)."""
+    res = morph.morph_any_device(pm_t, subdomain, erosion_radius, invert=False, fft=(mode == "morph"))
+    if inlet:
+        labels, count = cc.label_device(res, subdomain)
+        n = int(count.item())
+        fin = cc._face_flags(subdomain, labels, n, "inlet")
+        res = cc.keep_device(res, labels, fin, None, 0, None)
+    if mp_t is not None:
+        # porosimetry.py:137-148: keep erosion centres inside inlet-connected phase 1
+        labels, count = cc.label_device(mp_t, subdomain)
+        n = int(count.item())
+        fin = cc._face_flags(subdomain, labels, n, "inlet")
+        n_connected = cc.keep_device(mp_t, labels, fin, None, 0, 1)
+        both = torch.empty_like(res)
+        _lib.check(_lib.lib().pmx_both_one(_device.ptr(res), _device.ptr(n_connected), res.numel(),
+                                           _device.ptr(both), _device.stream()))
+        res = both
+    # phase_exists(img_results, 1) gates the dilation in the reference; the
+    # dilation of an empty set is empty, so it runs unconditionally here.
+    return morph.morph_any_device(res, subdomain, dilation_radius, invert=True, fft=(mode in ("morph", "hybrid")))
+
+
+def porosimetry(subdomain, porous_media, radius, inlet=False, multiphase=None, mode="hybrid"):
+    """porosimetry.py:84-160 (numpy in -> numpy out, tensor -> tensor)."""
+    erosion_radius, dilation_radius = _split_radius(radius)
+    morph.check_radii(subdomain, [erosion_radius, dilation_radius])
+    if mode not in ("hybrid", "distance", "morph"):
+        raise ValueError(f"mode {mode} not recognised")
+    pm_t, as_np = _device.as_device_u8(porous_media.img)
+    mp_t = None
+    if multiphase is not None and hasattr(multiphase, "img"):
+        mp_t, _ = _device.as_device_u8(multiphase.img)
+        if int((mp_t == 1).sum().item()) == 0:  # utils.phase_exists(multiphase.img, 1)
+            mp_t = None
+    out = porosimetry_device(subdomain, pm_t, erosion_radius, dilation_radius, inlet, mp_t, mode)
+    return _device.to_host(out, as_np)
+
+
+def _sorted_radii(subdomain, porous_media, radii, num_radii):
+    if radii is not None:
+        if isinstance(radii, (int, float)):
+            return [radii]
+        if isinstance(radii, list):
+            return sorted(radii, reverse=True)
+        return np.sort(np.asarray(radii))[::-1]
+    # porosimetry.py:193-201
+    edt = np.asarray(porous_media.distance)
+    global_max = float(np.max(edt)) - 1e-6 * porous_media.max_distance
+    r_min = np.nextafter(np.min(subdomain.domain.resolution), np.inf)
+    return get_sizes(r_min, global_max, num_radii, "linear")
+
+
+def psd_sweep(subdomain, pm_t: torch.Tensor, radii, inlet=False, mode="hybrid") -> torch.Tensor:
+    """Radius-code map: code i+1 where radii[i] (descending) first invades
+    a pore voxel, 0 where none does."""
+    code_map = torch.zeros(tuple(pm_t.shape), dtype=torch.int32, device=pm_t.device)
+    for i, r in enumerate(radii):
+        temp = porosimetry_device(subdomain, pm_t, r, r, inlet, None, mode)
+        _lib.check(_lib.lib().pmx_psd_update(_device.ptr(pm_t), _device.ptr(temp), pm_t.numel(), i + 1,
+                                             _device.ptr(code_map), _device.stream()))
+    return code_map
+
+
+def pore_size_distribution(subdomain, porous_media, radii=None, num_radii=25, inlet=False, mode="hybrid"):
+    """float64 map of the largest radius whose porosimetry image covers each
+    pore voxel (porosimetry.py:163-227)."""
+    radii = _sorted_radii(subdomain, porous_media, radii, num_radii)
+    morph.check_radii(subdomain, radii)
+    pm_t, as_np = _device.as_device_u8(porous_media.img)
+    for n, r in enumerate(radii):
+        logging.debug("Processing radius %i of %i with value: %f", n + 1, len(radii), r)
+    code_map = psd_sweep(subdomain, pm_t, radii, inlet, mode)
+    values = torch.tensor([float(r) for r in radii], dtype=torch.float64, device=pm_t.device)
+    out = torch.empty(tuple(pm_t.shape), dtype=torch.float64, device=pm_t.device)
+    _lib.check(_lib.lib().pmx_codes_to_values(_device.ptr(code_map), code_map.numel(), _device.ptr(values),
+                                              len(radii), _device.ptr(out), _device.stream()))
+    return _device.to_host(out, as_np)
+
+
+def saturation_curve(subdomain, pm_t: torch.Tensor, code_map: torch.Tensor, n_radii: int) -> np.ndarray:
+    """Wetting saturation after each radius of a sweep:
+    S_w(r_i) = (count(own pore & not invaded at radii[0..i]) / V) / (count(own pore) / V),
+    in float64 in that order (multiphase.py:56-95, porousmedia.py:56-66)."""
+    shape = tuple(pm_t.shape)
+    own = subdomain.get_own_voxels()
+    lo = (C.c_int64 * 3)(int(own[0]), int(own[2]), int(own[4]))
+    ext = (C.c_int64 * 3)(int(own[1] - own[0]), int(own[3] - own[2]), int(own[5] - own[4]))
+    counts = torch.empty(n_radii + 1, dtype=torch.int64, device=pm_t.device)
+    _lib.check(_lib.lib().pmx_code_histogram(_device.ptr(pm_t), _device.ptr(code_map), (C.c_int64 * 3)(*shape),
+                                             lo, ext, n_radii, _device.ptr(counts), _device.stream()))
+    c = counts.cpu().numpy()
+    vol = float(np.prod(subdomain.domain.voxels))
+    pore = int(c.sum())
+    invaded = np.cumsum(c[1:])
+    wet = pore - invaded
+    porosity = pore / vol
+    return np.array([(w / vol) / porosity for w in wet], dtype=np.float64)

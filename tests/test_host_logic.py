@@ -1,0 +1,52 @@
+"""CPU tests of the host-side logic of the drop-in (no GPU): argument
+checks and known answers the reference's tests pin, and the integer
+radius threshold against brute force."""
+
+import numpy as np
+import pytest
+
+from pmmoto_amd.filters import morphological_operators as M
+from pmmoto_amd.filters import porosimetry as PO
+
+
+def test_gen_struct_ratio_kats():
+    # tests/filters/test_morphological_operators.py:10-26
+    assert M.gen_struct_ratio(resolution=[0.5, 0.5, 0.5], radius=1) == (2, 2, 2)
+    with pytest.raises(ValueError):
+        M.gen_struct_ratio(resolution=[0.5], radius=1)
+    with pytest.raises(ValueError):
+        M.gen_struct_ratio(resolution=[0.5, 0.5, 0.5], radius=0.1)
+
+
+def test_gen_struct_element_kats():
+    # :29-50
+    _, se = M.gen_struct_element(resolution=[0.01, 0.01, 0.01], radius=0.03)
+    assert np.sum(se) == 123
+    for res, r in (([0.5], 1), ([0.5, 0.5, 0.5], 0.1), ([0.5, 0.25, 0.5], 0.1)):
+        with pytest.raises(ValueError):
+            M.gen_struct_element(resolution=res, radius=r)
+
+
+def test_get_sizes_kats():
+    # tests/filters/test_porosimetry.py:8-41
+    np.testing.assert_array_almost_equal(PO.get_sizes(0, 10, 4, "linear"), [10.0, 6.666667, 3.333333, 0.0])
+    np.testing.assert_array_almost_equal(PO.get_sizes(1, 10, 4, "log"), [10.0, 4.64158883, 2.15443469, 1.0])
+    for args in ((5, 1, 5), (0, 10, 0), (0, 10, 5, "log"), (0, 10, 5, "undefined")):
+        with pytest.raises(ValueError):
+            PO.get_sizes(*args)
+
+
+@pytest.mark.parametrize("radius", [0, 1, 2, 2.5, 3.0000001, 5, 7.999999, 31.5, np.float64(4.0000001),
+                                    np.float32(4.0000001), np.float64(2.9999999999), np.int64(6)])
+def test_radius_threshold_vs_brute_force(radius):
+    t = M.radius_threshold(radius)
+    d = np.arange(0, 4000, dtype=np.int64)
+    dist = np.sqrt(d.astype(np.float32))
+    # numpy's own comparison with the reference's operand types
+    ok = dist <= radius
+    want = int(d[ok].max()) if ok.any() else -1
+    assert t == want
+
+
+def test_radius_threshold_negative():
+    assert M.radius_threshold(-1.0) == -1

@@ -1,0 +1,166 @@
+"""GPU parity of morphology, labelling, inlet connectivity and porosimetry
+(through the C ABI) against the oracle's restatement of the reference
+drivers.  Bar: bit-exact uint8 images and labels."""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import E, P, W, padded_case, rand_binary
+
+pytestmark = pytest.mark.gpu
+
+BT_CASES = {
+    "end": ((E, E), (E, E), (E, E)),
+    "wall": ((W, W), (W, W), (W, W)),
+    "periodic": ((P, P), (P, P), (P, P)),
+    "mixed": ((E, E), (P, P), (W, W)),
+    "cfg3": ((E, E), (W, W), (W, W)),
+}
+
+
+@pytest.mark.parametrize("bts", list(BT_CASES))
+@pytest.mark.parametrize("radius", [1, 2.5, np.float64(3.0000001), np.float32(4.5), 5])
+def test_erode_dilate_match_oracle(bts, radius):
+    from pmmoto_amd.filters import morphological_operators as M
+    own = rand_binary((22, 25, 19), 0.06, 7)
+    img, sd, box = padded_case(own, BT_CASES[bts])
+    print(f"[pmx_morph] {bts} r={radius!r} shape={img.shape}", flush=True)
+    np.testing.assert_array_equal(M.erode(sd, img, radius), O.erode(box, img, radius))
+    sparse = rand_binary((22, 25, 19), 0.995, 8)
+    img2, _, _ = padded_case(sparse, BT_CASES[bts])
+    np.testing.assert_array_equal(M.dilate(sd, img2, radius), O.dilate(box, img2, radius))
+
+
+def test_opening_closing():
+    from pmmoto_amd.filters import morphological_operators as M
+    own = rand_binary((30, 30, 30), 0.1, 9)
+    img, sd, box = padded_case(own, BT_CASES["mixed"])
+    want_open = O.dilate(box, O.erode(box, img, 2.0), 2.0)
+    np.testing.assert_array_equal(M.opening(sd, img, 2.0), want_open)
+    want_close = O.erode(box, O.dilate(box, img, 2.0), 2.0)
+    np.testing.assert_array_equal(M.closing(sd, img, 2.0), want_close)
+
+
+@pytest.mark.parametrize("bts", ["end", "periodic", "mixed"])
+def test_connect_components_match_oracle(bts):
+    from pmmoto_amd.filters import connected_components as CC
+    rng = np.random.default_rng(3)
+    own = rng.choice([0, 1, 2], size=(30, 28, 33), p=[0.45, 0.45, 0.1]).astype(np.uint8)
+    img, sd, box = padded_case(own, BT_CASES[bts])
+    print(f"[pmx_ccl26] {bts} shape={img.shape}", flush=True)
+    labels, count = CC.connect_components(img, sd)
+    want, n = O.label_padded(box, img)
+    assert labels.dtype == np.uint64
+    np.testing.assert_array_equal(labels.astype(np.int64), want)
+    assert count == (n + 1 if bts in ("periodic", "mixed") else n)
+
+
+def test_connect_components_reference_kats():
+    from pmmoto_amd.filters import connected_components as CC
+    # tests/filters/test_connect_components.py:81-186 on a 6x5x4 domain
+    own = (np.arange(120) % 256).astype(np.uint8).reshape(6, 5, 4)
+    img, sd, _ = padded_case(own, BT_CASES["end"])
+    assert CC.connect_components(img, sd)[1] == 119
+    img, sd, _ = padded_case(own + 1, BT_CASES["end"])
+    assert CC.connect_components(img, sd)[1] == 120
+    for bts in (((P, P), (E, E), (E, E)), ((E, E), (P, P), (E, E)), ((E, E), (E, E), (P, P)),
+                ((P, P), (P, P), (E, E)), ((P, P), (E, E), (P, P))):
+        img, sd, _ = padded_case(own, bts)
+        assert CC.connect_components(img, sd)[1] == 120
+
+
+def test_inlet_outlet_isolated_reference_kats():
+    import pmmoto_amd
+    from pmmoto_amd.filters import connected_components as CC
+    # test_connect_components.py:250-279
+    sd = pmmoto_amd.initialize((20, 20, 20), box=((0, 20),) * 3, inlet=((1, 0), (0, 0), (0, 0)))
+    img = np.zeros(sd.voxels, dtype=np.uint8)
+    img[0:10, 5:10, 5:10] = 1
+    img[12:15, 12:15, 12:15] = 1
+    out = CC.inlet_connected_img(sd, img)
+    assert np.all(out[12:15, 12:15, 12:15] == 0) and np.all(out[0:10, 5:10, 5:10] == 1)
+    sd = pmmoto_amd.initialize((20, 20, 20), box=((0, 20),) * 3, outlet=((1, 0), (0, 0), (0, 0)))
+    out = CC.outlet_connected_img(sd, img)
+    assert np.all(out[12:15, 12:15, 12:15] == 0) and np.all(out[0:10, 5:10, 5:10] == 1)
+    # :314-344
+    sd = pmmoto_amd.initialize((20, 20, 20), box=((0, 20),) * 3, inlet=((1, 0), (0, 0), (0, 0)),
+                               outlet=((0, 1), (0, 0), (0, 0)))
+    img = np.zeros(sd.voxels, dtype=np.uint8)
+    img[:, 2:4, 5:10] = 1
+    img[:, 6:8, 1:2] = 2
+    out = CC.inlet_outlet_connected_img(sd, img)
+    assert np.all(out[:, 2:4, 5:10] == 1) and np.all(out[:, 6:8, 1:2] == 2)
+    assert sorted(CC.inlet_outlet_connections(sd, out)) == [1, 2]
+    out = CC.inlet_outlet_connected_img(sd, img, phase=2)
+    assert np.all(out[:, 2:4, 5:10] == 0) and np.all(out[:, 6:8, 1:2] == 2)
+    # :347-377
+    sd = pmmoto_amd.initialize((30, 30, 30), box=((0, 30),) * 3, inlet=((1, 0), (0, 0), (0, 0)),
+                               outlet=((0, 1), (0, 0), (0, 0)))
+    img = np.zeros(sd.voxels, dtype=np.uint8)
+    img[1:3, 2:4, 5:10] = 1
+    img[6:9, 6:8, 1:2] = 2
+    img[:, 6:8, 3:4] = 3
+    out = CC.isolated_img(sd, img)
+    assert np.all(out[1:3, 2:4, 5:10] == 1) and np.all(out[6:9, 6:8, 1:2] == 2)
+    assert np.all(out[:, 6:8, 3:4] == 0)
+    out = CC.isolated_img(sd, img, phase=2)
+    assert np.all(out[1:3, 2:4, 5:10] == 0) and np.all(out[6:9, 6:8, 1:2] == 2)
+
+
+@pytest.mark.parametrize("bts", ["cfg3", "mixed", "periodic"])
+def test_inlet_connected_match_oracle(bts):
+    from pmmoto_amd.filters import connected_components as CC
+    own = rand_binary((26, 24, 22), 0.35, 12)
+    inlet = ((True, False), (False, False), (False, False))
+    if bts == "periodic":
+        inlet = ((False, False),) * 3
+    img, sd, box = padded_case(own, BT_CASES[bts], inlet=inlet)
+    np.testing.assert_array_equal(CC.inlet_connected_img(sd, img), O.inlet_connected(box, img))
+
+
+@pytest.mark.parametrize("inlet", [False, True])
+def test_porosimetry_match_oracle(inlet):
+    from pmmoto_amd.domain_generation import gen_pm, pack_radii, sphere_pack
+    from pmmoto_amd.filters import porosimetry as PO
+    img_t, _ = sphere_pack((40, 36, 44), 0.45, pack_radii("uniform", lo=3, hi=7), seed=21)
+    own = img_t.cpu().numpy()
+    img, sd, box = padded_case(own, BT_CASES["cfg3"], inlet=((True, False), (False, False), (False, False)),
+                               outlet=((False, True), (False, False), (False, False)))
+    pm = gen_pm(sd, img)
+    for r in (1, 2.0, 3.5, np.float64(4.0)):
+        print(f"[porosimetry] inlet={inlet} r={r!r}", flush=True)
+        got = PO.porosimetry(sd, pm, r, inlet=inlet, mode="distance")
+        np.testing.assert_array_equal(got, O.porosimetry(box, img, r, inlet))
+
+
+def test_pore_size_distribution_match_oracle():
+    from pmmoto_amd.domain_generation import gen_pm, pack_radii, sphere_pack
+    from pmmoto_amd.filters import porosimetry as PO
+    img_t, _ = sphere_pack((36, 40, 32), 0.4, pack_radii("lognormal", mean=5.0, sigma=0.3), seed=31)
+    img, sd, box = padded_case(img_t.cpu().numpy(), BT_CASES["cfg3"],
+                               inlet=((True, False), (False, False), (False, False)))
+    pm = gen_pm(sd, img)
+    radii = np.arange(8, 0, -1).astype(np.float64)
+    got = PO.pore_size_distribution(sd, pm, radii, inlet=True, mode="distance")
+    np.testing.assert_array_equal(got, O.pore_size_distribution(box, img, radii, inlet=True))
+    got_all = PO.pore_size_distribution(sd, pm, [1, 3, 2], inlet=False)
+    np.testing.assert_array_equal(got_all, O.pore_size_distribution(box, img, [3, 2, 1], inlet=False))
+
+
+def test_porosimetry_modes_and_inlet_kat():
+    import pmmoto_amd
+    from pmmoto_amd.domain_generation import gen_pm
+    from pmmoto_amd.filters import porosimetry as PO
+    # test_porosimetry.py:81-103 in voxel units (radius 2 voxels)
+    sd = pmmoto_amd.initialize((100, 100, 100), box=((0, 100),) * 3, inlet=((1, 0), (0, 0), (0, 0)))
+    img = np.zeros(sd.voxels, dtype=np.uint8)
+    img[0:10, 5:10, 5:10] = 1
+    img[12:35, 12:35, 12:35] = 1
+    pm = gen_pm(sd, img)
+    no_inlet = PO.porosimetry(sd, pm, 2, mode="morph", inlet=False)
+    with_inlet = PO.porosimetry(sd, pm, 2, mode="morph", inlet=True)
+    assert not np.array_equal(no_inlet, with_inlet)
+    assert np.array_equal(PO.porosimetry(sd, pm, 2, mode="hybrid"), PO.porosimetry(sd, pm, [2, 2], mode="distance"))
+    with pytest.raises(ValueError):
+        PO.porosimetry(sd, pm, [2, 2, 2])

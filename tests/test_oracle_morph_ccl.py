@@ -1,0 +1,100 @@
+"""Pins the oracle's morphology and labelling before they are trusted:
+scipy's binary morphology with the reference's ball element
+(tests/filters/test_morphological_operators.py:195-225 uses the same
+check), scipy's 26-connected labelling, brute-force BFS on tiny tori, and
+the reference's connected-components known answers."""
+
+import itertools
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+from oracle import oracle as O
+from tests.helpers import rand_binary
+
+
+def ball(r):
+    R = int(np.ceil(r))
+    g = np.arange(-R, R + 1, dtype=np.float64)
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    return (x * x + y * y + z * z <= r * r).astype(np.uint8)
+
+
+@pytest.mark.parametrize("r", [1.0, 2.5, 3, 4.2])
+def test_erode_dilate_vs_scipy(r):
+    img = rand_binary((30, 26, 22), 0.08, 3)
+    box = O.Box(bc=((O.BC_END, O.BC_END),) * 3, pad=((0, 0),) * 3)
+    want_e = ndimage.binary_erosion(img, ball(r), border_value=1).astype(np.uint8)
+    np.testing.assert_array_equal(O.erode(box, img, r), want_e)
+    want_d = ndimage.binary_dilation(img, ball(r)).astype(np.uint8)
+    np.testing.assert_array_equal(O.dilate(box, img, r), want_d)
+
+
+def test_periodic_dilate_vs_tiled_scipy():
+    own = rand_binary((20, 18, 16), 0.97, 5)
+    r = 3.5
+    box = O.Box(bc=((O.BC_PERIODIC, O.BC_PERIODIC),) * 3, pad=((1, 1),) * 3)
+    padded = np.pad(own, 1, mode="wrap")
+    got = O.dilate(box, padded, r)[1:-1, 1:-1, 1:-1]
+    tiled = np.tile(own, (3, 3, 3))
+    want = ndimage.binary_dilation(tiled, ball(r)).astype(np.uint8)[20:40, 18:36, 16:32]
+    np.testing.assert_array_equal(got, want)
+
+
+def test_ccl_vs_scipy_label():
+    for seed, p in ((1, 0.5), (2, 0.7), (3, 0.3)):
+        img = rand_binary((25, 30, 20), p, seed)
+        lab, n = O.ccl26(img)
+        ref, m = ndimage.label(img, structure=np.ones((3, 3, 3)))
+        assert n == m
+        np.testing.assert_array_equal(lab, O.canonical_relabel(ref))
+
+
+def bfs_labels(img, periodic):
+    shape = img.shape
+    lab = np.zeros(shape, np.int64)
+    n = 0
+    for idx in itertools.product(*[range(s) for s in shape]):
+        if img[idx] == 0 or lab[idx]:
+            continue
+        n += 1
+        stack = [idx]
+        lab[idx] = n
+        while stack:
+            c = stack.pop()
+            for d in itertools.product((-1, 0, 1), repeat=3):
+                q = []
+                ok = True
+                for a in range(3):
+                    v = c[a] + d[a]
+                    if v < 0 or v >= shape[a]:
+                        if not periodic[a]:
+                            ok = False
+                            break
+                        v %= shape[a]
+                    q.append(v)
+                q = tuple(q)
+                if ok and img[q] == img[idx] and not lab[q]:
+                    lab[q] = n
+                    stack.append(q)
+    return lab, n
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_ccl_periodic_multilabel_vs_bfs(seed):
+    rng = np.random.default_rng(seed)
+    shape = tuple(int(s) for s in rng.integers(1, 7, 3))
+    img = rng.choice([0, 1, 2], size=shape, p=[0.4, 0.4, 0.2]).astype(np.uint8)
+    per = tuple(bool(b) for b in rng.integers(0, 2, 3))
+    lab, n = O.ccl26(img, per)
+    want, m = bfs_labels(img, per)
+    assert n == m
+    np.testing.assert_array_equal(lab, want)
+
+
+def test_ccl_reference_kats():
+    # test_connect_components.py:81-136: every voxel its own value
+    img = (np.arange(120) % 256).astype(np.uint8).reshape(6, 5, 4)
+    assert O.ccl26(img)[1] == 119
+    assert O.ccl26(img + 1)[1] == 120

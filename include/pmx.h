@@ -188,6 +188,19 @@ int pmx_code_histogram(const uint8_t *pm, const int32_t *map, const int64_t *sha
                        const int64_t *extent, int32_t n_codes, uint64_t *counts, hipStream_t stream);
 int pmx_both_one(const uint8_t *a, const uint8_t *b, int64_t n, uint8_t *out, hipStream_t stream);
 
+/* Drainage (equilibrium_distribution.py:92-123, multiphase.py:43-95).
+ * pmx_drain_update: img[i] = 1 where morph[i] == 1 && wet[i] == 2 (n voxels).
+ * pmx_box_op on the sub-box lo + [0, extent) of a uint8 array of `shape`:
+ * op 0 fills it with value (update_reservoir); op 1 writes the number of
+ * voxels equal to value to *count (device uint64; volume fractions).
+ * pmx_max_f32: *out (device) = max of n non-negative float32 values
+ * (determine_maximum, utils.py:148-162). */
+int pmx_drain_update(const uint8_t *morph, const uint8_t *wet, int64_t n, uint8_t *img,
+                     hipStream_t stream);
+int pmx_box_op(uint8_t *arr, const int64_t *shape, const int64_t *lo, const int64_t *extent, int op,
+               uint8_t value, uint64_t *count, hipStream_t stream);
+int pmx_max_f32(const float *a, int64_t n, float *out, hipStream_t stream);
+
 /* Sphere packs.  spheres: n_spheres x (cx, cy, cz, r) float64 in voxel
  * units (voxel (i,j,k) has its centroid at (i+.5, j+.5, k+.5)); a voxel is
  * covered by a sphere when dx*dx+dy*dy (+) dz*dz <= r*r in float64, periodic

@@ -262,10 +262,11 @@ class Box:
     inlet: tuple = ((False, False),) * 3
     outlet: tuple = ((False, False),) * 3
     resolution: tuple = (1.0, 1.0, 1.0)
+    reservoir: tuple = ((0, 0),) * 3
 
 
 def _own_slices(box: Box, shape):
-    return tuple(slice(p[0], s - p[1]) for p, s in zip(box.pad, shape))
+    return tuple(slice(p[0] + r[0], s - p[1] - r[1]) for p, r, s in zip(box.pad, box.reservoir, shape))
 
 
 def halo_image(box: Box, img, halo):
@@ -346,14 +347,16 @@ def label_padded(box: Box, img):
     return full, n
 
 
-def inlet_labels(box: Box, labels):
-    """gen_inlet_label_map (core/voxels.py:400-425) with the face own slabs
-    of core/features.py:76-139 (1 voxel thick on END faces)."""
+def inlet_labels(box: Box, labels, which="inlet"):
+    """gen_inlet_label_map / gen_outlet_label_map (core/voxels.py:400-454)
+    with the face own slabs of core/features.py:76-139 (1 voxel thick on
+    END faces, the outermost layer of a reservoir)."""
     own = _own_slices(box, labels.shape)
+    faces = box.inlet if which == "inlet" else box.outlet
     found = set()
     for ax in range(3):
         for side in (0, 1):
-            if box.inlet[ax][side] and box.bc[ax][side] == BC_END:
+            if faces[ax][side] and box.bc[ax][side] == BC_END:
                 idx = list(own)
                 p = box.pad[ax][side]
                 if side == 0:
@@ -366,12 +369,13 @@ def inlet_labels(box: Box, labels):
     return found
 
 
-def inlet_connected(box: Box, img, phase=None):
-    """inlet_connected_img (connected_components.py:254-296): voxels whose
-    component touches the inlet keep their phase value, others become 0."""
+def inlet_connected(box: Box, img, phase=None, which="inlet"):
+    """inlet_connected_img / outlet_connected_img
+    (connected_components.py:254-341): voxels whose component touches the
+    face keep their phase value, others become 0."""
     lab, _ = label_padded(box, img)
     keep = np.zeros(int(lab.max()) + 1, dtype=bool)
-    for l in inlet_labels(box, lab):
+    for l in inlet_labels(box, lab, which):
         keep[l] = True
     out = np.where(keep[lab], img, 0).astype(np.uint8)
     if phase is not None:
@@ -379,22 +383,92 @@ def inlet_connected(box: Box, img, phase=None):
     return out
 
 
-def porosimetry(box: Box, pm_img, radius, inlet=False):
-    """porosimetry, mode="distance" (porosimetry.py:84-160), no multiphase."""
-    res = erode(box, pm_img, radius)
+def struct_element(resolution, radius):
+    """gen_struct_element (morphological_operators.py:66-100)."""
+    ratio = struct_ratio(resolution, radius)
+    grids = [np.linspace(-r, r, r * 2 + 1) for r in ratio]
+    xg, yg, zg = np.meshgrid(*grids, indexing="ij")
+    s = xg**2 + yg**2 + zg**2
+    _radius = radius / resolution[0]
+    return ratio, np.array(s <= _radius * _radius, dtype=np.uint8)
+
+
+def erode_fft(box: Box, img, radius):
+    """subtraction, FFT mode (morphological_operators.py:203-217)."""
+    from scipy.signal import fftconvolve
+
+    ratio, se = struct_element(box.resolution, radius)
+    halo, pads = halo_image(box, img, ratio)
+    _pad = max(ratio) if any(b == BC_END for pair in box.bc for b in pair) else 1
+    _img = np.pad(halo, _pad, mode="constant", constant_values=1)
+    _img = fftconvolve(_img, se, mode="same") > (se.sum() - 0.1)
+    out = unpad(_img, ((_pad, _pad),) * 3).astype(np.uint8)
+    return set_wall_bcs(box, unpad(out, pads))
+
+
+def dilate_fft(box: Box, img, radius):
+    """addition, FFT mode (morphological_operators.py:131-133)."""
+    from scipy.signal import fftconvolve
+
+    ratio, se = struct_element(box.resolution, radius)
+    halo, pads = halo_image(box, img, ratio)
+    out = (fftconvolve(halo, se, mode="same") > 0.1).astype(np.uint8)
+    return set_wall_bcs(box, unpad(out, pads))
+
+
+def porosimetry(box: Box, pm_img, radius, inlet=False, mp_img=None, mode="distance"):
+    """porosimetry (porosimetry.py:84-160)."""
+    er, dr = (radius, radius) if not isinstance(radius, list) else radius
+    res = erode_fft(box, pm_img, er) if mode == "morph" else erode(box, pm_img, er)
     if inlet:
         res = inlet_connected(box, res)
     if np.count_nonzero(res == 1) > 0:  # utils.phase_exists (utils.py:127-145)
-        res = dilate(box, res, radius)
+        if mp_img is not None and np.count_nonzero(mp_img == 1) > 0:
+            n_connected = inlet_connected(box, mp_img, 1)
+            res = np.where((res != 1) | (n_connected != 1), 0, res)
+        res = dilate_fft(box, res, dr) if mode in ("morph", "hybrid") else dilate(box, res, dr)
     return res.astype(np.uint8)
 
 
-def pore_size_distribution(box: Box, pm_img, radii, inlet=False):
-    """pore_size_distribution, mode="distance" (porosimetry.py:163-227)."""
+def update_reservoir(box: Box, img, value):
+    """subdomain_padded.py:285-307."""
+    img = img.copy()
+    for a, (lo, hi) in enumerate(box.reservoir):
+        idx = [slice(None)] * 3
+        if lo:
+            idx[a] = slice(0, lo)
+            img[tuple(idx)] = value
+        if hi:
+            idx[a] = slice(img.shape[a] - hi, None)
+            img[tuple(idx)] = value
+    return img
+
+
+def drainage(box: Box, pm_img, mp_img, capillary_pressures, domain_voxels, max_distance, mode="hybrid"):
+    """drainage, standard method (equilibrium_distribution.py:25-179);
+    returns (wetting saturations, final phase image)."""
+    own = _own_slices(box, pm_img.shape)
+    vol = np.prod(domain_voxels)
+    porosity = np.count_nonzero(pm_img[own] == 1) / vol
+    out = np.zeros(len(capillary_pressures))
+    for n, pc in enumerate(capillary_pressures):
+        radius = 2.0 * 1 / pc * np.cos(np.deg2rad(0))
+        if radius < min(box.resolution) or radius > max_distance:
+            morph = np.zeros_like(pm_img, dtype=np.uint8)
+        else:
+            morph = porosimetry(box, pm_img, radius, inlet=True, mp_img=mp_img, mode=mode)
+        w = inlet_connected(box, mp_img, 2, which="outlet")
+        mp_img = update_reservoir(box, np.where((morph == 1) & (w == 2), 1, mp_img).astype(np.uint8), 1)
+        out[n] = float(np.count_nonzero(mp_img[own] == 2) / vol) / porosity
+    return out, mp_img
+
+
+def pore_size_distribution(box: Box, pm_img, radii, inlet=False, mode="distance"):
+    """pore_size_distribution (porosimetry.py:163-227)."""
     radii = np.sort(np.asarray(radii))[::-1]
     out = np.zeros(pm_img.shape, dtype=np.float64)
     for r in radii:
-        t = porosimetry(box, pm_img, r, inlet)
+        t = porosimetry(box, pm_img, r, inlet, mode=mode)
         if np.any(t):
             out = np.where((pm_img == 1) & (out == 0) & (t == 1), r, out)
     return out

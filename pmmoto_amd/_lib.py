@@ -54,6 +54,9 @@ SIGNATURES = {
     "pmx_codes_to_values": (C.c_int, [P, I64, P, I32, P, P]),
     "pmx_code_histogram": (C.c_int, [P, P, P, P, P, I32, P, P]),
     "pmx_both_one": (C.c_int, [P, P, I64, P, P]),
+    "pmx_drain_update": (C.c_int, [P, P, I64, P, P]),
+    "pmx_box_op": (C.c_int, [P, P, P, P, C.c_int, U8, P, P]),
+    "pmx_max_f32": (C.c_int, [P, I64, P, P]),
     "pmx_sphere_first_cover": (C.c_int, [P, I64, P, P, P, P, P]),
     "pmx_sphere_image": (C.c_int, [P, P, I32, P, P]),
 }

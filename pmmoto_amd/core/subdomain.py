@@ -92,6 +92,22 @@ class PaddedSubdomain:
             out[2 * a + 1] = self.voxels[a] - p[1] - r[1]
         return out
 
+    @property
+    def box(self) -> tuple:
+        """Physical extent including pads and reservoirs (subdomain_padded.py:204-216)."""
+        res = self.domain.resolution
+        return tuple((b[0] - (p[0] + r[0]) * res[a], b[1] + (p[1] + r[1]) * res[a])
+                     for a, (b, p, r) in enumerate(zip(self.domain.box, self.pad, self.reservoir_pad)))
+
+    @property
+    def coords(self) -> list:
+        """Voxel-centroid coordinates per axis (domain_discretization.py:70-90)."""
+        out = []
+        for (lo, hi), n, res in zip(self.box, self.voxels, self.domain.resolution):
+            half = 0.5 * res
+            out.append(np.linspace(lo + half, hi - half, n))
+        return out
+
     def check_boundary_type(self, bt: BoundaryType) -> bool:
         return any(b == bt for pair in self.boundary_types for b in pair)
 

@@ -69,6 +69,54 @@ __global__ __launch_bounds__(256) void k_both_one(const uint8_t *__restrict__ a,
         out[i] = (a[i] == 1 && b[i] == 1) ? 1 : 0;
 }
 
+// drainage step (equilibrium_distribution.py:98-101): img = 1 where the
+// morphology image is 1 and the voxel holds outlet-connected wetting phase
+__global__ __launch_bounds__(256) void k_drain_update(const uint8_t *__restrict__ morph,
+                                                     const uint8_t *__restrict__ wet, int64_t n,
+                                                     uint8_t *__restrict__ img)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (morph[i] == 1 && wet[i] == 2) img[i] = 1;
+}
+
+__global__ __launch_bounds__(256) void k_box_op(uint8_t *__restrict__ a, int64_t n1, int64_t n2, int64_t lo0,
+                                                int64_t lo1, int64_t lo2, int64_t e0, int64_t e1, int64_t e2,
+                                                int op, uint8_t value, unsigned long long *__restrict__ count)
+{
+    const int64_t n = e0 * e1 * e2;
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t z = i % e2, t = i / e2, y = t % e1, x = t / e1;
+        uint8_t *p = a + ((lo0 + x) * n1 + lo1 + y) * n2 + lo2 + z;
+        if (op == 0)
+            *p = value;
+        else
+            c += *p == value;
+    }
+    if (op != 0) {
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+    }
+}
+
+// max of non-negative float32 values (IEEE order equals integer order)
+__global__ __launch_bounds__(256) void k_max_f32(const float *__restrict__ a, int64_t n, unsigned int *__restrict__ out)
+{
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, a[i]);
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+__global__ __launch_bounds__(64) void k_fill_u32(unsigned int *a, int64_t n)
+{
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0;
+}
+
 __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long *a, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -120,6 +168,48 @@ int pmx_code_histogram(const uint8_t *pm, const int32_t *map, const int64_t *sha
                        pm, map, shape[1], shape[2], lo[0], lo[1], lo[2], extent[0], extent[1], extent[2],
                        n_codes, c);
     return pmx::check_launch("code histogram");
+}
+
+int pmx_drain_update(const uint8_t *morph, const uint8_t *wet, int64_t n, uint8_t *img, hipStream_t s)
+{
+    pmx::clear_error();
+    if (!morph || !wet || !img || n < 0) return pmx::fail(PMX_ERR_INVALID, "bad argument");
+    if (n == 0) return PMX_OK;
+    hipLaunchKernelGGL(pmx::k_drain_update, dim3(pmx::grid_for(n)), dim3(256), 0, s, morph, wet, n, img);
+    return pmx::check_launch("drainage update");
+}
+
+// op 0: fill the sub-box with value; op 1: *count = number of voxels == value
+int pmx_box_op(uint8_t *arr, const int64_t *shape, const int64_t *lo, const int64_t *extent, int op,
+               uint8_t value, uint64_t *count, hipStream_t s)
+{
+    pmx::clear_error();
+    if (!arr || !shape || !lo || !extent || (op != 0 && op != 1) || (op == 1 && !count))
+        return pmx::fail(PMX_ERR_INVALID, "bad argument");
+    for (int a = 0; a < 3; ++a)
+        if (lo[a] < 0 || extent[a] < 0 || lo[a] + extent[a] > shape[a])
+            return pmx::fail(PMX_ERR_INVALID, "sub-box outside the array on axis %d", a);
+    auto *c = reinterpret_cast<unsigned long long *>(count);
+    if (op == 1) {
+        hipLaunchKernelGGL(pmx::k_fill_u64, dim3(1), dim3(64), 0, s, c, (int64_t)1);
+        if (int e = pmx::check_launch("count clear")) return e;
+    }
+    const int64_t n = extent[0] * extent[1] * extent[2];
+    if (n == 0) return PMX_OK;
+    hipLaunchKernelGGL(pmx::k_box_op, dim3(std::min<unsigned>(pmx::grid_for(n), 4096)), dim3(256), 0, s, arr,
+                       shape[1], shape[2], lo[0], lo[1], lo[2], extent[0], extent[1], extent[2], op, value, c);
+    return pmx::check_launch("box op");
+}
+
+int pmx_max_f32(const float *a, int64_t n, float *out, hipStream_t s)
+{
+    pmx::clear_error();
+    if (!a || !out || n < 1) return pmx::fail(PMX_ERR_INVALID, "bad argument");
+    auto *o = reinterpret_cast<unsigned int *>(out);
+    hipLaunchKernelGGL(pmx::k_fill_u32, dim3(1), dim3(64), 0, s, o, (int64_t)1);
+    if (int e = pmx::check_launch("max clear")) return e;
+    hipLaunchKernelGGL(pmx::k_max_f32, dim3(std::min<unsigned>(pmx::grid_for(n), 4096)), dim3(256), 0, s, a, n, o);
+    return pmx::check_launch("max");
 }
 
 int pmx_both_one(const uint8_t *a, const uint8_t *b, int64_t n, uint8_t *out, hipStream_t s)

@@ -1,4 +1,6 @@
-from .domain_generation import gen_img_random_binary, pack_radii, sphere_pack
+from .domain_generation import gen_img_random_binary, gen_mp_constant, gen_pm_inkbottle, pack_radii, sphere_pack
+from .multiphase import Multiphase
 from .porousmedia import PorousMedia, gen_pm
 
-__all__ = ["PorousMedia", "gen_img_random_binary", "gen_pm", "pack_radii", "sphere_pack"]
+__all__ = ["Multiphase", "PorousMedia", "gen_img_random_binary", "gen_mp_constant", "gen_pm",
+           "gen_pm_inkbottle", "pack_radii", "sphere_pack"]

@@ -22,7 +22,7 @@ import torch
 
 from .. import _device, _lib
 
-__all__ = ["gen_img_random_binary", "sphere_pack", "pack_radii"]
+__all__ = ["gen_img_random_binary", "sphere_pack", "pack_radii", "gen_pm_inkbottle", "gen_mp_constant"]
 
 
 def gen_img_random_binary(shape, p_zero=0.5, seed=None):
@@ -93,3 +93,32 @@ def sphere_pack(shape, porosity, radii, seed, periodic=(False, False, False), de
     _lib.check(L.pmx_sphere_image(_device.ptr(first), sh, k, _device.ptr(img), _device.stream()))
     del first
     return (img if device_out else img.cpu().numpy()), spheres[:k]
+
+
+def gen_pm_inkbottle(subdomain, r_y: float = 1.0, r_z: float = 1.0):
+    """Ink-bottle pore (domain_generation.py:260-287, _domain_generation.pyx:
+    107-133): pore where y^2/ry^2 + z^2/rz^2 <= 1 with
+    r(x) = 0.01 cos(0.01 x) + 0.5 sin(x) + 0.75, pore for x < 0, and the
+    reservoir pads set to pore.  r(x) uses the C library's sin/cos per x,
+    the rest float64 numpy in the reference's operation order."""
+    from .porousmedia import gen_pm
+
+    x, y, z = subdomain.coords
+    r = np.array([0.01 * math.cos(0.01 * xi) + 0.5 * math.sin(xi) + 0.75 for xi in x], dtype=np.float64)
+    ry = r * r_y
+    rz = r * r_z
+    yy = (y * y)[None, :, None] / (ry * ry)[:, None, None]
+    zz = (z * z)[None, None, :] / (rz * rz)[:, None, None]
+    img = ((yy + zz) <= 1).astype(np.uint8)
+    img[x < 0] = 1
+    pm = gen_pm(subdomain, img)
+    pm.img = subdomain.update_reservoir(pm.img, np.uint8(1))
+    return pm
+
+
+def gen_mp_constant(porous_media, fluid_phase: int = 1):
+    """domain_generation.py:290-306: every pore voxel holds fluid_phase."""
+    from .multiphase import Multiphase
+
+    img = np.where(np.asarray(porous_media.img) == 1, fluid_phase, 0).astype(np.uint8)
+    return Multiphase(porous_media, img, 2)

@@ -1,3 +1,3 @@
-from . import connected_components, distance, morphological_operators, porosimetry
+from . import connected_components, distance, equilibrium_distribution, morphological_operators, porosimetry
 
-__all__ = ["connected_components", "distance", "morphological_operators", "porosimetry"]
+__all__ = ["connected_components", "distance", "equilibrium_distribution", "morphological_operators", "porosimetry"]

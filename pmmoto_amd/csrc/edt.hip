@@ -6,8 +6,8 @@
 // kernels over an integer working array:
 //
 //   pass Z  one wavefront per z-row: 1-D distance to the nearest solid
-//           from 64-bit ballot masks of 64-voxel chunks and wave scans;
-//           coalesced u8 loads, coalesced int32 stores.
+//           from per-lane 16-voxel solid masks and wave scans; 16-byte
+//           loads and stores.
 //   pass Y  one lane per (x, z) line, stride nz: lower envelope of
 //           parabolas (Felzenszwalb) with exact int64 intersection tests,
 //           stack in LDS.  A zero-height parabola (a solid voxel) can
@@ -65,58 +65,82 @@ __device__ __forceinline__ int wave_incl_min_rev(int v, int lane)
     return v;
 }
 
-// Rows of up to 4 * 64 * 64 = 16384 voxels: chunk c (64 voxels) has its
-// ballot mask held by lane c % 64 in slot c / 64.
+// One wavefront per row of up to 16 * 1024 voxels.  Lane l owns voxels
+// [s*1024 + 16l, s*1024 + 16l + 16) of super-chunk s: one 16-byte load and
+// four 16-byte stores per lane per super-chunk when the row is aligned.
+// A 16-bit solid mask per lane and super-chunk gives the in-block nearest
+// solids; wave prefix-max / suffix-min scans over the lanes' last / first
+// solid give the carries from other blocks.
+constexpr int kSup = 16;
+
+template <bool kInvert>
+__device__ __forceinline__ uint32_t solid_mask16(const uint8_t *src, int e0, int nz, bool vec)
+{
+    uint32_t m = 0;
+    if (vec && e0 + 16 <= nz) {
+        const uint4 q = *reinterpret_cast<const uint4 *>(src + e0);
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t v = (w[k] >> (8 * b)) & 0xffu;
+                if (kInvert ? (v != 0) : (v == 0)) m |= 1u << (4 * k + b);
+            }
+    } else {
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const int e = e0 + b;
+            if (e < nz) {
+                const uint8_t v = src[e];
+                if (kInvert ? (v != 0) : (v == 0)) m |= 1u << b;
+            }
+        }
+    }
+    return m;
+}
+
 template <bool kInvert>
 __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img, int64_t in_sx,
                                                 int64_t in_sy, int ny, int nz, int64_t rows,
-                                                int bc_lo, int bc_hi, int32_t *__restrict__ out,
+                                                int bc_lo, int bc_hi, int vec_in, int vec_out,
+                                                int32_t *__restrict__ out,
                                                 uint32_t *__restrict__ counters)
 {
+    __shared__ int sc[4][2][kSup];  // per wave: each super-chunk's last / first solid
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t row = (int64_t)blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    const int w = threadIdx.x / kWave;
+    const int64_t row = (int64_t)blockIdx.x * (blockDim.x / kWave) + w;
     if (blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0;  // overflow counts of passes Y, X
     if (row >= rows) return;
     const int64_t x = row / ny, y = row - x * ny;
     const uint8_t *src = img + x * in_sx + y * in_sy;
     int32_t *dst = out + row * (int64_t)nz;
-    const int nchunks = (nz + kWave - 1) / kWave;
+    const int nsup = (nz + 1023) / 1024;
+    const bool vin = vec_in != 0;
 
-    uint64_t mk[4] = {0, 0, 0, 0};
+    // sweep 1: each super-chunk's last and first solid
+    for (int s = 0; s < nsup; ++s) {
+        const int e0 = s * 1024 + lane * 16;
+        const uint32_t m = solid_mask16<kInvert>(src, e0, nz, vin);
+        int l = m ? e0 + 31 - __clz(m) : -kNone;
+        int f = m ? e0 + __ffs(m) - 1 : kNone;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        for (int cc = 0; cc < kWave; ++cc) {
-            const int c = s * kWave + cc;
-            if (c >= nchunks) break;
-            const int e = c * kWave + lane;
-            bool solid = false;
-            if (e < nz) {
-                const uint8_t v = src[e];
-                solid = kInvert ? (v != 0) : (v == 0);
-            }
-            const uint64_t m = __ballot(solid);
-            if (lane == cc) mk[s] = m;
+        for (int o = 32; o > 0; o >>= 1) {
+            l = max(l, __shfl_xor(l, o, kWave));
+            f = min(f, __shfl_xor(f, o, kWave));
+        }
+        if (lane == 0) {
+            sc[w][0][s] = l;
+            sc[w][1][s] = f;
         }
     }
-    // per-chunk first / last solid index (chunk c = s*64 + lane)
-    int last_in[4], first_in[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int c = s * kWave + lane;
-        last_in[s] = mk[s] ? c * kWave + 63 - __clzll(mk[s]) : -kNone;
-        first_in[s] = mk[s] ? c * kWave + __ffsll((unsigned long long)mk[s]) - 1 : kNone;
-    }
-    // row-wide first / last solid, for the periodic wrap
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     int row_last = -kNone, row_first = kNone;
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        row_last = max(row_last, last_in[s]);
-        row_first = min(row_first, first_in[s]);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        row_last = max(row_last, __shfl_xor(row_last, o, kWave));
-        row_first = min(row_first, __shfl_xor(row_first, o, kWave));
+    for (int s = 0; s < nsup; ++s) {
+        row_last = max(row_last, sc[w][0][s]);
+        row_first = min(row_first, sc[w][1][s]);
     }
     int prev0, next0;  // boundary defaults
     if (bc_lo == PMX_BC_PERIODIC) {
@@ -126,42 +150,44 @@ __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img,
         prev0 = bc_lo == PMX_BC_WALL ? -1 : -kNone;
         next0 = bc_hi == PMX_BC_WALL ? nz : kNone;
     }
-    // exclusive prefix max of last_in / suffix min of first_in in chunk order
-    int pc[4], nc[4];
-    int carry = prev0;
+    // sweep 2: distances, super-chunk by super-chunk
+    int pcar = prev0;
+    for (int s = 0; s < nsup; ++s) {
+        int ncar = next0;
+        for (int t = nsup - 1; t > s; --t) ncar = min(ncar, sc[w][1][t]);
+        const int e0 = s * 1024 + lane * 16;
+        const uint32_t m = solid_mask16<kInvert>(src, e0, nz, vin);
+        const int l = m ? e0 + 31 - __clz(m) : -kNone;
+        const int f = m ? e0 + __ffs(m) - 1 : kNone;
+        const int inc_l = wave_incl_max(l, lane);
+        int pc = __shfl_up(inc_l, 1, kWave);
+        if (lane == 0) pc = -kNone;
+        pc = max(pc, pcar);
+        const int inc_f = wave_incl_min_rev(f, lane);
+        int nc = __shfl_down(inc_f, 1, kWave);
+        if (lane == kWave - 1) nc = kNone;
+        nc = min(nc, ncar);
+        int d2v[16];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int inc = wave_incl_max(last_in[s], lane);
-        int exc = __shfl_up(inc, 1, kWave);
-        if (lane == 0) exc = -kNone;
-        pc[s] = max(carry, exc);
-        carry = max(carry, __shfl(inc, kWave - 1, kWave));
-    }
-    carry = next0;
-#pragma unroll
-    for (int s = 3; s >= 0; --s) {
-        const int inc = wave_incl_min_rev(first_in[s], lane);
-        int exc = __shfl_down(inc, 1, kWave);
-        if (lane == kWave - 1) exc = kNone;
-        nc[s] = min(carry, exc);
-        carry = min(carry, __shfl(inc, 0, kWave));
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        for (int cc = 0; cc < kWave; ++cc) {
-            const int c = s * kWave + cc;
-            if (c >= nchunks) break;
-            const uint64_t m = __shfl(mk[s], cc, kWave);
-            const int pcar = __shfl(pc[s], cc, kWave);
-            const int ncar = __shfl(nc[s], cc, kWave);
-            const int e = c * kWave + lane;
-            const uint64_t below = m & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
-            const uint64_t above = m & ~((1ull << lane) - 1);
-            const int prev = below ? c * kWave + 63 - __clzll(below) : pcar;
-            const int next = above ? c * kWave + __ffsll((unsigned long long)above) - 1 : ncar;
+        for (int b = 0; b < 16; ++b) {
+            const uint32_t below = m & ((2u << b) - 1u);
+            const uint32_t above = m & ~((1u << b) - 1u);
+            const int e = e0 + b;
+            const int prev = below ? e0 + 31 - __clz(below) : pc;
+            const int next = above ? e0 + __ffs(above) - 1 : nc;
             const int d = min(e - prev, next - e);
-            if (e < nz) dst[e] = d >= kNone / 2 ? kInf : d * d;
+            d2v[b] = d >= kNone / 2 ? kInf : d * d;
         }
+        if (vec_out && e0 + 16 <= nz) {
+            int4 *q = reinterpret_cast<int4 *>(dst + e0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) q[k] = make_int4(d2v[4 * k], d2v[4 * k + 1], d2v[4 * k + 2], d2v[4 * k + 3]);
+        } else {
+#pragma unroll
+            for (int b = 0; b < 16; ++b)
+                if (e0 + b < nz) dst[e0 + b] = d2v[b];
+        }
+        pcar = max(pcar, sc[w][0][s]);
     }
 }
 
@@ -469,12 +495,14 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     const dim3 gz((unsigned)((rows + 3) / 4));
     {
     ProfScope ps("edt_pass_z", s);
+    const int vec_in = ((uintptr_t)img % 16 == 0 && g->in_s[0] % 16 == 0 && g->in_s[1] % 16 == 0) ? 1 : 0;
+    const int vec_out = ((uintptr_t)p.a % 16 == 0 && nz % 4 == 0) ? 1 : 0;
     if (invert)
         hipLaunchKernelGGL(k_pass_z<true>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], p.a, p.counters);
+                           g->bc[4], g->bc[5], vec_in, vec_out, p.a, p.counters);
     else
         hipLaunchKernelGGL(k_pass_z<false>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], p.a, p.counters);
+                           g->bc[4], g->bc[5], vec_in, vec_out, p.a, p.counters);
     }
     if (int e = check_launch("pass z")) return e;
 

@@ -42,6 +42,10 @@ def gpu_edt(img, bc, outputs=("d2",)):
     ((2, 3, 1), 0.3, 5),
     ((40, 1, 50), 0.01, 6),
     ((5, 90, 300), 0.003, 7),
+    ((3, 5, 2100), 0.002, 8),
+    ((2, 4, 1024), 0.001, 9),
+    ((2, 3, 1025), 0.01, 10),
+    ((4, 3, 4099), 0.0005, 11),
 ])
 def test_d2_matches_exact_oracle(shape, p0, seed, bcname):
     img = rand_binary(shape, p0, seed)

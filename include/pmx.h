@@ -201,6 +201,21 @@ int pmx_box_op(uint8_t *arr, const int64_t *shape, const int64_t *lo, const int6
                uint8_t value, uint64_t *count, hipStream_t stream);
 int pmx_max_f32(const float *a, int64_t n, float *out, hipStream_t stream);
 
+/* The pore-size-distribution sweep of pore_size_distribution (porosimetry.py:
+ * 163-227) in one call, on a contiguous box (in and out strides equal the
+ * box; no WALL codes: pass padded images with END faces).  Level i (radii
+ * in descending order) erodes to d2 > t_erode[i] (non-increasing), keeps
+ * the part 26-connected to the inlet slab inlet_lo + [0, inlet_ext) (all of
+ * it when inlet_lo is NULL), dilates it to d2 <= t_dilate[i] and writes
+ * code i+1 into code_map where img == 1 and code_map is still 0.  One EDT
+ * of img, one union-find grown level by level, one EDT per level.  Blocks
+ * the calling thread once to read the level sizes.  n_levels <= 250. */
+int pmx_psd_workspace_size(const pmx_geom *g, size_t *bytes);
+int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo,
+                  const int64_t *inlet_ext, int n_levels, const int64_t *t_erode,
+                  const int64_t *t_dilate, uint32_t wall_mask, int32_t *code_map,
+                  void *workspace, size_t workspace_bytes, hipStream_t stream);
+
 /* Sphere packs.  spheres: n_spheres x (cx, cy, cz, r) float64 in voxel
  * units (voxel (i,j,k) has its centroid at (i+.5, j+.5, k+.5)); a voxel is
  * covered by a sphere when dx*dx+dy*dy (+) dz*dz <= r*r in float64, periodic

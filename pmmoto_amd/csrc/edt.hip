@@ -31,7 +31,7 @@
 // layer just beyond it (the reference's zero pad, subdomain.py:273-287),
 // PERIODIC = wrap (the reference's correctors and boundary hulls,
 // euclidean_distance_transform.py:395-528).
-#include "common.hpp"
+#include "edt_internal.hpp"
 
 namespace pmx {
 namespace {
@@ -335,17 +335,6 @@ __device__ bool envelope_line(int n, int bc_lo, int bc_hi, const Stack &st, bool
     return true;
 }
 
-struct PassXOut {
-    int32_t *d2;
-    float *f32sq;
-    float *f32;
-    double *f64;
-    uint8_t *u8;
-    int64_t sx, sy;
-    int32_t t;
-    uint8_t v_le, v_gt;
-    uint32_t wall_mask;
-};
 
 // pass Y: lines (x, z) along y.  in/out: contiguous box arrays.
 template <int K>
@@ -377,6 +366,10 @@ __device__ __forceinline__ void emit_x(const PassXOut &o, int64_t x, int64_t y, 
     if (o.f32sq) o.f32sq[idx] = v == kInf ? 3.40282347e+38f : (float)v;
     if (o.f32) o.f32[idx] = v == kInf ? sqrtf(3.40282347e+38f) : sqrtf((float)v);
     if (o.f64) o.f64[idx] = v == kInf ? sqrt((double)3.40282347e+38f) : sqrt((double)v);
+    if (o.psd_map) {
+        // first invasion (porosimetry.py:220-225): covered pore voxels not yet assigned
+        if (v <= o.t && o.psd_pm[idx] == 1 && o.psd_map[idx] == 0) o.psd_map[idx] = o.psd_code;
+    }
     if (o.u8) {
         uint8_t r = v <= o.t ? o.v_le : o.v_gt;
         const uint32_t w = o.wall_mask;
@@ -458,12 +451,7 @@ __global__ __launch_bounds__(256) void k_fallback_x(const int32_t *__restrict__ 
 
 constexpr int kStackK = 32;
 
-struct EdtPlan {
-    int32_t *a = nullptr, *b = nullptr;
-    uint32_t *counters = nullptr, *ovf_list = nullptr;
-    int *stk = nullptr;
-    int cap = 0;
-};
+}  // namespace
 
 size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p)
 {
@@ -533,7 +521,6 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     return check_launch("pass x fallback");
 }
 
-}  // namespace
 
 int validate_geom(const pmx_geom *g)
 {
@@ -575,8 +562,13 @@ int pmx_edt(const uint8_t *img, const pmx_geom *g, int32_t *d2, float *f32sq, fl
 {
     pmx::clear_error();
     if (!d2 && !f32sq && !f32 && !f64) return pmx::fail(PMX_ERR_INVALID, "no output requested");
-    pmx::PassXOut o{d2, f32sq, f32, f64, nullptr, g ? g->out_s[0] : 0, g ? g->out_s[1] : 0,
-                    0, 0, 0, 0};
+    pmx::PassXOut o{};
+    o.d2 = d2;
+    o.f32sq = f32sq;
+    o.f32 = f32;
+    o.f64 = f64;
+    o.sx = g ? g->out_s[0] : 0;
+    o.sy = g ? g->out_s[1] : 0;
     return pmx::run_edt(img, g, false, o, workspace, workspace_bytes, stream);
 }
 
@@ -588,8 +580,14 @@ int pmx_morph(const uint8_t *img, const pmx_geom *g, int invert, int64_t t, uint
     if (!out) return pmx::fail(PMX_ERR_INVALID, "out is NULL");
     if (wall_mask > 63u) return pmx::fail(PMX_ERR_INVALID, "wall_mask has bits above 5");
     const int32_t tt = t < 0 ? -1 : (t > (int64_t)INT32_MAX - 1 ? INT32_MAX - 1 : (int32_t)t);
-    pmx::PassXOut o{nullptr, nullptr, nullptr, nullptr, out, g ? g->out_s[0] : 0,
-                    g ? g->out_s[1] : 0, tt, v_le, v_gt, wall_mask};
+    pmx::PassXOut o{};
+    o.u8 = out;
+    o.sx = g ? g->out_s[0] : 0;
+    o.sy = g ? g->out_s[1] : 0;
+    o.t = tt;
+    o.v_le = v_le;
+    o.v_gt = v_gt;
+    o.wall_mask = wall_mask;
     return pmx::run_edt(img, g, invert != 0, o, workspace, workspace_bytes, stream);
 }
 

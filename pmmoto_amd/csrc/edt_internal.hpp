@@ -1,0 +1,41 @@
+// edt_internal.hpp -- the exact EDT pipeline shared by libpmx's entry points.
+#pragma once
+
+#include "common.hpp"
+
+namespace pmx {
+
+// Outputs of the last EDT pass; any subset may be set.  Element (x, y, z)
+// of the box is written at x*sx + y*sy + z.
+struct PassXOut {
+    int32_t *d2 = nullptr;       // squared distance
+    float *f32sq = nullptr;      // float(d2), FLT_MAX for "no solid"
+    float *f32 = nullptr;        // sqrtf of f32sq
+    double *f64 = nullptr;       // sqrt(double(d2))
+    uint8_t *u8 = nullptr;       // d2 <= t ? v_le : v_gt, WALL layers 0
+    int64_t sx = 0, sy = 0;
+    int32_t t = 0;
+    uint8_t v_le = 0, v_gt = 0;
+    uint32_t wall_mask = 0;
+    // porosimetry: psd_map = psd_code where d2 <= t, psd_pm == 1, psd_map == 0
+    const uint8_t *psd_pm = nullptr;
+    int32_t *psd_map = nullptr;
+    int32_t psd_code = 0;
+};
+
+struct EdtPlan {
+    int32_t *a = nullptr, *b = nullptr;
+    uint32_t *counters = nullptr, *ovf_list = nullptr;
+    int *stk = nullptr;
+    int cap = 0;
+};
+
+// Workspace layout of run_edt for geometry g (sizes only when ws is null).
+size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p);
+
+// Exact squared EDT of img over the box of g (distance to voxels == 0, or
+// != 0 with invert), delivered through `o`; stream-ordered on s.
+int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &o, void *ws,
+            size_t ws_bytes, hipStream_t s);
+
+}  // namespace pmx

@@ -1,0 +1,349 @@
+// psd.hip -- the whole pore-size-distribution sweep in one call.
+//
+// pore_size_distribution(radii, inlet=True, mode="distance")
+// (src/pmmoto/filters/porosimetry.py:163-227) runs, per radius r (in
+// descending order), porosimetry: erosion (EDT of the image <= r -> 0),
+// the inlet-connected part of the eroded set (cc3d + inlet labels), its
+// dilation by r, and the first-invasion update of the radius map.  Here:
+//
+//  * one exact EDT of the image; the erosion at radius i is d2 > Te[i];
+//  * voxels are bucketed by the first level at which they survive the
+//    erosion (Te decreases with i, so the eroded sets grow), and a single
+//    union-find over the sweep absorbs each level's new voxels: unions only
+//    ever merge, so the labelling of level i is incremental, and inlet
+//    connectivity is monotone (a root flagged once stays flagged);
+//  * per level: inlet-slab voxels flag their roots, the inlet-connected
+//    set is masked, and one exact EDT of the mask (distance to the set)
+//    feeds the first-invasion update directly from its last pass.
+#include "edt_internal.hpp"
+
+namespace pmx {
+namespace {
+
+constexpr int kInactive = INT32_MIN;
+
+__device__ __forceinline__ int ld(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+__device__ __forceinline__ int find_root(int *parent, int x)
+{
+    int p = ld(parent + x);
+    while (p != x) {
+        const int gp = ld(parent + p);
+        if (gp != p) parent[x] = gp;
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+__device__ __forceinline__ void unite(int *parent, int a, int b)
+{
+    while (true) {
+        a = find_root(parent, a);
+        b = find_root(parent, b);
+        if (a == b) return;
+        const int hi = max(a, b), lo = min(a, b);
+        const int old = atomicCAS(parent + hi, hi, lo);
+        if (old == hi) return;
+        a = old;
+        b = lo;
+    }
+}
+
+struct Dims {
+    int n0, n1, n2;
+    int per0, per1, per2;
+};
+
+__device__ __forceinline__ int64_t n_of(const Dims &d) { return (int64_t)d.n0 * d.n1 * d.n2; }
+
+// level[v] = first i with d2 > te[i] (the erosion at radius i keeps v),
+// or n_levels if none; counts[level]++
+__global__ __launch_bounds__(256) void k_levels(const int32_t *__restrict__ d2, int64_t n, const int64_t *__restrict__ te,
+                                                int n_levels, uint8_t *__restrict__ level,
+                                                unsigned int *__restrict__ counts, int *__restrict__ parent,
+                                                uint8_t *__restrict__ flag)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = d2[i];
+        int l = n_levels;
+        for (int k = 0; k < n_levels; ++k)
+            if (v > te[k]) {
+                l = k;
+                break;
+            }
+        level[i] = (uint8_t)l;
+        parent[i] = kInactive;
+        flag[i] = 0;
+        if (l < n_levels) atomicAdd(counts + l, 1u);
+    }
+}
+
+// exclusive scan of counts (n_levels + 1 entries) into offsets
+__global__ void k_level_offsets(const unsigned int *counts, int n_levels, unsigned int *offsets, unsigned int *cursor)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        unsigned int run = 0;
+        for (int k = 0; k <= n_levels; ++k) {
+            offsets[k] = run;
+            cursor[k] = run;
+            run += (k < n_levels) ? counts[k] : 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bucket(const uint8_t *__restrict__ level, int64_t n, int n_levels,
+                                                unsigned int *__restrict__ cursor, int *__restrict__ order)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int l = level[i];
+        if (l < n_levels) order[atomicAdd(cursor + l, 1u)] = (int)i;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_activate(const int *__restrict__ order, unsigned int begin, unsigned int end,
+                                                  int *__restrict__ parent)
+{
+    for (int64_t k = (int64_t)begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < end;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int v = order[k];
+        parent[v] = v;
+    }
+}
+
+// each new voxel unions with every active 26-neighbour (wrapping on periodic axes)
+__global__ __launch_bounds__(256) void k_merge_new(const int *__restrict__ order, unsigned int begin, unsigned int end,
+                                                   Dims d, int *__restrict__ parent)
+{
+    for (int64_t k = (int64_t)begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < end;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int v = order[k];
+        const int z = v % d.n2, t = v / d.n2, y = t % d.n1, x = t / d.n1;
+        for (int dx = -1; dx <= 1; ++dx) {
+            int nx = x + dx;
+            if (nx < 0 || nx >= d.n0) {
+                if (!d.per0) continue;
+                nx = nx < 0 ? nx + d.n0 : nx - d.n0;
+            }
+            for (int dy = -1; dy <= 1; ++dy) {
+                int ny = y + dy;
+                if (ny < 0 || ny >= d.n1) {
+                    if (!d.per1) continue;
+                    ny = ny < 0 ? ny + d.n1 : ny - d.n1;
+                }
+                for (int dz = -1; dz <= 1; ++dz) {
+                    if (!dx && !dy && !dz) continue;
+                    int nz = z + dz;
+                    if (nz < 0 || nz >= d.n2) {
+                        if (!d.per2) continue;
+                        nz = nz < 0 ? nz + d.n2 : nz - d.n2;
+                    }
+                    const int u = (nx * d.n1 + ny) * d.n2 + nz;
+                    if (ld(parent + u) != kInactive) unite(parent, v, u);
+                }
+            }
+        }
+    }
+}
+
+// active voxels of the inlet slab flag their current roots
+__global__ __launch_bounds__(256) void k_flag_inlet(Dims d, int64_t lo0, int64_t lo1, int64_t lo2, int64_t e0,
+                                                    int64_t e1, int64_t e2, int *__restrict__ parent,
+                                                    uint8_t *__restrict__ flag)
+{
+    const int64_t n = e0 * e1 * e2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t z = i % e2, t = i / e2, y = t % e1, x = t / e1;
+        const int v = (int)(((lo0 + x) * d.n1 + lo1 + y) * d.n2 + lo2 + z);
+        if (ld(parent + v) != kInactive) flag[find_root(parent, v)] = 1;
+    }
+}
+
+// mask = active and inlet-connected (the porosimetry image before dilation)
+__global__ __launch_bounds__(256) void k_mask(int64_t n, int *__restrict__ parent, const uint8_t *__restrict__ flag,
+                                              uint8_t *__restrict__ mask, unsigned int *__restrict__ any)
+{
+    bool seen = false;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint8_t m = 0;
+        if (ld(parent + i) != kInactive && flag[find_root(parent, (int)i)]) m = 1;
+        mask[i] = m;
+        seen |= m != 0;
+    }
+    if (__any(seen) && (threadIdx.x & 63) == 0) atomicOr(any, 1u);
+}
+
+__global__ void k_clear_u32(unsigned int *a, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_zero_map(int32_t *__restrict__ map, Dims d, int64_t s0, int64_t s1)
+{
+    const int64_t n = n_of(d);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t z = i % d.n2, t = i / d.n2, y = t % d.n1, x = t / d.n1;
+        map[x * s0 + y * s1 + z] = 0;
+    }
+}
+
+struct PsdPlan {
+    int32_t *d2 = nullptr;
+    uint8_t *level = nullptr, *flag = nullptr, *mask = nullptr;
+    int *parent = nullptr, *order = nullptr;
+    unsigned int *counts = nullptr, *offsets = nullptr, *cursor = nullptr, *any = nullptr;
+    int64_t *te = nullptr;
+    char *edt_ws = nullptr;
+    size_t edt_bytes = 0;
+};
+
+constexpr int kMaxLevels = 250;
+
+size_t plan_psd(const pmx_geom *g, void *ws, PsdPlan *p)
+{
+    Carver c(ws, SIZE_MAX);
+    const size_t n = (size_t)(g->n[0] * g->n[1] * g->n[2]);
+    p->d2 = c.take<int32_t>(n);
+    p->level = c.take<uint8_t>(n);
+    p->flag = c.take<uint8_t>(n);
+    p->mask = c.take<uint8_t>(n);
+    p->parent = c.take<int>(n);
+    p->order = c.take<int>(n);
+    p->counts = c.take<unsigned int>(kMaxLevels + 1);
+    p->offsets = c.take<unsigned int>(kMaxLevels + 1);
+    p->cursor = c.take<unsigned int>(kMaxLevels + 1);
+    p->any = c.take<unsigned int>(kMaxLevels);
+    p->te = c.take<int64_t>(kMaxLevels);
+    EdtPlan e;
+    p->edt_bytes = plan_edt(g, nullptr, &e);
+    p->edt_ws = c.take<char>(p->edt_bytes);
+    return c.need();
+}
+
+unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 65536); }
+
+}  // namespace
+}  // namespace pmx
+
+extern "C" {
+
+int pmx_psd_workspace_size(const pmx_geom *g, size_t *bytes)
+{
+    pmx::clear_error();
+    if (int e = pmx::validate_geom(g)) return e;
+    if (!bytes) return pmx::fail(PMX_ERR_INVALID, "bytes is NULL");
+    pmx::PsdPlan p;
+    *bytes = pmx::plan_psd(g, nullptr, &p);
+    return PMX_OK;
+}
+
+int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, const int64_t *inlet_ext,
+                  int n_levels, const int64_t *t_erode, const int64_t *t_dilate, uint32_t wall_mask,
+                  int32_t *code_map, void *workspace, size_t workspace_bytes, hipStream_t s)
+{
+    using namespace pmx;
+    clear_error();
+    if (int e = validate_geom(g)) return e;
+    if (!img || !t_erode || !t_dilate || !code_map) return fail(PMX_ERR_INVALID, "NULL argument");
+    if (n_levels < 1 || n_levels > kMaxLevels) return fail(PMX_ERR_INVALID, "n_levels must be in [1, %d]", kMaxLevels);
+    const int64_t n = g->n[0] * g->n[1] * g->n[2];
+    if (n >= INT32_MAX) return fail(PMX_ERR_INVALID, "box has %lld voxels (limit 2^31-1)", (long long)n);
+    if (g->in_s[0] != g->n[1] * g->n[2] || g->in_s[1] != g->n[2] || g->out_s[0] != g->in_s[0] ||
+        g->out_s[1] != g->in_s[1])
+        return fail(PMX_ERR_INVALID, "the sweep needs contiguous input and output boxes");
+    for (int k = 1; k < n_levels; ++k)
+        if (t_erode[k] > t_erode[k - 1])
+            return fail(PMX_ERR_INVALID, "erosion thresholds must not increase (radii in descending order)");
+    for (int a = 0; a < 3; ++a)
+        if (g->bc[2 * a] == PMX_BC_WALL || g->bc[2 * a + 1] == PMX_BC_WALL)
+            return fail(PMX_ERR_INVALID, "WALL faces: pass the padded image with END faces");
+    bool has_inlet = inlet_lo && inlet_ext;
+    if (has_inlet)
+        for (int a = 0; a < 3; ++a)
+            if (inlet_lo[a] < 0 || inlet_ext[a] < 0 || inlet_lo[a] + inlet_ext[a] > g->n[a])
+                return fail(PMX_ERR_INVALID, "inlet slab outside the box on axis %d", a);
+    PsdPlan p;
+    const size_t need = plan_psd(g, nullptr, &p);
+    if (!workspace || workspace_bytes < need)
+        return fail(PMX_ERR_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, need);
+    plan_psd(g, workspace, &p);
+    Dims d{(int)g->n[0], (int)g->n[1], (int)g->n[2], g->bc[0] == PMX_BC_PERIODIC, g->bc[2] == PMX_BC_PERIODIC,
+           g->bc[4] == PMX_BC_PERIODIC};
+    const unsigned gr = grid_for(n);
+    // contiguous box geometry for the internal EDTs
+    pmx_geom gc = *g;
+    gc.in_s[0] = gc.out_s[0] = g->n[1] * g->n[2];
+    gc.in_s[1] = gc.out_s[1] = g->n[2];
+
+    // 1. one exact EDT of the image; levels; buckets
+    {
+        ProfScope ps("psd_edt", s);
+        PassXOut o{};
+        o.d2 = p.d2;
+        o.sx = gc.out_s[0];
+        o.sy = gc.out_s[1];
+        if (int e = run_edt(img, &gc, false, o, p.edt_ws, p.edt_bytes, s)) return e;
+    }
+    if (hipMemcpyAsync(p.te, t_erode, sizeof(int64_t) * n_levels, hipMemcpyHostToDevice, s) != hipSuccess)
+        return fail(PMX_ERR_HIP, "threshold upload failed");
+    {
+        ProfScope ps("psd_levels", s);
+        hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(256), 0, s, p.counts, kMaxLevels + 1);
+        hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(256), 0, s, p.any, kMaxLevels);
+        hipLaunchKernelGGL(k_levels, dim3(gr), dim3(256), 0, s, p.d2, n, p.te, n_levels, p.level, p.counts,
+                           p.parent, p.flag);
+        hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, s, p.counts, n_levels, p.offsets, p.cursor);
+        hipLaunchKernelGGL(k_bucket, dim3(gr), dim3(256), 0, s, p.level, n, n_levels, p.cursor, p.order);
+        hipLaunchKernelGGL(k_zero_map, dim3(gr), dim3(256), 0, s, code_map, d, g->out_s[0], g->out_s[1]);
+    }
+    if (int e = check_launch("psd setup")) return e;
+    // the bucket bounds are needed on the host to size the per-level launches
+    unsigned int offs[kMaxLevels + 1];
+    if (hipMemcpyAsync(offs, p.offsets, sizeof(unsigned int) * (n_levels + 1), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return fail(PMX_ERR_HIP, "reading level offsets failed");
+
+    for (int lv = 0; lv < n_levels; ++lv) {
+        const unsigned int b = offs[lv], e = offs[lv + 1];
+        if (e > b) {
+            ProfScope ps("psd_union", s);
+            const unsigned gl = grid_for((int64_t)(e - b));
+            hipLaunchKernelGGL(k_activate, dim3(gl), dim3(256), 0, s, p.order, b, e, p.parent);
+            hipLaunchKernelGGL(k_merge_new, dim3(gl), dim3(256), 0, s, p.order, b, e, d, p.parent);
+        }
+        {
+            ProfScope ps("psd_mask", s);
+            if (has_inlet) {
+                const int64_t ni = inlet_ext[0] * inlet_ext[1] * inlet_ext[2];
+                hipLaunchKernelGGL(k_flag_inlet, dim3(grid_for(ni)), dim3(256), 0, s, d, inlet_lo[0], inlet_lo[1],
+                                   inlet_lo[2], inlet_ext[0], inlet_ext[1], inlet_ext[2], p.parent, p.flag);
+            } else {
+                // no inlet requirement: every active root counts
+                hipLaunchKernelGGL(k_flag_inlet, dim3(gr), dim3(256), 0, s, d, (int64_t)0, (int64_t)0, (int64_t)0,
+                                   (int64_t)d.n0, (int64_t)d.n1, (int64_t)d.n2, p.parent, p.flag);
+            }
+            hipLaunchKernelGGL(k_mask, dim3(gr), dim3(256), 0, s, n, p.parent, p.flag, p.mask, p.any + lv);
+        }
+        if (int rc = check_launch("psd level")) return rc;
+        // dilation of the mask by radius lv with the first-invasion update
+        // (an empty mask has no zero-distance voxels, so nothing is updated)
+        PassXOut o{};
+        o.sx = g->out_s[0];
+        o.sy = g->out_s[1];
+        o.t = (int32_t)std::min<int64_t>(std::max<int64_t>(t_dilate[lv], -1), INT32_MAX - 1);
+        o.wall_mask = wall_mask;
+        o.psd_pm = img;
+        o.psd_map = code_map;
+        o.psd_code = lv + 1;
+        ProfScope ps("psd_dilate", s);
+        if (int rc = run_edt(p.mask, &gc, true, o, p.edt_ws, p.edt_bytes, s)) return rc;
+    }
+    return PMX_OK;
+}
+
+}  // extern "C"

@@ -26,6 +26,7 @@ import torch
 from .. import _device, _lib
 from . import connected_components as cc
 from . import morphological_operators as morph
+from .distance.euclidean_distance_transform import _geom
 
 __all__ = ["get_sizes", "porosimetry", "pore_size_distribution", "psd_sweep", "saturation_curve"]
 
@@ -109,9 +110,60 @@ def _sorted_radii(subdomain, porous_media, radii, num_radii):
     return get_sizes(r_min, global_max, num_radii, "linear")
 
 
+def _thresholds(subdomain, radius, fft):
+    if fft:
+        return morph.fft_threshold(subdomain.domain.resolution, radius)
+    return morph.radius_threshold(radius)
+
+
+def fused_sweep_ok(subdomain, mode) -> bool:
+    """pmx_psd_sweep covers non-periodic domains whose erosion and dilation
+    are integer thresholds (unit resolution, or FFT semantics throughout)."""
+    from .distance.euclidean_distance_transform import _unit
+    if subdomain.domain.periodic:
+        return False
+    return _unit(subdomain.domain.resolution) or mode == "morph"
+
+
+def psd_sweep_fused(subdomain, pm_t: torch.Tensor, radii, inlet=False, mode="hybrid") -> torch.Tensor:
+    """The whole sweep in one pmx_psd_sweep call (same codes as psd_sweep)."""
+    shape = tuple(pm_t.shape)
+    te = [_thresholds(subdomain, r, mode == "morph") for r in radii]
+    td = [_thresholds(subdomain, r, mode in ("morph", "hybrid")) for r in radii]
+    g = _geom(shape, (0, 0, 0), shape, [_lib.BC_END] * 6)
+    code_map = torch.empty(shape, dtype=torch.int32, device=pm_t.device)
+    L = _lib.lib()
+    nbytes = C.c_size_t()
+    _lib.check(L.pmx_psd_workspace_size(C.byref(g), C.byref(nbytes)))
+    ws = _device.workspace(nbytes.value)
+    lo = ext = None
+    if inlet:
+        slabs = [subdomain.face_own_slab(a, s) for a in range(3) for s in (0, 1) if subdomain.inlet[a][s]]
+        if len(slabs) > 1:
+            raise NotImplementedError("the fused sweep takes one inlet face")
+        if slabs:
+            sl = slabs[0]
+            lo = (C.c_int64 * 3)(*[x.start for x in sl])
+            ext = (C.c_int64 * 3)(*[x.stop - x.start for x in sl])
+        else:
+            lo = (C.c_int64 * 3)(0, 0, 0)  # inlet requested but no inlet face: nothing connects
+            ext = (C.c_int64 * 3)(0, 0, 0)
+    _lib.check(L.pmx_psd_sweep(_device.ptr(pm_t), C.byref(g), lo, ext, len(radii), (C.c_int64 * len(te))(*te),
+                               (C.c_int64 * len(td))(*td), morph._wall_mask(subdomain), _device.ptr(code_map),
+                               _device.ptr(ws), nbytes.value, _device.stream()))
+    return code_map
+
+
 def psd_sweep(subdomain, pm_t: torch.Tensor, radii, inlet=False, mode="hybrid") -> torch.Tensor:
     """Radius-code map: code i+1 where radii[i] (descending) first invades
     a pore voxel, 0 where none does."""
+    if fused_sweep_ok(subdomain, mode):
+        return psd_sweep_fused(subdomain, pm_t, radii, inlet, mode)
+    return psd_sweep_levels(subdomain, pm_t, radii, inlet, mode)
+
+
+def psd_sweep_levels(subdomain, pm_t: torch.Tensor, radii, inlet=False, mode="hybrid") -> torch.Tensor:
+    """The sweep as one porosimetry call per radius (any domain)."""
     code_map = torch.zeros(tuple(pm_t.shape), dtype=torch.int32, device=pm_t.device)
     for i, r in enumerate(radii):
         temp = porosimetry_device(subdomain, pm_t, r, r, inlet, None, mode)

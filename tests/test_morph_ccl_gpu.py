@@ -145,7 +145,28 @@ def test_pore_size_distribution_match_oracle():
     got = PO.pore_size_distribution(sd, pm, radii, inlet=True, mode="distance")
     np.testing.assert_array_equal(got, O.pore_size_distribution(box, img, radii, inlet=True))
     got_all = PO.pore_size_distribution(sd, pm, [1, 3, 2], inlet=False)
-    np.testing.assert_array_equal(got_all, O.pore_size_distribution(box, img, [3, 2, 1], inlet=False))
+    np.testing.assert_array_equal(got_all, O.pore_size_distribution(box, img, [3, 2, 1], inlet=False,
+                                                                     mode="hybrid"))
+
+
+@pytest.mark.parametrize("mode", ["distance", "hybrid", "morph"])
+def test_fused_sweep_equals_per_radius_sweep_and_oracle(mode):
+    """pmx_psd_sweep (one EDT + incremental union-find + one EDT per level)
+    against the per-radius porosimetry path and the oracle."""
+    from pmmoto_amd.domain_generation import pack_radii, sphere_pack
+    from pmmoto_amd.filters import porosimetry as PO
+    img_t, _ = sphere_pack((48, 40, 44), 0.38, pack_radii("lognormal", mean=5.0, sigma=0.3), seed=51)
+    img, sd, box = padded_case(img_t.cpu().numpy(), BT_CASES["cfg3"],
+                               inlet=((True, False), (False, False), (False, False)))
+    import torch
+    pm_t = torch.from_numpy(img).cuda()
+    radii = [np.float64(r) for r in (7.5, 6.0, 5.2, 4.0, 3.3, 2.5, 2.0, 1.5, 1.0)]
+    fused = PO.psd_sweep_fused(sd, pm_t, radii, inlet=True, mode=mode).cpu().numpy()
+    levels = PO.psd_sweep_levels(sd, pm_t, radii, inlet=True, mode=mode).cpu().numpy()
+    np.testing.assert_array_equal(fused, levels)
+    want = O.pore_size_distribution(box, img, np.array(radii), inlet=True, mode=mode)
+    got = np.where(fused > 0, np.array([0.0] + [float(r) for r in radii])[fused], 0.0)
+    np.testing.assert_array_equal(got, want)
 
 
 def test_porosimetry_modes_and_inlet_kat():

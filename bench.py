@@ -49,6 +49,47 @@ def cpu_baseline(img_t, edge: int):
     }
 
 
+def porosimetry_sweep(img, n: int, reps: int):
+    """Exact drainage-radius sweep of the same domain: pore_size_distribution
+    with radii 32..1 voxels (numpy float64), inlet x = 0, mode "distance",
+    on the padded image (y, z WALL pads), as one pmx_psd_sweep call per
+    curve; the float64 wetting-saturation curve follows from the code map."""
+    import numpy as np
+    import torch
+
+    import pmmoto_amd
+    from pmmoto_amd import _lib
+    from pmmoto_amd.filters import porosimetry as PO
+
+    BT = pmmoto_amd.BoundaryType
+    sd = pmmoto_amd.initialize((n, n, n), box=((0, n),) * 3,
+                               boundary_types=((BT.END, BT.END), (BT.WALL, BT.WALL), (BT.WALL, BT.WALL)),
+                               inlet=((True, False), (False, False), (False, False)),
+                               outlet=((False, True), (False, False), (False, False)))
+    pm = torch.zeros(sd.voxels, dtype=torch.uint8, device="cuda")
+    pm[:, 1:-1, 1:-1] = img
+    radii = [np.float64(r) for r in range(32, 0, -1)]
+    code = PO.psd_sweep_fused(sd, pm, radii, inlet=True, mode="distance")  # warm-up
+    torch.cuda.synchronize()
+    _lib.profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        code = PO.psd_sweep_fused(sd, pm, radii, inlet=True, mode="distance")
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    prof = _lib.profile_end()
+    sw = PO.saturation_curve(sd, pm, code, len(radii))
+    del pm, code
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"pore_size_distribution radii 32..1 (32 levels), inlet x=0, mode distance, {n}^3 + y/z wall pads",
+        "curves_per_s": round(1.0 / dt, 4),
+        "ms_per_curve": round(dt * 1e3, 2),
+        "kernels_ms_per_curve": {k: round(t / reps, 3) for k, (t, c) in prof.items()},
+        "saturation_first_last": [float(sw[0]), float(sw[-1])],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +98,8 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--cpu-edge", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-psd", action="store_true")
+    ap.add_argument("--psd-reps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
@@ -116,6 +159,11 @@ def main() -> int:
     achieved = BYTES_PER_VOXEL[dom] * vox / (dom_ms * 1e-3) / 1e9
     print(f"[bench] {args.steps} steps: {ms:.3f} ms/step; kernels {per_kernel}", file=sys.stderr, flush=True)
 
+    psd = None
+    if not args.no_psd:
+        psd = porosimetry_sweep(img, n, args.psd_reps)
+        print(f"[bench] porosimetry: {psd}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(img, min(args.cpu_edge, n))
@@ -139,6 +187,7 @@ def main() -> int:
             "voxels": int(vox),
             "edt_hbm_frac_21B": round(EDT_BYTES_PER_VOXEL * vox / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "kernels": per_kernel,
+            "porosimetry": psd,
         },
         "roofline": {
             "bound": "hbm",

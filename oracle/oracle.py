@@ -464,8 +464,9 @@ def drainage(box: Box, pm_img, mp_img, capillary_pressures, domain_voxels, max_d
 
 
 def pore_size_distribution(box: Box, pm_img, radii, inlet=False, mode="distance"):
-    """pore_size_distribution (porosimetry.py:163-227)."""
-    radii = np.sort(np.asarray(radii))[::-1]
+    """pore_size_distribution (porosimetry.py:163-227); a list keeps its
+    Python scalars (sorted descending), an array becomes numpy scalars."""
+    radii = sorted(radii, reverse=True) if isinstance(radii, list) else np.sort(np.asarray(radii))[::-1]
     out = np.zeros(pm_img.shape, dtype=np.float64)
     for r in radii:
         t = porosimetry(box, pm_img, r, inlet, mode=mode)

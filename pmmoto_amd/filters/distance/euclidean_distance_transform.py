@@ -1,16 +1,18 @@
-"""Exact Euclidean distance transform on the GPU.
+"""Euclidean distance transform on the GPU.
 
 Same call surface as pmmoto.filters.distance
 (src/pmmoto/filters/distance/euclidean_distance_transform.py:25-303):
 ``edt``, ``edt2d``, ``edt3d``.  numpy in -> numpy out (the reference's
 contract); a torch tensor on the GPU in -> tensor out (device-resident use).
 
-All three run libpmx's exact integer EDT (pmx_edt).  At unit resolution the
+Two GPU paths.  pmx_edt is the exact integer EDT: at unit resolution the
 reference's float32 EDT holds exactly these integers wherever a solid is
-reachable and every axis is at most 1024 voxels, so the float32 outputs
-match the reference bit for bit there; periodic faces are exact torus
-distances (the reference keeps at most five boundary parabolas per line
-from across a periodic face, which matches on the reference's own tests).
+reachable and every axis is at most 1024 voxels, so non-periodic domains
+match the reference bit for bit on the fast path.  pmx_edt_ref runs the
+reference's float32 arithmetic line by line: it is the default at other
+resolutions and on periodic axes, where the reference keeps at most five
+boundary parabolas per line from across the face and so can exceed the
+exact torus distance near it (``mode="exact"`` gives the latter).
 """
 
 from __future__ import annotations
@@ -133,16 +135,31 @@ def _finish(t, squeeze, as_numpy):
     return _device.to_host(t, as_numpy)
 
 
+def _use_reference(mode, resolution, periodic) -> bool:
+    """mode "auto": the exact integer kernels where they equal the reference
+    bit for bit (unit resolution, no periodic axis); the reference's float32
+    arithmetic otherwise -- at periodic faces the reference keeps only five
+    boundary parabolas per line, which differs from the exact torus distance
+    near the face on some images.  "exact": the integer kernels (exact torus
+    distance on periodic axes); "reference": always the reference arithmetic."""
+    if mode == "reference":
+        return True
+    if mode == "exact":
+        if not _unit(resolution):
+            raise ValueError("mode='exact' needs unit resolution")
+        return False
+    if mode != "auto":
+        raise ValueError(f"mode {mode!r} must be 'auto', 'exact' or 'reference'")
+    return not _unit(resolution) or any(bool(p) for p in periodic)
+
+
 def edt3d(img, periodic=(False, False, False), resolution=(1.0, 1.0, 1.0), squared=False, mode="auto"):
     """edt3d (euclidean_distance_transform.py:208-303): float32 distance (or
-    squared distance with ``squared=True``) to the nearest zero voxel.
-    mode "auto": exact integer kernels at unit resolution, the reference's
-    float32 arithmetic (pmx_edt_ref) otherwise; "reference" forces the
-    latter (bitwise the reference's periodic five-parabola hull handling)."""
+    squared distance with ``squared=True``) to the nearest zero voxel."""
     t, as_np = _device.as_device_u8(img)
     if t.dim() != 3:
         raise ValueError(f"edt3d needs a 3-D image, got {tuple(t.shape)}")
-    if mode == "reference" or not _unit(resolution):
+    if _use_reference(mode, resolution, periodic):
         out = run_edt_ref(t, (0, 0, 0), t.shape, 0, periodic, resolution, not squared)
         return _device.to_host(out, as_np)
     origin, n, bc, pads = compute_box(tuple(t.shape), periodic=periodic)
@@ -156,7 +173,7 @@ def edt2d(img, periodic=(False, False), resolution=(1.0, 1.0), mode="auto"):
     if t.dim() != 2:
         raise ValueError(f"edt2d needs a 2-D image, got {tuple(t.shape)}")
     t3, _ = _as3d(t)
-    if mode == "reference" or not _unit(resolution):
+    if _use_reference(mode, resolution, periodic):
         # the reference sweeps axis 1 first, then axis 0: box (1, X, Y), axis 2 first
         out = run_edt_ref(t3, (0, 0, 0), t3.shape, 2, (False,) + tuple(periodic),
                           (1.0,) + tuple(resolution), True)
@@ -165,7 +182,7 @@ def edt2d(img, periodic=(False, False), resolution=(1.0, 1.0), mode="auto"):
     return _finish(run_edt(t3, origin, n, bc, pads, ("f32",))["f32"], True, as_np)
 
 
-def edt(img, subdomain=None):
+def edt(img, subdomain=None, mode="auto"):
     """edt (euclidean_distance_transform.py:25-53): with a subdomain, the
     domain's resolution and periodic faces apply to the padded image and the
     periodic pads of the result hold periodic copies; without one, edt3d or
@@ -182,10 +199,10 @@ def edt(img, subdomain=None):
         raise ValueError(f"Wrong img dimension {tuple(t.shape)}")
     origin, n, bc, pads = compute_box(tuple(t.shape), subdomain=subdomain)
     res = subdomain.domain.resolution
-    if not _unit(res):
+    per = [bc[2 * a] == _lib.BC_PERIODIC for a in range(3)]
+    if _use_reference(mode, res, per):
         # non-periodic: edt3d of the padded image, as the reference; periodic:
         # the reference arithmetic on the own box with wrap, pads copied
-        per = [bc[2 * a] == _lib.BC_PERIODIC for a in range(3)]
         out = run_edt_ref(t, origin, n, 0, per, res, True)
         fill_periodic_pads([out], tuple(t.shape), pads)
         return _device.to_host(out, as_np)

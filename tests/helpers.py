@@ -40,3 +40,30 @@ def padded_case(own, bts, inlet=((False, False),) * 3, outlet=((False, False),) 
     box = O.Box(bc=tuple((CODE[lo], CODE[hi]) for lo, hi in bts), pad=tuple(pads),
                 inlet=sd.inlet, outlet=sd.outlet)
     return np.ascontiguousarray(img), sd, box
+
+
+def cpu_sphere_pack(shape, porosity, kind, seed, periodic=(False, False, False), **kw):
+    """The product's seeded draw sequence (domain_generation.sphere_pack),
+    rasterised by the oracle's sequential rule on the CPU."""
+    import math
+    rng = np.random.default_rng(seed)
+    if kind == "fixed":
+        sampler, mean_r3 = (lambda g, n: np.full(n, float(kw["r"]))), float(kw["r"]) ** 3
+    elif kind == "uniform":
+        lo, hi = float(kw["lo"]), float(kw["hi"])
+        sampler, mean_r3 = (lambda g, n: g.uniform(lo, hi, n)), (hi ** 4 - lo ** 4) / (4 * (hi - lo))
+    else:
+        mean, sigma = float(kw["mean"]), float(kw["sigma"])
+        mu = math.log(mean) - sigma * sigma / 2
+        sampler, mean_r3 = (lambda g, n: g.lognormal(mu, sigma, n)), math.exp(3 * mu + 4.5 * sigma * sigma)
+    vol = float(np.prod(shape))
+    est = int(math.ceil(-math.log(porosity) * vol / (4.0 / 3.0 * math.pi * mean_r3) * 1.25)) + 64
+    batches = []
+    while True:
+        m = est if not batches else max(64, est // 4)
+        centres = rng.uniform(0.0, 1.0, (m, 3)) * np.array(shape, dtype=np.float64)
+        batches.append(np.column_stack([centres, np.asarray(sampler(rng, m), dtype=np.float64)]))
+        spheres = np.concatenate(batches)
+        img, n = O.rasterize_spheres(shape, periodic, spheres, porosity)
+        if n > 0:
+            return img, spheres[:n]

@@ -100,14 +100,33 @@ def test_reference_api_edt3d_edt2d_edt():
     img2 = rand_binary((10, 50), 0.1, 1)
     np.testing.assert_array_equal(distance.edt(img2), O.ref_edt(img2))
     np.testing.assert_array_equal(distance.edt2d(img2, periodic=(True, True)),
+                                  np.sqrt(O.ref_edt_sq(img2, (True, True))))
+    np.testing.assert_array_equal(distance.edt2d(img2, periodic=(True, True), mode="exact"),
                                   np.sqrt(O.d2_to_f32(O.exact_edt_sq(img2, [O.BC_PERIODIC] * 4))))
     img3 = rand_binary((20, 22, 24), 0.1, 3)
     np.testing.assert_array_equal(distance.edt3d(img3, squared=True), O.ref_edt_sq(img3))
-    # periodic: the exact torus EDT; the reference's five-parabola hull
-    # exchange agrees on these inputs (checked here against its restatement)
+    # periodic: the reference's five-parabola hull exchange by default, the
+    # exact torus EDT with mode="exact" (they agree on this dense image)
     got = distance.edt3d(img3, periodic=(True, True, True), squared=True)
-    np.testing.assert_array_equal(got, O.d2_to_f32(O.exact_edt_sq(img3, BCS["periodic"])))
     np.testing.assert_array_equal(got, O.ref_edt_sq(img3, (True, True, True)))
+    got = distance.edt3d(img3, periodic=(True, True, True), squared=True, mode="exact")
+    np.testing.assert_array_equal(got, O.d2_to_f32(O.exact_edt_sq(img3, BCS["periodic"])))
+
+
+def test_periodic_sphere_pack_reference_truncation_reproduced():
+    """On a periodic sphere pack the reference's five-member boundary hulls
+    exceed the exact torus distance on voxels near the faces; the default
+    path reproduces the reference there, mode="exact" the true distance."""
+    from pmmoto_amd.filters import distance
+    from tests.helpers import cpu_sphere_pack
+    img, _ = cpu_sphere_pack((96, 96, 96), 0.5, "uniform", 2, periodic=(True, True, True), lo=4, hi=12)
+    ref = O.ref_edt_sq(img, (True, True, True))
+    exact = O.d2_to_f32(O.exact_edt_sq(img, BCS["periodic"]))
+    got = distance.edt3d(img, periodic=(True, True, True), squared=True)
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+    got_exact = distance.edt3d(img, periodic=(True, True, True), squared=True, mode="exact")
+    np.testing.assert_array_equal(got_exact, exact)
+    print(f"voxels where the reference exceeds the exact torus EDT: {int((ref != exact).sum())}")
 
 
 def test_subdomain_padded_periodic_and_wall():
@@ -124,7 +143,7 @@ def test_subdomain_padded_periodic_and_wall():
     padded = np.pad(img, ((1, 1), (1, 1), (0, 0)), mode="wrap")
     padded[:, 0, :] = 0
     padded[:, -1, :] = 0
-    got = distance.edt(padded, sd)
+    got = distance.edt(padded, sd, mode="exact")
     # own region: periodic in x, WALL in y (the zero pad), END in z
     want = np.sqrt(O.d2_to_f32(O.exact_edt_sq(img, [O.BC_PERIODIC] * 2 + [O.BC_WALL] * 2 + [O.BC_END] * 2)))
     np.testing.assert_array_equal(got[1:-1, 1:-1, :], want)

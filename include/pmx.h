@@ -158,19 +158,19 @@ int pmx_ccl_workspace_size(const pmx_geom *g, size_t *bytes);
 int pmx_ccl26(const uint8_t *img, const pmx_geom *g, int32_t *labels32, uint64_t *labels64,
               int64_t *count, void *workspace, size_t workspace_bytes, hipStream_t stream);
 
-/* flags[l] = 1 for every non-zero label l in the sub-box lo[3] + [0,
- * extent[3]) of a C-order label array (label_bytes 4 or 8, element
- * strides[2] for x and y).  flags must hold max label + 1 bytes. */
+/* flags[l] = 1 for every label l in [1, n_flags) in the sub-box lo[3] +
+ * [0, extent[3]) of a C-order label array (label_bytes 4 or 8, element
+ * strides[2] for x and y); flags holds n_flags bytes (count + 1). */
 int pmx_mark_labels(const void *labels, int label_bytes, const int64_t *strides, const int64_t *lo,
-                    const int64_t *extent, uint8_t *flags, hipStream_t stream);
+                    const int64_t *extent, uint8_t *flags, uint64_t n_flags, hipStream_t stream);
 
 /* out[i] = keep ? img[i] : 0 over n contiguous voxels, keep decided from
- * the voxel's label l != 0: mode 0 flags_a[l]; mode 1 flags_a[l] &&
- * flags_b[l]; mode 2 !flags_a[l] && !flags_b[l]; and, if phase >= 0,
- * img[i] == phase. */
+ * the voxel's label l in [1, n_flags): mode 0 flags_a[l]; mode 1
+ * flags_a[l] && flags_b[l]; mode 2 !flags_a[l] && !flags_b[l]; and, if
+ * phase >= 0, img[i] == phase. */
 int pmx_keep_labels(const uint8_t *img, const void *labels, int label_bytes, int64_t n,
-                    const uint8_t *flags_a, const uint8_t *flags_b, int mode, int phase,
-                    uint8_t *out, hipStream_t stream);
+                    const uint8_t *flags_a, const uint8_t *flags_b, uint64_t n_flags, int mode,
+                    int phase, uint8_t *out, hipStream_t stream);
 
 /* Porosimetry steps over n contiguous voxels.  pmx_psd_update: map[i] =
  * code where pm[i] == 1 && temp[i] == 1 && map[i] == 0 (the first-invasion

@@ -118,13 +118,29 @@ __global__ __launch_bounds__(256) void k_merge(const uint8_t *__restrict__ img, 
     }
 }
 
+// Read-only walk to the root.  Compression must not halve other voxels'
+// paths: a halving write computed from a stale read could overwrite a
+// pointer another lane has just set to the root with a non-root ancestor,
+// and the label gather relies on every parent being a root.
+__device__ __forceinline__ int root_of(const int *parent, int x)
+{
+    int p = ld(parent + x);
+    while (p != x) {
+        x = p;
+        p = ld(parent + x);
+    }
+    return x;
+}
+
 __global__ __launch_bounds__(256) void k_compress(int *__restrict__ parent, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int p = parent[i];
+        const int p = ld(parent + i);
         if (p == kBg || p == (int)i) continue;
-        parent[i] = find_root(parent, (int)i);
+        // each lane writes only its own entry, so roots (and every pointer
+        // already set to a root) stay as written
+        __hip_atomic_store(parent + i, root_of(parent, (int)i), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -227,17 +243,21 @@ __global__ __launch_bounds__(256) void k_labels(const int *__restrict__ parent, 
 }
 
 // flags[label] = 1 for every label inside the sub-box [lo, hi)
+// labels outside [1, n_flags) are skipped (they cannot come from pmx_ccl26
+// with a matching count; the check keeps a stale count from writing past
+// the flag array)
 template <class T>
 __global__ __launch_bounds__(256) void k_mark(const T *__restrict__ labels, int64_t s0, int64_t s1,
                                               int64_t lo0, int64_t lo1, int64_t lo2, int64_t e0,
-                                              int64_t e1, int64_t e2, uint8_t *__restrict__ flags)
+                                              int64_t e1, int64_t e2, uint8_t *__restrict__ flags,
+                                              uint64_t n_flags)
 {
     const int64_t n = e0 * e1 * e2;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t z = i % e2, t = i / e2, y = t % e1, x = t / e1;
-        const T l = labels[(lo0 + x) * s0 + (lo1 + y) * s1 + lo2 + z];
-        if (l) flags[l] = 1;
+        const uint64_t l = (uint64_t)labels[(lo0 + x) * s0 + (lo1 + y) * s1 + lo2 + z];
+        if (l && l < n_flags) flags[l] = 1;
     }
 }
 
@@ -245,15 +265,15 @@ __global__ __launch_bounds__(256) void k_mark(const T *__restrict__ labels, int6
 template <class T>
 __global__ __launch_bounds__(256) void k_keep(const uint8_t *__restrict__ img, const T *__restrict__ labels,
                                               int64_t n, const uint8_t *__restrict__ fa,
-                                              const uint8_t *__restrict__ fb, int mode, int phase,
-                                              uint8_t *__restrict__ out)
+                                              const uint8_t *__restrict__ fb, uint64_t n_flags, int mode,
+                                              int phase, uint8_t *__restrict__ out)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const T l = labels[i];
+        const uint64_t l = (uint64_t)labels[i];
         const uint8_t v = img[i];
         bool keep = false;
-        if (l) {
+        if (l && l < n_flags) {
             const bool a = fa[l] != 0, bb = fb ? fb[l] != 0 : false;
             keep = mode == 0 ? a : (mode == 1 ? (a && bb) : (!a && !bb));
             if (phase >= 0 && v != phase) keep = false;
@@ -350,7 +370,7 @@ int pmx_ccl26(const uint8_t *img, const pmx_geom *g, int32_t *labels32, uint64_t
 }
 
 int pmx_mark_labels(const void *labels, int label_bytes, const int64_t *strides, const int64_t *lo,
-                    const int64_t *extent, uint8_t *flags, hipStream_t s)
+                    const int64_t *extent, uint8_t *flags, uint64_t n_flags, hipStream_t s)
 {
     using namespace pmx;
     clear_error();
@@ -360,19 +380,19 @@ int pmx_mark_labels(const void *labels, int label_bytes, const int64_t *strides,
     if (label_bytes == 4)
         hipLaunchKernelGGL(k_mark<int32_t>, dim3(grid_for(n)), dim3(256), 0, s,
                            static_cast<const int32_t *>(labels), strides[0], strides[1], lo[0], lo[1],
-                           lo[2], extent[0], extent[1], extent[2], flags);
+                           lo[2], extent[0], extent[1], extent[2], flags, n_flags);
     else if (label_bytes == 8)
         hipLaunchKernelGGL(k_mark<uint64_t>, dim3(grid_for(n)), dim3(256), 0, s,
                            static_cast<const uint64_t *>(labels), strides[0], strides[1], lo[0], lo[1],
-                           lo[2], extent[0], extent[1], extent[2], flags);
+                           lo[2], extent[0], extent[1], extent[2], flags, n_flags);
     else
         return fail(PMX_ERR_INVALID, "label_bytes must be 4 or 8");
     return check_launch("mark labels");
 }
 
 int pmx_keep_labels(const uint8_t *img, const void *labels, int label_bytes, int64_t n,
-                    const uint8_t *flags_a, const uint8_t *flags_b, int mode, int phase, uint8_t *out,
-                    hipStream_t s)
+                    const uint8_t *flags_a, const uint8_t *flags_b, uint64_t n_flags, int mode, int phase,
+                    uint8_t *out, hipStream_t s)
 {
     using namespace pmx;
     clear_error();
@@ -382,10 +402,10 @@ int pmx_keep_labels(const uint8_t *img, const void *labels, int label_bytes, int
     if (n == 0) return PMX_OK;
     if (label_bytes == 4)
         hipLaunchKernelGGL(k_keep<int32_t>, dim3(grid_for(n)), dim3(256), 0, s, img,
-                           static_cast<const int32_t *>(labels), n, flags_a, flags_b, mode, phase, out);
+                           static_cast<const int32_t *>(labels), n, flags_a, flags_b, n_flags, mode, phase, out);
     else if (label_bytes == 8)
         hipLaunchKernelGGL(k_keep<uint64_t>, dim3(grid_for(n)), dim3(256), 0, s, img,
-                           static_cast<const uint64_t *>(labels), n, flags_a, flags_b, mode, phase, out);
+                           static_cast<const uint64_t *>(labels), n, flags_a, flags_b, n_flags, mode, phase, out);
     else
         return fail(PMX_ERR_INVALID, "label_bytes must be 4 or 8");
     return check_launch("keep labels");

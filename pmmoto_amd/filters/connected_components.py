@@ -70,15 +70,17 @@ def _face_flags(subdomain, labels: torch.Tensor, n_labels: int, which: str) -> t
                 lo = (C.c_int64 * 3)(*[x.start for x in sl])
                 ext = (C.c_int64 * 3)(*[x.stop - x.start for x in sl])
                 _lib.check(_lib.lib().pmx_mark_labels(_device.ptr(labels), labels.element_size(), strides, lo,
-                                                      ext, _device.ptr(flags), _device.stream()))
+                                                      ext, _device.ptr(flags), flags.numel(), _device.stream()))
     return flags
 
 
 def keep_device(img_t, labels, fa, fb, mode, phase):
     out = torch.empty_like(img_t)
+    if fb is not None and fb.numel() != fa.numel():
+        raise ValueError("flag arrays differ in size")
     _lib.check(_lib.lib().pmx_keep_labels(_device.ptr(img_t), _device.ptr(labels), labels.element_size(),
                                           img_t.numel(), _device.ptr(fa), _device.ptr(fb) if fb is not None else None,
-                                          mode, -1 if phase is None else int(phase), _device.ptr(out),
+                                          fa.numel(), mode, -1 if phase is None else int(phase), _device.ptr(out),
                                           _device.stream()))
     return out
 

@@ -55,13 +55,13 @@ void prof_mark(const char *name, hipStream_t s, bool start)
     std::lock_guard<std::mutex> lk(g_prof_mu);
     if (start) {
         ProfRec r{name, g_prof.get(), nullptr};
-        if (r.a) hipEventRecord(r.a, s);
+        if (r.a) (void)hipEventRecord(r.a, s);
         g_prof.recs.push_back(r);
     } else {
         for (auto it = g_prof.recs.rbegin(); it != g_prof.recs.rend(); ++it)
             if (it->b == nullptr && it->name == name) {
                 it->b = g_prof.get();
-                if (it->b) hipEventRecord(it->b, s);
+                if (it->b) (void)hipEventRecord(it->b, s);
                 break;
             }
     }

@@ -13,6 +13,7 @@
 namespace pmx {
 
 constexpr int kWave = 64;
+constexpr int kXcds = 8;  // MI355X: workgroups go round-robin over 8 XCDs, each with its own L2
 
 // Thread-local last-error text behind pmx_last_error().
 void set_error(const char *fmt, ...);

@@ -8,19 +8,21 @@
 //   pass Z  one wavefront per z-row: 1-D distance to the nearest solid
 //           from per-lane 16-voxel solid masks and wave scans; 16-byte
 //           loads and stores.
-//   pass Y  one lane per (x, z) line, stride nz: lower envelope of
-//           parabolas (Felzenszwalb) with exact int64 intersection tests,
-//           stack in LDS.  A zero-height parabola (a solid voxel) can
-//           never be popped and dominates everything behind it, so the
-//           envelope is evaluated and the stack reset at every solid: the
-//           stack only spans the gap between consecutive solids.
+//   pass Y  windowed minimum of parabolas along y: a workgroup stages a
+//           tile of 64 rows x 64 z-columns plus 32 halo rows each side in
+//           LDS; each voxel scans outward until k^2 >= its best value, so
+//           the work per voxel is about its distance to the nearest solid.
 //   pass X  the same along x (stride ny*nz), writing the requested
 //           outputs (int32 d2, float32 / float64 sqrt, or a u8 threshold).
 //
-// Lines whose gap between solids outgrows the LDS stack are appended to an
-// overflow list and redone by a persistent fallback kernel whose stacks
-// live in the workspace (capacity 3n+4, enough for a periodic line with no
-// solid at all, which needs the neighbouring periods in full).
+// A voxel not settled within the tile continues its scan in global memory
+// up to 128 rows; one still open after that defers its whole line to the
+// exact lower envelope of parabolas (Felzenszwalb, int64 intersection
+// tests), run by a persistent fallback kernel with stacks in the workspace
+// (capacity 3n+4, enough for a periodic line with no solid at all, which
+// needs the neighbouring periods in full).  The envelope is evaluated and
+// its stack reset at every solid: a zero-height parabola dominates
+// everything behind it.
 //
 // The result is the exact squared EDT.  At unit resolution the reference's
 // float32 arithmetic yields the same integers whenever a solid is
@@ -38,8 +40,6 @@ namespace {
 
 constexpr int kInf = PMX_D2_INF;
 constexpr int kNone = 1 << 29;  // "no solid on this side" marker in pass Z
-constexpr int kBlock = 128;     // lanes per block in the envelope passes
-constexpr int kFallbackThreads = 4096;
 
 // ---------------------------------------------------------------------------
 // pass Z: 1-D squared distance along contiguous rows
@@ -336,28 +336,6 @@ __device__ bool envelope_line(int n, int bc_lo, int bc_hi, const Stack &st, bool
 }
 
 
-// pass Y: lines (x, z) along y.  in/out: contiguous box arrays.
-template <int K>
-__global__ __launch_bounds__(kBlock) void k_pass_y(const int32_t *__restrict__ in,
-                                                   int32_t *__restrict__ out, int nx, int ny, int nz,
-                                                   int bc_lo, int bc_hi,
-                                                   uint32_t *__restrict__ ovf_count,
-                                                   uint32_t *__restrict__ ovf_list)
-{
-    __shared__ int sv[K * kBlock], sh[K * kBlock], sb[kChunk * kBlock];
-    const int64_t line = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (line >= (int64_t)nx * nz) return;
-    const int64_t x = line / nz, z = line - x * nz;
-    const int64_t base = x * (int64_t)ny * nz + z;
-    Stack st{sv + threadIdx.x, sh + threadIdx.x, kBlock, K, sb + threadIdx.x};
-    const int32_t *src = in + base;
-    int32_t *dst = out + base;
-    const bool ok = envelope_line(
-        ny, bc_lo, bc_hi, st, false, [&](int i) { return src[(int64_t)i * nz]; },
-        [&](int i, int v) { dst[(int64_t)i * nz] = v; });
-    if (!ok) ovf_list[atomicAdd(ovf_count, 1u)] = (uint32_t)line;
-}
-
 __device__ __forceinline__ void emit_x(const PassXOut &o, int64_t x, int64_t y, int64_t z,
                                        int nx, int ny, int nz, int v)
 {
@@ -382,39 +360,184 @@ __device__ __forceinline__ void emit_x(const PassXOut &o, int64_t x, int64_t y, 
     }
 }
 
-// pass X: lines (y, z) along x, reading the pass-Y array.
-template <int K>
-__global__ __launch_bounds__(kBlock) void k_pass_x(const int32_t *__restrict__ in, PassXOut o,
-                                                   int nx, int ny, int nz, int bc_lo, int bc_hi,
-                                                   uint32_t *__restrict__ ovf_count,
-                                                   uint32_t *__restrict__ ovf_list)
+// ---------------------------------------------------------------------------
+// passes Y and X: windowed minimum over an LDS tile
+// ---------------------------------------------------------------------------
+//
+// d(y) = min_k f(y + k) + k^2.  Scanning k = 1, 2, ... outward from y can
+// stop as soon as k^2 >= best: no farther candidate can win.  A solid
+// (f = 0) stops the scan at its distance, so the work per voxel is about
+// its distance to the nearest solid (a few voxels in a porous medium).
+// A workgroup owns a tile of T rows (the line axis) by 64 z-columns plus W
+// halo rows on each side in LDS; one lane per z-column, so every LDS row
+// is one coalesced 256-byte global segment.  Voxels not settled within W
+// continue the scan in global memory up to kReach rows; a voxel still
+// open after that flags its line for the envelope fallback.
+//
+// Values are clamped at `cap` on load.  Clamping commutes with the
+// passes: min(d, cap) of the whole EDT equals the EDT of the clamped
+// input, so a caller that only compares against a threshold t passes
+// cap = t + 1 and no scan ever runs past sqrt(t + 1).
+
+constexpr int kBig = 1 << 30;   // "no solid" inside the windowed passes
+constexpr int kTileZ = 64;
+constexpr int kWinThreads = 256;
+constexpr int kReach = 128;     // farthest row offset scanned before a line is deferred
+constexpr int kRowWall = -1, kRowNone = -2;
+
+// row r of a line of n rows under the boundary rule: an in-range row,
+// kRowWall (the solid layer just beyond a WALL face) or kRowNone
+__device__ __forceinline__ int map_row(int r, int n, int bc_lo, int bc_hi)
 {
-    __shared__ int sv[K * kBlock], sh[K * kBlock], sb[kChunk * kBlock];
-    const int64_t line = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (line >= (int64_t)ny * nz) return;
-    const int64_t y = line / nz, z = line - y * nz;
-    const int64_t pstride = (int64_t)ny * nz;
-    const int32_t *src = in + line;
-    Stack st{sv + threadIdx.x, sh + threadIdx.x, kBlock, K, sb + threadIdx.x};
-    const bool ok = envelope_line(
-        nx, bc_lo, bc_hi, st, false, [&](int i) { return src[(int64_t)i * pstride]; },
-        [&](int i, int v) { emit_x(o, i, y, z, nx, ny, nz, v); });
-    if (!ok) ovf_list[atomicAdd(ovf_count, 1u)] = (uint32_t)line;
+    if (r >= 0 && r < n) return r;
+    if (bc_lo == PMX_BC_PERIODIC) {
+        r %= n;
+        return r < 0 ? r + n : r;
+    }
+    if ((r == -1 && bc_lo == PMX_BC_WALL) || (r == n && bc_hi == PMX_BC_WALL)) return kRowWall;
+    return kRowNone;
 }
 
-// Fallback for deferred lines: persistent threads, stacks in the workspace.
+// XCD-aware tile order: workgroup b runs on XCD b % 8, so give each XCD a
+// contiguous range of logical tiles (neighbouring tiles share halo rows in
+// that XCD's L2).
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb)
+{
+    const int64_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds, i = b / kXcds;
+    return x * q + min(x, r) + i;
+}
+
+struct WinGeom {
+    int n;        // rows per line (the pass axis)
+    int64_t rs;   // element stride between rows
+    int64_t os;   // element stride between outer indices (x in pass Y, y in pass X)
+    int nouter, nz;
+    int bc_lo, bc_hi;
+    int cap;      // values are clamped to [0, cap], cap <= kBig
+    int vec;      // 16-byte loads allowed (aligned base, nz % 4 == 0)
+};
+
+__device__ __forceinline__ int fetch_row(const int32_t *line, const WinGeom &g, int r)
+{
+    const int m = map_row(r, g.n, g.bc_lo, g.bc_hi);
+    if (m >= 0) return min(line[(int64_t)m * g.rs], g.cap);
+    return m == kRowWall ? 0 : g.cap;
+}
+
+// kX: the pass along x writing PassXOut; otherwise pass Y writing `out`.
+template <int T, int W, bool kX>
+__global__ __launch_bounds__(kWinThreads) void k_window(const int32_t *__restrict__ in,
+                                                        int32_t *__restrict__ out, PassXOut o,
+                                                        WinGeom g, int nx, int ny,
+                                                        uint32_t *__restrict__ line_bits,
+                                                        uint32_t *__restrict__ ovf_count,
+                                                        uint32_t *__restrict__ ovf_list)
+{
+    static_assert(W % 4 == 0, "the scan steps four rows at a time");
+    constexpr int kRows = T + 2 * W;
+    __shared__ __align__(16) int tile[kRows * kTileZ];
+    const int nzt = (g.nz + kTileZ - 1) / kTileZ, nrt = (g.n + T - 1) / T;
+    const int64_t ntiles = (int64_t)g.nouter * nzt * nrt;
+    const int64_t t = xcd_tile(blockIdx.x, ntiles);
+    const int rt = (int)(t % nrt);
+    const int64_t t2 = t / nrt;
+    const int zt = (int)(t2 % nzt);
+    const int outer = (int)(t2 / nzt);
+    const int r0 = rt * T, z0 = zt * kTileZ;
+    const int32_t *base = in + (int64_t)outer * g.os + z0;
+
+    // stage rows r0 - W .. r0 + T + W - 1; columns past nz hold 0 (settled at once)
+    for (int i = threadIdx.x; i < kRows * (kTileZ / 4); i += kWinThreads) {
+        const int row = i / (kTileZ / 4), q = i % (kTileZ / 4);
+        const int m = map_row(r0 - W + row, g.n, g.bc_lo, g.bc_hi);
+        const int z = z0 + 4 * q;
+        int4 v;
+        if (m >= 0) {
+            const int32_t *src = base + (int64_t)m * g.rs + 4 * q;
+            if (g.vec && z + 3 < g.nz) {
+                v = *reinterpret_cast<const int4 *>(src);
+            } else {
+                v.x = z + 0 < g.nz ? src[0] : 0;
+                v.y = z + 1 < g.nz ? src[1] : 0;
+                v.z = z + 2 < g.nz ? src[2] : 0;
+                v.w = z + 3 < g.nz ? src[3] : 0;
+            }
+            v.x = min(v.x, g.cap);
+            v.y = min(v.y, g.cap);
+            v.z = min(v.z, g.cap);
+            v.w = min(v.w, g.cap);
+        } else {
+            const int c = m == kRowWall ? 0 : g.cap;
+            v = make_int4(c, c, c, c);
+        }
+        *reinterpret_cast<int4 *>(tile + row * kTileZ + 4 * q) = v;
+    }
+    __syncthreads();
+
+    const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int z = z0 + lane;
+    const int *col = tile + lane;
+    const int64_t line = (int64_t)outer * g.nz + z;
+    for (int j = wv; j < T; j += kWinThreads / kWave) {
+        const int y = r0 + j;
+        if (y >= g.n) break;
+        const int *c = col + (W + j) * kTileZ;
+        int best = c[0];
+        int k = 1;
+        for (; k <= W; k += 4) {
+            if (k * k >= best) break;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int kk = k + u;
+                best = min(best, min(c[-kk * kTileZ], c[kk * kTileZ]) + kk * kk);
+            }
+        }
+        bool ok = k * k >= best;
+        if (!ok && z < g.nz) {
+            // beyond the tile: scan the line in global memory, unless the
+            // line is already deferred
+            const uint32_t bit = 1u << (line & 31);
+            if (!(line_bits[line >> 5] & bit)) {
+                const int32_t *ln = in + (int64_t)outer * g.os + z;
+                const int kmax = g.bc_lo == PMX_BC_PERIODIC ? g.n / 2 : max(y + 1, g.n - y);
+                for (; k <= kReach && k <= kmax; ++k) {
+                    if (k * k >= best) break;
+                    best = min(best, min(fetch_row(ln, g, y - k), fetch_row(ln, g, y + k)) + k * k);
+                }
+                ok = k * k >= best || k > kmax;
+            }
+            if (!ok && !(atomicOr(line_bits + (line >> 5), bit) & bit))
+                ovf_list[atomicAdd(ovf_count, 1u)] = (uint32_t)line;
+        }
+        if (!ok || z >= g.nz) continue;
+        if constexpr (kX) {
+            emit_x(o, y, outer, z, nx, ny, g.nz, best >= kBig ? kInf : best);
+        } else {
+            out[(int64_t)outer * g.os + (int64_t)y * g.rs + z] = best >= kBig ? kInf : best;
+        }
+    }
+}
+
+// Fallback for deferred lines: the exact envelope, persistent threads,
+// stacks in the workspace; results clamped like the windowed passes.
+__device__ __forceinline__ int finish(int v, int cap)
+{
+    v = min(v, cap);
+    return v >= kBig ? kInf : v;
+}
+
 __global__ __launch_bounds__(256) void k_fallback_y(const int32_t *__restrict__ in,
                                                     int32_t *__restrict__ out, int nx, int ny,
-                                                    int nz, int bc_lo, int bc_hi,
+                                                    int nz, int bc_lo, int bc_hi, int cap,
                                                     const uint32_t *__restrict__ ovf_count,
                                                     const uint32_t *__restrict__ ovf_list,
-                                                    int *__restrict__ stk, int cap)
+                                                    int *__restrict__ stk, int stk_cap)
 {
     const uint32_t count = *ovf_count;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
-    Stack st{stk + tid, stk + (int64_t)cap * nthreads + tid, nthreads, cap,
-             stk + (int64_t)2 * cap * nthreads + tid};
+    Stack st{stk + tid, stk + (int64_t)stk_cap * nthreads + tid, nthreads, stk_cap,
+             stk + (int64_t)2 * stk_cap * nthreads + tid};
     for (int64_t i = tid; i < (int64_t)count; i += nthreads) {
         const int64_t line = ovf_list[i];
         const int64_t x = line / nz, z = line - x * nz;
@@ -423,33 +546,41 @@ __global__ __launch_bounds__(256) void k_fallback_y(const int32_t *__restrict__ 
         int32_t *dst = out + base;
         envelope_line(
             ny, bc_lo, bc_hi, st, true, [&](int j) { return src[(int64_t)j * nz]; },
-            [&](int j, int v) { dst[(int64_t)j * nz] = v; });
+            [&](int j, int v) { dst[(int64_t)j * nz] = finish(v, cap); });
     }
 }
 
 __global__ __launch_bounds__(256) void k_fallback_x(const int32_t *__restrict__ in, PassXOut o,
-                                                    int nx, int ny, int nz, int bc_lo, int bc_hi,
+                                                    int nx, int ny, int nz, int bc_lo, int bc_hi, int cap,
                                                     const uint32_t *__restrict__ ovf_count,
                                                     const uint32_t *__restrict__ ovf_list,
-                                                    int *__restrict__ stk, int cap)
+                                                    int *__restrict__ stk, int stk_cap)
 {
     const uint32_t count = *ovf_count;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
     const int64_t pstride = (int64_t)ny * nz;
-    Stack st{stk + tid, stk + (int64_t)cap * nthreads + tid, nthreads, cap,
-             stk + (int64_t)2 * cap * nthreads + tid};
+    Stack st{stk + tid, stk + (int64_t)stk_cap * nthreads + tid, nthreads, stk_cap,
+             stk + (int64_t)2 * stk_cap * nthreads + tid};
     for (int64_t i = tid; i < (int64_t)count; i += nthreads) {
         const int64_t line = ovf_list[i];
         const int64_t y = line / nz, z = line - y * nz;
         const int32_t *src = in + line;
         envelope_line(
             nx, bc_lo, bc_hi, st, true, [&](int j) { return src[(int64_t)j * pstride]; },
-            [&](int j, int v) { emit_x(o, j, y, z, nx, ny, nz, v); });
+            [&](int j, int v) { emit_x(o, j, y, z, nx, ny, nz, finish(v, cap)); });
     }
 }
 
-constexpr int kStackK = 32;
+__global__ __launch_bounds__(256) void k_clear_words(uint32_t *__restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+constexpr int kWinT = 64, kWinW = 32;
+constexpr size_t kFallbackStackBytes = size_t(256) << 20;
 
 }  // namespace
 
@@ -460,15 +591,24 @@ size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p)
     p->a = c.take<int32_t>(vox);
     p->b = c.take<int32_t>(vox);
     p->counters = c.take<uint32_t>(4);
-    const size_t lines = (size_t)std::max(g->n[0] * g->n[2], g->n[1] * g->n[2]);
-    p->ovf_list = c.take<uint32_t>(lines);
+    const size_t ly = (size_t)(g->n[0] * g->n[2]), lx = (size_t)(g->n[1] * g->n[2]);
+    p->n_bits = (ly + 31) / 32 + (lx + 31) / 32;
+    p->bits_y = c.take<uint32_t>(p->n_bits);
+    p->bits_x = p->bits_y + (ly + 31) / 32;
+    p->ovf_list = c.take<uint32_t>(std::max(ly, lx));
     p->cap = (int)(3 * std::max(g->n[0], g->n[1]) + 4);
-    p->stk = c.take<int>((size_t)(2 * p->cap + kChunk) * kFallbackThreads);
+    // persistent fallback lanes: one per deferred line up to 32768, with
+    // the stacks bounded to kFallbackStackBytes
+    const size_t per_lane = (size_t)(2 * p->cap + kChunk) * sizeof(int);
+    size_t lanes = std::min<size_t>(32768, (std::max(ly, lx) + 255) / 256 * 256);
+    lanes = std::min(lanes, std::max<size_t>(256, kFallbackStackBytes / per_lane / 256 * 256));
+    p->fallback_lanes = (int)lanes;
+    p->stk = c.take<int>((size_t)(2 * p->cap + kChunk) * lanes);
     return c.need();
 }
 
-int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &o, void *ws,
-            size_t ws_bytes, hipStream_t s)
+int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &o, int64_t cap,
+            void *ws, size_t ws_bytes, hipStream_t s)
 {
     if (int e = validate_geom(g)) return e;
     if (!img) return fail(PMX_ERR_INVALID, "img is NULL");
@@ -477,8 +617,11 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     if (!ws || ws_bytes < need)
         return fail(PMX_ERR_WORKSPACE, "workspace %zu bytes < required %zu", ws_bytes, need);
     plan_edt(g, ws, &p);
+    const int ccap = (int)std::min<int64_t>(std::max<int64_t>(cap, 0), kBig);
     const int nx = (int)g->n[0], ny = (int)g->n[1], nz = (int)g->n[2];
     const int64_t rows = (int64_t)nx * ny;
+    hipLaunchKernelGGL(k_clear_words, dim3((unsigned)std::min<int64_t>((p.n_bits + 255) / 256 + 1, 4096)),
+                       dim3(256), 0, s, p.bits_y, (int64_t)p.n_bits);
     const dim3 bz(256);
     const dim3 gz((unsigned)((rows + 3) / 4));
     {
@@ -494,29 +637,34 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     }
     if (int e = check_launch("pass z")) return e;
 
-    const int64_t ly = (int64_t)nx * nz, lx = (int64_t)ny * nz;
+    const int vec = ((uintptr_t)p.a % 16 == 0 && (uintptr_t)p.b % 16 == 0 && nz % 4 == 0) ? 1 : 0;
+    const int nzt = (nz + kTileZ - 1) / kTileZ;
     {
     ProfScope ps("edt_pass_y", s);
-    hipLaunchKernelGGL(k_pass_y<kStackK>, dim3((unsigned)((ly + kBlock - 1) / kBlock)), dim3(kBlock),
-                       0, s, p.a, p.b, nx, ny, nz, g->bc[2], g->bc[3], p.counters + 0, p.ovf_list);
+    const WinGeom wg{ny, nz, (int64_t)ny * nz, nx, nz, g->bc[2], g->bc[3], ccap, vec};
+    const int64_t tiles = (int64_t)nx * nzt * ((ny + kWinT - 1) / kWinT);
+    hipLaunchKernelGGL((k_window<kWinT, kWinW, false>), dim3((unsigned)tiles), dim3(kWinThreads), 0, s, p.a,
+                       p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list);
     }
     if (int e = check_launch("pass y")) return e;
     {
     ProfScope ps("edt_fallback_y", s);
-    hipLaunchKernelGGL(k_fallback_y, dim3(kFallbackThreads / 256), dim3(256), 0, s, p.a, p.b, nx,
-                       ny, nz, g->bc[2], g->bc[3], p.counters + 0, p.ovf_list, p.stk, p.cap);
+    hipLaunchKernelGGL(k_fallback_y, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.a, p.b, nx, ny, nz,
+                       g->bc[2], g->bc[3], ccap, p.counters + 0, p.ovf_list, p.stk, p.cap);
     }
     if (int e = check_launch("pass y fallback")) return e;
     {
     ProfScope ps("edt_pass_x", s);
-    hipLaunchKernelGGL(k_pass_x<kStackK>, dim3((unsigned)((lx + kBlock - 1) / kBlock)), dim3(kBlock),
-                       0, s, p.b, o, nx, ny, nz, g->bc[0], g->bc[1], p.counters + 1, p.ovf_list);
+    const WinGeom wg{nx, (int64_t)ny * nz, nz, ny, nz, g->bc[0], g->bc[1], ccap, vec};
+    const int64_t tiles = (int64_t)ny * nzt * ((nx + kWinT - 1) / kWinT);
+    hipLaunchKernelGGL((k_window<kWinT, kWinW, true>), dim3((unsigned)tiles), dim3(kWinThreads), 0, s, p.b,
+                       nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1, p.ovf_list);
     }
     if (int e = check_launch("pass x")) return e;
     {
     ProfScope ps("edt_fallback_x", s);
-    hipLaunchKernelGGL(k_fallback_x, dim3(kFallbackThreads / 256), dim3(256), 0, s, p.b, o, nx, ny,
-                       nz, g->bc[0], g->bc[1], p.counters + 1, p.ovf_list, p.stk, p.cap);
+    hipLaunchKernelGGL(k_fallback_x, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.b, o, nx, ny, nz,
+                       g->bc[0], g->bc[1], ccap, p.counters + 1, p.ovf_list, p.stk, p.cap);
     }
     return check_launch("pass x fallback");
 }
@@ -569,7 +717,7 @@ int pmx_edt(const uint8_t *img, const pmx_geom *g, int32_t *d2, float *f32sq, fl
     o.f64 = f64;
     o.sx = g ? g->out_s[0] : 0;
     o.sy = g ? g->out_s[1] : 0;
-    return pmx::run_edt(img, g, false, o, workspace, workspace_bytes, stream);
+    return pmx::run_edt(img, g, false, o, INT64_MAX, workspace, workspace_bytes, stream);
 }
 
 int pmx_morph(const uint8_t *img, const pmx_geom *g, int invert, int64_t t, uint8_t v_le,
@@ -588,7 +736,8 @@ int pmx_morph(const uint8_t *img, const pmx_geom *g, int invert, int64_t t, uint
     o.v_le = v_le;
     o.v_gt = v_gt;
     o.wall_mask = wall_mask;
-    return pmx::run_edt(img, g, invert != 0, o, workspace, workspace_bytes, stream);
+    // only "d2 <= t" is asked: clamp at t + 1
+    return pmx::run_edt(img, g, invert != 0, o, (int64_t)tt + 1, workspace, workspace_bytes, stream);
 }
 
 }  // extern "C"

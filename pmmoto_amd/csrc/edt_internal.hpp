@@ -26,16 +26,21 @@ struct PassXOut {
 struct EdtPlan {
     int32_t *a = nullptr, *b = nullptr;
     uint32_t *counters = nullptr, *ovf_list = nullptr;
+    uint32_t *bits_y = nullptr, *bits_x = nullptr;  // deferred-line bitmaps (one allocation)
+    size_t n_bits = 0;                              // words in both bitmaps
     int *stk = nullptr;
-    int cap = 0;
+    int cap = 0;            // fallback stack entries per lane
+    int fallback_lanes = 0;
 };
 
 // Workspace layout of run_edt for geometry g (sizes only when ws is null).
 size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p);
 
 // Exact squared EDT of img over the box of g (distance to voxels == 0, or
-// != 0 with invert), delivered through `o`; stream-ordered on s.
-int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &o, void *ws,
-            size_t ws_bytes, hipStream_t s);
+// != 0 with invert), clamped to min(d2, cap) (cap >= 2^30: unclamped),
+// delivered through `o`; stream-ordered on s.  A caller that only tests
+// d2 <= t passes cap = t + 1.
+int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &o, int64_t cap,
+            void *ws, size_t ws_bytes, hipStream_t s);
 
 }  // namespace pmx

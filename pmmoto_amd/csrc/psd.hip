@@ -287,7 +287,8 @@ int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo
         o.d2 = p.d2;
         o.sx = gc.out_s[0];
         o.sy = gc.out_s[1];
-        if (int e = run_edt(img, &gc, false, o, p.edt_ws, p.edt_bytes, s)) return e;
+        // only comparisons with the erosion thresholds follow: clamp above the largest
+        if (int e = run_edt(img, &gc, false, o, t_erode[0] < INT64_MAX ? t_erode[0] + 1 : INT64_MAX, p.edt_ws, p.edt_bytes, s)) return e;
     }
     if (hipMemcpyAsync(p.te, t_erode, sizeof(int64_t) * n_levels, hipMemcpyHostToDevice, s) != hipSuccess)
         return fail(PMX_ERR_HIP, "threshold upload failed");
@@ -341,7 +342,7 @@ int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo
         o.psd_map = code_map;
         o.psd_code = lv + 1;
         ProfScope ps("psd_dilate", s);
-        if (int rc = run_edt(p.mask, &gc, true, o, p.edt_ws, p.edt_bytes, s)) return rc;
+        if (int rc = run_edt(p.mask, &gc, true, o, (int64_t)o.t + 1, p.edt_ws, p.edt_bytes, s)) return rc;
     }
     return PMX_OK;
 }

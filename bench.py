@@ -100,6 +100,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-psd", action="store_true")
     ap.add_argument("--psd-reps", type=int, default=2)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed steps, copy a 1 GiB device buffer 3 times (the known-size "
+                         "reference kernel for HBM counter scaling under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
@@ -158,6 +161,14 @@ def main() -> int:
     dom_ms = prof[dom][0] / prof[dom][1]
     achieved = BYTES_PER_VOXEL[dom] * vox / (dom_ms * 1e-3) / 1e9
     print(f"[bench] {args.steps} steps: {ms:.3f} ms/step; kernels {per_kernel}", file=sys.stderr, flush=True)
+
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     psd = None
     if not args.no_psd:

@@ -13,7 +13,8 @@ import os
 
 import numpy as np
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpmx.so")
+# PMX_LIBRARY may name another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("PMX_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpmx.so")
 
 OK, ERR_INVALID, ERR_HIP, ERR_WORKSPACE = 0, -1, -2, -3
 BC_END, BC_WALL, BC_PERIODIC = 0, 1, 2

@@ -40,18 +40,22 @@ inline int check_launch(const char *what)
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-// Carves aligned sub-buffers out of the caller's workspace.
+// Carves 256-byte aligned sub-buffers out of the caller's workspace (which
+// may itself have any alignment: need() includes the slack to align it).
 struct Carver {
     char *base;
     size_t cap, used = 0;
-    Carver(void *b, size_t c) : base(static_cast<char *>(b)), cap(c) {}
+    Carver(void *b, size_t c)
+        : base(b ? reinterpret_cast<char *>(align_up(reinterpret_cast<uintptr_t>(b), 256)) : nullptr), cap(c)
+    {
+    }
     template <class T> T *take(size_t count)
     {
         size_t off = align_up(used, 256);
         used = off + count * sizeof(T);
         return base ? reinterpret_cast<T *>(base + off) : nullptr;
     }
-    size_t need() const { return align_up(used, 256); }
+    size_t need() const { return align_up(used, 256) + 256; }
 };
 
 int validate_geom(const pmx_geom *g);

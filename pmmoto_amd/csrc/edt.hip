@@ -2,17 +2,19 @@
 //
 // Replaces the reference's three per-line passes (_distance.hpp:135-272,
 // driven by _distance.pyx:86-127,308-353 from
-// euclidean_distance_transform.py:56-101,208-303) with three whole-volume
-// kernels over an integer working array:
+// euclidean_distance_transform.py:56-101,208-303) with three kernels:
 //
-//   pass Z  one wavefront per z-row: 1-D distance to the nearest solid
-//           from per-lane 16-voxel solid masks and wave scans; 16-byte
-//           loads and stores.
-//   pass Y  windowed minimum of parabolas along y: a workgroup stages a
-//           tile of 64 rows x 64 z-columns plus 32 halo rows each side in
-//           LDS; each voxel scans outward until k^2 >= its best value, so
-//           the work per voxel is about its distance to the nearest solid.
-//   pass X  the same along x (stride ny*nz), writing the requested
+//   rows    one wavefront per z-row compresses the image row into 16-byte
+//           summaries, one per 64-voxel chunk: the chunk's solid bitmask
+//           and the nearest solid before and after the chunk (boundary
+//           rules included).  The 1-D distance along z of any voxel follows
+//           from its chunk's summary with a few bit operations.
+//   pass Y  windowed minimum of parabolas along y: a workgroup builds the
+//           1-D z distances of a tile of rows x 64 z-columns (plus halo
+//           rows) in LDS from the summaries, then scans each voxel outward
+//           until k^2 >= its best value, so the work per voxel is about its
+//           distance to the nearest solid.  Output stored [y][x][z].
+//   pass X  the same along x over the pass-Y array, writing the requested
 //           outputs (int32 d2, float32 / float64 sqrt, or a u8 threshold).
 //
 // A voxel not settled within the tile continues its scan in global memory
@@ -39,10 +41,10 @@ namespace pmx {
 namespace {
 
 constexpr int kInf = PMX_D2_INF;
-constexpr int kNone = 1 << 29;  // "no solid on this side" marker in pass Z
+constexpr int kNone = 1 << 29;  // "no solid on this side" position marker
 
 // ---------------------------------------------------------------------------
-// pass Z: 1-D squared distance along contiguous rows
+// row summaries
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ int wave_incl_max(int v, int lane)
@@ -74,52 +76,79 @@ __device__ __forceinline__ int wave_incl_min_rev(int v, int lane)
 constexpr int kSup = 16;
 
 template <bool kInvert>
-__device__ __forceinline__ uint32_t solid_mask16(const uint8_t *src, int e0, int nz, bool vec)
+__device__ __forceinline__ uint32_t bytes_mask(const uint32_t w[4])
 {
     uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t v = (w[k] >> (8 * b)) & 0xffu;
+            if (kInvert ? (v != 0) : (v == 0)) m |= 1u << (4 * k + b);
+        }
+    return m;
+}
+
+// Solid mask of row bytes e0 .. e0+15 (bits past nz stay 0).  Aligned rows
+// take one 16-byte load; rows at any other byte offset read the five
+// aligned dwords around the chunk and shift them into place, as long as
+// those dwords lie inside the row; the ends of such rows go byte by byte.
+template <bool kInvert>
+__device__ __forceinline__ uint32_t solid_mask16(const uint8_t *src, int e0, int nz, bool vec)
+{
     if (vec && e0 + 16 <= nz) {
         const uint4 q = *reinterpret_cast<const uint4 *>(src + e0);
         const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+        return bytes_mask<kInvert>(w);
+    }
+    const int sh = (int)((uintptr_t)(src + e0) & 3u);
+    if (!vec && e0 - sh >= 0 && e0 - sh + 20 <= nz) {
+        const uint32_t *a = reinterpret_cast<const uint32_t *>(src + e0 - sh);
+        const uint32_t d0 = a[0], d1 = a[1], d2 = a[2], d3 = a[3], d4 = a[4];
+        const uint32_t w[4] = {__builtin_amdgcn_alignbyte(d1, d0, sh), __builtin_amdgcn_alignbyte(d2, d1, sh),
+                               __builtin_amdgcn_alignbyte(d3, d2, sh), __builtin_amdgcn_alignbyte(d4, d3, sh)};
+        return bytes_mask<kInvert>(w);
+    }
+    uint32_t m = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const uint32_t v = (w[k] >> (8 * b)) & 0xffu;
-                if (kInvert ? (v != 0) : (v == 0)) m |= 1u << (4 * k + b);
-            }
-    } else {
-#pragma unroll
-        for (int b = 0; b < 16; ++b) {
-            const int e = e0 + b;
-            if (e < nz) {
-                const uint8_t v = src[e];
-                if (kInvert ? (v != 0) : (v == 0)) m |= 1u << b;
-            }
+    for (int b = 0; b < 16; ++b) {
+        const int e = e0 + b;
+        if (e < nz) {
+            const uint8_t v = src[e];
+            if (kInvert ? (v != 0) : (v == 0)) m |= 1u << b;
         }
     }
     return m;
 }
 
+// Summary of chunk c (voxels 64c .. 64c+63) of a row: {P, N, lo, hi} with
+// P the last solid before the chunk, N the first solid after it (kNone
+// magnitudes where there is none; WALL = a solid at -1 / nz, PERIODIC =
+// the row's own solids one period away) and lo | hi << 32 the chunk's
+// solid bitmask (bits past nz are 0).
+//
+// One wavefront per row; lane l owns bytes 16l .. 16l+15 of each 1024-byte
+// super-chunk, four lanes per chunk.  Sweep 1 finds each super-chunk's
+// last / first solid, sweep 2 writes the chunk summaries with wave scans
+// for the carries.
 template <bool kInvert>
-__global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img, int64_t in_sx,
-                                                int64_t in_sy, int ny, int nz, int64_t rows,
-                                                int bc_lo, int bc_hi, int vec_in, int vec_out,
-                                                int32_t *__restrict__ out,
-                                                uint32_t *__restrict__ counters)
+__global__ __launch_bounds__(256) void k_row_summary(const uint8_t *__restrict__ img, int64_t in_sx,
+                                                     int64_t in_sy, int ny, int nz, int64_t rows,
+                                                     int bc_lo, int bc_hi, int vec_in,
+                                                     int4 *__restrict__ summary,
+                                                     uint32_t *__restrict__ counters)
 {
     __shared__ int sc[4][2][kSup];  // per wave: each super-chunk's last / first solid
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
     const int64_t row = (int64_t)blockIdx.x * (blockDim.x / kWave) + w;
-    if (blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0;  // overflow counts of passes Y, X
+    if (blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0;  // deferred-line counts
     if (row >= rows) return;
     const int64_t x = row / ny, y = row - x * ny;
     const uint8_t *src = img + x * in_sx + y * in_sy;
-    int32_t *dst = out + row * (int64_t)nz;
-    const int nsup = (nz + 1023) / 1024;
+    const int nsup = (nz + 1023) / 1024, nzt = (nz + 63) / 64;
     const bool vin = vec_in != 0;
 
-    // sweep 1: each super-chunk's last and first solid
     for (int s = 0; s < nsup; ++s) {
         const int e0 = s * 1024 + lane * 16;
         const uint32_t m = solid_mask16<kInvert>(src, e0, nz, vin);
@@ -142,7 +171,7 @@ __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img,
         row_last = max(row_last, sc[w][0][s]);
         row_first = min(row_first, sc[w][1][s]);
     }
-    int prev0, next0;  // boundary defaults
+    int prev0, next0;  // the solids beyond the row's ends
     if (bc_lo == PMX_BC_PERIODIC) {
         prev0 = row_last > -kNone ? row_last - nz : -kNone;
         next0 = row_first < kNone ? row_first + nz : kNone;
@@ -150,7 +179,7 @@ __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img,
         prev0 = bc_lo == PMX_BC_WALL ? -1 : -kNone;
         next0 = bc_hi == PMX_BC_WALL ? nz : kNone;
     }
-    // sweep 2: distances, super-chunk by super-chunk
+    int4 *dst = summary + row * nzt;
     int pcar = prev0;
     for (int s = 0; s < nsup; ++s) {
         int ncar = next0;
@@ -160,35 +189,33 @@ __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img,
         const int l = m ? e0 + 31 - __clz(m) : -kNone;
         const int f = m ? e0 + __ffs(m) - 1 : kNone;
         const int inc_l = wave_incl_max(l, lane);
-        int pc = __shfl_up(inc_l, 1, kWave);
-        if (lane == 0) pc = -kNone;
-        pc = max(pc, pcar);
+        int before = __shfl_up(inc_l, 1, kWave);  // last solid of lanes < this one
+        if (lane == 0) before = -kNone;
         const int inc_f = wave_incl_min_rev(f, lane);
-        int nc = __shfl_down(inc_f, 1, kWave);
-        if (lane == kWave - 1) nc = kNone;
-        nc = min(nc, ncar);
-        int d2v[16];
-#pragma unroll
-        for (int b = 0; b < 16; ++b) {
-            const uint32_t below = m & ((2u << b) - 1u);
-            const uint32_t above = m & ~((1u << b) - 1u);
-            const int e = e0 + b;
-            const int prev = below ? e0 + 31 - __clz(below) : pc;
-            const int next = above ? e0 + __ffs(above) - 1 : nc;
-            const int d = min(e - prev, next - e);
-            d2v[b] = d >= kNone / 2 ? kInf : d * d;
-        }
-        if (vec_out && e0 + 16 <= nz) {
-            int4 *q = reinterpret_cast<int4 *>(dst + e0);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) q[k] = make_int4(d2v[4 * k], d2v[4 * k + 1], d2v[4 * k + 2], d2v[4 * k + 3]);
-        } else {
-#pragma unroll
-            for (int b = 0; b < 16; ++b)
-                if (e0 + b < nz) dst[e0 + b] = d2v[b];
-        }
+        int after = __shfl_down(inc_f, 4, kWave);  // first solid of lanes >= this one + 4
+        if (lane + 4 >= kWave) after = kNone;
+        const uint32_t m1 = __shfl_down(m, 1, kWave), m2 = __shfl_down(m, 2, kWave),
+                       m3 = __shfl_down(m, 3, kWave);
+        const int chunk = s * 16 + lane / 4;
+        if ((lane & 3) == 0 && chunk < nzt)
+            dst[chunk] = make_int4(max(before, pcar), min(after, ncar), (int)(m | (m1 << 16)), (int)(m2 | (m3 << 16)));
         pcar = max(pcar, sc[w][0][s]);
     }
+}
+
+// squared 1-D distance along z at column z of a row from its summary
+// array; -1 when the row has no reachable solid
+__device__ __forceinline__ int zdist2(const int4 *__restrict__ row_summary, int z)
+{
+    const int4 s = row_summary[z >> 6];
+    const uint64_t m = ((uint64_t)(uint32_t)s.w << 32) | (uint32_t)s.z;
+    const int bit = z & 63, z64 = z - bit;
+    const uint64_t below = m & (bit == 63 ? ~0ull : ((2ull << bit) - 1ull));
+    const uint64_t above = m & (~0ull << bit);
+    const int prev = below ? z64 + 63 - __clzll(below) : s.x;
+    const int next = above ? z64 + __ffsll((unsigned long long)above) - 1 : s.y;
+    const int d = min(z - prev, next - z);
+    return d >= kNone / 2 ? -1 : d * d;
 }
 
 // ---------------------------------------------------------------------------
@@ -335,7 +362,6 @@ __device__ bool envelope_line(int n, int bc_lo, int bc_hi, const Stack &st, bool
     return true;
 }
 
-
 __device__ __forceinline__ void emit_x(const PassXOut &o, int64_t x, int64_t y, int64_t z,
                                        int nx, int ny, int nz, int v)
 {
@@ -360,6 +386,7 @@ __device__ __forceinline__ void emit_x(const PassXOut &o, int64_t x, int64_t y, 
     }
 }
 
+
 // ---------------------------------------------------------------------------
 // passes Y and X: windowed minimum over an LDS tile
 // ---------------------------------------------------------------------------
@@ -368,9 +395,10 @@ __device__ __forceinline__ void emit_x(const PassXOut &o, int64_t x, int64_t y, 
 // stop as soon as k^2 >= best: no farther candidate can win.  A solid
 // (f = 0) stops the scan at its distance, so the work per voxel is about
 // its distance to the nearest solid (a few voxels in a porous medium).
-// A workgroup owns a tile of T rows (the line axis) by 64 z-columns plus W
-// halo rows on each side in LDS; one lane per z-column, so every LDS row
-// is one coalesced 256-byte global segment.  Voxels not settled within W
+// A workgroup owns a strip of 64 z-columns of one line bundle and walks
+// kTilesPerBlock row tiles of T rows, each staged in LDS with W halo rows
+// on each side; one lane per z-column.  The next tile's loads are issued
+// before the current tile is scanned.  Voxels not settled within W
 // continue the scan in global memory up to kReach rows; a voxel still
 // open after that flags its line for the envelope fallback.
 //
@@ -384,6 +412,10 @@ constexpr int kTileZ = 64;
 constexpr int kWinThreads = 256;
 constexpr int kReach = 128;     // farthest row offset scanned before a line is deferred
 constexpr int kRowWall = -1, kRowNone = -2;
+#ifndef PMX_WIN_TILES
+#define PMX_WIN_TILES 8
+#endif
+constexpr int kTilesPerBlock = PMX_WIN_TILES;
 
 // row r of a line of n rows under the boundary rule: an in-range row,
 // kRowWall (the solid layer just beyond a WALL face) or kRowNone
@@ -398,9 +430,9 @@ __device__ __forceinline__ int map_row(int r, int n, int bc_lo, int bc_hi)
     return kRowNone;
 }
 
-// XCD-aware tile order: workgroup b runs on XCD b % 8, so give each XCD a
-// contiguous range of logical tiles (neighbouring tiles share halo rows in
-// that XCD's L2).
+// XCD-aware block order: workgroup b runs on XCD b % 8, so give each XCD a
+// contiguous range of logical blocks (neighbouring tiles share halo rows
+// in that XCD's L2).
 __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb)
 {
     const int64_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds, i = b / kXcds;
@@ -408,113 +440,264 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb)
 }
 
 struct WinGeom {
-    int n;        // rows per line (the pass axis)
-    int64_t rs;   // element stride between rows
-    int64_t os;   // element stride between outer indices (x in pass Y, y in pass X)
+    int n;          // rows per line (the pass axis)
+    int64_t rs;     // input element stride between rows (pass X)
+    int64_t os;     // input element stride between outer indices (pass X)
     int nouter, nz;
     int bc_lo, bc_hi;
-    int cap;      // values are clamped to [0, cap], cap <= kBig
-    int vec;      // 16-byte loads allowed (aligned base, nz % 4 == 0)
+    int cap;        // values are clamped to [0, cap], cap <= kBig
+    int64_t pitch;  // input row length in memory: a multiple of 4 >= nz (pass X)
+    int64_t out_os, out_rs;  // output strides of the outer index and of a row (pass Y)
+    int ny;         // rows per x of the summary array (pass Y)
 };
 
-__device__ __forceinline__ int fetch_row(const int32_t *line, const WinGeom &g, int r)
+// pass-Y input at row r of the line (outer, z): the clamped z distance
+__device__ __forceinline__ int fetch_y(const int4 *__restrict__ summary, const WinGeom &g, int outer, int z,
+                                       int r)
+{
+    const int m = map_row(r, g.n, g.bc_lo, g.bc_hi);
+    if (m < 0) return m == kRowWall ? 0 : g.cap;
+    const int nzt = (g.nz + kTileZ - 1) / kTileZ;
+    const int d = zdist2(summary + ((int64_t)outer * g.ny + m) * nzt, z);
+    return d < 0 ? g.cap : min(d, g.cap);
+}
+
+// pass-X input at row r of the line starting at `line`
+__device__ __forceinline__ int fetch_x(const int32_t *line, const WinGeom &g, int r)
 {
     const int m = map_row(r, g.n, g.bc_lo, g.bc_hi);
     if (m >= 0) return min(line[(int64_t)m * g.rs], g.cap);
     return m == kRowWall ? 0 : g.cap;
 }
 
-// kX: the pass along x writing PassXOut; otherwise pass Y writing `out`.
-template <int T, int W, bool kX>
-__global__ __launch_bounds__(kWinThreads) void k_window(const int32_t *__restrict__ in,
+// Staging of tile rows.  Pass X: thread (q, row0) = (t % 16, t / 16)
+// owns columns 4q .. 4q+3 of rows row0 + 16 i, loaded as 16-byte vectors.
+// Pass Y: lane i of wave w holds the row summary of row w + 4 i, and the
+// wave rebuilds that row's z distances (one column per lane).  Loads are
+// issued (addresses clamped into the line) well before their commit.
+template <int N>
+__device__ __forceinline__ void issue_x(const int32_t *__restrict__ base, const WinGeom &g, int first, int row0,
+                                        int c4, int (&mrow)[N], int4 (&raw)[N])
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) mrow[i] = map_row(first + row0 + 16 * i, g.n, g.bc_lo, g.bc_hi);
+#pragma unroll
+    for (int i = 0; i < N; ++i) raw[i] = *reinterpret_cast<const int4 *>(base + (int64_t)max(mrow[i], 0) * g.rs + c4);
+}
+
+// clamp at cap (columns past nz hold 0, settled at once) and store at
+// tile rows dst + row0 + 16 i; acc keeps the per-column minimum
+template <int N>
+__device__ __forceinline__ void commit_x(int *tile, const WinGeom &g, int dst, int row0, int q, int zl,
+                                         const int (&mrow)[N], const int4 (&raw)[N], int4 &acc)
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const bool in_line = mrow[i] >= 0;
+        const int c = mrow[i] == kRowWall ? 0 : g.cap;
+        int4 v;
+        v.x = in_line ? (zl >= 0 ? min(raw[i].x, g.cap) : 0) : c;
+        v.y = in_line ? (zl >= 1 ? min(raw[i].y, g.cap) : 0) : c;
+        v.z = in_line ? (zl >= 2 ? min(raw[i].z, g.cap) : 0) : c;
+        v.w = in_line ? (zl >= 3 ? min(raw[i].w, g.cap) : 0) : c;
+        acc.x = min(acc.x, v.x);
+        acc.y = min(acc.y, v.y);
+        acc.z = min(acc.z, v.z);
+        acc.w = min(acc.w, v.w);
+        *reinterpret_cast<int4 *>(tile + (dst + row0 + 16 * i) * kTileZ + 4 * q) = v;
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void issue_y(const int4 *__restrict__ sbase, int nzt, const WinGeom &g, int first,
+                                        int wv, int lane, int &m, int4 &sm)
+{
+    static_assert(N <= kWave, "one row summary per lane");
+    m = map_row(first + wv + 4 * lane, g.n, g.bc_lo, g.bc_hi);
+    if (lane < N) sm = sbase[(int64_t)max(m, 0) * nzt];
+}
+
+// z distances of tile rows dst + wv + 4 i from the summaries held by lane i
+template <int N>
+__device__ __forceinline__ void commit_y(int *tile, const WinGeom &g, int dst, int wv, int lane, int z0, int m,
+                                         int4 sm, int &acc)
+{
+    const int z = z0 + lane;
+    const uint64_t lo_mask = lane == 63 ? ~0ull : ((2ull << lane) - 1ull), hi_mask = ~0ull << lane;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const int mi = __builtin_amdgcn_readlane(m, i);
+        const int px = __builtin_amdgcn_readlane(sm.x, i), nx_ = __builtin_amdgcn_readlane(sm.y, i);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(sm.z, i), hi = (uint32_t)__builtin_amdgcn_readlane(sm.w, i);
+        const uint64_t msk = ((uint64_t)hi << 32) | lo;
+        const uint64_t below = msk & lo_mask, above = msk & hi_mask;
+        const int prev = below ? z0 + 63 - __clzll(below) : px;
+        const int next = above ? z0 + __ffsll((unsigned long long)above) - 1 : nx_;
+        const int d = min(z - prev, next - z);
+        int f = d >= kNone / 2 ? g.cap : min(d * d, g.cap);
+        f = mi >= 0 ? f : (mi == kRowWall ? 0 : g.cap);
+        f = z < g.nz ? f : 0;
+        tile[(dst + wv + 4 * i) * kTileZ + lane] = f;
+        acc = min(acc, f);
+    }
+}
+
+// kY: pass Y (input: row summaries, output: the int32 array `out`);
+// otherwise pass X (input: that array, output: PassXOut).
+//
+// The tile is a sliding window: after tile i is scanned its last 2W rows
+// move to the top and only the T new rows are loaded (issued while tile i
+// is scanned), so each row is loaded and built once per block.
+template <int T, int W, bool kY>
+__global__ __launch_bounds__(kWinThreads) void k_window(const int4 *__restrict__ summary,
+                                                        const int32_t *__restrict__ in,
                                                         int32_t *__restrict__ out, PassXOut o,
                                                         WinGeom g, int nx, int ny,
                                                         uint32_t *__restrict__ line_bits,
                                                         uint32_t *__restrict__ ovf_count,
                                                         uint32_t *__restrict__ ovf_list)
 {
-    static_assert(W % 4 == 0, "the scan steps four rows at a time");
+    static_assert(W % 8 == 0 && T % 16 == 0, "tile layout");
     constexpr int kRows = T + 2 * W;
+    constexpr int kWaves = kWinThreads / kWave;
+    static_assert(kWaves == 4 && kWinThreads == 256, "four waves, 16 threads per row of int4");
+    static_assert(kRows / 4 <= kWave, "one summary per lane");
+    constexpr int kHalo = 2 * W / 16;  // int4 rows per thread of the halo move
     __shared__ __align__(16) int tile[kRows * kTileZ];
+    __shared__ __align__(16) int part[2][kWaves][kTileZ];
+
     const int nzt = (g.nz + kTileZ - 1) / kTileZ, nrt = (g.n + T - 1) / T;
-    const int64_t ntiles = (int64_t)g.nouter * nzt * nrt;
-    const int64_t t = xcd_tile(blockIdx.x, ntiles);
-    const int rt = (int)(t % nrt);
-    const int64_t t2 = t / nrt;
+    const int nchunk = (nrt + kTilesPerBlock - 1) / kTilesPerBlock;
+    const int64_t nblocks = (int64_t)g.nouter * nzt * nchunk;
+    const int64_t t = xcd_tile(blockIdx.x, nblocks);
+    const int chunk = (int)(t % nchunk);
+    const int64_t t2 = t / nchunk;
     const int zt = (int)(t2 % nzt);
     const int outer = (int)(t2 / nzt);
-    const int r0 = rt * T, z0 = zt * kTileZ;
-    const int32_t *base = in + (int64_t)outer * g.os + z0;
-
-    // stage rows r0 - W .. r0 + T + W - 1; columns past nz hold 0 (settled at once)
-    for (int i = threadIdx.x; i < kRows * (kTileZ / 4); i += kWinThreads) {
-        const int row = i / (kTileZ / 4), q = i % (kTileZ / 4);
-        const int m = map_row(r0 - W + row, g.n, g.bc_lo, g.bc_hi);
-        const int z = z0 + 4 * q;
-        int4 v;
-        if (m >= 0) {
-            const int32_t *src = base + (int64_t)m * g.rs + 4 * q;
-            if (g.vec && z + 3 < g.nz) {
-                v = *reinterpret_cast<const int4 *>(src);
-            } else {
-                v.x = z + 0 < g.nz ? src[0] : 0;
-                v.y = z + 1 < g.nz ? src[1] : 0;
-                v.z = z + 2 < g.nz ? src[2] : 0;
-                v.w = z + 3 < g.nz ? src[3] : 0;
-            }
-            v.x = min(v.x, g.cap);
-            v.y = min(v.y, g.cap);
-            v.z = min(v.z, g.cap);
-            v.w = min(v.w, g.cap);
-        } else {
-            const int c = m == kRowWall ? 0 : g.cap;
-            v = make_int4(c, c, c, c);
-        }
-        *reinterpret_cast<int4 *>(tile + row * kTileZ + 4 * q) = v;
-    }
-    __syncthreads();
+    const int rt_end = min(nrt, (chunk + 1) * kTilesPerBlock);
+    const int z0 = zt * kTileZ;
 
     const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int z = z0 + lane;
     const int *col = tile + lane;
     const int64_t line = (int64_t)outer * g.nz + z;
-    for (int j = wv; j < T; j += kWinThreads / kWave) {
-        const int y = r0 + j;
-        if (y >= g.n) break;
-        const int *c = col + (W + j) * kTileZ;
-        int best = c[0];
-        int k = 1;
-        for (; k <= W; k += 4) {
-            if (k * k >= best) break;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int kk = k + u;
-                best = min(best, min(c[-kk * kTileZ], c[kk * kTileZ]) + kk * kk);
-            }
-        }
-        bool ok = k * k >= best;
-        if (!ok && z < g.nz) {
-            // beyond the tile: scan the line in global memory, unless the
-            // line is already deferred
-            const uint32_t bit = 1u << (line & 31);
-            if (!(line_bits[line >> 5] & bit)) {
-                const int32_t *ln = in + (int64_t)outer * g.os + z;
-                const int kmax = g.bc_lo == PMX_BC_PERIODIC ? g.n / 2 : max(y + 1, g.n - y);
-                for (; k <= kReach && k <= kmax; ++k) {
-                    if (k * k >= best) break;
-                    best = min(best, min(fetch_row(ln, g, y - k), fetch_row(ln, g, y + k)) + k * k);
-                }
-                ok = k * k >= best || k > kmax;
-            }
-            if (!ok && !(atomicOr(line_bits + (line >> 5), bit) & bit))
-                ovf_list[atomicAdd(ovf_count, 1u)] = (uint32_t)line;
-        }
-        if (!ok || z >= g.nz) continue;
-        if constexpr (kX) {
-            emit_x(o, y, outer, z, nx, ny, g.nz, best >= kBig ? kInf : best);
+    const int q = threadIdx.x % (kTileZ / 4), row0 = threadIdx.x / (kTileZ / 4);
+    const int zl = g.nz - 1 - z0 - 4 * q;  // valid columns of this thread's int4: 0 .. zl
+    // pass X rows are padded to a multiple of 4 columns: a 16-byte load at
+    // a clamped column stays inside the row (columns >= nz are dropped)
+    const int c4 = kY ? 0 : min(4 * q, (int)(g.pitch - 4 - z0));
+    const int32_t *base = kY ? nullptr : in + (int64_t)outer * g.os + z0;
+    const int4 *sbase = kY ? summary + (int64_t)outer * g.ny * nzt + zt : nullptr;
+
+    int4 acc4 = make_int4(kBig, kBig, kBig, kBig);
+    int acc = kBig;
+    int r0 = chunk * kTilesPerBlock * T;
+    int ym = 0;
+    int4 ys = make_int4(0, 0, 0, 0);
+    int xm[T / 16];
+    int4 xr[T / 16];
+    {
+        // the first tile: all kRows rows
+        if constexpr (kY) {
+            issue_y<kRows / 4>(sbase, nzt, g, r0 - W, wv, lane, ym, ys);
+            commit_y<kRows / 4>(tile, g, 0, wv, lane, z0, ym, ys, acc);
         } else {
-            out[(int64_t)outer * g.os + (int64_t)y * g.rs + z] = best >= kBig ? kInf : best;
+            int fm[kRows / 16];
+            int4 fr[kRows / 16];
+            issue_x<kRows / 16>(base, g, r0 - W, row0, c4, fm, fr);
+            commit_x<kRows / 16>(tile, g, 0, row0, q, zl, fm, fr, acc4);
         }
+    }
+    for (int rt = chunk * kTilesPerBlock; rt < rt_end; ++rt, r0 += T) {
+#pragma unroll
+        for (int s = 16; s < kWave; s <<= 1) {
+            acc4.x = min(acc4.x, __shfl_xor(acc4.x, s, kWave));
+            acc4.y = min(acc4.y, __shfl_xor(acc4.y, s, kWave));
+            acc4.z = min(acc4.z, __shfl_xor(acc4.z, s, kWave));
+            acc4.w = min(acc4.w, __shfl_xor(acc4.w, s, kWave));
+        }
+        if (lane < 16) *reinterpret_cast<int4 *>(&part[0][wv][4 * lane]) = acc4;
+        part[1][wv][lane] = acc;
+        __syncthreads();
+        // every tile row of this column is >= cm: a scan can stop once cm + k^2 >= best
+        int cm = kBig;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) cm = min(cm, min(part[0][w][lane], part[1][w][lane]));
+        const bool more = rt + 1 < rt_end;
+        if (more) {
+            if constexpr (kY)
+                issue_y<T / 4>(sbase, nzt, g, r0 + T + W, wv, lane, ym, ys);
+            else
+                issue_x<T / 16>(base, g, r0 + T + W, row0, c4, xm, xr);
+        }
+
+        for (int j = wv; j < T; j += kWaves) {
+            const int y = r0 + j;
+            if (y >= g.n) break;
+            const int *c = col + (W + j) * kTileZ;
+            int best = c[0];
+            for (int k = 1; k <= W; k += 4) {
+                if (cm + k * k >= best) break;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int kk = k + u;
+                    best = min(best, min(c[-kk * kTileZ], c[kk * kTileZ]) + kk * kk);
+                }
+            }
+            // the tile is done; rows beyond it are at least W + 1 away
+            int k = W + 1;
+            bool ok = k * k >= best;
+            if (!ok && z < g.nz) {
+                // beyond the tile: scan the line in global memory, unless the
+                // line is already deferred
+                const uint32_t bit = 1u << (line & 31);
+                if (!(line_bits[line >> 5] & bit)) {
+                    const int kmax = g.bc_lo == PMX_BC_PERIODIC ? g.n / 2 : max(y + 1, g.n - y);
+                    for (; k <= kReach && k <= kmax; ++k) {
+                        if (k * k >= best) break;
+                        int a, b;
+                        if constexpr (kY) {
+                            a = fetch_y(summary, g, outer, z, y - k);
+                            b = fetch_y(summary, g, outer, z, y + k);
+                        } else {
+                            const int32_t *ln = in + (int64_t)outer * g.os + z;
+                            a = fetch_x(ln, g, y - k);
+                            b = fetch_x(ln, g, y + k);
+                        }
+                        best = min(best, min(a, b) + k * k);
+                    }
+                    ok = k * k >= best || k > kmax;
+                }
+                if (!ok && !(atomicOr(line_bits + (line >> 5), bit) & bit))
+                    ovf_list[atomicAdd(ovf_count, 1u)] = (uint32_t)line;
+            }
+            if (!ok || z >= g.nz) continue;
+            if constexpr (kY) {
+                out[(int64_t)outer * g.out_os + (int64_t)y * g.out_rs + z] = best >= kBig ? kInf : best;
+            } else {
+                emit_x(o, y, outer, z, nx, ny, g.nz, best >= kBig ? kInf : best);
+            }
+        }
+        if (!more) break;
+        // slide: rows T .. T+2W-1 become rows 0 .. 2W-1, the new rows follow
+        int4 h[kHalo];
+#pragma unroll
+        for (int i = 0; i < kHalo; ++i) h[i] = *reinterpret_cast<const int4 *>(tile + (T + row0 + 16 * i) * kTileZ + 4 * q);
+        __syncthreads();
+        acc4 = make_int4(kBig, kBig, kBig, kBig);
+        acc = kBig;
+#pragma unroll
+        for (int i = 0; i < kHalo; ++i) {
+            *reinterpret_cast<int4 *>(tile + (row0 + 16 * i) * kTileZ + 4 * q) = h[i];
+            acc4.x = min(acc4.x, h[i].x);
+            acc4.y = min(acc4.y, h[i].y);
+            acc4.z = min(acc4.z, h[i].z);
+            acc4.w = min(acc4.w, h[i].w);
+        }
+        if constexpr (kY)
+            commit_y<T / 4>(tile, g, 2 * W, wv, lane, z0, ym, ys, acc);
+        else
+            commit_x<T / 16>(tile, g, 2 * W, row0, q, zl, xm, xr, acc4);
     }
 }
 
@@ -526,9 +709,9 @@ __device__ __forceinline__ int finish(int v, int cap)
     return v >= kBig ? kInf : v;
 }
 
-__global__ __launch_bounds__(256) void k_fallback_y(const int32_t *__restrict__ in,
-                                                    int32_t *__restrict__ out, int nx, int ny,
-                                                    int nz, int bc_lo, int bc_hi, int cap,
+__global__ __launch_bounds__(256) void k_fallback_y(const int4 *__restrict__ summary, int32_t *__restrict__ out,
+                                                    int nx, int ny, int nz, int64_t out_sx, int64_t out_sy,
+                                                    int bc_lo, int bc_hi, int cap,
                                                     const uint32_t *__restrict__ ovf_count,
                                                     const uint32_t *__restrict__ ovf_list,
                                                     int *__restrict__ stk, int stk_cap)
@@ -536,22 +719,27 @@ __global__ __launch_bounds__(256) void k_fallback_y(const int32_t *__restrict__ 
     const uint32_t count = *ovf_count;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
+    const int nzt = (nz + kTileZ - 1) / kTileZ;
     Stack st{stk + tid, stk + (int64_t)stk_cap * nthreads + tid, nthreads, stk_cap,
              stk + (int64_t)2 * stk_cap * nthreads + tid};
     for (int64_t i = tid; i < (int64_t)count; i += nthreads) {
         const int64_t line = ovf_list[i];
         const int64_t x = line / nz, z = line - x * nz;
-        const int64_t base = x * (int64_t)ny * nz + z;
-        const int32_t *src = in + base;
-        int32_t *dst = out + base;
+        const int4 *rows = summary + x * ny * nzt;
+        int32_t *dst = out + x * out_sx + z;
         envelope_line(
-            ny, bc_lo, bc_hi, st, true, [&](int j) { return src[(int64_t)j * nz]; },
-            [&](int j, int v) { dst[(int64_t)j * nz] = finish(v, cap); });
+            ny, bc_lo, bc_hi, st, true,
+            [&](int j) {
+                const int d = zdist2(rows + (int64_t)j * nzt, (int)z);
+                return d < 0 ? kInf : d;
+            },
+            [&](int j, int v) { dst[(int64_t)j * out_sy] = finish(v, cap); });
     }
 }
 
 __global__ __launch_bounds__(256) void k_fallback_x(const int32_t *__restrict__ in, PassXOut o,
-                                                    int nx, int ny, int nz, int bc_lo, int bc_hi, int cap,
+                                                    int nx, int ny, int nz, int64_t in_sx, int64_t in_sy,
+                                                    int bc_lo, int bc_hi, int cap,
                                                     const uint32_t *__restrict__ ovf_count,
                                                     const uint32_t *__restrict__ ovf_list,
                                                     int *__restrict__ stk, int stk_cap)
@@ -559,15 +747,14 @@ __global__ __launch_bounds__(256) void k_fallback_x(const int32_t *__restrict__ 
     const uint32_t count = *ovf_count;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
-    const int64_t pstride = (int64_t)ny * nz;
     Stack st{stk + tid, stk + (int64_t)stk_cap * nthreads + tid, nthreads, stk_cap,
              stk + (int64_t)2 * stk_cap * nthreads + tid};
     for (int64_t i = tid; i < (int64_t)count; i += nthreads) {
         const int64_t line = ovf_list[i];
         const int64_t y = line / nz, z = line - y * nz;
-        const int32_t *src = in + line;
+        const int32_t *src = in + y * in_sy + z;
         envelope_line(
-            nx, bc_lo, bc_hi, st, true, [&](int j) { return src[(int64_t)j * pstride]; },
+            nx, bc_lo, bc_hi, st, true, [&](int j) { return src[(int64_t)j * in_sx]; },
             [&](int j, int v) { emit_x(o, j, y, z, nx, ny, nz, finish(v, cap)); });
     }
 }
@@ -579,17 +766,33 @@ __global__ __launch_bounds__(256) void k_clear_words(uint32_t *__restrict__ p, i
         p[i] = 0u;
 }
 
-constexpr int kWinT = 64, kWinW = 32;
+#ifndef PMX_WIN_T
+#define PMX_WIN_T 32  // rows per tile of the windowed passes
+#endif
+constexpr int kWinT = PMX_WIN_T;
 constexpr size_t kFallbackStackBytes = size_t(256) << 20;
+
+// launches one windowed pass with halo W
+template <int W, bool kY>
+void launch_window(const int4 *summary, const int32_t *in, int32_t *out, const PassXOut &o, const WinGeom &wg,
+                   int nx, int ny, uint32_t *bits, uint32_t *count, uint32_t *list, hipStream_t s)
+{
+    const int nzt = (wg.nz + kTileZ - 1) / kTileZ;
+    const int64_t blocks = (int64_t)wg.nouter * nzt * (((wg.n + kWinT - 1) / kWinT + kTilesPerBlock - 1) / kTilesPerBlock);
+    hipLaunchKernelGGL((k_window<kWinT, W, kY>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, summary, in, out, o,
+                       wg, nx, ny, bits, count, list);
+}
 
 }  // namespace
 
 size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p)
 {
     Carver c(ws, SIZE_MAX);
-    const size_t vox = (size_t)(g->n[0] * g->n[1] * g->n[2]);
-    p->a = c.take<int32_t>(vox);
-    p->b = c.take<int32_t>(vox);
+    const int64_t nzt = (g->n[2] + kTileZ - 1) / kTileZ;
+    p->summary = c.take<int4>((size_t)(g->n[0] * g->n[1] * nzt));
+    // pass-Y output, rows padded to a multiple of 4 voxels (16-byte aligned)
+    p->pitch = (g->n[2] + 3) / 4 * 4;
+    p->b = c.take<int32_t>((size_t)(g->n[0] * g->n[1] * p->pitch));
     p->counters = c.take<uint32_t>(4);
     const size_t ly = (size_t)(g->n[0] * g->n[2]), lx = (size_t)(g->n[1] * g->n[2]);
     p->n_bits = (ly + 31) / 32 + (lx + 31) / 32;
@@ -622,49 +825,54 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     const int64_t rows = (int64_t)nx * ny;
     hipLaunchKernelGGL(k_clear_words, dim3((unsigned)std::min<int64_t>((p.n_bits + 255) / 256 + 1, 4096)),
                        dim3(256), 0, s, p.bits_y, (int64_t)p.n_bits);
+    {
+    ProfScope ps("edt_rows", s);
     const dim3 bz(256);
     const dim3 gz((unsigned)((rows + 3) / 4));
-    {
-    ProfScope ps("edt_pass_z", s);
     const int vec_in = ((uintptr_t)img % 16 == 0 && g->in_s[0] % 16 == 0 && g->in_s[1] % 16 == 0) ? 1 : 0;
-    const int vec_out = ((uintptr_t)p.a % 16 == 0 && nz % 4 == 0) ? 1 : 0;
     if (invert)
-        hipLaunchKernelGGL(k_pass_z<true>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, vec_out, p.a, p.counters);
+        hipLaunchKernelGGL(k_row_summary<true>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
+                           g->bc[4], g->bc[5], vec_in, p.summary, p.counters);
     else
-        hipLaunchKernelGGL(k_pass_z<false>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, vec_out, p.a, p.counters);
+        hipLaunchKernelGGL(k_row_summary<false>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
+                           g->bc[4], g->bc[5], vec_in, p.summary, p.counters);
     }
-    if (int e = check_launch("pass z")) return e;
+    if (int e = check_launch("row summaries")) return e;
 
-    const int vec = ((uintptr_t)p.a % 16 == 0 && (uintptr_t)p.b % 16 == 0 && nz % 4 == 0) ? 1 : 0;
-    const int nzt = (nz + kTileZ - 1) / kTileZ;
+    // pass-Y output stored [y][x][z]: pass X then reads rows 4*pitch bytes apart
+    const int64_t pitch = p.pitch;
+    const int64_t bsx = pitch, bsy = (int64_t)nx * pitch;
+    // halo: 16 rows settle nearly every voxel of a porous medium; a clamped
+    // EDT whose threshold reaches past 17^2 scans up to 32 rows in LDS
+    const bool wide = ccap < kBig && ccap > 17 * 17;
     {
     ProfScope ps("edt_pass_y", s);
-    const WinGeom wg{ny, nz, (int64_t)ny * nz, nx, nz, g->bc[2], g->bc[3], ccap, vec};
-    const int64_t tiles = (int64_t)nx * nzt * ((ny + kWinT - 1) / kWinT);
-    hipLaunchKernelGGL((k_window<kWinT, kWinW, false>), dim3((unsigned)tiles), dim3(kWinThreads), 0, s, p.a,
-                       p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list);
+    WinGeom wg{ny, 0, 0, nx, nz, g->bc[2], g->bc[3], ccap, 0, bsx, bsy, ny};
+    if (wide)
+        launch_window<32, true>(p.summary, nullptr, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, s);
+    else
+        launch_window<16, true>(p.summary, nullptr, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, s);
     }
     if (int e = check_launch("pass y")) return e;
     {
     ProfScope ps("edt_fallback_y", s);
-    hipLaunchKernelGGL(k_fallback_y, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.a, p.b, nx, ny, nz,
-                       g->bc[2], g->bc[3], ccap, p.counters + 0, p.ovf_list, p.stk, p.cap);
+    hipLaunchKernelGGL(k_fallback_y, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.summary, p.b, nx, ny, nz,
+                       bsx, bsy, g->bc[2], g->bc[3], ccap, p.counters + 0, p.ovf_list, p.stk, p.cap);
     }
     if (int e = check_launch("pass y fallback")) return e;
     {
     ProfScope ps("edt_pass_x", s);
-    const WinGeom wg{nx, (int64_t)ny * nz, nz, ny, nz, g->bc[0], g->bc[1], ccap, vec};
-    const int64_t tiles = (int64_t)ny * nzt * ((nx + kWinT - 1) / kWinT);
-    hipLaunchKernelGGL((k_window<kWinT, kWinW, true>), dim3((unsigned)tiles), dim3(kWinThreads), 0, s, p.b,
-                       nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1, p.ovf_list);
+    WinGeom wg{nx, bsx, bsy, ny, nz, g->bc[0], g->bc[1], ccap, pitch, 0, 0, ny};
+    if (wide)
+        launch_window<32, false>(nullptr, p.b, nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1, p.ovf_list, s);
+    else
+        launch_window<16, false>(nullptr, p.b, nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1, p.ovf_list, s);
     }
     if (int e = check_launch("pass x")) return e;
     {
     ProfScope ps("edt_fallback_x", s);
     hipLaunchKernelGGL(k_fallback_x, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.b, o, nx, ny, nz,
-                       g->bc[0], g->bc[1], ccap, p.counters + 1, p.ovf_list, p.stk, p.cap);
+                       bsx, bsy, g->bc[0], g->bc[1], ccap, p.counters + 1, p.ovf_list, p.stk, p.cap);
     }
     return check_launch("pass x fallback");
 }

@@ -24,7 +24,9 @@ struct PassXOut {
 };
 
 struct EdtPlan {
-    int32_t *a = nullptr, *b = nullptr;
+    int4 *summary = nullptr;  // per z-row, per 64-voxel chunk: nearest solids and bitmask
+    int32_t *b = nullptr;     // pass-Y output, stored [y][x][z] with row pitch `pitch` voxels
+    int64_t pitch = 0;
     uint32_t *counters = nullptr, *ovf_list = nullptr;
     uint32_t *bits_y = nullptr, *bits_x = nullptr;  // deferred-line bitmaps (one allocation)
     size_t n_bits = 0;                              // words in both bitmaps

@@ -57,27 +57,43 @@ struct Dims {
 
 __device__ __forceinline__ int64_t n_of(const Dims &d) { return (int64_t)d.n0 * d.n1 * d.n2; }
 
+constexpr int kMaxLevels = 250;
+constexpr int kBucketItems = 16;  // voxels per thread between bucket reservations
+
 // level[v] = first i with d2 > te[i] (the erosion at radius i keeps v),
-// or n_levels if none; counts[level]++
+// or n_levels if none (te does not increase, so a binary search finds
+// it); counts[level] += per-block totals from an LDS histogram
 __global__ __launch_bounds__(256) void k_levels(const int32_t *__restrict__ d2, int64_t n, const int64_t *__restrict__ te,
                                                 int n_levels, uint8_t *__restrict__ level,
                                                 unsigned int *__restrict__ counts, int *__restrict__ parent,
                                                 uint8_t *__restrict__ flag)
 {
+    __shared__ int64_t ste[kMaxLevels];
+    __shared__ unsigned int hist[kMaxLevels];
+    for (int k = threadIdx.x; k < n_levels; k += blockDim.x) {
+        ste[k] = te[k];
+        hist[k] = 0;
+    }
+    __syncthreads();
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t v = d2[i];
-        int l = n_levels;
-        for (int k = 0; k < n_levels; ++k)
-            if (v > te[k]) {
-                l = k;
-                break;
-            }
-        level[i] = (uint8_t)l;
+        int lo = 0, hi = n_levels;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (v > ste[mid])
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        level[i] = (uint8_t)lo;
         parent[i] = kInactive;
         flag[i] = 0;
-        if (l < n_levels) atomicAdd(counts + l, 1u);
+        if (lo < n_levels) atomicAdd(hist + lo, 1u);
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < n_levels; k += blockDim.x)
+        if (hist[k]) atomicAdd(counts + k, hist[k]);
 }
 
 // exclusive scan of counts (n_levels + 1 entries) into offsets
@@ -93,13 +109,33 @@ __global__ void k_level_offsets(const unsigned int *counts, int n_levels, unsign
     }
 }
 
+// order[] = voxel indices grouped by level.  A block ranks a chunk of
+// 256 * kBucketItems voxels per level in LDS, then reserves each level's
+// range with one global atomic.
 __global__ __launch_bounds__(256) void k_bucket(const uint8_t *__restrict__ level, int64_t n, int n_levels,
                                                 unsigned int *__restrict__ cursor, int *__restrict__ order)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int l = level[i];
-        if (l < n_levels) order[atomicAdd(cursor + l, 1u)] = (int)i;
+    __shared__ unsigned int cnt[kMaxLevels], base[kMaxLevels];
+    const int64_t chunk = (int64_t)blockDim.x * kBucketItems;
+    for (int64_t c0 = (int64_t)blockIdx.x * chunk; c0 < n; c0 += (int64_t)gridDim.x * chunk) {
+        for (int k = threadIdx.x; k < n_levels; k += blockDim.x) cnt[k] = 0;
+        __syncthreads();
+        int lv[kBucketItems];
+        unsigned int rk[kBucketItems];
+#pragma unroll
+        for (int j = 0; j < kBucketItems; ++j) {
+            const int64_t i = c0 + (int64_t)j * blockDim.x + threadIdx.x;
+            lv[j] = i < n ? level[i] : n_levels;
+            rk[j] = lv[j] < n_levels ? atomicAdd(cnt + lv[j], 1u) : 0u;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < n_levels; k += blockDim.x)
+            if (cnt[k]) base[k] = atomicAdd(cursor + k, cnt[k]);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kBucketItems; ++j)
+            if (lv[j] < n_levels) order[base[lv[j]] + rk[j]] = (int)(c0 + (int64_t)j * blockDim.x + threadIdx.x);
+        __syncthreads();
     }
 }
 
@@ -202,8 +238,6 @@ struct PsdPlan {
     size_t edt_bytes = 0;
 };
 
-constexpr int kMaxLevels = 250;
-
 size_t plan_psd(const pmx_geom *g, void *ws, PsdPlan *p)
 {
     Carver c(ws, SIZE_MAX);
@@ -299,7 +333,8 @@ int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo
         hipLaunchKernelGGL(k_levels, dim3(gr), dim3(256), 0, s, p.d2, n, p.te, n_levels, p.level, p.counts,
                            p.parent, p.flag);
         hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, s, p.counts, n_levels, p.offsets, p.cursor);
-        hipLaunchKernelGGL(k_bucket, dim3(gr), dim3(256), 0, s, p.level, n, n_levels, p.cursor, p.order);
+        hipLaunchKernelGGL(k_bucket, dim3(grid_for((n + kBucketItems - 1) / kBucketItems)), dim3(256), 0, s, p.level,
+                           n, n_levels, p.cursor, p.order);
         hipLaunchKernelGGL(k_zero_map, dim3(gr), dim3(256), 0, s, code_map, d, g->out_s[0], g->out_s[1]);
     }
     if (int e = check_launch("psd setup")) return e;

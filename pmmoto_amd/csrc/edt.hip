@@ -4,16 +4,14 @@
 // driven by _distance.pyx:86-127,308-353 from
 // euclidean_distance_transform.py:56-101,208-303) with three kernels:
 //
-//   rows    one wavefront per z-row compresses the image row into 16-byte
-//           summaries, one per 64-voxel chunk: the chunk's solid bitmask
-//           and the nearest solid before and after the chunk (boundary
-//           rules included).  The 1-D distance along z of any voxel follows
-//           from its chunk's summary with a few bit operations.
-//   pass Y  windowed minimum of parabolas along y: a workgroup builds the
-//           1-D z distances of a tile of rows x 64 z-columns (plus halo
-//           rows) in LDS from the summaries, then scans each voxel outward
-//           until k^2 >= its best value, so the work per voxel is about its
-//           distance to the nearest solid.  Output stored [y][x][z].
+//   pass Z  one wavefront per z-row: 1-D distance to the nearest solid
+//           from per-lane 16-voxel solid masks and wave scans; stored as
+//           uint16 (distances are at most 16384).
+//   pass Y  windowed minimum of parabolas along y: a workgroup stages a
+//           tile of rows x 64 z-columns (plus halo rows) in LDS and scans
+//           each voxel outward until k^2 >= its best value, so the work per
+//           voxel is about its distance to the nearest solid.  Output
+//           stored [y][x][z].
 //   pass X  the same along x over the pass-Y array, writing the requested
 //           outputs (int32 d2, float32 / float64 sqrt, or a u8 threshold).
 //
@@ -44,7 +42,7 @@ constexpr int kInf = PMX_D2_INF;
 constexpr int kNone = 1 << 29;  // "no solid on this side" position marker
 
 // ---------------------------------------------------------------------------
-// row summaries
+// pass Z
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ int wave_incl_max(int v, int lane)
@@ -121,24 +119,26 @@ __device__ __forceinline__ uint32_t solid_mask16(const uint8_t *src, int e0, int
     return m;
 }
 
-// Summary of chunk c (voxels 64c .. 64c+63) of a row: {P, N, lo, hi} with
-// P the last solid before the chunk, N the first solid after it (kNone
-// magnitudes where there is none; WALL = a solid at -1 / nz, PERIODIC =
-// the row's own solids one period away) and lo | hi << 32 the chunk's
-// solid bitmask (bits past nz are 0).
+constexpr uint16_t kNoSolid = 0xffff;  // pass-Z output where no solid is reachable
+
+// Pass Z: 1-D distance along z (not squared; kNoSolid where none), as
+// uint16 -- exact, since a distance is at most 16384.
 //
 // One wavefront per row; lane l owns bytes 16l .. 16l+15 of each 1024-byte
-// super-chunk, four lanes per chunk.  Sweep 1 finds each super-chunk's
-// last / first solid, sweep 2 writes the chunk summaries with wave scans
-// for the carries.
+// super-chunk: one 16-byte load (its solid mask is kept in LDS for sweep
+// 2) and two 16-byte stores per lane when the row is aligned.  A 16-bit solid mask per lane gives the in-lane nearest
+// solids; wave prefix-max / suffix-min scans over the lanes' last / first
+// solid give the carries from other lanes, and sweep 1's per-super-chunk
+// summaries the carries from other super-chunks.
 template <bool kInvert>
-__global__ __launch_bounds__(256) void k_row_summary(const uint8_t *__restrict__ img, int64_t in_sx,
-                                                     int64_t in_sy, int ny, int nz, int64_t rows,
-                                                     int bc_lo, int bc_hi, int vec_in,
-                                                     int4 *__restrict__ summary,
-                                                     uint32_t *__restrict__ counters)
+__global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img, int64_t in_sx,
+                                                int64_t in_sy, int ny, int nz, int64_t rows,
+                                                int bc_lo, int bc_hi, int vec_in, int64_t pitch,
+                                                uint16_t *__restrict__ out,
+                                                uint32_t *__restrict__ counters)
 {
-    __shared__ int sc[4][2][kSup];  // per wave: each super-chunk's last / first solid
+    __shared__ int sc[4][2][kSup];           // per wave: each super-chunk's last / first solid
+    __shared__ uint32_t smask[4][kSup][kWave];  // per wave: each lane's masks (the row is read once)
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
     const int64_t row = (int64_t)blockIdx.x * (blockDim.x / kWave) + w;
@@ -146,12 +146,15 @@ __global__ __launch_bounds__(256) void k_row_summary(const uint8_t *__restrict__
     if (row >= rows) return;
     const int64_t x = row / ny, y = row - x * ny;
     const uint8_t *src = img + x * in_sx + y * in_sy;
-    const int nsup = (nz + 1023) / 1024, nzt = (nz + 63) / 64;
+    uint16_t *dst = out + row * pitch;
+    const int nsup = (nz + 1023) / 1024;
     const bool vin = vec_in != 0;
 
+    // sweep 1: each super-chunk's solid masks, last and first solid
     for (int s = 0; s < nsup; ++s) {
         const int e0 = s * 1024 + lane * 16;
         const uint32_t m = solid_mask16<kInvert>(src, e0, nz, vin);
+        smask[w][s][lane] = m;
         int l = m ? e0 + 31 - __clz(m) : -kNone;
         int f = m ? e0 + __ffs(m) - 1 : kNone;
 #pragma unroll
@@ -179,43 +182,49 @@ __global__ __launch_bounds__(256) void k_row_summary(const uint8_t *__restrict__
         prev0 = bc_lo == PMX_BC_WALL ? -1 : -kNone;
         next0 = bc_hi == PMX_BC_WALL ? nz : kNone;
     }
-    int4 *dst = summary + row * nzt;
+    // sweep 2: distances, super-chunk by super-chunk
     int pcar = prev0;
     for (int s = 0; s < nsup; ++s) {
         int ncar = next0;
         for (int t = nsup - 1; t > s; --t) ncar = min(ncar, sc[w][1][t]);
         const int e0 = s * 1024 + lane * 16;
-        const uint32_t m = solid_mask16<kInvert>(src, e0, nz, vin);
+        const uint32_t m = smask[w][s][lane];
         const int l = m ? e0 + 31 - __clz(m) : -kNone;
         const int f = m ? e0 + __ffs(m) - 1 : kNone;
         const int inc_l = wave_incl_max(l, lane);
-        int before = __shfl_up(inc_l, 1, kWave);  // last solid of lanes < this one
-        if (lane == 0) before = -kNone;
+        int pc = __shfl_up(inc_l, 1, kWave);
+        if (lane == 0) pc = -kNone;
+        pc = max(pc, pcar);
         const int inc_f = wave_incl_min_rev(f, lane);
-        int after = __shfl_down(inc_f, 4, kWave);  // first solid of lanes >= this one + 4
-        if (lane + 4 >= kWave) after = kNone;
-        const uint32_t m1 = __shfl_down(m, 1, kWave), m2 = __shfl_down(m, 2, kWave),
-                       m3 = __shfl_down(m, 3, kWave);
-        const int chunk = s * 16 + lane / 4;
-        if ((lane & 3) == 0 && chunk < nzt)
-            dst[chunk] = make_int4(max(before, pcar), min(after, ncar), (int)(m | (m1 << 16)), (int)(m2 | (m3 << 16)));
+        int nc = __shfl_down(inc_f, 1, kWave);
+        if (lane == kWave - 1) nc = kNone;
+        nc = min(nc, ncar);
+        uint32_t dv[8];  // two distances per word
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const uint32_t below = m & ((2u << b) - 1u);
+            const uint32_t above = m & ~((1u << b) - 1u);
+            const int e = e0 + b;
+            const int prev = below ? e0 + 31 - __clz(below) : pc;
+            const int next = above ? e0 + __ffs(above) - 1 : nc;
+            const int d = min(e - prev, next - e);
+            const uint32_t v = d >= kNone / 2 ? kNoSolid : (uint32_t)d;
+            if (b & 1)
+                dv[b >> 1] |= v << 16;
+            else
+                dv[b >> 1] = v;
+        }
+        if (e0 + 16 <= nz) {
+            uint4 *q = reinterpret_cast<uint4 *>(dst + e0);  // rows are 16-byte aligned
+            q[0] = make_uint4(dv[0], dv[1], dv[2], dv[3]);
+            q[1] = make_uint4(dv[4], dv[5], dv[6], dv[7]);
+        } else {
+#pragma unroll
+            for (int b = 0; b < 16; ++b)
+                if (e0 + b < nz) dst[e0 + b] = (uint16_t)(dv[b >> 1] >> (16 * (b & 1)));
+        }
         pcar = max(pcar, sc[w][0][s]);
     }
-}
-
-// squared 1-D distance along z at column z of a row from its summary
-// array; -1 when the row has no reachable solid
-__device__ __forceinline__ int zdist2(const int4 *__restrict__ row_summary, int z)
-{
-    const int4 s = row_summary[z >> 6];
-    const uint64_t m = ((uint64_t)(uint32_t)s.w << 32) | (uint32_t)s.z;
-    const int bit = z & 63, z64 = z - bit;
-    const uint64_t below = m & (bit == 63 ? ~0ull : ((2ull << bit) - 1ull));
-    const uint64_t above = m & (~0ull << bit);
-    const int prev = below ? z64 + 63 - __clzll(below) : s.x;
-    const int next = above ? z64 + __ffsll((unsigned long long)above) - 1 : s.y;
-    const int d = min(z - prev, next - z);
-    return d >= kNone / 2 ? -1 : d * d;
 }
 
 // ---------------------------------------------------------------------------
@@ -441,65 +450,78 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb)
 
 struct WinGeom {
     int n;          // rows per line (the pass axis)
-    int64_t rs;     // input element stride between rows (pass X)
-    int64_t os;     // input element stride between outer indices (pass X)
+    int64_t rs;     // input element stride between rows
+    int64_t os;     // input element stride between outer indices (x in pass Y, y in pass X)
     int nouter, nz;
     int bc_lo, bc_hi;
     int cap;        // values are clamped to [0, cap], cap <= kBig
-    int64_t pitch;  // input row length in memory: a multiple of 4 >= nz (pass X)
+    int64_t pitch;  // input row length in memory: a multiple of 8 >= nz, 16-byte aligned rows
     int64_t out_os, out_rs;  // output strides of the outer index and of a row (pass Y)
-    int ny;         // rows per x of the summary array (pass Y)
 };
 
-// pass-Y input at row r of the line (outer, z): the clamped z distance
-__device__ __forceinline__ int fetch_y(const int4 *__restrict__ summary, const WinGeom &g, int outer, int z,
-                                       int r)
+// Input element -> clamped squared distance: pass Y reads pass Z's uint16
+// distances (kNoSolid: none), pass X the int32 squared distances of pass Y.
+__device__ __forceinline__ int decode(uint16_t d, int cap)
 {
-    const int m = map_row(r, g.n, g.bc_lo, g.bc_hi);
-    if (m < 0) return m == kRowWall ? 0 : g.cap;
-    const int nzt = (g.nz + kTileZ - 1) / kTileZ;
-    const int d = zdist2(summary + ((int64_t)outer * g.ny + m) * nzt, z);
-    return d < 0 ? g.cap : min(d, g.cap);
+    return d == kNoSolid ? cap : min((int)d * (int)d, cap);
 }
+__device__ __forceinline__ int decode(int32_t v, int cap) { return min(v, cap); }
 
-// pass-X input at row r of the line starting at `line`
-__device__ __forceinline__ int fetch_x(const int32_t *line, const WinGeom &g, int r)
+// input at row r of the line starting at `line`, under the boundary rule
+template <class E>
+__device__ __forceinline__ int fetch(const E *line, const WinGeom &g, int r)
 {
     const int m = map_row(r, g.n, g.bc_lo, g.bc_hi);
-    if (m >= 0) return min(line[(int64_t)m * g.rs], g.cap);
+    if (m >= 0) return decode(line[(int64_t)m * g.rs], g.cap);
     return m == kRowWall ? 0 : g.cap;
 }
 
-// Staging of tile rows.  Pass X: thread (q, row0) = (t % 16, t / 16)
-// owns columns 4q .. 4q+3 of rows row0 + 16 i, loaded as 16-byte vectors.
-// Pass Y: lane i of wave w holds the row summary of row w + 4 i, and the
-// wave rebuilds that row's z distances (one column per lane).  Loads are
-// issued (addresses clamped into the line) well before their commit.
-template <int N>
-__device__ __forceinline__ void issue_x(const int32_t *__restrict__ base, const WinGeom &g, int first, int row0,
-                                        int c4, int (&mrow)[N], int4 (&raw)[N])
+// Staging of tile rows: thread (q, row0) = (t % 16, t / 16) owns columns
+// 4q .. 4q+3 of rows row0 + 16 i, loaded as one 8-byte (uint16 input) or
+// 16-byte (int32 input) vector.  Loads are issued (addresses clamped into
+// the line) well before their commit.
+template <class E> struct Quad;
+template <> struct Quad<uint16_t> { using type = uint2; };
+template <> struct Quad<int32_t> { using type = int4; };
+
+template <int N, class E>
+__device__ __forceinline__ void issue_rows(const E *__restrict__ base, const WinGeom &g, int first, int row0,
+                                           int c4, int (&mrow)[N], typename Quad<E>::type (&raw)[N])
 {
+    using V = typename Quad<E>::type;
 #pragma unroll
     for (int i = 0; i < N; ++i) mrow[i] = map_row(first + row0 + 16 * i, g.n, g.bc_lo, g.bc_hi);
 #pragma unroll
-    for (int i = 0; i < N; ++i) raw[i] = *reinterpret_cast<const int4 *>(base + (int64_t)max(mrow[i], 0) * g.rs + c4);
+    for (int i = 0; i < N; ++i) raw[i] = *reinterpret_cast<const V *>(base + (int64_t)max(mrow[i], 0) * g.rs + c4);
 }
 
-// clamp at cap (columns past nz hold 0, settled at once) and store at
-// tile rows dst + row0 + 16 i; acc keeps the per-column minimum
-template <int N>
-__device__ __forceinline__ void commit_x(int *tile, const WinGeom &g, int dst, int row0, int q, int zl,
-                                         const int (&mrow)[N], const int4 (&raw)[N], int4 &acc)
+__device__ __forceinline__ int4 unpack(uint2 r, int cap)
+{
+    return make_int4(decode((uint16_t)(r.x & 0xffffu), cap), decode((uint16_t)(r.x >> 16), cap),
+                     decode((uint16_t)(r.y & 0xffffu), cap), decode((uint16_t)(r.y >> 16), cap));
+}
+__device__ __forceinline__ int4 unpack(int4 r, int cap)
+{
+    return make_int4(min(r.x, cap), min(r.y, cap), min(r.z, cap), min(r.w, cap));
+}
+
+// clamp (columns past nz hold 0, settled at once; rows beyond the line the
+// boundary value) and store at tile rows dst + row0 + 16 i; acc keeps the
+// per-column minimum
+template <int N, class V>
+__device__ __forceinline__ void commit_rows(int *tile, const WinGeom &g, int dst, int row0, int q, int zl,
+                                            const int (&mrow)[N], const V (&raw)[N], int4 &acc)
 {
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const bool in_line = mrow[i] >= 0;
         const int c = mrow[i] == kRowWall ? 0 : g.cap;
+        const int4 u = unpack(raw[i], g.cap);
         int4 v;
-        v.x = in_line ? (zl >= 0 ? min(raw[i].x, g.cap) : 0) : c;
-        v.y = in_line ? (zl >= 1 ? min(raw[i].y, g.cap) : 0) : c;
-        v.z = in_line ? (zl >= 2 ? min(raw[i].z, g.cap) : 0) : c;
-        v.w = in_line ? (zl >= 3 ? min(raw[i].w, g.cap) : 0) : c;
+        v.x = in_line ? (zl >= 0 ? u.x : 0) : c;
+        v.y = in_line ? (zl >= 1 ? u.y : 0) : c;
+        v.z = in_line ? (zl >= 2 ? u.z : 0) : c;
+        v.w = in_line ? (zl >= 3 ? u.w : 0) : c;
         acc.x = min(acc.x, v.x);
         acc.y = min(acc.y, v.y);
         acc.z = min(acc.z, v.z);
@@ -508,63 +530,28 @@ __device__ __forceinline__ void commit_x(int *tile, const WinGeom &g, int dst, i
     }
 }
 
-template <int N>
-__device__ __forceinline__ void issue_y(const int4 *__restrict__ sbase, int nzt, const WinGeom &g, int first,
-                                        int wv, int lane, int &m, int4 &sm)
-{
-    static_assert(N <= kWave, "one row summary per lane");
-    m = map_row(first + wv + 4 * lane, g.n, g.bc_lo, g.bc_hi);
-    if (lane < N) sm = sbase[(int64_t)max(m, 0) * nzt];
-}
-
-// z distances of tile rows dst + wv + 4 i from the summaries held by lane i
-template <int N>
-__device__ __forceinline__ void commit_y(int *tile, const WinGeom &g, int dst, int wv, int lane, int z0, int m,
-                                         int4 sm, int &acc)
-{
-    const int z = z0 + lane;
-    const uint64_t lo_mask = lane == 63 ? ~0ull : ((2ull << lane) - 1ull), hi_mask = ~0ull << lane;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        const int mi = __builtin_amdgcn_readlane(m, i);
-        const int px = __builtin_amdgcn_readlane(sm.x, i), nx_ = __builtin_amdgcn_readlane(sm.y, i);
-        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(sm.z, i), hi = (uint32_t)__builtin_amdgcn_readlane(sm.w, i);
-        const uint64_t msk = ((uint64_t)hi << 32) | lo;
-        const uint64_t below = msk & lo_mask, above = msk & hi_mask;
-        const int prev = below ? z0 + 63 - __clzll(below) : px;
-        const int next = above ? z0 + __ffsll((unsigned long long)above) - 1 : nx_;
-        const int d = min(z - prev, next - z);
-        int f = d >= kNone / 2 ? g.cap : min(d * d, g.cap);
-        f = mi >= 0 ? f : (mi == kRowWall ? 0 : g.cap);
-        f = z < g.nz ? f : 0;
-        tile[(dst + wv + 4 * i) * kTileZ + lane] = f;
-        acc = min(acc, f);
-    }
-}
-
-// kY: pass Y (input: row summaries, output: the int32 array `out`);
-// otherwise pass X (input: that array, output: PassXOut).
+// E = uint16_t: pass Y (input: pass Z's distances, output: the int32 array
+// `out`); E = int32_t: pass X (input: that array, output: PassXOut).
 //
 // The tile is a sliding window: after tile i is scanned its last 2W rows
 // move to the top and only the T new rows are loaded (issued while tile i
-// is scanned), so each row is loaded and built once per block.
-template <int T, int W, bool kY>
-__global__ __launch_bounds__(kWinThreads) void k_window(const int4 *__restrict__ summary,
-                                                        const int32_t *__restrict__ in,
-                                                        int32_t *__restrict__ out, PassXOut o,
-                                                        WinGeom g, int nx, int ny,
+// is scanned), so each row is loaded once per block.
+template <int T, int W, class E>
+__global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in, int32_t *__restrict__ out,
+                                                        PassXOut o, WinGeom g, int nx, int ny,
                                                         uint32_t *__restrict__ line_bits,
                                                         uint32_t *__restrict__ ovf_count,
                                                         uint32_t *__restrict__ ovf_list)
 {
+    constexpr bool kY = sizeof(E) == 2;
+    using V = typename Quad<E>::type;
     static_assert(W % 8 == 0 && T % 16 == 0, "tile layout");
     constexpr int kRows = T + 2 * W;
     constexpr int kWaves = kWinThreads / kWave;
-    static_assert(kWaves == 4 && kWinThreads == 256, "four waves, 16 threads per row of int4");
-    static_assert(kRows / 4 <= kWave, "one summary per lane");
-    constexpr int kHalo = 2 * W / 16;  // int4 rows per thread of the halo move
+    static_assert(kWaves == 4 && kWinThreads == 256, "four waves, 16 threads per row of four columns");
+    constexpr int kHalo = 2 * W / 16;  // rows of four columns per thread in the halo move
     __shared__ __align__(16) int tile[kRows * kTileZ];
-    __shared__ __align__(16) int part[2][kWaves][kTileZ];
+    __shared__ __align__(16) int part[kWaves][kTileZ];
 
     const int nzt = (g.nz + kTileZ - 1) / kTileZ, nrt = (g.n + T - 1) / T;
     const int nchunk = (nrt + kTilesPerBlock - 1) / kTilesPerBlock;
@@ -577,59 +564,47 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const int4 *__restrict__
     const int rt_end = min(nrt, (chunk + 1) * kTilesPerBlock);
     const int z0 = zt * kTileZ;
 
-    const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    // the wave index is uniform: keep it (and every row index derived from
+    // it) in scalar registers, so per-row address arithmetic runs on the
+    // scalar unit
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
     const int z = z0 + lane;
     const int *col = tile + lane;
     const int64_t line = (int64_t)outer * g.nz + z;
     const int q = threadIdx.x % (kTileZ / 4), row0 = threadIdx.x / (kTileZ / 4);
-    const int zl = g.nz - 1 - z0 - 4 * q;  // valid columns of this thread's int4: 0 .. zl
-    // pass X rows are padded to a multiple of 4 columns: a 16-byte load at
-    // a clamped column stays inside the row (columns >= nz are dropped)
-    const int c4 = kY ? 0 : min(4 * q, (int)(g.pitch - 4 - z0));
-    const int32_t *base = kY ? nullptr : in + (int64_t)outer * g.os + z0;
-    const int4 *sbase = kY ? summary + (int64_t)outer * g.ny * nzt + zt : nullptr;
+    const int zl = g.nz - 1 - z0 - 4 * q;  // valid columns of this thread's quad: 0 .. zl
+    // rows are padded to a multiple of 8 columns: a quad load at a clamped
+    // column stays inside the row (columns >= nz are dropped)
+    const int c4 = min(4 * q, (int)(g.pitch - 4 - z0));
+    const E *base = in + (int64_t)outer * g.os + z0;
 
-    int4 acc4 = make_int4(kBig, kBig, kBig, kBig);
-    int acc = kBig;
+    int4 acc = make_int4(kBig, kBig, kBig, kBig);
     int r0 = chunk * kTilesPerBlock * T;
-    int ym = 0;
-    int4 ys = make_int4(0, 0, 0, 0);
     int xm[T / 16];
-    int4 xr[T / 16];
+    V xr[T / 16];
     {
         // the first tile: all kRows rows
-        if constexpr (kY) {
-            issue_y<kRows / 4>(sbase, nzt, g, r0 - W, wv, lane, ym, ys);
-            commit_y<kRows / 4>(tile, g, 0, wv, lane, z0, ym, ys, acc);
-        } else {
-            int fm[kRows / 16];
-            int4 fr[kRows / 16];
-            issue_x<kRows / 16>(base, g, r0 - W, row0, c4, fm, fr);
-            commit_x<kRows / 16>(tile, g, 0, row0, q, zl, fm, fr, acc4);
-        }
+        int fm[kRows / 16];
+        V fr[kRows / 16];
+        issue_rows<kRows / 16>(base, g, r0 - W, row0, c4, fm, fr);
+        commit_rows<kRows / 16>(tile, g, 0, row0, q, zl, fm, fr, acc);
     }
     for (int rt = chunk * kTilesPerBlock; rt < rt_end; ++rt, r0 += T) {
 #pragma unroll
         for (int s = 16; s < kWave; s <<= 1) {
-            acc4.x = min(acc4.x, __shfl_xor(acc4.x, s, kWave));
-            acc4.y = min(acc4.y, __shfl_xor(acc4.y, s, kWave));
-            acc4.z = min(acc4.z, __shfl_xor(acc4.z, s, kWave));
-            acc4.w = min(acc4.w, __shfl_xor(acc4.w, s, kWave));
+            acc.x = min(acc.x, __shfl_xor(acc.x, s, kWave));
+            acc.y = min(acc.y, __shfl_xor(acc.y, s, kWave));
+            acc.z = min(acc.z, __shfl_xor(acc.z, s, kWave));
+            acc.w = min(acc.w, __shfl_xor(acc.w, s, kWave));
         }
-        if (lane < 16) *reinterpret_cast<int4 *>(&part[0][wv][4 * lane]) = acc4;
-        part[1][wv][lane] = acc;
+        if (lane < 16) *reinterpret_cast<int4 *>(&part[wv][4 * lane]) = acc;
         __syncthreads();
         // every tile row of this column is >= cm: a scan can stop once cm + k^2 >= best
-        int cm = kBig;
+        int cm = part[0][lane];
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) cm = min(cm, min(part[0][w][lane], part[1][w][lane]));
+        for (int w = 1; w < kWaves; ++w) cm = min(cm, part[w][lane]);
         const bool more = rt + 1 < rt_end;
-        if (more) {
-            if constexpr (kY)
-                issue_y<T / 4>(sbase, nzt, g, r0 + T + W, wv, lane, ym, ys);
-            else
-                issue_x<T / 16>(base, g, r0 + T + W, row0, c4, xm, xr);
-        }
+        if (more) issue_rows<T / 16>(base, g, r0 + T + W, row0, c4, xm, xr);
 
         for (int j = wv; j < T; j += kWaves) {
             const int y = r0 + j;
@@ -652,19 +627,11 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const int4 *__restrict__
                 // line is already deferred
                 const uint32_t bit = 1u << (line & 31);
                 if (!(line_bits[line >> 5] & bit)) {
+                    const E *ln = in + (int64_t)outer * g.os + z;
                     const int kmax = g.bc_lo == PMX_BC_PERIODIC ? g.n / 2 : max(y + 1, g.n - y);
                     for (; k <= kReach && k <= kmax; ++k) {
                         if (k * k >= best) break;
-                        int a, b;
-                        if constexpr (kY) {
-                            a = fetch_y(summary, g, outer, z, y - k);
-                            b = fetch_y(summary, g, outer, z, y + k);
-                        } else {
-                            const int32_t *ln = in + (int64_t)outer * g.os + z;
-                            a = fetch_x(ln, g, y - k);
-                            b = fetch_x(ln, g, y + k);
-                        }
-                        best = min(best, min(a, b) + k * k);
+                        best = min(best, min(fetch(ln, g, y - k), fetch(ln, g, y + k)) + k * k);
                     }
                     ok = k * k >= best || k > kmax;
                 }
@@ -684,20 +651,16 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const int4 *__restrict__
 #pragma unroll
         for (int i = 0; i < kHalo; ++i) h[i] = *reinterpret_cast<const int4 *>(tile + (T + row0 + 16 * i) * kTileZ + 4 * q);
         __syncthreads();
-        acc4 = make_int4(kBig, kBig, kBig, kBig);
-        acc = kBig;
+        acc = make_int4(kBig, kBig, kBig, kBig);
 #pragma unroll
         for (int i = 0; i < kHalo; ++i) {
             *reinterpret_cast<int4 *>(tile + (row0 + 16 * i) * kTileZ + 4 * q) = h[i];
-            acc4.x = min(acc4.x, h[i].x);
-            acc4.y = min(acc4.y, h[i].y);
-            acc4.z = min(acc4.z, h[i].z);
-            acc4.w = min(acc4.w, h[i].w);
+            acc.x = min(acc.x, h[i].x);
+            acc.y = min(acc.y, h[i].y);
+            acc.z = min(acc.z, h[i].z);
+            acc.w = min(acc.w, h[i].w);
         }
-        if constexpr (kY)
-            commit_y<T / 4>(tile, g, 2 * W, wv, lane, z0, ym, ys, acc);
-        else
-            commit_x<T / 16>(tile, g, 2 * W, row0, q, zl, xm, xr, acc4);
+        commit_rows<T / 16>(tile, g, 2 * W, row0, q, zl, xm, xr, acc);
     }
 }
 
@@ -709,9 +672,9 @@ __device__ __forceinline__ int finish(int v, int cap)
     return v >= kBig ? kInf : v;
 }
 
-__global__ __launch_bounds__(256) void k_fallback_y(const int4 *__restrict__ summary, int32_t *__restrict__ out,
-                                                    int nx, int ny, int nz, int64_t out_sx, int64_t out_sy,
-                                                    int bc_lo, int bc_hi, int cap,
+__global__ __launch_bounds__(256) void k_fallback_y(const uint16_t *__restrict__ in, int32_t *__restrict__ out,
+                                                    int nx, int ny, int nz, int64_t pitch, int64_t out_sx,
+                                                    int64_t out_sy, int bc_lo, int bc_hi, int cap,
                                                     const uint32_t *__restrict__ ovf_count,
                                                     const uint32_t *__restrict__ ovf_list,
                                                     int *__restrict__ stk, int stk_cap)
@@ -719,19 +682,18 @@ __global__ __launch_bounds__(256) void k_fallback_y(const int4 *__restrict__ sum
     const uint32_t count = *ovf_count;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
-    const int nzt = (nz + kTileZ - 1) / kTileZ;
     Stack st{stk + tid, stk + (int64_t)stk_cap * nthreads + tid, nthreads, stk_cap,
              stk + (int64_t)2 * stk_cap * nthreads + tid};
     for (int64_t i = tid; i < (int64_t)count; i += nthreads) {
         const int64_t line = ovf_list[i];
         const int64_t x = line / nz, z = line - x * nz;
-        const int4 *rows = summary + x * ny * nzt;
+        const uint16_t *src = in + x * ny * pitch + z;
         int32_t *dst = out + x * out_sx + z;
         envelope_line(
             ny, bc_lo, bc_hi, st, true,
             [&](int j) {
-                const int d = zdist2(rows + (int64_t)j * nzt, (int)z);
-                return d < 0 ? kInf : d;
+                const uint16_t d = src[(int64_t)j * pitch];
+                return d == kNoSolid ? kInf : (int)d * (int)d;
             },
             [&](int j, int v) { dst[(int64_t)j * out_sy] = finish(v, cap); });
     }
@@ -767,20 +729,20 @@ __global__ __launch_bounds__(256) void k_clear_words(uint32_t *__restrict__ p, i
 }
 
 #ifndef PMX_WIN_T
-#define PMX_WIN_T 32  // rows per tile of the windowed passes
+#define PMX_WIN_T 64  // rows per tile of the windowed passes
 #endif
 constexpr int kWinT = PMX_WIN_T;
 constexpr size_t kFallbackStackBytes = size_t(256) << 20;
 
 // launches one windowed pass with halo W
-template <int W, bool kY>
-void launch_window(const int4 *summary, const int32_t *in, int32_t *out, const PassXOut &o, const WinGeom &wg,
-                   int nx, int ny, uint32_t *bits, uint32_t *count, uint32_t *list, hipStream_t s)
+template <int W, class E>
+void launch_window(const E *in, int32_t *out, const PassXOut &o, const WinGeom &wg, int nx, int ny,
+                   uint32_t *bits, uint32_t *count, uint32_t *list, hipStream_t s)
 {
     const int nzt = (wg.nz + kTileZ - 1) / kTileZ;
     const int64_t blocks = (int64_t)wg.nouter * nzt * (((wg.n + kWinT - 1) / kWinT + kTilesPerBlock - 1) / kTilesPerBlock);
-    hipLaunchKernelGGL((k_window<kWinT, W, kY>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, summary, in, out, o,
-                       wg, nx, ny, bits, count, list);
+    hipLaunchKernelGGL((k_window<kWinT, W, E>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, in, out, o, wg, nx,
+                       ny, bits, count, list);
 }
 
 }  // namespace
@@ -788,11 +750,12 @@ void launch_window(const int4 *summary, const int32_t *in, int32_t *out, const P
 size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p)
 {
     Carver c(ws, SIZE_MAX);
-    const int64_t nzt = (g->n[2] + kTileZ - 1) / kTileZ;
-    p->summary = c.take<int4>((size_t)(g->n[0] * g->n[1] * nzt));
-    // pass-Y output, rows padded to a multiple of 4 voxels (16-byte aligned)
-    p->pitch = (g->n[2] + 3) / 4 * 4;
-    p->b = c.take<int32_t>((size_t)(g->n[0] * g->n[1] * p->pitch));
+    // working rows padded to a multiple of 8 voxels: 16-byte aligned rows
+    // for both the uint16 pass-Z array and the int32 pass-Y array
+    p->pitch = (g->n[2] + 7) / 8 * 8;
+    const size_t vox = (size_t)(g->n[0] * g->n[1] * p->pitch);
+    p->dz = c.take<uint16_t>(vox);
+    p->b = c.take<int32_t>(vox);
     p->counters = c.take<uint32_t>(4);
     const size_t ly = (size_t)(g->n[0] * g->n[2]), lx = (size_t)(g->n[1] * g->n[2]);
     p->n_bits = (ly + 31) / 32 + (lx + 31) / 32;
@@ -823,50 +786,52 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     const int ccap = (int)std::min<int64_t>(std::max<int64_t>(cap, 0), kBig);
     const int nx = (int)g->n[0], ny = (int)g->n[1], nz = (int)g->n[2];
     const int64_t rows = (int64_t)nx * ny;
+    const int64_t pitch = p.pitch;
     hipLaunchKernelGGL(k_clear_words, dim3((unsigned)std::min<int64_t>((p.n_bits + 255) / 256 + 1, 4096)),
                        dim3(256), 0, s, p.bits_y, (int64_t)p.n_bits);
     {
-    ProfScope ps("edt_rows", s);
+    ProfScope ps("edt_pass_z", s);
     const dim3 bz(256);
     const dim3 gz((unsigned)((rows + 3) / 4));
     const int vec_in = ((uintptr_t)img % 16 == 0 && g->in_s[0] % 16 == 0 && g->in_s[1] % 16 == 0) ? 1 : 0;
     if (invert)
-        hipLaunchKernelGGL(k_row_summary<true>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, p.summary, p.counters);
+        hipLaunchKernelGGL(k_pass_z<true>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
+                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters);
     else
-        hipLaunchKernelGGL(k_row_summary<false>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, p.summary, p.counters);
+        hipLaunchKernelGGL(k_pass_z<false>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
+                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters);
     }
-    if (int e = check_launch("row summaries")) return e;
+    if (int e = check_launch("pass z")) return e;
 
     // pass-Y output stored [y][x][z]: pass X then reads rows 4*pitch bytes apart
-    const int64_t pitch = p.pitch;
     const int64_t bsx = pitch, bsy = (int64_t)nx * pitch;
     // halo: 16 rows settle nearly every voxel of a porous medium; a clamped
     // EDT whose threshold reaches past 17^2 scans up to 32 rows in LDS
     const bool wide = ccap < kBig && ccap > 17 * 17;
     {
     ProfScope ps("edt_pass_y", s);
-    WinGeom wg{ny, 0, 0, nx, nz, g->bc[2], g->bc[3], ccap, 0, bsx, bsy, ny};
+    const WinGeom wg{ny, pitch, ny * pitch, nx, nz, g->bc[2], g->bc[3], ccap, pitch, bsx, bsy};
     if (wide)
-        launch_window<32, true>(p.summary, nullptr, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, s);
+        launch_window<32>(p.dz, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, s);
     else
-        launch_window<16, true>(p.summary, nullptr, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, s);
+        launch_window<16>(p.dz, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, s);
     }
     if (int e = check_launch("pass y")) return e;
     {
     ProfScope ps("edt_fallback_y", s);
-    hipLaunchKernelGGL(k_fallback_y, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.summary, p.b, nx, ny, nz,
+    hipLaunchKernelGGL(k_fallback_y, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.dz, p.b, nx, ny, nz, pitch,
                        bsx, bsy, g->bc[2], g->bc[3], ccap, p.counters + 0, p.ovf_list, p.stk, p.cap);
     }
     if (int e = check_launch("pass y fallback")) return e;
     {
     ProfScope ps("edt_pass_x", s);
-    WinGeom wg{nx, bsx, bsy, ny, nz, g->bc[0], g->bc[1], ccap, pitch, 0, 0, ny};
+    const WinGeom wg{nx, bsx, bsy, ny, nz, g->bc[0], g->bc[1], ccap, pitch, 0, 0};
     if (wide)
-        launch_window<32, false>(nullptr, p.b, nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1, p.ovf_list, s);
+        launch_window<32>((const int32_t *)p.b, (int32_t *)nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1,
+                          p.ovf_list, s);
     else
-        launch_window<16, false>(nullptr, p.b, nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1, p.ovf_list, s);
+        launch_window<16>((const int32_t *)p.b, (int32_t *)nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1,
+                          p.ovf_list, s);
     }
     if (int e = check_launch("pass x")) return e;
     {

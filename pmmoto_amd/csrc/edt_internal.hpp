@@ -24,8 +24,8 @@ struct PassXOut {
 };
 
 struct EdtPlan {
-    int4 *summary = nullptr;  // per z-row, per 64-voxel chunk: nearest solids and bitmask
-    int32_t *b = nullptr;     // pass-Y output, stored [y][x][z] with row pitch `pitch` voxels
+    uint16_t *dz = nullptr;   // pass-Z distances, [x][y][z], row pitch `pitch` voxels
+    int32_t *b = nullptr;     // pass-Y squared distances, stored [y][x][z], same pitch
     int64_t pitch = 0;
     uint32_t *counters = nullptr, *ovf_list = nullptr;
     uint32_t *bits_y = nullptr, *bits_x = nullptr;  // deferred-line bitmaps (one allocation)

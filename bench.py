@@ -21,9 +21,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per voxel of each EDT kernel (u8 in, int32 working array)
-BYTES_PER_VOXEL = {"edt_pass_z": 1 + 4, "edt_pass_y": 4 + 4, "edt_pass_x": 4 + 4}
+# algorithmic bytes per voxel of each EDT kernel as built: pass Z reads the
+# u8 image and writes uint16 z distances, pass Y reads those and writes
+# int32 squared distances, pass X reads those and writes the int32 output
+BYTES_PER_VOXEL = {"edt_pass_z": 1 + 2, "edt_pass_y": 2 + 4, "edt_pass_x": 4 + 4}
+# the squared-EDT figure of the three-pass algorithm with int32 passes
+# (1 R + 4 W, 4 R + 4 W, 4 R + 4 W): the %HBM definition of the metric
 EDT_BYTES_PER_VOXEL = 21
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "edt_1024_hbm_traffic.json")
+
+
+def traffic_for(kernel: str, n: int):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE (scaled by a 1 GiB copy) at 1024^3, from the
+    committed profile summary; None for other sizes or when absent."""
+    if n != 1024 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k["hbm_bytes_per_launch"]
 
 
 def cpu_baseline(img_t, edge: int):
@@ -197,6 +214,7 @@ def main() -> int:
                         "(mean r 10, sigma 0.3, porosity 0.35, seed 3), x END / y,z WALL",
             "voxels": int(vox),
             "edt_hbm_frac_21B": round(EDT_BYTES_PER_VOXEL * vox / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "edt_bytes_per_voxel_moved": sum(BYTES_PER_VOXEL.values()),
             "kernels": per_kernel,
             "porosimetry": psd,
         },
@@ -207,7 +225,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic_for(dom, n),
         },
         "cpu_baseline": cpu,
     }

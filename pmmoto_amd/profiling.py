@@ -1,0 +1,123 @@
+"""Summaries of rocprofv3 runs of bench.py (kernel times and HBM traffic).
+
+    rocprofv3 --kernel-trace --stats -d DIR_STATS -o run -- python3 bench.py --no-cpu --no-psd --steps 5 --warmup 1
+    rocprofv3 --pmc FETCH_SIZE -d DIR_FETCH -o run -- python3 bench.py --no-cpu --no-psd --steps 2 --warmup 1 --calibrate-copy
+    rocprofv3 --pmc WRITE_SIZE -d DIR_WRITE -o run -- python3 bench.py --no-cpu --no-psd --steps 2 --warmup 1 --calibrate-copy
+    python -m pmmoto_amd.profiling DIR_STATS DIR_FETCH DIR_WRITE OUT.json OUT.md
+
+FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what the same
+counter reads for the copy of a 1 GiB buffer that ``--calibrate-copy``
+runs (1 GiB read, 1 GiB written), so the traffic is in bytes of HBM.
+"""
+
+from __future__ import annotations
+
+import glob
+import json
+import os
+import re
+import sqlite3
+import sys
+from collections import defaultdict
+
+GIB = 1 << 30
+
+# kernel symbol -> the name bench.py reports it under
+BENCH_NAMES = (
+    (re.compile(r"k_pass_z<"), "edt_pass_z"),
+    (re.compile(r"k_window<.*unsigned short>"), "edt_pass_y"),
+    (re.compile(r"k_window<.*int>"), "edt_pass_x"),
+    (re.compile(r"k_fallback_y"), "edt_fallback_y"),
+    (re.compile(r"k_fallback_x"), "edt_fallback_x"),
+)
+COPY = re.compile(r"copyBuffer")
+
+
+def _short(name: str) -> str:
+    name = name.replace("(anonymous namespace)", "")
+    name = name.split("(")[0]
+    return name.split("::")[-1]
+
+
+def _db(d: str) -> str:
+    found = sorted(glob.glob(os.path.join(d, "**", "*_results.db"), recursive=True))
+    if not found:
+        raise FileNotFoundError(f"no rocprofv3 results database under {d}")
+    return found[0]
+
+
+def kernel_times(stats_dir: str) -> dict:
+    con = sqlite3.connect(_db(stats_dir))
+    out = {}
+    for name, n, avg in con.execute("select name, count(*), avg(duration) from kernels group by name"):
+        out[_short(name)] = {"calls": n, "avg_us": avg / 1e3}
+    return out
+
+
+def counter(pmc_dir: str, counter_name: str) -> dict:
+    con = sqlite3.connect(_db(pmc_dir))
+    acc = defaultdict(list)
+    for k, v in con.execute("select kernel_name, value from counters_collection where counter_name = ?",
+                            (counter_name,)):
+        acc[_short(k)].append(v)
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def summarize(stats_dir: str, fetch_dir: str, write_dir: str) -> dict:
+    times = kernel_times(stats_dir)
+    fetch = counter(fetch_dir, "FETCH_SIZE")
+    write = counter(write_dir, "WRITE_SIZE")
+    copy_f = [v for k, v in fetch.items() if COPY.search(k)]
+    copy_w = [v for k, v in write.items() if COPY.search(k)]
+    if not copy_f or not copy_w:
+        raise RuntimeError("the 1 GiB copy kernel is missing: run bench.py with --calibrate-copy")
+    f_fetch = GIB / (copy_f[0] * 1024.0)  # bytes of HBM per KiB the counter reports
+    f_write = GIB / (copy_w[0] * 1024.0)
+    kernels = {}
+    for sym in sorted(set(times) | set(fetch) | set(write)):
+        bench = next((b for rx, b in BENCH_NAMES if rx.search(sym)), None)
+        if bench is None:
+            continue
+        rd = fetch.get(sym, 0.0) * 1024.0 * f_fetch
+        wr = write.get(sym, 0.0) * 1024.0 * f_write
+        kernels[bench] = {
+            "symbol": sym,
+            "avg_us": round(times.get(sym, {}).get("avg_us", float("nan")), 2),
+            "calls": times.get(sym, {}).get("calls", 0),
+            "hbm_read_bytes_per_launch": int(rd),
+            "hbm_write_bytes_per_launch": int(wr),
+            "hbm_bytes_per_launch": int(rd + wr),
+        }
+    return {"scale": {"fetch_bytes_per_counter_kib": f_fetch * 1024.0, "write_bytes_per_counter_kib": f_write * 1024.0},
+            "kernels": kernels, "all_kernels_us": {k: round(v["avg_us"], 2) for k, v in times.items()}}
+
+
+def to_markdown(s: dict, command: str) -> str:
+    lines = [f"Command: `{command}`", "",
+             f"Counter scale (from a 1 GiB copy): FETCH_SIZE x{s['scale']['fetch_bytes_per_counter_kib'] / 1024:.3f}, "
+             f"WRITE_SIZE x{s['scale']['write_bytes_per_counter_kib'] / 1024:.3f} (bytes per reported byte)", "",
+             "| kernel | avg us | HBM read GB | HBM write GB | HBM GB/s |", "|---|---|---|---|---|"]
+    for b, k in s["kernels"].items():
+        tot = k["hbm_bytes_per_launch"]
+        rate = tot / (k["avg_us"] * 1e-6) / 1e9 if k["avg_us"] == k["avg_us"] and k["avg_us"] > 0 else float("nan")
+        lines.append(f"| {b} (`{k['symbol']}`) | {k['avg_us']:.1f} | {k['hbm_read_bytes_per_launch'] / 1e9:.3f} | "
+                     f"{k['hbm_write_bytes_per_launch'] / 1e9:.3f} | {rate:.0f} |")
+    lines += ["", "All kernels (avg us per launch):", ""]
+    lines += [f"- `{k}`: {v}" for k, v in sorted(s["all_kernels_us"].items(), key=lambda kv: -kv[1])]
+    return "\n".join(lines) + "\n"
+
+
+def main(argv) -> int:
+    if len(argv) != 6:
+        print(__doc__)
+        return 2
+    s = summarize(argv[1], argv[2], argv[3])
+    with open(argv[4], "w") as f:
+        json.dump(s, f, indent=1)
+    with open(argv[5], "w") as f:
+        f.write(to_markdown(s, "python -m pmmoto_amd.profiling " + " ".join(argv[1:])))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv))

@@ -135,8 +135,10 @@ __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img,
                                                 int64_t in_sy, int ny, int nz, int64_t rows,
                                                 int bc_lo, int bc_hi, int vec_in, int64_t pitch,
                                                 uint16_t *__restrict__ out,
-                                                uint32_t *__restrict__ counters)
+                                                uint32_t *__restrict__ counters,
+                                                const unsigned int *__restrict__ proceed)
 {
+    if (proceed && *proceed == 0u) return;
     __shared__ int sc[4][2][kSup];           // per wave: each super-chunk's last / first solid
     __shared__ uint32_t smask[4][kSup][kWave];  // per wave: each lane's masks (the row is read once)
     const int lane = threadIdx.x & (kWave - 1);
@@ -541,8 +543,10 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
                                                         PassXOut o, WinGeom g, int nx, int ny,
                                                         uint32_t *__restrict__ line_bits,
                                                         uint32_t *__restrict__ ovf_count,
-                                                        uint32_t *__restrict__ ovf_list)
+                                                        uint32_t *__restrict__ ovf_list,
+                                                        const unsigned int *__restrict__ proceed)
 {
+    if (proceed && *proceed == 0u) return;
     constexpr bool kY = sizeof(E) == 2;
     using V = typename Quad<E>::type;
     static_assert(W % 8 == 0 && T % 16 == 0, "tile layout");
@@ -677,8 +681,10 @@ __global__ __launch_bounds__(256) void k_fallback_y(const uint16_t *__restrict__
                                                     int64_t out_sy, int bc_lo, int bc_hi, int cap,
                                                     const uint32_t *__restrict__ ovf_count,
                                                     const uint32_t *__restrict__ ovf_list,
-                                                    int *__restrict__ stk, int stk_cap)
+                                                    int *__restrict__ stk, int stk_cap,
+                                                    const unsigned int *__restrict__ proceed)
 {
+    if (proceed && *proceed == 0u) return;
     const uint32_t count = *ovf_count;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
@@ -704,8 +710,10 @@ __global__ __launch_bounds__(256) void k_fallback_x(const int32_t *__restrict__ 
                                                     int bc_lo, int bc_hi, int cap,
                                                     const uint32_t *__restrict__ ovf_count,
                                                     const uint32_t *__restrict__ ovf_list,
-                                                    int *__restrict__ stk, int stk_cap)
+                                                    int *__restrict__ stk, int stk_cap,
+                                                    const unsigned int *__restrict__ proceed)
 {
+    if (proceed && *proceed == 0u) return;
     const uint32_t count = *ovf_count;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
@@ -737,12 +745,12 @@ constexpr size_t kFallbackStackBytes = size_t(256) << 20;
 // launches one windowed pass with halo W
 template <int W, class E>
 void launch_window(const E *in, int32_t *out, const PassXOut &o, const WinGeom &wg, int nx, int ny,
-                   uint32_t *bits, uint32_t *count, uint32_t *list, hipStream_t s)
+                   uint32_t *bits, uint32_t *count, uint32_t *list, const unsigned int *proceed, hipStream_t s)
 {
     const int nzt = (wg.nz + kTileZ - 1) / kTileZ;
     const int64_t blocks = (int64_t)wg.nouter * nzt * (((wg.n + kWinT - 1) / kWinT + kTilesPerBlock - 1) / kTilesPerBlock);
     hipLaunchKernelGGL((k_window<kWinT, W, E>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, in, out, o, wg, nx,
-                       ny, bits, count, list);
+                       ny, bits, count, list, proceed);
 }
 
 }  // namespace
@@ -774,7 +782,7 @@ size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p)
 }
 
 int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &o, int64_t cap,
-            void *ws, size_t ws_bytes, hipStream_t s)
+            void *ws, size_t ws_bytes, hipStream_t s, const unsigned int *proceed)
 {
     if (int e = validate_geom(g)) return e;
     if (!img) return fail(PMX_ERR_INVALID, "img is NULL");
@@ -796,10 +804,10 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     const int vec_in = ((uintptr_t)img % 16 == 0 && g->in_s[0] % 16 == 0 && g->in_s[1] % 16 == 0) ? 1 : 0;
     if (invert)
         hipLaunchKernelGGL(k_pass_z<true>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters);
+                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);
     else
         hipLaunchKernelGGL(k_pass_z<false>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters);
+                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);
     }
     if (int e = check_launch("pass z")) return e;
 
@@ -812,15 +820,15 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     ProfScope ps("edt_pass_y", s);
     const WinGeom wg{ny, pitch, ny * pitch, nx, nz, g->bc[2], g->bc[3], ccap, pitch, bsx, bsy};
     if (wide)
-        launch_window<32>(p.dz, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, s);
+        launch_window<32>(p.dz, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, proceed, s);
     else
-        launch_window<16>(p.dz, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, s);
+        launch_window<16>(p.dz, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, proceed, s);
     }
     if (int e = check_launch("pass y")) return e;
     {
     ProfScope ps("edt_fallback_y", s);
     hipLaunchKernelGGL(k_fallback_y, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.dz, p.b, nx, ny, nz, pitch,
-                       bsx, bsy, g->bc[2], g->bc[3], ccap, p.counters + 0, p.ovf_list, p.stk, p.cap);
+                       bsx, bsy, g->bc[2], g->bc[3], ccap, p.counters + 0, p.ovf_list, p.stk, p.cap, proceed);
     }
     if (int e = check_launch("pass y fallback")) return e;
     {
@@ -828,16 +836,16 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     const WinGeom wg{nx, bsx, bsy, ny, nz, g->bc[0], g->bc[1], ccap, pitch, 0, 0};
     if (wide)
         launch_window<32>((const int32_t *)p.b, (int32_t *)nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1,
-                          p.ovf_list, s);
+                          p.ovf_list, proceed, s);
     else
         launch_window<16>((const int32_t *)p.b, (int32_t *)nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1,
-                          p.ovf_list, s);
+                          p.ovf_list, proceed, s);
     }
     if (int e = check_launch("pass x")) return e;
     {
     ProfScope ps("edt_fallback_x", s);
     hipLaunchKernelGGL(k_fallback_x, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.b, o, nx, ny, nz,
-                       bsx, bsy, g->bc[0], g->bc[1], ccap, p.counters + 1, p.ovf_list, p.stk, p.cap);
+                       bsx, bsy, g->bc[0], g->bc[1], ccap, p.counters + 1, p.ovf_list, p.stk, p.cap, proceed);
     }
     return check_launch("pass x fallback");
 }

@@ -41,8 +41,9 @@ size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p);
 // Exact squared EDT of img over the box of g (distance to voxels == 0, or
 // != 0 with invert), clamped to min(d2, cap) (cap >= 2^30: unclamped),
 // delivered through `o`; stream-ordered on s.  A caller that only tests
-// d2 <= t passes cap = t + 1.
+// d2 <= t passes cap = t + 1.  With `proceed` (device memory) set, every
+// kernel does nothing when *proceed == 0 at run time.
 int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &o, int64_t cap,
-            void *ws, size_t ws_bytes, hipStream_t s);
+            void *ws, size_t ws_bytes, hipStream_t s, const unsigned int *proceed = nullptr);
 
 }  // namespace pmx

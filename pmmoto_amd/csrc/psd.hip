@@ -66,7 +66,7 @@ constexpr int kBucketItems = 16;  // voxels per thread between bucket reservatio
 __global__ __launch_bounds__(256) void k_levels(const int32_t *__restrict__ d2, int64_t n, const int64_t *__restrict__ te,
                                                 int n_levels, uint8_t *__restrict__ level,
                                                 unsigned int *__restrict__ counts, int *__restrict__ parent,
-                                                uint8_t *__restrict__ flag)
+                                                uint8_t *__restrict__ flag, uint8_t *__restrict__ in_c)
 {
     __shared__ int64_t ste[kMaxLevels];
     __shared__ unsigned int hist[kMaxLevels];
@@ -89,6 +89,7 @@ __global__ __launch_bounds__(256) void k_levels(const int32_t *__restrict__ d2, 
         level[i] = (uint8_t)lo;
         parent[i] = kInactive;
         flag[i] = 0;
+        in_c[i] = 0;
         if (lo < n_levels) atomicAdd(hist + lo, 1u);
     }
     __syncthreads();
@@ -198,17 +199,25 @@ __global__ __launch_bounds__(256) void k_flag_inlet(Dims d, int64_t lo0, int64_t
     }
 }
 
-// mask = active and inlet-connected (the porosimetry image before dilation)
+// mask = the voxels that joined the inlet-connected eroded set at this
+// level (in_c remembers the set of the previous level; the sets only grow).
+// A pore voxel left unassigned by level i-1 is farther than r_{i-1} > r_i
+// from every older member, so only the new members can invade it at level
+// i: they are the only centres the level's dilation needs.  any: set when
+// the level has a new member (otherwise its dilation is skipped on the GPU).
 __global__ __launch_bounds__(256) void k_mask(int64_t n, int *__restrict__ parent, const uint8_t *__restrict__ flag,
-                                              uint8_t *__restrict__ mask, unsigned int *__restrict__ any)
+                                              uint8_t *__restrict__ in_c, uint8_t *__restrict__ mask,
+                                              unsigned int *__restrict__ any)
 {
     bool seen = false;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         uint8_t m = 0;
         if (ld(parent + i) != kInactive && flag[find_root(parent, (int)i)]) m = 1;
-        mask[i] = m;
-        seen |= m != 0;
+        const uint8_t fresh = m && !in_c[i];
+        if (m) in_c[i] = 1;
+        mask[i] = fresh;
+        seen |= fresh != 0;
     }
     if (__any(seen) && (threadIdx.x & 63) == 0) atomicOr(any, 1u);
 }
@@ -230,7 +239,7 @@ __global__ __launch_bounds__(256) void k_zero_map(int32_t *__restrict__ map, Dim
 
 struct PsdPlan {
     int32_t *d2 = nullptr;
-    uint8_t *level = nullptr, *flag = nullptr, *mask = nullptr;
+    uint8_t *level = nullptr, *flag = nullptr, *mask = nullptr, *in_c = nullptr;
     int *parent = nullptr, *order = nullptr;
     unsigned int *counts = nullptr, *offsets = nullptr, *cursor = nullptr, *any = nullptr;
     int64_t *te = nullptr;
@@ -246,6 +255,7 @@ size_t plan_psd(const pmx_geom *g, void *ws, PsdPlan *p)
     p->level = c.take<uint8_t>(n);
     p->flag = c.take<uint8_t>(n);
     p->mask = c.take<uint8_t>(n);
+    p->in_c = c.take<uint8_t>(n);
     p->parent = c.take<int>(n);
     p->order = c.take<int>(n);
     p->counts = c.take<unsigned int>(kMaxLevels + 1);
@@ -331,7 +341,7 @@ int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo
         hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(256), 0, s, p.counts, kMaxLevels + 1);
         hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(256), 0, s, p.any, kMaxLevels);
         hipLaunchKernelGGL(k_levels, dim3(gr), dim3(256), 0, s, p.d2, n, p.te, n_levels, p.level, p.counts,
-                           p.parent, p.flag);
+                           p.parent, p.flag, p.in_c);
         hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, s, p.counts, n_levels, p.offsets, p.cursor);
         hipLaunchKernelGGL(k_bucket, dim3(grid_for((n + kBucketItems - 1) / kBucketItems)), dim3(256), 0, s, p.level,
                            n, n_levels, p.cursor, p.order);
@@ -363,11 +373,10 @@ int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo
                 hipLaunchKernelGGL(k_flag_inlet, dim3(gr), dim3(256), 0, s, d, (int64_t)0, (int64_t)0, (int64_t)0,
                                    (int64_t)d.n0, (int64_t)d.n1, (int64_t)d.n2, p.parent, p.flag);
             }
-            hipLaunchKernelGGL(k_mask, dim3(gr), dim3(256), 0, s, n, p.parent, p.flag, p.mask, p.any + lv);
+            hipLaunchKernelGGL(k_mask, dim3(gr), dim3(256), 0, s, n, p.parent, p.flag, p.in_c, p.mask, p.any + lv);
         }
         if (int rc = check_launch("psd level")) return rc;
-        // dilation of the mask by radius lv with the first-invasion update
-        // (an empty mask has no zero-distance voxels, so nothing is updated)
+        // dilation of the level's new centres with the first-invasion update
         PassXOut o{};
         o.sx = g->out_s[0];
         o.sy = g->out_s[1];
@@ -377,7 +386,8 @@ int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo
         o.psd_map = code_map;
         o.psd_code = lv + 1;
         ProfScope ps("psd_dilate", s);
-        if (int rc = run_edt(p.mask, &gc, true, o, (int64_t)o.t + 1, p.edt_ws, p.edt_bytes, s)) return rc;
+        // skipped on the GPU when the level has no new centre
+        if (int rc = run_edt(p.mask, &gc, true, o, (int64_t)o.t + 1, p.edt_ws, p.edt_bytes, s, p.any + lv)) return rc;
     }
     return PMX_OK;
 }

@@ -203,18 +203,56 @@ int pmx_max_f32(const float *a, int64_t n, float *out, hipStream_t stream);
 
 /* The pore-size-distribution sweep of pore_size_distribution (porosimetry.py:
  * 163-227) in one call, on a contiguous box (in and out strides equal the
- * box; no WALL codes: pass padded images with END faces).  Level i (radii
+ * box).  A padded PMMoTo image with END faces, or the own region with
+ * WALL / PERIODIC faces (the torus for the erosions, dilations and the
+ * inlet connectivity) give the same codes on the own voxels.  Level i (radii
  * in descending order) erodes to d2 > t_erode[i] (non-increasing), keeps
  * the part 26-connected to the inlet slab inlet_lo + [0, inlet_ext) (all of
  * it when inlet_lo is NULL), dilates it to d2 <= t_dilate[i] and writes
  * code i+1 into code_map where img == 1 and code_map is still 0.  One EDT
- * of img, one union-find grown level by level, one EDT per level.  Blocks
- * the calling thread once to read the level sizes.  n_levels <= 250. */
+ * of img, one union-find grown level by level, one EDT per level (of the
+ * centres that joined at that level only).  No host synchronisation: the
+ * launches can be captured in a graph.  n_levels <= 250. */
 int pmx_psd_workspace_size(const pmx_geom *g, size_t *bytes);
 int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo,
                   const int64_t *inlet_ext, int n_levels, const int64_t *t_erode,
                   const int64_t *t_dilate, uint32_t wall_mask, int32_t *code_map,
                   void *workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* Batched sweeps (no counterpart in the reference, which runs one domain
+ * per process: for callers that sweep hundreds of unevenly sized domains).
+ * Every domain runs pmx_psd_sweep's algorithm with the shared thresholds;
+ * domains are spread over the caller's streams `lanes` (longest first, to
+ * the least loaded lane), each lane's domains run in turn in the lane's
+ * slice of the workspace, and nothing synchronises with the host, so the
+ * whole batch can be captured in one graph.  Outputs are bit-equal to
+ * per-domain pmx_psd_sweep calls. */
+typedef struct {
+    const uint8_t *img;    /* padded image, contiguous box with END/PERIODIC faces  */
+    pmx_geom g;            /* its box (in and out strides equal the box)            */
+    int64_t inlet_lo[3];   /* inlet slab, used when has_inlet != 0                  */
+    int64_t inlet_ext[3];
+    int32_t has_inlet;
+    uint32_t wall_mask;    /* faces whose layer is zeroed in the dilations          */
+    int32_t *code_map;     /* output: int32, the box                                */
+    int32_t tile_rows;     /* rows per EDT tile: 0 (default), 32 or 64              */
+} pmx_psd_domain;
+int pmx_psd_batch_workspace_size(int n_domains, const pmx_psd_domain *domains, int n_lanes,
+                                 size_t *bytes);
+int pmx_psd_sweep_batch(int n_domains, const pmx_psd_domain *domains, int n_levels,
+                        const int64_t *t_erode, const int64_t *t_dilate, int n_lanes,
+                        const hipStream_t *lanes, void *workspace, size_t workspace_bytes);
+
+/* Tile autotuner (no counterpart in the reference): times the windowed EDT
+ * passes of img with 32- and 64-row tiles (a threshold EDT clamped at cap;
+ * invert as for pmx_morph) and returns the faster in *tile_rows.  The
+ * choice is kept per (shape, boundary, clamp class) for the life of the
+ * process, so later calls for the same key return at once.  Synchronises
+ * with the stream; not for graph capture. */
+int pmx_edt_autotune_workspace_size(const pmx_geom *g, size_t *bytes);
+int pmx_edt_autotune(const uint8_t *img, const pmx_geom *g, int64_t cap, int invert,
+                     int32_t *tile_rows, void *workspace, size_t workspace_bytes,
+                     hipStream_t stream);
 
 /* Sphere packs.  spheres: n_spheres x (cx, cy, cz, r) float64 in voxel
  * units (voxel (i,j,k) has its centroid at (i+.5, j+.5, k+.5)); a voxel is

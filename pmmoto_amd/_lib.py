@@ -32,6 +32,21 @@ class Geom(C.Structure):
     ]
 
 
+class PsdDomain(C.Structure):
+    """pmx_psd_domain: one domain of a batched sweep."""
+
+    _fields_ = [
+        ("img", C.c_void_p),
+        ("g", Geom),
+        ("inlet_lo", C.c_int64 * 3),
+        ("inlet_ext", C.c_int64 * 3),
+        ("has_inlet", C.c_int32),
+        ("wall_mask", C.c_uint32),
+        ("code_map", C.c_void_p),
+        ("tile_rows", C.c_int32),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/pmx.h declares
 P, I32, I64, U8, U32, SZ = C.c_void_p, C.c_int32, C.c_int64, C.c_uint8, C.c_uint32, C.c_size_t
 SIGNATURES = {
@@ -53,6 +68,10 @@ SIGNATURES = {
     "pmx_keep_labels": (C.c_int, [P, P, C.c_int, I64, P, P, C.c_uint64, C.c_int, C.c_int, P, P]),
     "pmx_psd_workspace_size": (C.c_int, [P, P]),
     "pmx_psd_sweep": (C.c_int, [P, P, P, P, C.c_int, P, P, U32, P, P, SZ, P]),
+    "pmx_psd_batch_workspace_size": (C.c_int, [C.c_int, P, C.c_int, P]),
+    "pmx_psd_sweep_batch": (C.c_int, [C.c_int, P, C.c_int, P, P, C.c_int, P, P, SZ]),
+    "pmx_edt_autotune_workspace_size": (C.c_int, [P, P]),
+    "pmx_edt_autotune": (C.c_int, [P, P, I64, C.c_int, P, P, SZ, P]),
     "pmx_psd_update": (C.c_int, [P, P, I64, I32, P, P]),
     "pmx_codes_to_values": (C.c_int, [P, I64, P, I32, P, P]),
     "pmx_code_histogram": (C.c_int, [P, P, P, P, P, I32, P, P]),

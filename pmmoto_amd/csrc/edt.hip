@@ -736,21 +736,36 @@ __global__ __launch_bounds__(256) void k_clear_words(uint32_t *__restrict__ p, i
         p[i] = 0u;
 }
 
-#ifndef PMX_WIN_T
-#define PMX_WIN_T 64  // rows per tile of the windowed passes
-#endif
-constexpr int kWinT = PMX_WIN_T;
+constexpr int kDefaultTileRows = 64;  // rows per tile of the windowed passes (32 is the other choice)
 constexpr size_t kFallbackStackBytes = size_t(256) << 20;
 
-// launches one windowed pass with halo W
-template <int W, class E>
+// launches one windowed pass with tile rows T and halo W
+template <int T, int W, class E>
 void launch_window(const E *in, int32_t *out, const PassXOut &o, const WinGeom &wg, int nx, int ny,
                    uint32_t *bits, uint32_t *count, uint32_t *list, const unsigned int *proceed, hipStream_t s)
 {
     const int nzt = (wg.nz + kTileZ - 1) / kTileZ;
-    const int64_t blocks = (int64_t)wg.nouter * nzt * (((wg.n + kWinT - 1) / kWinT + kTilesPerBlock - 1) / kTilesPerBlock);
-    hipLaunchKernelGGL((k_window<kWinT, W, E>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, in, out, o, wg, nx,
+    const int64_t blocks = (int64_t)wg.nouter * nzt * (((wg.n + T - 1) / T + kTilesPerBlock - 1) / kTilesPerBlock);
+    hipLaunchKernelGGL((k_window<T, W, E>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, in, out, o, wg, nx,
                        ny, bits, count, list, proceed);
+}
+
+template <class E>
+void dispatch_window(int tile_rows, bool wide, const E *in, int32_t *out, const PassXOut &o, const WinGeom &wg,
+                     int nx, int ny, uint32_t *bits, uint32_t *count, uint32_t *list, const unsigned int *proceed,
+                     hipStream_t s)
+{
+    if (tile_rows == 32) {
+        if (wide)
+            launch_window<32, 32>(in, out, o, wg, nx, ny, bits, count, list, proceed, s);
+        else
+            launch_window<32, 16>(in, out, o, wg, nx, ny, bits, count, list, proceed, s);
+    } else {
+        if (wide)
+            launch_window<64, 32>(in, out, o, wg, nx, ny, bits, count, list, proceed, s);
+        else
+            launch_window<64, 16>(in, out, o, wg, nx, ny, bits, count, list, proceed, s);
+    }
 }
 
 }  // namespace
@@ -782,8 +797,11 @@ size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p)
 }
 
 int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &o, int64_t cap,
-            void *ws, size_t ws_bytes, hipStream_t s, const unsigned int *proceed)
+            void *ws, size_t ws_bytes, hipStream_t s, const unsigned int *proceed, int tile_rows)
 {
+    if (tile_rows != 0 && tile_rows != 32 && tile_rows != 64)
+        return fail(PMX_ERR_INVALID, "tile_rows %d: use 0 (default), 32 or 64", tile_rows);
+    const int tr = tile_rows == 0 ? kDefaultTileRows : tile_rows;
     if (int e = validate_geom(g)) return e;
     if (!img) return fail(PMX_ERR_INVALID, "img is NULL");
     EdtPlan p;
@@ -819,10 +837,7 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     {
     ProfScope ps("edt_pass_y", s);
     const WinGeom wg{ny, pitch, ny * pitch, nx, nz, g->bc[2], g->bc[3], ccap, pitch, bsx, bsy};
-    if (wide)
-        launch_window<32>(p.dz, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, proceed, s);
-    else
-        launch_window<16>(p.dz, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, proceed, s);
+    dispatch_window(tr, wide, p.dz, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, proceed, s);
     }
     if (int e = check_launch("pass y")) return e;
     {
@@ -834,12 +849,8 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     {
     ProfScope ps("edt_pass_x", s);
     const WinGeom wg{nx, bsx, bsy, ny, nz, g->bc[0], g->bc[1], ccap, pitch, 0, 0};
-    if (wide)
-        launch_window<32>((const int32_t *)p.b, (int32_t *)nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1,
-                          p.ovf_list, proceed, s);
-    else
-        launch_window<16>((const int32_t *)p.b, (int32_t *)nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1,
-                          p.ovf_list, proceed, s);
+    dispatch_window(tr, wide, (const int32_t *)p.b, (int32_t *)nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1,
+                    p.ovf_list, proceed, s);
     }
     if (int e = check_launch("pass x")) return e;
     {

@@ -15,7 +15,7 @@
 //  * per level: inlet-slab voxels flag their roots, the inlet-connected
 //    set is masked, and one exact EDT of the mask (distance to the set)
 //    feeds the first-invasion update directly from its last pass.
-#include "edt_internal.hpp"
+#include "psd_internal.hpp"
 
 namespace pmx {
 namespace {
@@ -140,9 +140,13 @@ __global__ __launch_bounds__(256) void k_bucket(const uint8_t *__restrict__ leve
     }
 }
 
-__global__ __launch_bounds__(256) void k_activate(const int *__restrict__ order, unsigned int begin, unsigned int end,
-                                                  int *__restrict__ parent)
+// Level lv's voxels are order[offsets[lv] .. offsets[lv+1]): the range is
+// read on the device, so the sweep issues a fixed sequence of launches
+// with no host synchronisation (it can be captured in a graph).
+__global__ __launch_bounds__(256) void k_activate(const int *__restrict__ order, const unsigned int *__restrict__ offsets,
+                                                  int lv, int *__restrict__ parent)
 {
+    const unsigned int begin = offsets[lv], end = offsets[lv + 1];
     for (int64_t k = (int64_t)begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < end;
          k += (int64_t)gridDim.x * blockDim.x) {
         const int v = order[k];
@@ -151,9 +155,10 @@ __global__ __launch_bounds__(256) void k_activate(const int *__restrict__ order,
 }
 
 // each new voxel unions with every active 26-neighbour (wrapping on periodic axes)
-__global__ __launch_bounds__(256) void k_merge_new(const int *__restrict__ order, unsigned int begin, unsigned int end,
-                                                   Dims d, int *__restrict__ parent)
+__global__ __launch_bounds__(256) void k_merge_new(const int *__restrict__ order, const unsigned int *__restrict__ offsets,
+                                                   int lv, Dims d, int *__restrict__ parent)
 {
+    const unsigned int begin = offsets[lv], end = offsets[lv + 1];
     for (int64_t k = (int64_t)begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < end;
          k += (int64_t)gridDim.x * blockDim.x) {
         const int v = order[k];
@@ -222,6 +227,17 @@ __global__ __launch_bounds__(256) void k_mask(int64_t n, int *__restrict__ paren
     if (__any(seen) && (threadIdx.x & 63) == 0) atomicOr(any, 1u);
 }
 
+// the erosion thresholds travel as a kernel argument (a graph-capturable
+// upload: no host-to-device copy from pageable memory)
+struct Thresholds {
+    int64_t t[kMaxLevels];
+};
+
+__global__ void k_store_thresholds(Thresholds th, int n_levels, int64_t *te)
+{
+    for (int i = threadIdx.x; i < n_levels; i += blockDim.x) te[i] = th.t[i];
+}
+
 __global__ void k_clear_u32(unsigned int *a, int n)
 {
     for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0;
@@ -272,26 +288,17 @@ size_t plan_psd(const pmx_geom *g, void *ws, PsdPlan *p)
 unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 65536); }
 
 }  // namespace
-}  // namespace pmx
 
-extern "C" {
-
-int pmx_psd_workspace_size(const pmx_geom *g, size_t *bytes)
+size_t psd_workspace_bytes(const pmx_geom *g)
 {
-    pmx::clear_error();
-    if (int e = pmx::validate_geom(g)) return e;
-    if (!bytes) return pmx::fail(PMX_ERR_INVALID, "bytes is NULL");
-    pmx::PsdPlan p;
-    *bytes = pmx::plan_psd(g, nullptr, &p);
-    return PMX_OK;
+    PsdPlan p;
+    return plan_psd(g, nullptr, &p);
 }
 
-int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, const int64_t *inlet_ext,
-                  int n_levels, const int64_t *t_erode, const int64_t *t_dilate, uint32_t wall_mask,
-                  int32_t *code_map, void *workspace, size_t workspace_bytes, hipStream_t s)
+// argument checks shared by the single and batched sweeps
+int check_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, const int64_t *inlet_ext, int n_levels,
+              const int64_t *t_erode, const int64_t *t_dilate, const int32_t *code_map)
 {
-    using namespace pmx;
-    clear_error();
     if (int e = validate_geom(g)) return e;
     if (!img || !t_erode || !t_dilate || !code_map) return fail(PMX_ERR_INVALID, "NULL argument");
     if (n_levels < 1 || n_levels > kMaxLevels) return fail(PMX_ERR_INVALID, "n_levels must be in [1, %d]", kMaxLevels);
@@ -303,14 +310,20 @@ int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo
     for (int k = 1; k < n_levels; ++k)
         if (t_erode[k] > t_erode[k - 1])
             return fail(PMX_ERR_INVALID, "erosion thresholds must not increase (radii in descending order)");
-    for (int a = 0; a < 3; ++a)
-        if (g->bc[2 * a] == PMX_BC_WALL || g->bc[2 * a + 1] == PMX_BC_WALL)
-            return fail(PMX_ERR_INVALID, "WALL faces: pass the padded image with END faces");
-    bool has_inlet = inlet_lo && inlet_ext;
+    const bool has_inlet = inlet_lo && inlet_ext;
     if (has_inlet)
         for (int a = 0; a < 3; ++a)
             if (inlet_lo[a] < 0 || inlet_ext[a] < 0 || inlet_lo[a] + inlet_ext[a] > g->n[a])
                 return fail(PMX_ERR_INVALID, "inlet slab outside the box on axis %d", a);
+    return PMX_OK;
+}
+
+int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, const int64_t *inlet_ext, int n_levels,
+            const int64_t *t_erode, const int64_t *t_dilate, uint32_t wall_mask, int32_t *code_map, void *workspace,
+            size_t workspace_bytes, hipStream_t s, int tile_rows)
+{
+    const int64_t n = g->n[0] * g->n[1] * g->n[2];
+    const bool has_inlet = inlet_lo && inlet_ext;
     PsdPlan p;
     const size_t need = plan_psd(g, nullptr, &p);
     if (!workspace || workspace_bytes < need)
@@ -323,6 +336,13 @@ int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo
     pmx_geom gc = *g;
     gc.in_s[0] = gc.out_s[0] = g->n[1] * g->n[2];
     gc.in_s[1] = gc.out_s[1] = g->n[2];
+    // A WALL face's layer is solid: a feature of the erosion EDT (distance
+    // to solid) but not of the dilations (distance to the centres), which
+    // see nothing beyond the face (morphological_operators.py:103-172 pads
+    // with solid and dilates from the pore-phase centres only).
+    pmx_geom gd = gc;
+    for (int f = 0; f < 6; ++f)
+        if (gd.bc[f] == PMX_BC_WALL) gd.bc[f] = PMX_BC_END;
 
     // 1. one exact EDT of the image; levels; buckets
     {
@@ -332,10 +352,14 @@ int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo
         o.sx = gc.out_s[0];
         o.sy = gc.out_s[1];
         // only comparisons with the erosion thresholds follow: clamp above the largest
-        if (int e = run_edt(img, &gc, false, o, t_erode[0] < INT64_MAX ? t_erode[0] + 1 : INT64_MAX, p.edt_ws, p.edt_bytes, s)) return e;
+        if (int e = run_edt(img, &gc, false, o, t_erode[0] < INT64_MAX ? t_erode[0] + 1 : INT64_MAX, p.edt_ws, p.edt_bytes, s,
+                            nullptr, tile_rows)) return e;
     }
-    if (hipMemcpyAsync(p.te, t_erode, sizeof(int64_t) * n_levels, hipMemcpyHostToDevice, s) != hipSuccess)
-        return fail(PMX_ERR_HIP, "threshold upload failed");
+    {
+        Thresholds th{};
+        for (int k = 0; k < n_levels; ++k) th.t[k] = t_erode[k];
+        hipLaunchKernelGGL(k_store_thresholds, dim3(1), dim3(256), 0, s, th, n_levels, p.te);
+    }
     {
         ProfScope ps("psd_levels", s);
         hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(256), 0, s, p.counts, kMaxLevels + 1);
@@ -348,19 +372,13 @@ int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo
         hipLaunchKernelGGL(k_zero_map, dim3(gr), dim3(256), 0, s, code_map, d, g->out_s[0], g->out_s[1]);
     }
     if (int e = check_launch("psd setup")) return e;
-    // the bucket bounds are needed on the host to size the per-level launches
-    unsigned int offs[kMaxLevels + 1];
-    if (hipMemcpyAsync(offs, p.offsets, sizeof(unsigned int) * (n_levels + 1), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return fail(PMX_ERR_HIP, "reading level offsets failed");
 
+    const unsigned gu = std::min<unsigned>(gr, 2048u);  // grid-stride over each level's bucket
     for (int lv = 0; lv < n_levels; ++lv) {
-        const unsigned int b = offs[lv], e = offs[lv + 1];
-        if (e > b) {
+        {
             ProfScope ps("psd_union", s);
-            const unsigned gl = grid_for((int64_t)(e - b));
-            hipLaunchKernelGGL(k_activate, dim3(gl), dim3(256), 0, s, p.order, b, e, p.parent);
-            hipLaunchKernelGGL(k_merge_new, dim3(gl), dim3(256), 0, s, p.order, b, e, d, p.parent);
+            hipLaunchKernelGGL(k_activate, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, p.parent);
+            hipLaunchKernelGGL(k_merge_new, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, d, p.parent);
         }
         {
             ProfScope ps("psd_mask", s);
@@ -387,9 +405,35 @@ int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo
         o.psd_code = lv + 1;
         ProfScope ps("psd_dilate", s);
         // skipped on the GPU when the level has no new centre
-        if (int rc = run_edt(p.mask, &gc, true, o, (int64_t)o.t + 1, p.edt_ws, p.edt_bytes, s, p.any + lv)) return rc;
+        if (int rc = run_edt(p.mask, &gd, true, o, (int64_t)o.t + 1, p.edt_ws, p.edt_bytes, s, p.any + lv,
+                             tile_rows)) return rc;
     }
     return PMX_OK;
+}
+
+}  // namespace pmx
+
+extern "C" {
+
+int pmx_psd_workspace_size(const pmx_geom *g, size_t *bytes)
+{
+    pmx::clear_error();
+    if (int e = pmx::validate_geom(g)) return e;
+    if (!bytes) return pmx::fail(PMX_ERR_INVALID, "bytes is NULL");
+    pmx::PsdPlan p;
+    *bytes = pmx::plan_psd(g, nullptr, &p);
+    return PMX_OK;
+}
+
+int pmx_psd_sweep(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, const int64_t *inlet_ext,
+                  int n_levels, const int64_t *t_erode, const int64_t *t_dilate, uint32_t wall_mask,
+                  int32_t *code_map, void *workspace, size_t workspace_bytes, hipStream_t s)
+{
+    using namespace pmx;
+    clear_error();
+    if (int e = check_psd(img, g, inlet_lo, inlet_ext, n_levels, t_erode, t_dilate, code_map)) return e;
+    return run_psd(img, g, inlet_lo, inlet_ext, n_levels, t_erode, t_dilate, wall_mask, code_map, workspace,
+                   workspace_bytes, s, 0);
 }
 
 }  // extern "C"

@@ -70,3 +70,24 @@ def test_edt_dimension_errors():
     from pmmoto_amd.filters import distance
     with pytest.raises(ValueError):
         distance.edt(np.ones(4))  # test_edt.py:369-373
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """ctypes mirrors of pmx_geom and pmx_psd_domain have the C layout."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "pmx.h"\n'
+                   'int main(void) { printf("%zu %zu %zu %zu\\n", sizeof(pmx_geom), sizeof(pmx_psd_domain), '
+                   'offsetof(pmx_psd_domain, code_map), offsetof(pmx_psd_domain, tile_rows)); return 0; }\n')
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include",
+                        str(src), "-o", str(exe)], capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip("header does not compile with gcc here: " + r.stderr[-200:])
+    sizes = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    assert sizes == [C.sizeof(_lib.Geom), C.sizeof(_lib.PsdDomain), _lib.PsdDomain.code_map.offset,
+                     _lib.PsdDomain.tile_rows.offset]

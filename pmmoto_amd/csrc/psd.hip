@@ -200,7 +200,10 @@ __global__ __launch_bounds__(256) void k_flag_inlet(Dims d, int64_t lo0, int64_t
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t z = i % e2, t = i / e2, y = t % e1, x = t / e1;
         const int v = (int)(((lo0 + x) * d.n1 + lo1 + y) * d.n2 + lo2 + z);
-        if (ld(parent + v) != kInactive) flag[find_root(parent, v)] = 1;
+        if (ld(parent + v) != kInactive) {
+            const int r = find_root(parent, v);
+            if (!flag[r]) flag[r] = 1;  // read first: a large component's voxels share one root
+        }
     }
 }
 
@@ -224,7 +227,8 @@ __global__ __launch_bounds__(256) void k_mask(int64_t n, int *__restrict__ paren
         mask[i] = fresh;
         seen |= fresh != 0;
     }
-    if (__any(seen) && (threadIdx.x & 63) == 0) atomicOr(any, 1u);
+    // one lane per wave, and only while the flag still reads 0 (one shared word)
+    if (__any(seen) && (threadIdx.x & 63) == 0 && ld(reinterpret_cast<const int *>(any)) == 0) atomicOr(any, 1u);
 }
 
 // the erosion thresholds travel as a kernel argument (a graph-capturable
@@ -236,6 +240,22 @@ struct Thresholds {
 __global__ void k_store_thresholds(Thresholds th, int n_levels, int64_t *te)
 {
     for (int i = threadIdx.x; i < n_levels; i += blockDim.x) te[i] = th.t[i];
+}
+
+// Without an inlet requirement the eroded sets themselves are the centres:
+// level lv's new centres are the voxels whose erosion level is lv.
+__global__ __launch_bounds__(256) void k_mask_level(int64_t n, const uint8_t *__restrict__ level, int lv,
+                                                    uint8_t *__restrict__ mask, unsigned int *__restrict__ any)
+{
+    bool seen = false;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint8_t m = level[i] == lv;
+        mask[i] = m;
+        seen |= m != 0;
+    }
+    // one lane per wave, and only while the flag still reads 0 (one shared word)
+    if (__any(seen) && (threadIdx.x & 63) == 0 && ld(reinterpret_cast<const int *>(any)) == 0) atomicOr(any, 1u);
 }
 
 __global__ void k_clear_u32(unsigned int *a, int n)
@@ -364,33 +384,34 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
         ProfScope ps("psd_levels", s);
         hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(256), 0, s, p.counts, kMaxLevels + 1);
         hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(256), 0, s, p.any, kMaxLevels);
-        hipLaunchKernelGGL(k_levels, dim3(gr), dim3(256), 0, s, p.d2, n, p.te, n_levels, p.level, p.counts,
+        // few blocks: each adds its per-level totals with one global atomic per level
+        hipLaunchKernelGGL(k_levels, dim3(std::min<unsigned>(gr, 2048u)), dim3(256), 0, s, p.d2, n, p.te, n_levels,
+                           p.level, p.counts,
                            p.parent, p.flag, p.in_c);
-        hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, s, p.counts, n_levels, p.offsets, p.cursor);
-        hipLaunchKernelGGL(k_bucket, dim3(grid_for((n + kBucketItems - 1) / kBucketItems)), dim3(256), 0, s, p.level,
-                           n, n_levels, p.cursor, p.order);
+        if (has_inlet) {  // the union-find absorbs each level's voxels from these buckets
+            hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, s, p.counts, n_levels, p.offsets, p.cursor);
+            hipLaunchKernelGGL(k_bucket, dim3(grid_for((n + kBucketItems - 1) / kBucketItems)), dim3(256), 0, s,
+                               p.level, n, n_levels, p.cursor, p.order);
+        }
         hipLaunchKernelGGL(k_zero_map, dim3(gr), dim3(256), 0, s, code_map, d, g->out_s[0], g->out_s[1]);
     }
     if (int e = check_launch("psd setup")) return e;
 
     const unsigned gu = std::min<unsigned>(gr, 2048u);  // grid-stride over each level's bucket
     for (int lv = 0; lv < n_levels; ++lv) {
-        {
-            ProfScope ps("psd_union", s);
-            hipLaunchKernelGGL(k_activate, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, p.parent);
-            hipLaunchKernelGGL(k_merge_new, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, d, p.parent);
-        }
-        {
+        if (!has_inlet) {
             ProfScope ps("psd_mask", s);
-            if (has_inlet) {
-                const int64_t ni = inlet_ext[0] * inlet_ext[1] * inlet_ext[2];
-                hipLaunchKernelGGL(k_flag_inlet, dim3(grid_for(ni)), dim3(256), 0, s, d, inlet_lo[0], inlet_lo[1],
-                                   inlet_lo[2], inlet_ext[0], inlet_ext[1], inlet_ext[2], p.parent, p.flag);
-            } else {
-                // no inlet requirement: every active root counts
-                hipLaunchKernelGGL(k_flag_inlet, dim3(gr), dim3(256), 0, s, d, (int64_t)0, (int64_t)0, (int64_t)0,
-                                   (int64_t)d.n0, (int64_t)d.n1, (int64_t)d.n2, p.parent, p.flag);
+            hipLaunchKernelGGL(k_mask_level, dim3(gr), dim3(256), 0, s, n, p.level, lv, p.mask, p.any + lv);
+        } else {
+            {
+                ProfScope ps("psd_union", s);
+                hipLaunchKernelGGL(k_activate, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, p.parent);
+                hipLaunchKernelGGL(k_merge_new, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, d, p.parent);
             }
+            ProfScope ps("psd_mask", s);
+            const int64_t ni = inlet_ext[0] * inlet_ext[1] * inlet_ext[2];
+            hipLaunchKernelGGL(k_flag_inlet, dim3(grid_for(ni)), dim3(256), 0, s, d, inlet_lo[0], inlet_lo[1],
+                               inlet_lo[2], inlet_ext[0], inlet_ext[1], inlet_ext[2], p.parent, p.flag);
             hipLaunchKernelGGL(k_mask, dim3(gr), dim3(256), 0, s, n, p.parent, p.flag, p.in_c, p.mask, p.any + lv);
         }
         if (int rc = check_launch("psd level")) return rc;

@@ -254,6 +254,43 @@ int pmx_edt_autotune(const uint8_t *img, const pmx_geom *g, int64_t cap, int inv
                      int32_t *tile_rows, void *workspace, size_t workspace_bytes,
                      hipStream_t stream);
 
+/* Learned-surrogate kernels (no counterpart in the reference, which has no
+ * learned component; for the porosimetry surrogate).  BF16 operands are
+ * raw bfloat16 bits (uint16), accumulation is FP32.
+ * pmx_gemm_bf16: C[M x N] (+)= op(A) . op(B) on the matrix cores, op(A)(m,k)
+ * = A[m*lda+k] or A[k*lda+m] when a_trans, likewise B; k_splits > 1 splits
+ * K over workgroups that add into C atomically (accumulate must be 1).
+ * pmx_im2col3d / pmx_col2im3d: kernel-3, padding-1 patches of a
+ * channels-last [N][D][H][W][C] tensor (in_shape) with stride 1 or 2,
+ * patch rows of ldk >= 27C, k = ((kz*3+ky)*3+kx)*C + c; col2im is the
+ * adjoint (a gather).  pmx_bias_act: y + bias, optional ReLU, to BF16 and/or
+ * FP32.  pmx_relu_backward: dy where act > 0.  pmx_col_sum: out[j] += sum_i
+ * a[i][j].  pmx_avgpool(_backward): mean over the p positions of each
+ * sample.  pmx_gnll_loss: per-sample Gaussian negative log-likelihood of
+ * [curve(q), log variance] and the gradient of its batch mean.  pmx_adam:
+ * one Adam step on FP32 weights, refreshing their BF16 copy. */
+int pmx_gemm_bf16(int64_t m, int n, int k, const uint16_t *a, int64_t lda, int a_trans,
+                  const uint16_t *b, int64_t ldb, int b_trans, float *c, int64_t ldc,
+                  int accumulate, int k_splits, hipStream_t stream);
+int pmx_im2col3d(const uint16_t *x, const int32_t *in_shape, int stride, int64_t ldk,
+                 uint16_t *cols, hipStream_t stream);
+int pmx_col2im3d(const float *dcols, const int32_t *in_shape, int stride, int64_t ldk, float *dx,
+                 hipStream_t stream);
+int pmx_bias_act(const float *y, const float *bias, int64_t m, int c, int relu, uint16_t *out_bf16,
+                 float *out_f32, hipStream_t stream);
+int pmx_relu_backward(const float *dy, const uint16_t *act, int64_t n, float *dx, uint16_t *dx_bf16,
+                      hipStream_t stream);
+int pmx_to_bf16(const float *a, int64_t n, uint16_t *out, hipStream_t stream);
+int pmx_u8_to_bf16(const uint8_t *a, int64_t n, uint16_t *out, hipStream_t stream);
+int pmx_col_sum(const float *a, int64_t m, int n, float *out, hipStream_t stream);
+int pmx_avgpool(const uint16_t *x, int n, int64_t p, int c, float *out_f32, uint16_t *out_bf16,
+                hipStream_t stream);
+int pmx_avgpool_backward(const float *dy, int n, int64_t p, int c, float *dx, hipStream_t stream);
+int pmx_gnll_loss(const float *pred, const float *target, int n, int q, float *loss, float *dpred,
+                  hipStream_t stream);
+int pmx_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2,
+             float eps, int step, uint16_t *w_bf16, hipStream_t stream);
+
 /* Sphere packs.  spheres: n_spheres x (cx, cy, cz, r) float64 in voxel
  * units (voxel (i,j,k) has its centroid at (i+.5, j+.5, k+.5)); a voxel is
  * covered by a sphere when dx*dx+dy*dy (+) dz*dz <= r*r in float64, periodic

@@ -79,6 +79,18 @@ SIGNATURES = {
     "pmx_drain_update": (C.c_int, [P, P, I64, P, P]),
     "pmx_box_op": (C.c_int, [P, P, P, P, C.c_int, U8, P, P]),
     "pmx_max_f32": (C.c_int, [P, I64, P, P]),
+    "pmx_gemm_bf16": (C.c_int, [I64, C.c_int, C.c_int, P, I64, C.c_int, P, I64, C.c_int, P, I64, C.c_int, C.c_int, P]),
+    "pmx_im2col3d": (C.c_int, [P, P, C.c_int, I64, P, P]),
+    "pmx_col2im3d": (C.c_int, [P, P, C.c_int, I64, P, P]),
+    "pmx_bias_act": (C.c_int, [P, P, I64, C.c_int, C.c_int, P, P, P]),
+    "pmx_relu_backward": (C.c_int, [P, P, I64, P, P, P]),
+    "pmx_to_bf16": (C.c_int, [P, I64, P, P]),
+    "pmx_u8_to_bf16": (C.c_int, [P, I64, P, P]),
+    "pmx_col_sum": (C.c_int, [P, I64, C.c_int, P, P]),
+    "pmx_avgpool": (C.c_int, [P, C.c_int, I64, C.c_int, P, P, P]),
+    "pmx_avgpool_backward": (C.c_int, [P, C.c_int, I64, C.c_int, P, P]),
+    "pmx_gnll_loss": (C.c_int, [P, P, C.c_int, C.c_int, P, P, P]),
+    "pmx_adam": (C.c_int, [P, P, P, P, I64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, P, P]),
     "pmx_sphere_first_cover": (C.c_int, [P, I64, P, P, P, P, P]),
     "pmx_sphere_image": (C.c_int, [P, P, I32, P, P]),
 }

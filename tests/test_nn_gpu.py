@@ -1,0 +1,118 @@
+"""GPU checks of the surrogate's kernels against the oracle
+(oracle/nn_oracle.py): the BF16 matrix-core GEMM (all operand layouts,
+split K), im2col/col2im, and the whole network's forward values and
+gradients with the same BF16 rounding points emulated in float64.
+Tolerances are in each assertion (FP32 accumulation vs float64)."""
+
+import numpy as np
+import pytest
+
+from oracle import nn_oracle as N
+
+pytestmark = pytest.mark.gpu
+
+
+def tbf(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(N.bf16(a)).astype(np.float32)).cuda().to(torch.bfloat16)
+
+
+@pytest.mark.parametrize("m,n,k,splits", [(70, 37, 45, 1), (130, 65, 300, 1), (33, 129, 5000, 7), (1, 1, 1, 1)])
+@pytest.mark.parametrize("at,bt", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_gemm_bf16_matches_float64(m, n, k, splits, at, bt):
+    import torch
+
+    from pmmoto_amd.surrogate.model import gemm
+    rng = np.random.default_rng(m * 7 + n + k)
+    a = N.bf16(rng.normal(size=(m, k)))
+    b = N.bf16(rng.normal(size=(k, n)))
+    A = tbf(a.T if at else a)
+    B = tbf(b.T if bt else b)
+    c = torch.zeros((m, n), dtype=torch.float32, device="cuda")
+    print(f"[gemm] m={m} n={n} k={k} at={at} bt={bt} splits={splits}", flush=True)
+    gemm(m, n, k, A, m if at else k, at, B, k if bt else n, bt, c, n, accumulate=splits > 1, k_splits=splits)
+    want = a @ b
+    np.testing.assert_allclose(c.cpu().numpy(), want, rtol=1e-4, atol=1e-4 * np.sqrt(k))
+
+
+@pytest.mark.parametrize("s", [1, 2])
+def test_im2col_col2im_match_oracle(s):
+    import ctypes as C
+
+    import torch
+
+    from pmmoto_amd import _device, _lib
+    rng = np.random.default_rng(5 + s)
+    x = N.bf16(rng.normal(size=(2, 5, 7, 6, 3)))
+    cols_want = N.im2col(x, s)
+    ldk = cols_want.shape[1]
+    cols = torch.empty(cols_want.shape, dtype=torch.bfloat16, device="cuda")
+    shape = (C.c_int32 * 5)(*x.shape)
+    _lib.check(_lib.lib().pmx_im2col3d(_device.ptr(tbf(x)), shape, s, ldk, _device.ptr(cols), _device.stream()))
+    np.testing.assert_array_equal(cols.float().cpu().numpy(), cols_want)
+    g = rng.normal(size=cols_want.shape).astype(np.float32)
+    dx = torch.empty(x.shape, dtype=torch.float32, device="cuda")
+    _lib.check(_lib.lib().pmx_col2im3d(_device.ptr(torch.from_numpy(g).cuda()), shape, s, ldk, _device.ptr(dx),
+                                       _device.stream()))
+    np.testing.assert_allclose(dx.cpu().numpy(), N.col2im(g.astype(np.float64), x.shape, s), rtol=1e-5, atol=1e-5)
+
+
+def emulated(net_state, channels, x, target, q):
+    """float64 forward/backward with the GPU's BF16 rounding points."""
+    W = {k: v for k, v in net_state.items()}
+    wb = {k: N.bf16(v) if k.endswith(".w") else v for k, v in W.items()}
+    h = x.astype(np.float64)[..., None]
+    acts, colss = [h], []
+    for i in range(len(channels)):
+        y = N.conv_fwd(h, wb[f"conv{i}.w"], W[f"conv{i}.b"], 2)
+        h = N.bf16(np.maximum(y, 0.0))
+        acts.append(h)
+    pooled = N.bf16(h.reshape(h.shape[0], -1, h.shape[-1]).mean(axis=1))
+    z1 = N.bf16(np.maximum(pooled @ wb["fc1.w"].T + W["fc1.b"], 0.0))
+    pred = z1 @ wb["fc2.w"].T + W["fc2.b"]
+    _, _, dpred = N.gnll(pred, target)
+    g = {}
+    dpb = N.bf16(dpred)
+    g["fc2.w"] = dpb.T @ z1
+    g["fc2.b"] = dpred.sum(axis=0)
+    dz1 = dpb @ wb["fc2.w"]
+    dy1 = dz1 * (z1 > 0)
+    dy1b = N.bf16(dy1)
+    g["fc1.w"] = dy1b.T @ pooled
+    g["fc1.b"] = dy1.sum(axis=0)
+    dpool = dy1b @ wb["fc1.w"]
+    p = int(np.prod(h.shape[1:4]))
+    dh = np.repeat(dpool[:, None, :] / p, p, axis=1).reshape(h.shape)
+    for i in range(len(channels) - 1, -1, -1):
+        dz = dh * (acts[i + 1] > 0)
+        dzb = N.bf16(dz)
+        cols = N.im2col(acts[i], 2)
+        g[f"conv{i}.w"] = (dzb.reshape(-1, channels[i]).T @ cols).reshape(W[f"conv{i}.w"].shape)
+        g[f"conv{i}.b"] = dz.reshape(-1, channels[i]).sum(axis=0)
+        if i > 0:
+            dh = N.col2im(dzb.reshape(-1, channels[i]) @ wb[f"conv{i}.w"].reshape(channels[i], -1), acts[i].shape, 2)
+    return pred, g
+
+
+def test_network_forward_and_gradients_match_oracle():
+    import torch
+
+    from pmmoto_amd.surrogate import SurrogateNet
+    rng = np.random.default_rng(11)
+    channels, q = (4, 6, 8), 5
+    net = SurrogateNet(size=12, channels=channels, hidden=7, q=q, seed=3)
+    for name, w, wb in net.params:  # non-zero biases exercise every term
+        if name.endswith(".b"):
+            w.copy_(torch.from_numpy(rng.normal(size=w.shape).astype(np.float32) * 0.1))
+    x = (rng.random(size=(3, 12, 12, 12)) < 0.6).astype(np.uint8)
+    target = rng.uniform(size=(3, q)).astype(np.float32)
+    pred, cache = net.forward(torch.from_numpy(x).cuda())
+    loss, dpred = net.loss_and_grad(pred, torch.from_numpy(target).cuda())
+    grads = net.backward(cache, dpred)
+    want_pred, want_g = emulated(net.state(), channels, x, target.astype(np.float64), q)
+    np.testing.assert_allclose(pred.cpu().numpy(), want_pred, rtol=2e-3, atol=2e-3)
+    for (name, _, _), gg in zip(net.params, grads):
+        got = gg.cpu().numpy().astype(np.float64)
+        want = want_g[name]
+        scale = max(np.abs(want).max(), 1e-6)
+        np.testing.assert_allclose(got, want, rtol=2e-2, atol=2e-2 * scale, err_msg=name)

@@ -49,7 +49,7 @@ def _splits(k_total: int) -> int:
 class SurrogateNet:
     """channels: output channels of the six stride-2 blocks; q: curve points."""
 
-    def __init__(self, size=128, channels=(24, 48, 64, 96, 128, 192), hidden=256, q=32, seed=0, device="cuda"):
+    def __init__(self, size=128, channels=(8, 16, 48, 96, 160, 256), hidden=256, q=32, seed=0, device="cuda"):
         _device.require_gpu()
         self.size, self.channels, self.hidden, self.q = size, tuple(channels), hidden, q
         self.strides = (2,) * len(channels)

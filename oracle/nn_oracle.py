@@ -99,20 +99,28 @@ def conv_direct(x, w, b, s):
 
 
 def gnll(pred, target):
-    """Per-sample loss sum_j (p_j - t_j)^2 / (2 e^s) + q s / 2 (s = log variance,
-    clamped to [-12, 12]), its batch mean and d(mean)/d(pred)."""
+    """Per-sample loss se/2 + (se e^-s + q s)/2 (se = sum_j (p_j - t_j)^2, s =
+    log variance clamped to [-12, 12]), its batch mean and the gradient the
+    training uses: the curve by squared error, s by the Gaussian negative
+    log-likelihood with se held fixed and the clamp passed straight through."""
     q = target.shape[1]
     s = np.clip(pred[:, q], -12.0, 12.0)
     inv = np.exp(-s)
     e = pred[:, :q] - target
     se = (e * e).sum(axis=1)
-    loss = 0.5 * se * inv + 0.5 * q * s
+    loss = 0.5 * se + 0.5 * (se * inv + q * s)
     n = pred.shape[0]
     d = np.zeros_like(pred)
-    d[:, :q] = e * inv[:, None] / n
-    inside = (pred[:, q] > -12.0) & (pred[:, q] < 12.0)
-    d[:, q] = np.where(inside, (-0.5 * se * inv + 0.5 * q) / n, 0.0)
+    d[:, :q] = e / n
+    d[:, q] = (-0.5 * se * inv + 0.5 * q) / n
     return loss, loss.mean(), d
+
+
+def quadratic(pred, target):
+    """0.5 mean_b sum_j (pred - target)^2 over all outputs and its gradient
+    (a loss whose gradient is exact, for finite-difference checks)."""
+    e = pred - target
+    return 0.5 * (e * e).sum() / pred.shape[0], e / pred.shape[0]
 
 
 def network_forward(params, x, strides):

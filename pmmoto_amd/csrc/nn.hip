@@ -293,8 +293,13 @@ __global__ __launch_bounds__(256) void k_avgpool_back(const float *__restrict__ 
     }
 }
 
-// Gaussian negative log-likelihood per sample: pred = (curve[q], s = log var)
-// L = sum_j (y_j - t_j)^2 / (2 e^s) + q s / 2, averaged over the batch
+// Per-sample loss of pred = (curve[q], s = log variance) against the exact
+// curve t: the curve learns by squared error, the variance head by the
+// Gaussian negative log-likelihood of that (fixed) error,
+//   L = se / 2 + (se e^-s + q s) / 2,   se = sum_j (y_j - t_j)^2,
+// with s clamped to [-12, 12] in the value and passed straight through in
+// the gradient.  dpred is the gradient of the batch mean (se held fixed in
+// the variance term).
 __global__ __launch_bounds__(64) void k_gnll(const float *__restrict__ pred, const float *__restrict__ target, int n,
                                              int q, float *__restrict__ loss, float *__restrict__ dpred)
 {
@@ -307,10 +312,10 @@ __global__ __launch_bounds__(64) void k_gnll(const float *__restrict__ pred, con
     for (int j = 0; j < q; ++j) {
         const float e = p[j] - target[(int64_t)b * q + j];
         se += e * e;
-        dpred[(int64_t)b * (q + 1) + j] = e * inv / (float)n;
+        dpred[(int64_t)b * (q + 1) + j] = e / (float)n;
     }
-    dpred[(int64_t)b * (q + 1) + q] = (p[q] > -12.f && p[q] < 12.f) ? (-0.5f * se * inv + 0.5f * q) / (float)n : 0.f;
-    loss[b] = 0.5f * se * inv + 0.5f * q * s;
+    dpred[(int64_t)b * (q + 1) + q] = (-0.5f * se * inv + 0.5f * q) / (float)n;
+    loss[b] = 0.5f * se + 0.5f * (se * inv + q * s);
 }
 
 // Adam on FP32 master weights, refreshing their BF16 copy

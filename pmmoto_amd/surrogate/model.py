@@ -3,7 +3,9 @@
 Input: a cubic uint8 pore image (1 = pore).  Six convolution blocks
 (kernel 3, padding 1, stride 2, bias, ReLU) halve the grid each time; a
 global average pool, a dense layer with ReLU and a dense output layer give
-q saturation values and a log variance (the network's own uncertainty).
+q saturation values and a log variance (the network's own uncertainty;
+its gradient trains only its own output row, the trunk learns from the
+curve error alone).
 All arithmetic runs in the library's kernels (BF16 operands on the matrix
 cores, FP32 accumulation, FP32 master weights with Adam); torch only holds
 the buffers.
@@ -159,8 +161,12 @@ class SurrogateNet:
         gb2 = torch.zeros(self.q + 1, dtype=torch.float32, device=dev)
         _lib.check(_L().pmx_col_sum(_p(dpred), b, self.q + 1, _p(gb2), _device.stream()))
         grads["fc2.w"], grads["fc2.b"] = gw2, gb2
+        # the uncertainty output reads the features but does not shape them:
+        # only the curve error flows back into the trunk
+        trunk_bf = dpred_bf.clone()
+        trunk_bf[:, self.q] = 0
         dz1 = torch.empty((b, self.hidden), dtype=torch.float32, device=dev)
-        gemm(b, self.hidden, self.q + 1, dpred_bf, self.q + 1, 0, w2[2], self.hidden, 0, dz1, self.hidden)
+        gemm(b, self.hidden, self.q + 1, trunk_bf, self.q + 1, 0, w2[2], self.hidden, 0, dz1, self.hidden)
         # fc1 (ReLU)
         dy1 = torch.empty_like(dz1)
         dy1_bf = torch.empty((b, self.hidden), dtype=BF16, device=dev)

@@ -36,14 +36,14 @@ def test_network_gradients_match_finite_differences():
     params = tiny_params(rng)
     strides = (2, 2)
     x = rng.normal(size=(3, 6, 7, 5, 1))
-    t = rng.uniform(size=(3, 3))
+    t = rng.uniform(size=(3, 4))
 
     def loss_of(p):
         pred, _ = N.network_forward(p, x, strides)
-        return N.gnll(pred, t)[1]
+        return N.quadratic(pred, t)[0]
 
     pred, cache = N.network_forward(params, x, strides)
-    _, _, dpred = N.gnll(pred, t)
+    _, dpred = N.quadratic(pred, t)
     grads = N.network_backward(params, cache, dpred, strides)
     eps = 1e-6
     checks = [("conv", 0, 0), ("conv", 1, 0), ("conv", 0, 1), ("conv", 1, 1), ("fc1", None, 0), ("fc1", None, 1),
@@ -60,6 +60,27 @@ def test_network_gradients_match_finite_differences():
             lm = loss_of(params)
             arr[idx] = old
             assert np.isclose(g[idx], (lp - lm) / (2 * eps), rtol=1e-5, atol=1e-8), (name, li, which, idx)
+
+
+def test_training_loss_gradient():
+    """The curve part of the training gradient is the squared error's; the
+    variance part is d/ds of the likelihood term with se fixed."""
+    rng = np.random.default_rng(3)
+    pred = rng.normal(size=(4, 6))
+    t = rng.uniform(size=(4, 5))
+    _, _, d = N.gnll(pred, t)
+    eps = 1e-6
+    for b in range(4):
+        for j in range(5):
+            p2 = pred.copy()
+            p2[b, j] += eps
+            lp = 0.5 * ((p2[:, :5] - t) ** 2).sum() / 4
+            p2[b, j] -= 2 * eps
+            lm = 0.5 * ((p2[:, :5] - t) ** 2).sum() / 4
+            assert np.isclose(d[b, j], (lp - lm) / (2 * eps), rtol=1e-6)
+        se = ((pred[b, :5] - t[b]) ** 2).sum()
+        f = lambda s: 0.5 * (se * np.exp(-s) + 5 * s) / 4  # noqa: E731
+        assert np.isclose(d[b, 5], (f(pred[b, 5] + eps) - f(pred[b, 5] - eps)) / (2 * eps), rtol=1e-6)
 
 
 def test_bf16_rounding_is_nearest_even():

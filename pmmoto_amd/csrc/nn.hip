@@ -336,6 +336,195 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ w, const float
     }
 }
 
+// ---------------------------------------------------------------------------
+// inference convolutions (kernel 3, padding 1, stride s; bias + ReLU fused)
+// ---------------------------------------------------------------------------
+
+// First layer, one input channel: a thread per output voxel reads its 27
+// input voxels (the u8 image) and writes co (<= 32) BF16 channels.
+
+// First layer on the matrix cores: out^T (channels x positions) = W (channels
+// x K) . P^T (K x positions) with K = 48 slots, slot 4r + kx for tap row r =
+// 3kz + ky (slot 4r + 3 and rows 9..11 hold zero), so in each 16-deep MFMA
+// step lane l (position l % 16, slots 4(l/16)..+3) needs the three bytes of
+// one input row.  Wave = 64 output columns of one output row (four MFMA
+// column blocks); block = four output rows.  FAST (stride 2, w % 4 == 0,
+// 4-byte aligned image): a row's three bytes come from two aligned dwords.
+constexpr int kFirstRows = 4, kFirstCols = 64;
+
+__device__ __forceinline__ uint32_t bf16_pair(uint32_t lo, uint32_t hi)
+{
+    return (__float_as_uint((float)lo) >> 16) | (__float_as_uint((float)hi) & 0xffff0000u);
+}
+
+template <int MB, bool FAST>
+__global__ __launch_bounds__(256) void k_conv_first(const uint8_t *__restrict__ x, ConvDims cd, int gy, int gx, int co,
+                                                    const uint16_t *__restrict__ w, const float *__restrict__ bias,
+                                                    uint16_t *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63, g = lane >> 4, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    unsigned r = blockIdx.x;
+    const int bx = (int)(r % (unsigned)gx);
+    r /= (unsigned)gx;
+    const int by = (int)(r % (unsigned)gy);
+    r /= (unsigned)gy;
+    const int oz = (int)(r % (unsigned)cd.od);
+    const int b = (int)(r / (unsigned)cd.od);
+    const int oy = by * kFirstRows + wv;
+    if (oy >= cd.oh) return;
+    // weights: A[ch = l % 16 + 16 mb][slot 16 j + 4 g + kx]
+    bf16x4 wa[MB][3];
+    float bs[MB][4];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+        const int ch = 16 * mb + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int row = 4 * j + g;
+            short e[4] = {0, 0, 0, 0};
+            if (ch < co && row < 9) {
+#pragma unroll
+                for (int kx = 0; kx < 3; ++kx) e[kx] = (short)w[ch * 27 + row * 3 + kx];
+            }
+            wa[mb][j] = bf16x4{e[0], e[1], e[2], e[3]};
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = 16 * mb + 4 * g + v;
+            bs[mb][v] = c < co ? bias[c] : 0.f;
+        }
+    }
+    const int64_t out_row = (((int64_t)b * cd.od + oz) * cd.oh + oy) * cd.ow;
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+        const int ox = bx * kFirstCols + 16 * nb + (lane & 15);
+        bf16x4 tp[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int row = 4 * j + g;
+            const int kz = row / 3, ky = row - 3 * kz;
+            const int iz = oz * cd.s - 1 + kz, iy = oy * cd.s - 1 + ky;
+            const bool in = row < 9 && ox < cd.ow && iz >= 0 && iz < cd.d && iy >= 0 && iy < cd.h;
+            const uint8_t *src = x + (((int64_t)b * cd.d + (in ? iz : 0)) * cd.h + (in ? iy : 0)) * cd.w;
+            uint32_t b0 = 0u, b1 = 0u, b2 = 0u;
+            if (FAST) {
+                // bytes 2ox-1 .. 2ox+1 lie in the dwords at a and a + 4
+                const int x0 = 2 * ox - 1;
+                const int a = x0 & ~3;
+                uint32_t lo = 0u, hi = 0u;
+                if (in && a >= 0) lo = *reinterpret_cast<const uint32_t *>(src + a);
+                if (in && a + 4 < cd.w) hi = *reinterpret_cast<const uint32_t *>(src + a + 4);
+                const uint32_t v = __builtin_amdgcn_alignbyte(hi, lo, (uint32_t)(x0 & 3));
+                b0 = v & 0xffu;
+                b1 = (v >> 8) & 0xffu;
+                b2 = (v >> 16) & 0xffu;
+            } else if (in) {
+                const int x0 = ox * cd.s - 1;
+                if (x0 >= 0) b0 = src[x0];
+                if (x0 + 1 < cd.w) b1 = src[x0 + 1];
+                if (x0 + 2 < cd.w) b2 = src[x0 + 2];
+            }
+            const uint32_t p01 = bf16_pair(b0, b1), p2 = __float_as_uint((float)b2) >> 16;
+            tp[j] = bf16x4{(short)(p01 & 0xffffu), (short)(p01 >> 16), (short)p2, (short)0};
+        }
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int j = 0; j < 3; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[mb][j], tp[j], acc, 0, 0, 0);
+            const int c0 = 16 * mb + 4 * g;
+            if (ox < cd.ow && c0 < co) {
+                const uint32_t q0 = f2bf(fmaxf(acc[0] + bs[mb][0], 0.f)) | ((uint32_t)f2bf(fmaxf(acc[1] + bs[mb][1], 0.f)) << 16);
+                const uint32_t q1 = f2bf(fmaxf(acc[2] + bs[mb][2], 0.f)) | ((uint32_t)f2bf(fmaxf(acc[3] + bs[mb][3], 0.f)) << 16);
+                *reinterpret_cast<uint2 *>(out + (out_row + ox) * co + c0) = make_uint2(q0, q1);
+            }
+        }
+    }
+}
+
+// Implicit-GEMM convolution for c % 8 == 0: out[m][co] = ReLU(bias + sum_k
+// patch(m, k) w[co][k]).  A workgroup owns 64 output positions x NT output
+// channels; each K step of 32 gathers one 16-byte run of 8 channels per
+// thread straight from the input (k = tap * c + ci, a run never crosses a
+// tap), so no patch matrix is written.  Wave w computes rows 16w .. 16w+15.
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv_igemm(const uint16_t *__restrict__ x, ConvDims cd, int co,
+                                                    const uint16_t *__restrict__ w, const float *__restrict__ bias,
+                                                    int relu, uint16_t *__restrict__ out)
+{
+    __shared__ __align__(16) uint16_t As[64][kTK + 8];
+    __shared__ __align__(16) uint16_t Bs[NT][kTK + 8];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t M = (int64_t)cd.n * cd.od * cd.oh * cd.ow;
+    const int K = 27 * cd.c;
+    const int64_t m0 = (int64_t)blockIdx.x * 64;
+    const int n0 = blockIdx.y * NT;
+    // this thread's staged row and 8-channel run within each K step
+    const int ar = tid >> 2, ac = (tid & 3) * 8;
+    const int64_t am = m0 + ar;
+    int ab = 0, az = 0, ay = 0, ax = 0;
+    {
+        int64_t r = am < M ? am : 0;
+        ax = (int)(r % cd.ow) * cd.s - 1; r /= cd.ow;
+        ay = (int)(r % cd.oh) * cd.s - 1; r /= cd.oh;
+        az = (int)(r % cd.od) * cd.s - 1;
+        ab = (int)(r / cd.od);
+    }
+    floatx4 acc[NT / 16];
+#pragma unroll
+    for (int j = 0; j < NT / 16; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += kTK) {
+        {
+            const int k = k0 + ac;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (am < M && k < K) {
+                const int tap = k / cd.c, ci = k - tap * cd.c;
+                const int iz = az + tap / 9, iy = ay + (tap / 3) % 3, ix = ax + tap % 3;
+                if (iz >= 0 && iz < cd.d && iy >= 0 && iy < cd.h && ix >= 0 && ix < cd.w)
+                    v = *reinterpret_cast<const uint4 *>(
+                        x + ((((int64_t)ab * cd.d + iz) * cd.h + iy) * cd.w + ix) * cd.c + ci);
+            }
+            *reinterpret_cast<uint4 *>(&As[ar][ac]) = v;
+        }
+        for (int e = tid; e < NT * (kTK / 8); e += 256) {
+            const int c = e / (kTK / 8), kc = (e % (kTK / 8)) * 8;
+            const int gn = n0 + c, gk = k0 + kc;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gn < co && gk < K) v = *reinterpret_cast<const uint4 *>(w + (int64_t)gn * K + gk);
+            *reinterpret_cast<uint4 *>(&Bs[c][kc]) = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kTK; kk += 16) {
+            const int kq = kk + 4 * (lane >> 4);
+            const uint16_t *pa = &As[16 * wv + (lane & 15)][kq];
+            const bf16x4 a = bf16x4{(short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3]};
+#pragma unroll
+            for (int j = 0; j < NT / 16; ++j) {
+                const uint16_t *pb = &Bs[16 * j + (lane & 15)][kq];
+                const bf16x4 b = bf16x4{(short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3]};
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < NT / 16; ++j) {
+        const int col = n0 + 16 * j + (lane & 15);
+        if (col >= co) continue;
+        const float bb = bias[col];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = m0 + 16 * wv + 4 * (lane >> 4) + v;
+            if (row < M) {
+                float y = acc[j][v] + bb;
+                if (relu) y = fmaxf(y, 0.f);
+                out[row * co + col] = f2bf(y);
+            }
+        }
+    }
+}
+
 unsigned grid1(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384)); }
 
 int conv_dims(const int32_t *in_shape, int stride, int64_t ldk, ConvDims *cd)
@@ -485,6 +674,66 @@ int pmx_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, 
     hipLaunchKernelGGL(pmx::k_adam, dim3(pmx::grid1(n)), dim3(256), 0, s, w, g, m, v, n, lr, b1, b2, eps, c1, c2,
                        w_bf16);
     return pmx::check_launch("adam");
+}
+
+int pmx_conv3d_first(const uint8_t *x, const int32_t *in_shape, int stride, int co, const uint16_t *w_bf16,
+                     const float *bias, uint16_t *out, hipStream_t s)
+{
+    pmx::clear_error();
+    pmx::ConvDims cd;
+    if (int e = pmx::conv_dims(in_shape, stride, 27 * (int64_t)in_shape[4], &cd)) return e;
+    if (cd.c != 1) return pmx::fail(PMX_ERR_INVALID, "the first-layer kernel takes one input channel");
+    if (co < 4 || co > 32 || co % 4) return pmx::fail(PMX_ERR_INVALID, "first-layer co must be a multiple of 4 in [4, 32]");
+    if (!x || !w_bf16 || !bias || !out) return pmx::fail(PMX_ERR_INVALID, "NULL buffer");
+    if ((uintptr_t)out % 8) return pmx::fail(PMX_ERR_INVALID, "out must be 8-byte aligned");
+    const int gy = (cd.oh + pmx::kFirstRows - 1) / pmx::kFirstRows, gx = (cd.ow + pmx::kFirstCols - 1) / pmx::kFirstCols;
+    const int64_t blocks = (int64_t)cd.n * cd.od * gy * gx;
+    if (blocks > UINT32_MAX / 2) return pmx::fail(PMX_ERR_INVALID, "input too large for one launch");
+    const bool fast = stride == 2 && cd.w % 4 == 0 && (uintptr_t)x % 4 == 0;
+    const dim3 grid((unsigned)blocks);
+#define PMX_FIRST(MB)                                                                                              \
+    do {                                                                                                           \
+        if (fast)                                                                                                  \
+            hipLaunchKernelGGL((pmx::k_conv_first<MB, true>), grid, dim3(256), 0, s, x, cd, gy, gx, co, w_bf16, bias, \
+                               out);                                                                               \
+        else                                                                                                       \
+            hipLaunchKernelGGL((pmx::k_conv_first<MB, false>), grid, dim3(256), 0, s, x, cd, gy, gx, co, w_bf16,    \
+                               bias, out);                                                                         \
+    } while (0)
+    if (co <= 16)
+        PMX_FIRST(1);
+    else
+        PMX_FIRST(2);
+#undef PMX_FIRST
+    return pmx::check_launch("conv3d_first");
+}
+
+int pmx_conv3d_bf16(const uint16_t *x, const int32_t *in_shape, int stride, int co, const uint16_t *w_bf16,
+                    const float *bias, int relu, uint16_t *out, hipStream_t s)
+{
+    pmx::clear_error();
+    pmx::ConvDims cd;
+    if (int e = pmx::conv_dims(in_shape, stride, 27 * (int64_t)in_shape[4], &cd)) return e;
+    if (cd.c % 8) return pmx::fail(PMX_ERR_INVALID, "input channels must be a multiple of 8");
+    if (co < 1) return pmx::fail(PMX_ERR_INVALID, "co must be positive");
+    if (!x || !w_bf16 || !bias || !out) return pmx::fail(PMX_ERR_INVALID, "NULL buffer");
+    if ((uintptr_t)x % 16 || (uintptr_t)w_bf16 % 16) return pmx::fail(PMX_ERR_INVALID, "x and w must be 16-byte aligned");
+    const int64_t M = (int64_t)cd.n * cd.od * cd.oh * cd.ow;
+    const int64_t mt = (M + 63) / 64;
+    if (mt > INT32_MAX) return pmx::fail(PMX_ERR_INVALID, "too many output positions");
+    if (co <= 16)
+        hipLaunchKernelGGL(pmx::k_conv_igemm<16>, dim3((unsigned)mt, 1), dim3(256), 0, s, x, cd, co, w_bf16, bias, relu,
+                           out);
+    else if (co <= 32)
+        hipLaunchKernelGGL(pmx::k_conv_igemm<32>, dim3((unsigned)mt, 1), dim3(256), 0, s, x, cd, co, w_bf16, bias, relu,
+                           out);
+    else if (co <= 48)
+        hipLaunchKernelGGL(pmx::k_conv_igemm<48>, dim3((unsigned)mt, 1), dim3(256), 0, s, x, cd, co, w_bf16, bias, relu,
+                           out);
+    else
+        hipLaunchKernelGGL(pmx::k_conv_igemm<64>, dim3((unsigned)mt, (unsigned)((co + 63) / 64)), dim3(256), 0, s, x, cd,
+                           co, w_bf16, bias, relu, out);
+    return pmx::check_launch("conv3d_bf16");
 }
 
 }  // extern "C"

@@ -128,20 +128,7 @@ class SurrogateNet:
                 del cols
                 cache["acts"] = [a]
             h = a
-        p = shape[1] * shape[2] * shape[3]
-        cf = shape[4]
-        pooled = torch.empty((b, cf), dtype=BF16, device=x_u8.device)
-        _lib.check(_L().pmx_avgpool(_p(h), b, p, cf, None, _p(pooled), _device.stream()))
-        w1, b1 = self.param("fc1.w"), self.param("fc1.b")[1]
-        y1 = torch.empty((b, self.hidden), dtype=torch.float32, device=x_u8.device)
-        gemm(b, self.hidden, cf, pooled, cf, 0, w1[2], cf, 1, y1, self.hidden)
-        z1 = torch.empty((b, self.hidden), dtype=BF16, device=x_u8.device)
-        _lib.check(_L().pmx_bias_act(_p(y1), _p(b1), b, self.hidden, 1, _p(z1), None, _device.stream()))
-        w2, b2 = self.param("fc2.w"), self.param("fc2.b")[1]
-        y2 = torch.empty((b, self.q + 1), dtype=torch.float32, device=x_u8.device)
-        gemm(b, self.q + 1, self.hidden, z1, self.hidden, 0, w2[2], self.hidden, 1, y2, self.q + 1)
-        pred = torch.empty((b, self.q + 1), dtype=torch.float32, device=x_u8.device)
-        _lib.check(_L().pmx_bias_act(_p(y2), _p(b2), b, self.q + 1, 0, None, _p(pred), _device.stream()))
+        pred, pooled, z1, p, cf = self._head(h, shape)
         cache.update(pooled=pooled, z1=z1, p=p, cf=cf, b=b)
         return pred, cache
 
@@ -233,13 +220,71 @@ class SurrogateNet:
         self.adam_step(grads, lr)
         return loss
 
+    # -- inference ---------------------------------------------------------
+    def infer(self, x_u8: torch.Tensor):
+        """Forward pass without the training cache: the convolutions run as
+        fused kernels (direct first layer on the uint8 image, implicit GEMM
+        on the matrix cores after it) with bias, ReLU and BF16 rounding in
+        their epilogue.  Same arithmetic as forward() up to FP32 summation
+        order.  Returns (pred [B, q+1] FP32, pooled, z1, p, cf)."""
+        b = x_u8.shape[0]
+        dev = x_u8.device
+        shape = [b, x_u8.shape[1], x_u8.shape[2], x_u8.shape[3], 1]
+        h = None
+        for i, co in enumerate(self.channels):
+            w = self.param(f"conv{i}.w")
+            bias = self.param(f"conv{i}.b")[1]
+            s = self.strides[i]
+            od, oh, ow = [(v - 1) // s + 1 for v in shape[1:4]]
+            a = torch.empty((b, od, oh, ow, co), dtype=BF16, device=dev)
+            dims = (C.c_int32 * 5)(*shape)
+            if i == 0 and co % 4 == 0 and co <= 32:
+                _lib.check(_L().pmx_conv3d_first(_p(x_u8), dims, s, co, _p(w[2]), _p(bias), _p(a), _device.stream()))
+            else:
+                if h is None:
+                    h = torch.empty(tuple(shape), dtype=BF16, device=dev)
+                    _lib.check(_L().pmx_u8_to_bf16(_p(x_u8), x_u8.numel(), _p(h), _device.stream()))
+                if shape[4] % 8 == 0:
+                    _lib.check(_L().pmx_conv3d_bf16(_p(h), dims, s, co, _p(w[2]), _p(bias), 1, _p(a),
+                                                    _device.stream()))
+                else:
+                    kk = 27 * shape[4]
+                    m = b * od * oh * ow
+                    cols = torch.empty((m, kk), dtype=BF16, device=dev)
+                    _lib.check(_L().pmx_im2col3d(_p(h), dims, s, kk, _p(cols), _device.stream()))
+                    y = torch.empty((m, co), dtype=torch.float32, device=dev)
+                    gemm(m, co, kk, cols, kk, 0, w[2], kk, 1, y, co)
+                    del cols
+                    _lib.check(_L().pmx_bias_act(_p(y), _p(bias), m, co, 1, _p(a), None, _device.stream()))
+                    del y
+            h = a
+            shape = [b, od, oh, ow, co]
+        return self._head(h, shape)
+
+    def _head(self, h, shape):
+        b, dev = shape[0], h.device
+        p = shape[1] * shape[2] * shape[3]
+        cf = shape[4]
+        pooled = torch.empty((b, cf), dtype=BF16, device=dev)
+        _lib.check(_L().pmx_avgpool(_p(h), b, p, cf, None, _p(pooled), _device.stream()))
+        w1, b1 = self.param("fc1.w"), self.param("fc1.b")[1]
+        y1 = torch.empty((b, self.hidden), dtype=torch.float32, device=dev)
+        gemm(b, self.hidden, cf, pooled, cf, 0, w1[2], cf, 1, y1, self.hidden)
+        z1 = torch.empty((b, self.hidden), dtype=BF16, device=dev)
+        _lib.check(_L().pmx_bias_act(_p(y1), _p(b1), b, self.hidden, 1, _p(z1), None, _device.stream()))
+        w2, b2 = self.param("fc2.w"), self.param("fc2.b")[1]
+        y2 = torch.empty((b, self.q + 1), dtype=torch.float32, device=dev)
+        gemm(b, self.q + 1, self.hidden, z1, self.hidden, 0, w2[2], self.hidden, 1, y2, self.q + 1)
+        pred = torch.empty((b, self.q + 1), dtype=torch.float32, device=dev)
+        _lib.check(_L().pmx_bias_act(_p(y2), _p(b2), b, self.q + 1, 0, None, _p(pred), _device.stream()))
+        return pred, pooled, z1, p, cf
+
     @torch.no_grad()
     def predict(self, x_u8: torch.Tensor, batch=64):
         """(curve [B, q] clipped to [0, 1], sigma [B]) in FP32."""
         outs = []
         for i in range(0, x_u8.shape[0], batch):
-            pred, _ = self.forward(x_u8[i:i + batch], keep=False)
-            outs.append(pred)
+            outs.append(self.infer(x_u8[i:i + batch].contiguous())[0])
         pred = torch.cat(outs)
         curve = pred[:, :self.q].clamp(0.0, 1.0)
         sigma = torch.exp(0.5 * pred[:, self.q].clamp(-12.0, 12.0))

@@ -48,13 +48,79 @@ def test_im2col_col2im_match_oracle(s):
     ldk = cols_want.shape[1]
     cols = torch.empty(cols_want.shape, dtype=torch.bfloat16, device="cuda")
     shape = (C.c_int32 * 5)(*x.shape)
-    _lib.check(_lib.lib().pmx_im2col3d(_device.ptr(tbf(x)), shape, s, ldk, _device.ptr(cols), _device.stream()))
+    xd = tbf(x)
+    _lib.check(_lib.lib().pmx_im2col3d(_device.ptr(xd), shape, s, ldk, _device.ptr(cols), _device.stream()))
     np.testing.assert_array_equal(cols.float().cpu().numpy(), cols_want)
     g = rng.normal(size=cols_want.shape).astype(np.float32)
     dx = torch.empty(x.shape, dtype=torch.float32, device="cuda")
-    _lib.check(_lib.lib().pmx_col2im3d(_device.ptr(torch.from_numpy(g).cuda()), shape, s, ldk, _device.ptr(dx),
+    gd = torch.from_numpy(g).cuda()
+    _lib.check(_lib.lib().pmx_col2im3d(_device.ptr(gd), shape, s, ldk, _device.ptr(dx),
                                        _device.stream()))
     np.testing.assert_allclose(dx.cpu().numpy(), N.col2im(g.astype(np.float64), x.shape, s), rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("co,s,shape", [(8, 2, (2, 9, 7, 10)), (16, 1, (1, 5, 6, 7)), (32, 2, (3, 4, 4, 4)),
+                                        (8, 2, (2, 6, 19, 16)), (16, 2, (1, 3, 5, 136)), (8, 1, (1, 3, 4, 130)),
+                                        (12, 2, (1, 4, 6, 8)), (24, 1, (2, 3, 3, 5))])
+def test_first_layer_conv_matches_oracle(co, s, shape):
+    """uint8 first-layer convolution on the matrix cores, bias + ReLU + BF16 fused."""
+    import ctypes as C
+
+    import torch
+
+    from pmmoto_amd import _device, _lib
+    rng = np.random.default_rng(co + s)
+    x = (rng.random(shape) < 0.5).astype(np.uint8)
+    w = N.bf16(rng.normal(size=(co, 3, 3, 3, 1)))
+    b = rng.normal(size=co).astype(np.float32).astype(np.float64)
+    want = N.bf16(np.maximum(N.conv_direct(x[..., None].astype(np.float64), w, b, s), 0.0))
+    out = torch.empty(want.shape, dtype=torch.bfloat16, device="cuda")
+    dims = (C.c_int32 * 5)(*shape, 1)
+    print(f"[conv_first] co={co} s={s} shape={shape}", flush=True)
+    # operands held in names: a temporary freed before the launch would hand its memory to the next one
+    xd, wd, bd = torch.from_numpy(x).cuda(), tbf(w), torch.from_numpy(b.astype(np.float32)).cuda()
+    _lib.check(_lib.lib().pmx_conv3d_first(_device.ptr(xd), dims, s, co, _device.ptr(wd), _device.ptr(bd),
+                                           _device.ptr(out), _device.stream()))
+    # FP32 vs float64 sums may round to neighbouring BF16 values
+    np.testing.assert_allclose(out.float().cpu().numpy(), want, rtol=2 ** -7, atol=1e-6)
+
+
+@pytest.mark.parametrize("c,co,s,shape,relu", [(8, 16, 2, (2, 9, 7, 10), 1), (16, 48, 2, (1, 6, 5, 8), 1),
+                                               (8, 24, 1, (1, 5, 4, 6), 0), (48, 96, 2, (2, 4, 4, 4), 1),
+                                               (16, 130, 1, (1, 3, 3, 3), 1)])
+def test_implicit_gemm_conv_matches_oracle(c, co, s, shape, relu):
+    """Matrix-core convolution without a patch matrix, bias/ReLU/BF16 fused."""
+    import ctypes as C
+
+    import torch
+
+    from pmmoto_amd import _device, _lib
+    rng = np.random.default_rng(c * 3 + co)
+    x = N.bf16(rng.normal(size=shape + (c,)))
+    w = N.bf16(rng.normal(size=(co, 3, 3, 3, c)) / np.sqrt(27 * c))
+    b = rng.normal(size=co).astype(np.float32).astype(np.float64)
+    y = N.conv_fwd(x, w, b, s)
+    want = N.bf16(np.maximum(y, 0.0) if relu else y)
+    out = torch.empty(want.shape, dtype=torch.bfloat16, device="cuda")
+    dims = (C.c_int32 * 5)(*shape, c)
+    print(f"[conv_igemm] c={c} co={co} s={s} shape={shape}", flush=True)
+    xd, wd, bd = tbf(x), tbf(w), torch.from_numpy(b.astype(np.float32)).cuda()
+    _lib.check(_lib.lib().pmx_conv3d_bf16(_device.ptr(xd), dims, s, co, _device.ptr(wd), _device.ptr(bd), relu,
+                                          _device.ptr(out), _device.stream()))
+    np.testing.assert_allclose(out.float().cpu().numpy(), want, rtol=2 ** -7, atol=1e-5)
+
+
+def test_fused_inference_matches_training_forward():
+    import torch
+
+    from pmmoto_amd.surrogate.model import SurrogateNet
+    net = SurrogateNet(size=24, channels=(8, 16, 24), hidden=32, q=6, seed=3)
+    rng = np.random.default_rng(11)
+    x = torch.from_numpy((rng.random((5, 24, 24, 24)) < 0.45).astype(np.uint8)).cuda()
+    print("[infer] size=24 channels=(8,16,24)", flush=True)
+    want, _ = net.forward(x, keep=False)
+    got = net.infer(x)[0]
+    np.testing.assert_allclose(got.cpu().numpy(), want.cpu().numpy(), rtol=1e-2, atol=1e-2)
 
 
 def emulated(net_state, channels, x, target, q):

@@ -273,17 +273,21 @@ int pmx_edt_autotune(const uint8_t *img, const pmx_geom *g, int64_t cap, int inv
  * padding 1, stride 1 or 2) with the bias, the ReLU and the BF16 rounding
  * fused, writing channels-last BF16 [N][od][oh][ow][co] without a patch
  * matrix; w is BF16 [co][3][3][3][C] and bias FP32 [co], both on the device.
- * _first reads a one-channel uint8 image (co a multiple of 4 up to 32;
- * ReLU always; out 8-byte aligned);
- * _bf16 reads a BF16 tensor whose C is a multiple of 8 (x, w 16-byte
- * aligned) on the matrix cores. */
+ * in_shape is the tensor's own shape; with in_pad / out_pad = 1 the buffer
+ * is [N][D+2][H+2][W+2][C] holding the tensor in its interior (the caller
+ * keeps the one-voxel border zero; the kernels write only the interior).
+ * _first reads a one-channel uint8 image (co a multiple of 4 up to 32; ReLU
+ * always; out 8-byte aligned); _bf16 reads a BF16 tensor whose C is a
+ * multiple of 8 (x, w, out 8-byte aligned) on the matrix cores.  One sample
+ * of the input must be below 2 GiB. */
 int pmx_gemm_bf16(int64_t m, int n, int k, const uint16_t *a, int64_t lda, int a_trans,
                   const uint16_t *b, int64_t ldb, int b_trans, float *c, int64_t ldc,
                   int accumulate, int k_splits, hipStream_t stream);
 int pmx_conv3d_first(const uint8_t *x, const int32_t *in_shape, int stride, int co, const uint16_t *w_bf16,
-                     const float *bias, uint16_t *out, hipStream_t stream);
-int pmx_conv3d_bf16(const uint16_t *x, const int32_t *in_shape, int stride, int co, const uint16_t *w_bf16,
-                    const float *bias, int relu, uint16_t *out, hipStream_t stream);
+                     const float *bias, uint16_t *out, int out_pad, hipStream_t stream);
+int pmx_conv3d_bf16(const uint16_t *x, const int32_t *in_shape, int in_pad, int stride, int co,
+                    const uint16_t *w_bf16, const float *bias, int relu, uint16_t *out, int out_pad,
+                    hipStream_t stream);
 int pmx_im2col3d(const uint16_t *x, const int32_t *in_shape, int stride, int64_t ldk,
                  uint16_t *cols, hipStream_t stream);
 int pmx_col2im3d(const float *dcols, const int32_t *in_shape, int stride, int64_t ldk, float *dx,

@@ -80,8 +80,8 @@ SIGNATURES = {
     "pmx_box_op": (C.c_int, [P, P, P, P, C.c_int, U8, P, P]),
     "pmx_max_f32": (C.c_int, [P, I64, P, P]),
     "pmx_gemm_bf16": (C.c_int, [I64, C.c_int, C.c_int, P, I64, C.c_int, P, I64, C.c_int, P, I64, C.c_int, C.c_int, P]),
-    "pmx_conv3d_first": (C.c_int, [P, P, C.c_int, C.c_int, P, P, P, P]),
-    "pmx_conv3d_bf16": (C.c_int, [P, P, C.c_int, C.c_int, P, P, C.c_int, P, P]),
+    "pmx_conv3d_first": (C.c_int, [P, P, C.c_int, C.c_int, P, P, P, C.c_int, P]),
+    "pmx_conv3d_bf16": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, P, C.c_int, P, C.c_int, P]),
     "pmx_im2col3d": (C.c_int, [P, P, C.c_int, I64, P, P]),
     "pmx_col2im3d": (C.c_int, [P, P, C.c_int, I64, P, P]),
     "pmx_bias_act": (C.c_int, [P, P, I64, C.c_int, C.c_int, P, P, P]),

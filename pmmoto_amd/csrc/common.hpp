@@ -15,6 +15,15 @@ namespace pmx {
 constexpr int kWave = 64;
 constexpr int kXcds = 8;  // MI355X: workgroups go round-robin over 8 XCDs, each with its own L2
 
+// XCD-aware block order: workgroup b runs on XCD b % 8, so give each XCD a
+// contiguous range of logical blocks (neighbouring tiles share halo data
+// in that XCD's L2).  A bijection on [0, nb).
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb)
+{
+    const int64_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds, i = b / kXcds;
+    return x * q + min(x, r) + i;
+}
+
 // Thread-local last-error text behind pmx_last_error().
 void set_error(const char *fmt, ...);
 void clear_error();

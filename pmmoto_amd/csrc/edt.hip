@@ -441,14 +441,6 @@ __device__ __forceinline__ int map_row(int r, int n, int bc_lo, int bc_hi)
     return kRowNone;
 }
 
-// XCD-aware block order: workgroup b runs on XCD b % 8, so give each XCD a
-// contiguous range of logical blocks (neighbouring tiles share halo rows
-// in that XCD's L2).
-__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb)
-{
-    const int64_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds, i = b / kXcds;
-    return x * q + min(x, r) + i;
-}
 
 struct WinGeom {
     int n;          // rows per line (the pass axis)

@@ -337,41 +337,62 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ w, const float
 }
 
 // ---------------------------------------------------------------------------
-// inference convolutions (kernel 3, padding 1, stride s; bias + ReLU fused)
+// inference convolutions (kernel 3, padding 1, stride s; bias + ReLU + BF16
+// rounding fused).  Operands are read through buffer resources: a load whose
+// offset is kOob returns zero, so padding taps and tail lanes need no
+// branch.  A launch covers a run of samples whose input is below 2 GiB.
+// Activations may carry a one-voxel zero border ("padded"): the
+// [n][d+2][h+2][w+2][c] buffer holds the tensor in its interior.
 // ---------------------------------------------------------------------------
 
-// First layer, one input channel: a thread per output voxel reads its 27
-// input voxels (the u8 image) and writes co (<= 32) BF16 channels.
+constexpr uint32_t kOob = 0x80000000u;  // >= every buffer's size
+constexpr int kBufFlags = 0x00020000;   // raw buffer, 32-bit data format (CDNA)
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer(const void *p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, kBufFlags);
+}
+
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+
+// two floats to a bf16 pair (round to nearest even; one v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t{lo, hi}), bf16x2_t));
+}
+
+// max(v, 0) on the bits: a negative float is a negative int (no NaN canonicalisation)
+__device__ __forceinline__ float relu_bits(float v) { return __int_as_float(max(__float_as_int(v), 0)); }
 
 // First layer on the matrix cores: out^T (channels x positions) = W (channels
 // x K) . P^T (K x positions) with K = 48 slots, slot 4r + kx for tap row r =
 // 3kz + ky (slot 4r + 3 and rows 9..11 hold zero), so in each 16-deep MFMA
 // step lane l (position l % 16, slots 4(l/16)..+3) needs the three bytes of
-// one input row.  Wave = 64 output columns of one output row (four MFMA
-// column blocks); block = four output rows.  FAST (stride 2, w % 4 == 0,
-// 4-byte aligned image): a row's three bytes come from two aligned dwords.
-constexpr int kFirstRows = 4, kFirstCols = 64;
-
-__device__ __forceinline__ uint32_t bf16_pair(uint32_t lo, uint32_t hi)
-{
-    return (__float_as_uint((float)lo) >> 16) | (__float_as_uint((float)hi) & 0xffff0000u);
-}
+// one input row.  Wave = kFirstWaveRows output rows x 64 output columns, the
+// loads of row r + 1 in flight while row r is multiplied; block = four waves.
+// FAST (stride 2, w % 4 == 0): a row's three bytes come from two aligned
+// dwords, else from three byte loads.
+constexpr int kFirstWaveRows = 4, kFirstRows = 4 * kFirstWaveRows, kFirstCols = 64;
 
 template <int MB, bool FAST>
-__global__ __launch_bounds__(256) void k_conv_first(const uint8_t *__restrict__ x, ConvDims cd, int gy, int gx, int co,
-                                                    const uint16_t *__restrict__ w, const float *__restrict__ bias,
-                                                    uint16_t *__restrict__ out)
+__global__ __launch_bounds__(256) void k_conv_first(const uint8_t *__restrict__ x, uint32_t x_bytes, ConvDims cd,
+                                                    int gy, int gx, int co, const uint16_t *__restrict__ w,
+                                                    const float *__restrict__ bias, uint16_t *__restrict__ out,
+                                                    int out_pad)
 {
     const int lane = threadIdx.x & 63, g = lane >> 4, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    unsigned r = blockIdx.x;
+    unsigned r = (unsigned)xcd_tile(blockIdx.x, gridDim.x);
     const int bx = (int)(r % (unsigned)gx);
     r /= (unsigned)gx;
     const int by = (int)(r % (unsigned)gy);
     r /= (unsigned)gy;
     const int oz = (int)(r % (unsigned)cd.od);
     const int b = (int)(r / (unsigned)cd.od);
-    const int oy = by * kFirstRows + wv;
-    if (oy >= cd.oh) return;
+    const int oy0 = by * kFirstRows + wv * kFirstWaveRows;
+    if (oy0 >= cd.oh) return;
+    const __amdgpu_buffer_rsrc_t xb = buffer(x, x_bytes);
     // weights: A[ch = l % 16 + 16 mb][slot 16 j + 4 g + kx]
     bf16x4 wa[MB][3];
     float bs[MB][4];
@@ -394,132 +415,268 @@ __global__ __launch_bounds__(256) void k_conv_first(const uint8_t *__restrict__ 
             bs[mb][v] = c < co ? bias[c] : 0.f;
         }
     }
-    const int64_t out_row = (((int64_t)b * cd.od + oz) * cd.oh + oy) * cd.ow;
+    // this lane's tap rows for the three MFMA steps: input plane/row offsets
+    int tz[3], ty[3];
 #pragma unroll
-    for (int nb = 0; nb < 4; ++nb) {
-        const int ox = bx * kFirstCols + 16 * nb + (lane & 15);
-        bf16x4 tp[3];
+    for (int j = 0; j < 3; ++j) {
+        const int row = 4 * j + g;
+        tz[j] = row < 9 ? row / 3 : 1 << 20;  // rows 9..11: always out of range
+        ty[j] = row - 3 * (row / 3);
+    }
+    const int plane = cd.h * cd.w;
+    const int sample_off = b * cd.d * plane;
+    const int ox_l = bx * kFirstCols + (lane & 15);
+    const int x0_l = ox_l * cd.s - 1;
+    const uint32_t shift = (uint32_t)(x0_l & 3);
+    // raw operands of the output row in flight: 4 column blocks x 3 tap rows.  One
+    // offset per tap row; the column blocks sit at constant distances (16 outputs =
+    // 16 s bytes).  Loads past the row's end only feed outputs >= ow (never stored)
+    // or the unused fourth byte; past the buffer they read zero.
+    uint32_t rlo[4][3], rhi[4][3];
+    const bool left = x0_l < 0;  // the lane of output column 0: its first byte is padding
+    auto issue = [&](int rr) {
+        const int oy = oy0 + rr;
+        const bool row_ok = rr < kFirstWaveRows && oy < cd.oh;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const int row = 4 * j + g;
-            const int kz = row / 3, ky = row - 3 * kz;
-            const int iz = oz * cd.s - 1 + kz, iy = oy * cd.s - 1 + ky;
-            const bool in = row < 9 && ox < cd.ow && iz >= 0 && iz < cd.d && iy >= 0 && iy < cd.h;
-            const uint8_t *src = x + (((int64_t)b * cd.d + (in ? iz : 0)) * cd.h + (in ? iy : 0)) * cd.w;
-            uint32_t b0 = 0u, b1 = 0u, b2 = 0u;
+            const int iz = oz * cd.s - 1 + tz[j], iy = oy * cd.s - 1 + ty[j];
+            const bool in = row_ok && iz >= 0 && iz < cd.d && iy >= 0 && iy < cd.h;
+            const int rowoff = sample_off + iz * plane + iy * cd.w;
             if (FAST) {
-                // bytes 2ox-1 .. 2ox+1 lie in the dwords at a and a + 4
-                const int x0 = 2 * ox - 1;
-                const int a = x0 & ~3;
-                uint32_t lo = 0u, hi = 0u;
-                if (in && a >= 0) lo = *reinterpret_cast<const uint32_t *>(src + a);
-                if (in && a + 4 < cd.w) hi = *reinterpret_cast<const uint32_t *>(src + a + 4);
-                const uint32_t v = __builtin_amdgcn_alignbyte(hi, lo, (uint32_t)(x0 & 3));
-                b0 = v & 0xffu;
-                b1 = (v >> 8) & 0xffu;
-                b2 = (v >> 16) & 0xffu;
-            } else if (in) {
-                const int x0 = ox * cd.s - 1;
-                if (x0 >= 0) b0 = src[x0];
-                if (x0 + 1 < cd.w) b1 = src[x0 + 1];
-                if (x0 + 2 < cd.w) b2 = src[x0 + 2];
+                // bytes 2ox-1 .. 2ox+1 lie in the dwords at a, a + 4 (a = -4 for output column 0:
+                // that lane loads at 0 and moves the dword up, the padding byte comes from zero;
+                // no offset may wrap, the two loads can be fused into one range-checked load)
+                const uint32_t vo = in ? (uint32_t)(rowoff + (x0_l & ~3)) : kOob;  // blocks 1..3: >= 28
+                const uint32_t v0 = in ? (uint32_t)(rowoff + max(x0_l & ~3, 0)) : kOob;
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb) {
+                    const uint32_t o = nb == 0 ? v0 : vo + 32 * nb;
+                    rlo[nb][j] = __builtin_amdgcn_raw_buffer_load_b32(xb, o, 0, 0);
+                    rhi[nb][j] = __builtin_amdgcn_raw_buffer_load_b32(xb, o + 4, 0, 0);
+                }
+                rhi[0][j] = left ? rlo[0][j] : rhi[0][j];
+                rlo[0][j] = left ? 0u : rlo[0][j];
+            } else {
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb) {
+                    const int x0 = (ox_l + 16 * nb) * cd.s - 1;
+                    const bool ok = in && ox_l + 16 * nb < cd.ow;
+                    const uint32_t o = (uint32_t)(rowoff + x0);
+                    const uint32_t b0 = __builtin_amdgcn_raw_buffer_load_b8(xb, ok && x0 >= 0 ? o : kOob, 0, 0);
+                    const uint32_t b1 = __builtin_amdgcn_raw_buffer_load_b8(xb, ok && x0 + 1 < cd.w ? o + 1 : kOob, 0, 0);
+                    const uint32_t b2 = __builtin_amdgcn_raw_buffer_load_b8(xb, ok && x0 + 2 < cd.w ? o + 2 : kOob, 0, 0);
+                    rlo[nb][j] = b0 | (b1 << 8) | (b2 << 16);
+                    rhi[nb][j] = 0u;
+                }
             }
-            const uint32_t p01 = bf16_pair(b0, b1), p2 = __float_as_uint((float)b2) >> 16;
-            tp[j] = bf16x4{(short)(p01 & 0xffffu), (short)(p01 >> 16), (short)p2, (short)0};
         }
+    };
+    const int odp = cd.od + 2 * out_pad, ohp = cd.oh + 2 * out_pad, owp = cd.ow + 2 * out_pad;
+    issue(0);
+    for (int rr = 0; rr < kFirstWaveRows; ++rr) {
+        const int oy = oy0 + rr;
+        bf16x4 tp[4][3];
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) {
-            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int nb = 0; nb < 4; ++nb) {
 #pragma unroll
-            for (int j = 0; j < 3; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[mb][j], tp[j], acc, 0, 0, 0);
-            const int c0 = 16 * mb + 4 * g;
-            if (ox < cd.ow && c0 < co) {
-                const uint32_t q0 = f2bf(fmaxf(acc[0] + bs[mb][0], 0.f)) | ((uint32_t)f2bf(fmaxf(acc[1] + bs[mb][1], 0.f)) << 16);
-                const uint32_t q1 = f2bf(fmaxf(acc[2] + bs[mb][2], 0.f)) | ((uint32_t)f2bf(fmaxf(acc[3] + bs[mb][3], 0.f)) << 16);
-                *reinterpret_cast<uint2 *>(out + (out_row + ox) * co + c0) = make_uint2(q0, q1);
+            for (int j = 0; j < 3; ++j) {
+                const uint32_t v = FAST ? __builtin_amdgcn_alignbyte(rhi[nb][j], rlo[nb][j], shift) : rlo[nb][j];
+                // byte -> float -> bf16 (exact for 0..255); perm takes the high halves
+                const float f0 = (float)(v & 0xffu), f1 = (float)((v >> 8) & 0xffu), f2 = (float)((v >> 16) & 0xffu);
+                const uint32_t p01 = __builtin_amdgcn_perm(__float_as_uint(f1), __float_as_uint(f0), 0x07060302u);
+                const uint32_t p2 = __float_as_uint(f2) >> 16;
+                tp[nb][j] = bf16x4{(short)(p01 & 0xffffu), (short)(p01 >> 16), (short)p2, (short)0};
+            }
+        }
+        issue(rr + 1);
+        if (oy >= cd.oh) continue;  // wave-uniform
+        uint16_t *orow = out + ((((int64_t)b * odp + oz + out_pad) * ohp + oy + out_pad) * owp + out_pad + ox_l) * co;
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            const int ox = ox_l + 16 * nb;
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                floatx4 acc = {bs[mb][0], bs[mb][1], bs[mb][2], bs[mb][3]};  // the bias rides in the accumulator
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[mb][j], tp[nb][j], acc, 0, 0, 0);
+                const int c0 = 16 * mb + 4 * g;
+                if (ox < cd.ow && c0 < co) {
+                    const uint32_t q0 = pack_bf16(relu_bits(acc[0]), relu_bits(acc[1]));
+                    const uint32_t q1 = pack_bf16(relu_bits(acc[2]), relu_bits(acc[3]));
+                    *reinterpret_cast<uint2 *>(orow + 16 * nb * co + c0) = make_uint2(q0, q1);
+                }
             }
         }
     }
 }
 
-// Implicit-GEMM convolution for c % 8 == 0: out[m][co] = ReLU(bias + sum_k
-// patch(m, k) w[co][k]).  A workgroup owns 64 output positions x NT output
-// channels; each K step of 32 gathers one 16-byte run of 8 channels per
-// thread straight from the input (k = tap * c + ci, a run never crosses a
-// tap), so no patch matrix is written.  Wave w computes rows 16w .. 16w+15.
-template <int NT>
-__global__ __launch_bounds__(256) void k_conv_igemm(const uint16_t *__restrict__ x, ConvDims cd, int co,
-                                                    const uint16_t *__restrict__ w, const float *__restrict__ bias,
-                                                    int relu, uint16_t *__restrict__ out)
+// Implicit-GEMM convolution for c % 8 == 0 on the matrix cores, computed as
+// out^T (channels x positions) = W (channels x K) . P^T (K x positions) with
+// k = tap * c + ci.  Wave = 16 P output positions x 16 CB output channels;
+// no LDS, no barrier: in each 16-deep step lane l gathers the four channels
+// k = 16 j + 4 (l / 16) .. + 3 of its position (one 8-byte load: a run never
+// crosses a tap) and the same k of its weight row, and after the loop holds
+// four consecutive channels of one position, stored as one 8-byte write.
+// The tap part of each step's offset is wave-uniform (with PAD a scalar
+// offset); the lanes whose four channels start in the next tap (c % 16 != 0)
+// add the difference.  PAD: the input is padded, every tap in range, no
+// masks.
+template <int P, int CB, bool PAD>
+__global__ __launch_bounds__(256) void k_conv_igemm(const uint16_t *__restrict__ x, uint32_t x_bytes, ConvDims cd,
+                                                    int co, const uint16_t *__restrict__ w,
+                                                    const float *__restrict__ bias, int relu,
+                                                    uint16_t *__restrict__ out, int out_pad)
 {
-    __shared__ __align__(16) uint16_t As[64][kTK + 8];
-    __shared__ __align__(16) uint16_t Bs[NT][kTK + 8];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int64_t M = (int64_t)cd.n * cd.od * cd.oh * cd.ow;
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const int M = cd.n * cd.od * cd.oh * cd.ow;  // < 2^31 (checked on the host)
     const int K = 27 * cd.c;
-    const int64_t m0 = (int64_t)blockIdx.x * 64;
-    const int n0 = blockIdx.y * NT;
-    // this thread's staged row and 8-channel run within each K step
-    const int ar = tid >> 2, ac = (tid & 3) * 8;
-    const int64_t am = m0 + ar;
-    int ab = 0, az = 0, ay = 0, ax = 0;
-    {
-        int64_t r = am < M ? am : 0;
-        ax = (int)(r % cd.ow) * cd.s - 1; r /= cd.ow;
-        ay = (int)(r % cd.oh) * cd.s - 1; r /= cd.oh;
-        az = (int)(r % cd.od) * cd.s - 1;
-        ab = (int)(r / cd.od);
+    const int p0 = ((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6)) * 16 * P;
+    if (p0 >= M) return;
+    const int n0 = blockIdx.y * 16 * CB;
+    const __amdgpu_buffer_rsrc_t xb = buffer(x, x_bytes);
+    const __amdgpu_buffer_rsrc_t wb = buffer(w, (uint32_t)co * (uint32_t)K * 2u);
+    const int dp = cd.d + (PAD ? 2 : 0), hp = cd.h + (PAD ? 2 : 0), wp = cd.w + (PAD ? 2 : 0);
+    const int sz = hp * wp * cd.c, sy = wp * cd.c;  // element strides of z and y
+    // per position block: the byte offset of tap (0,0,0) + this lane's 4g channels,
+    // and (unpadded input) the in-range taps per axis
+    uint32_t base[P];
+    uint32_t msk[P];  // bits 0-2: z taps, 3-5: y taps, 6-8: x taps
+#pragma unroll
+    for (int pb = 0; pb < P; ++pb) {
+        const int m = p0 + 16 * pb + (lane & 15);
+        int rq = m < M ? m : 0;
+        const int ox = rq % cd.ow; rq /= cd.ow;
+        const int oy = rq % cd.oh; rq /= cd.oh;
+        const int oz = rq % cd.od;
+        const int b = rq / cd.od;
+        const int iz = oz * cd.s - 1, iy = oy * cd.s - 1, ix = ox * cd.s - 1;
+        uint32_t mk = 0u;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            mk |= (uint32_t)(iz + t >= 0 && iz + t < cd.d) << t;
+            mk |= (uint32_t)(iy + t >= 0 && iy + t < cd.h) << (3 + t);
+            mk |= (uint32_t)(ix + t >= 0 && ix + t < cd.w) << (6 + t);
+        }
+        msk[pb] = PAD ? 0x1ffu : mk;
+        const int o = PAD ? (((b * dp + iz + 1) * hp + iy + 1) * wp + ix + 1) * cd.c
+                          : (((b * cd.d + iz) * cd.h + iy) * cd.w + ix) * cd.c;
+        base[pb] = m < M ? (uint32_t)(2 * (o + 4 * g)) : kOob;
     }
-    floatx4 acc[NT / 16];
+    uint32_t wbase[CB];
 #pragma unroll
-    for (int j = 0; j < NT / 16; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += kTK) {
-        {
-            const int k = k0 + ac;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (am < M && k < K) {
-                const int tap = k / cd.c, ci = k - tap * cd.c;
-                const int iz = az + tap / 9, iy = ay + (tap / 3) % 3, ix = ax + tap % 3;
-                if (iz >= 0 && iz < cd.d && iy >= 0 && iy < cd.h && ix >= 0 && ix < cd.w)
-                    v = *reinterpret_cast<const uint4 *>(
-                        x + ((((int64_t)ab * cd.d + iz) * cd.h + iy) * cd.w + ix) * cd.c + ci);
-            }
-            *reinterpret_cast<uint4 *>(&As[ar][ac]) = v;
-        }
-        for (int e = tid; e < NT * (kTK / 8); e += 256) {
-            const int c = e / (kTK / 8), kc = (e % (kTK / 8)) * 8;
-            const int gn = n0 + c, gk = k0 + kc;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (gn < co && gk < K) v = *reinterpret_cast<const uint4 *>(w + (int64_t)gn * K + gk);
-            *reinterpret_cast<uint4 *>(&Bs[c][kc]) = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < kTK; kk += 16) {
-            const int kq = kk + 4 * (lane >> 4);
-            const uint16_t *pa = &As[16 * wv + (lane & 15)][kq];
-            const bf16x4 a = bf16x4{(short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3]};
-#pragma unroll
-            for (int j = 0; j < NT / 16; ++j) {
-                const uint16_t *pb = &Bs[16 * j + (lane & 15)][kq];
-                const bf16x4 b = bf16x4{(short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3]};
-                acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
-            }
-        }
-        __syncthreads();
+    for (int cb = 0; cb < CB; ++cb) {
+        const int n = n0 + 16 * cb + (lane & 15);
+        wbase[cb] = n < co ? (uint32_t)(2 * (n * K + 4 * g)) : kOob;
     }
+    floatx4 acc[P][CB];
 #pragma unroll
-    for (int j = 0; j < NT / 16; ++j) {
-        const int col = n0 + 16 * j + (lane & 15);
-        if (col >= co) continue;
-        const float bb = bias[col];
+    for (int pb = 0; pb < P; ++pb)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int64_t row = m0 + 16 * wv + 4 * (lane >> 4) + v;
-            if (row < M) {
-                float y = acc[j][v] + bb;
-                if (relu) y = fmaxf(y, 0.f);
-                out[row * co + col] = f2bf(y);
+        for (int cb = 0; cb < CB; ++cb) acc[pb][cb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int steps = (K + 15) / 16;
+    // software pipeline: the raw operands of steps j + 2, j + 3 are loaded before the
+    // products of steps j, j + 1
+    constexpr int S = 2;
+    u32x2 rp[S][P], rw[S][CB];
+    auto tap_off = [&](int tap) {  // element offset of tap (dz, dy, dx); tap < 27
+        const int dz = (tap * 29) >> 8, rem = tap - 9 * dz;
+        const int dy = (rem * 11) >> 5, dx = rem - 3 * dy;
+        return dz * sz + dy * sy + dx * cd.c;
+    };
+    auto tap_bits = [&](int tap) {
+        const int dz = (tap * 29) >> 8, rem = tap - 9 * dz;
+        const int dy = (rem * 11) >> 5, dx = rem - 3 * dy;
+        return (1u << dz) | (8u << dy) | (64u << dx);
+    };
+    // (tap, first channel) of the next step to issue, advanced by 16 channels per step
+    // (wave-uniform, no division)
+    int t_nx = 0, ci_nx = 0;
+    auto issue = [&](int j0) {
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            const int ku = 16 * (j0 + q);  // wave-uniform
+            const int tap = t_nx, ci0 = ci_nx;
+            ci_nx += 16;
+            while (ci_nx >= cd.c) {
+                ci_nx -= cd.c;
+                ++t_nx;
+            }
+            const bool kin = ku + 4 * g < K;
+            const bool cross = ci0 + 4 * g >= cd.c;  // this lane's channels start in tap + 1
+            const int t_lo = tap < 27 ? tap : 0, t_hi = tap + 1 < 27 ? tap + 1 : 0;
+            const int off_lo = tap_off(t_lo) + ci0, off_hi = tap_off(t_hi) + ci0 - cd.c;
+            const uint32_t need = cross ? tap_bits(t_hi) : tap_bits(t_lo);
+            const uint32_t dlane = cross ? (uint32_t)(2 * (off_hi - off_lo)) : 0u;
+#pragma unroll
+            for (int pb = 0; pb < P; ++pb) {
+                // the range check covers the vector offset only: without a border the tap
+                // (0,0,0) offset of an edge position is negative, so the whole sum goes there
+                const bool ok = kin && (PAD || (msk[pb] & need) == need);
+                if (PAD)
+                    rp[q][pb] = __builtin_amdgcn_raw_buffer_load_b64(xb, ok ? base[pb] + dlane : kOob, 2 * off_lo, 0);
+                else
+                    rp[q][pb] = __builtin_amdgcn_raw_buffer_load_b64(
+                        xb, ok ? base[pb] + dlane + (uint32_t)(2 * off_lo) : kOob, 0, 0);
+            }
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb)
+                rw[q][cb] = __builtin_amdgcn_raw_buffer_load_b64(wb, kin ? wbase[cb] : kOob, 2 * ku, 0);
+        }
+    };
+    auto frag = [](u32x2 v) {
+        return bf16x4{(short)(v.x & 0xffffu), (short)(v.x >> 16), (short)(v.y & 0xffffu), (short)(v.y >> 16)};
+    };
+    issue(0);
+    for (int j = 0; j < steps; j += S) {
+        bf16x4 pf[S][P], wf[S][CB];
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+#pragma unroll
+            for (int pb = 0; pb < P; ++pb) pf[q][pb] = frag(rp[q][pb]);
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb) wf[q][cb] = frag(rw[q][cb]);
+        }
+        issue(j + S);
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+#pragma unroll
+            for (int pb = 0; pb < P; ++pb)
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb)
+                    acc[pb][cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wf[q][cb], pf[q][pb], acc[pb][cb], 0, 0, 0);
+    }
+    const int odp = cd.od + 2 * out_pad, ohp = cd.oh + 2 * out_pad, owp = cd.ow + 2 * out_pad;
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) {
+        const int c0 = n0 + 16 * cb + 4 * g;
+        if (c0 >= co) continue;
+        float bb[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) bb[v] = c0 + v < co ? bias[c0 + v] : 0.f;
+#pragma unroll
+        for (int pb = 0; pb < P; ++pb) {
+            const int m = p0 + 16 * pb + (lane & 15);
+            if (m >= M) continue;
+            int rq = m;
+            const int ox = rq % cd.ow; rq /= cd.ow;
+            const int oy = rq % cd.oh; rq /= cd.oh;
+            const int oz = rq % cd.od;
+            const int b = rq / cd.od;
+            const int64_t pos = (((int64_t)b * odp + oz + out_pad) * ohp + oy + out_pad) * owp + ox + out_pad;
+            float y[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                y[v] = acc[pb][cb][v] + bb[v];
+                if (relu) y[v] = relu_bits(y[v]);
+            }
+            uint16_t *o = out + pos * co + c0;
+            if ((co & 3) == 0) {
+                *reinterpret_cast<uint2 *>(o) = make_uint2(pack_bf16(y[0], y[1]), pack_bf16(y[2], y[3]));
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (c0 + v < co) o[v] = f2bf(y[v]);
             }
         }
     }
@@ -677,63 +834,111 @@ int pmx_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, 
 }
 
 int pmx_conv3d_first(const uint8_t *x, const int32_t *in_shape, int stride, int co, const uint16_t *w_bf16,
-                     const float *bias, uint16_t *out, hipStream_t s)
+                     const float *bias, uint16_t *out, int out_pad, hipStream_t s)
 {
     pmx::clear_error();
     pmx::ConvDims cd;
     if (int e = pmx::conv_dims(in_shape, stride, 27 * (int64_t)in_shape[4], &cd)) return e;
     if (cd.c != 1) return pmx::fail(PMX_ERR_INVALID, "the first-layer kernel takes one input channel");
     if (co < 4 || co > 32 || co % 4) return pmx::fail(PMX_ERR_INVALID, "first-layer co must be a multiple of 4 in [4, 32]");
+    if (out_pad != 0 && out_pad != 1) return pmx::fail(PMX_ERR_INVALID, "out_pad must be 0 or 1");
     if (!x || !w_bf16 || !bias || !out) return pmx::fail(PMX_ERR_INVALID, "NULL buffer");
     if ((uintptr_t)out % 8) return pmx::fail(PMX_ERR_INVALID, "out must be 8-byte aligned");
+    const int64_t sample = (int64_t)cd.d * cd.h * cd.w;
+    if (sample >= pmx::kOob) return pmx::fail(PMX_ERR_INVALID, "one sample exceeds 2 GiB");
+    const int64_t chunk = std::max<int64_t>(1, (int64_t)(pmx::kOob - 1) / sample);
+    const bool fast = stride == 2 && cd.w % 4 == 0;
     const int gy = (cd.oh + pmx::kFirstRows - 1) / pmx::kFirstRows, gx = (cd.ow + pmx::kFirstCols - 1) / pmx::kFirstCols;
-    const int64_t blocks = (int64_t)cd.n * cd.od * gy * gx;
-    if (blocks > UINT32_MAX / 2) return pmx::fail(PMX_ERR_INVALID, "input too large for one launch");
-    const bool fast = stride == 2 && cd.w % 4 == 0 && (uintptr_t)x % 4 == 0;
-    const dim3 grid((unsigned)blocks);
-#define PMX_FIRST(MB)                                                                                              \
-    do {                                                                                                           \
-        if (fast)                                                                                                  \
-            hipLaunchKernelGGL((pmx::k_conv_first<MB, true>), grid, dim3(256), 0, s, x, cd, gy, gx, co, w_bf16, bias, \
-                               out);                                                                               \
-        else                                                                                                       \
-            hipLaunchKernelGGL((pmx::k_conv_first<MB, false>), grid, dim3(256), 0, s, x, cd, gy, gx, co, w_bf16,    \
-                               bias, out);                                                                         \
-    } while (0)
-    if (co <= 16)
-        PMX_FIRST(1);
-    else
-        PMX_FIRST(2);
+    const int64_t out_sample = (int64_t)(cd.od + 2 * out_pad) * (cd.oh + 2 * out_pad) * (cd.ow + 2 * out_pad) * co;
+    for (int64_t n0 = 0; n0 < cd.n; n0 += chunk) {
+        pmx::ConvDims c = cd;
+        c.n = (int)std::min<int64_t>(chunk, cd.n - n0);
+        const int64_t blocks = (int64_t)c.n * c.od * gy * gx;
+        if (blocks > INT32_MAX) return pmx::fail(PMX_ERR_INVALID, "input too large for one launch");
+        const uint8_t *xc = x + n0 * sample;
+        uint16_t *oc = out + n0 * out_sample;
+        const uint32_t xbytes = (uint32_t)(c.n * sample);
+        const dim3 grid((unsigned)blocks);
+#define PMX_FIRST(MB, F) \
+    hipLaunchKernelGGL((pmx::k_conv_first<MB, F>), grid, dim3(256), 0, s, xc, xbytes, c, gy, gx, co, w_bf16, bias, oc, out_pad)
+        if (co <= 16) {
+            if (fast) PMX_FIRST(1, true); else PMX_FIRST(1, false);
+        } else {
+            if (fast) PMX_FIRST(2, true); else PMX_FIRST(2, false);
+        }
 #undef PMX_FIRST
-    return pmx::check_launch("conv3d_first");
+        if (int e = pmx::check_launch("conv3d_first")) return e;
+    }
+    return PMX_OK;
 }
 
-int pmx_conv3d_bf16(const uint16_t *x, const int32_t *in_shape, int stride, int co, const uint16_t *w_bf16,
-                    const float *bias, int relu, uint16_t *out, hipStream_t s)
+int pmx_conv3d_bf16(const uint16_t *x, const int32_t *in_shape, int in_pad, int stride, int co,
+                    const uint16_t *w_bf16, const float *bias, int relu, uint16_t *out, int out_pad, hipStream_t s)
 {
     pmx::clear_error();
     pmx::ConvDims cd;
     if (int e = pmx::conv_dims(in_shape, stride, 27 * (int64_t)in_shape[4], &cd)) return e;
     if (cd.c % 8) return pmx::fail(PMX_ERR_INVALID, "input channels must be a multiple of 8");
     if (co < 1) return pmx::fail(PMX_ERR_INVALID, "co must be positive");
+    if ((in_pad != 0 && in_pad != 1) || (out_pad != 0 && out_pad != 1))
+        return pmx::fail(PMX_ERR_INVALID, "in_pad and out_pad must be 0 or 1");
     if (!x || !w_bf16 || !bias || !out) return pmx::fail(PMX_ERR_INVALID, "NULL buffer");
-    if ((uintptr_t)x % 16 || (uintptr_t)w_bf16 % 16) return pmx::fail(PMX_ERR_INVALID, "x and w must be 16-byte aligned");
-    const int64_t M = (int64_t)cd.n * cd.od * cd.oh * cd.ow;
-    const int64_t mt = (M + 63) / 64;
-    if (mt > INT32_MAX) return pmx::fail(PMX_ERR_INVALID, "too many output positions");
-    if (co <= 16)
-        hipLaunchKernelGGL(pmx::k_conv_igemm<16>, dim3((unsigned)mt, 1), dim3(256), 0, s, x, cd, co, w_bf16, bias, relu,
-                           out);
-    else if (co <= 32)
-        hipLaunchKernelGGL(pmx::k_conv_igemm<32>, dim3((unsigned)mt, 1), dim3(256), 0, s, x, cd, co, w_bf16, bias, relu,
-                           out);
-    else if (co <= 48)
-        hipLaunchKernelGGL(pmx::k_conv_igemm<48>, dim3((unsigned)mt, 1), dim3(256), 0, s, x, cd, co, w_bf16, bias, relu,
-                           out);
-    else
-        hipLaunchKernelGGL(pmx::k_conv_igemm<64>, dim3((unsigned)mt, (unsigned)((co + 63) / 64)), dim3(256), 0, s, x, cd,
-                           co, w_bf16, bias, relu, out);
-    return pmx::check_launch("conv3d_bf16");
+    if ((uintptr_t)x % 8 || (uintptr_t)w_bf16 % 8 || (uintptr_t)out % 8)
+        return pmx::fail(PMX_ERR_INVALID, "x, w and out must be 8-byte aligned");
+    const int64_t K = 27 * (int64_t)cd.c;
+    if (2 * K * co >= pmx::kOob) return pmx::fail(PMX_ERR_INVALID, "weights exceed 2 GiB");
+    const int64_t sample = 2 * (int64_t)(cd.d + 2 * in_pad) * (cd.h + 2 * in_pad) * (cd.w + 2 * in_pad) * cd.c;
+    if (sample >= pmx::kOob) return pmx::fail(PMX_ERR_INVALID, "one sample exceeds 2 GiB");
+    const int64_t chunk = std::max<int64_t>(1, (int64_t)(pmx::kOob - 1) / sample);
+    const int64_t per = (int64_t)cd.od * cd.oh * cd.ow;
+    const int64_t out_sample = (int64_t)(cd.od + 2 * out_pad) * (cd.oh + 2 * out_pad) * (cd.ow + 2 * out_pad) * co;
+    // channel blocks per wave: the largest of 4, 3, 2, 1 that tiles ceil(co / 16) exactly, else 4
+    const int c16 = (co + 15) / 16;
+    const int cb = c16 % 4 == 0 ? 4 : c16 % 3 == 0 ? 3 : c16 % 2 == 0 ? 2 : c16 == 1 ? 1 : 4;
+    const int64_t ny = (c16 + cb - 1) / cb;
+    for (int64_t n0 = 0; n0 < cd.n; n0 += chunk) {
+        pmx::ConvDims c = cd;
+        c.n = (int)std::min<int64_t>(chunk, cd.n - n0);
+        const int64_t M = c.n * per;
+        if (M > INT32_MAX / 2) return pmx::fail(PMX_ERR_INVALID, "too many output positions");
+        // positions per wave: 64, fewer when that leaves under 4096 waves
+        int pos = 4;
+        while (pos > 1 && (M + 16 * pos - 1) / (16 * pos) * ny < 4096) pos /= 2;
+        const int64_t blocks = (M + 4 * 16 * pos - 1) / (4 * 16 * pos);
+        const dim3 grid((unsigned)blocks, (unsigned)ny);
+        const uint16_t *xc = x + n0 * (sample / 2);
+        uint16_t *oc = out + n0 * out_sample;
+        const uint32_t xbytes = (uint32_t)(c.n * sample);
+#define PMX_IG(P, CB, PD)                                                                                          \
+    hipLaunchKernelGGL((pmx::k_conv_igemm<P, CB, PD>), grid, dim3(256), 0, s, xc, xbytes, c, co, w_bf16, bias, relu, \
+                       oc, out_pad)
+#define PMX_IGP(CB, PD)       \
+    do {                      \
+        if (pos == 4)         \
+            PMX_IG(4, CB, PD); \
+        else if (pos == 2)    \
+            PMX_IG(2, CB, PD); \
+        else                  \
+            PMX_IG(1, CB, PD); \
+    } while (0)
+#define PMX_IGC(PD)                    \
+    switch (cb) {                      \
+        case 1: PMX_IGP(1, PD); break; \
+        case 2: PMX_IGP(2, PD); break; \
+        case 3: PMX_IGP(3, PD); break; \
+        default: PMX_IGP(4, PD);       \
+    }
+        if (in_pad) {
+            PMX_IGC(true);
+        } else {
+            PMX_IGC(false);
+        }
+#undef PMX_IGC
+#undef PMX_IGP
+#undef PMX_IG
+        if (int e = pmx::check_launch("conv3d_bf16")) return e;
+    }
+    return PMX_OK;
 }
 
 }  // extern "C"

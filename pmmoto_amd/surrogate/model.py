@@ -77,6 +77,7 @@ class SurrogateNet:
         self.m = [torch.zeros_like(p[1]) for p in self.params]
         self.v = [torch.zeros_like(p[1]) for p in self.params]
         self.step_count = 0
+        self._infer_bufs = {}
 
     def n_params(self) -> int:
         return sum(p[1].numel() for p in self.params)
@@ -223,29 +224,52 @@ class SurrogateNet:
     # -- inference ---------------------------------------------------------
     def infer(self, x_u8: torch.Tensor):
         """Forward pass without the training cache: the convolutions run as
-        fused kernels (direct first layer on the uint8 image, implicit GEMM
-        on the matrix cores after it) with bias, ReLU and BF16 rounding in
-        their epilogue.  Same arithmetic as forward() up to FP32 summation
-        order.  Returns (pred [B, q+1] FP32, pooled, z1, p, cf)."""
+        fused kernels (the first layer on the uint8 image, implicit GEMM on
+        the matrix cores after it) with bias, ReLU and BF16 rounding in their
+        epilogue, and the activations between them keep a one-voxel zero
+        border so that the next layer's taps need no range checks.  Same
+        arithmetic as forward() up to FP32 summation order.  Returns (pred
+        [B, q+1] FP32, pooled, z1, p, cf)."""
         b = x_u8.shape[0]
         dev = x_u8.device
         shape = [b, x_u8.shape[1], x_u8.shape[2], x_u8.shape[3], 1]
-        h = None
+        kinds = []
+        cin = 1
+        for i, co in enumerate(self.channels):
+            kinds.append("first" if i == 0 and co % 4 == 0 and co <= 32 else "igemm" if cin % 8 == 0 else "im2col")
+            cin = co
+        key = (b,) + tuple(shape[1:4])
+        bufs = self._infer_bufs.get(key)
+        if bufs is None:
+            # padded (zero border, kernels write the interior only) when the next layer reads it directly
+            bufs, shp = [], list(shape)
+            for i, co in enumerate(self.channels):
+                s = self.strides[i]
+                od, oh, ow = [(v - 1) // s + 1 for v in shp[1:4]]
+                if i + 1 < len(self.channels) and kinds[i + 1] == "igemm":
+                    bufs.append(torch.zeros((b, od + 2, oh + 2, ow + 2, co), dtype=BF16, device=dev))
+                else:
+                    bufs.append(torch.empty((b, od, oh, ow, co), dtype=BF16, device=dev))
+                shp = [b, od, oh, ow, co]
+            self._infer_bufs = {key: bufs}  # one batch shape cached at a time
+        h, h_pad = None, 0
         for i, co in enumerate(self.channels):
             w = self.param(f"conv{i}.w")
             bias = self.param(f"conv{i}.b")[1]
             s = self.strides[i]
             od, oh, ow = [(v - 1) // s + 1 for v in shape[1:4]]
-            a = torch.empty((b, od, oh, ow, co), dtype=BF16, device=dev)
+            a = bufs[i]
+            a_pad = 1 if a.shape[1] == od + 2 else 0
             dims = (C.c_int32 * 5)(*shape)
-            if i == 0 and co % 4 == 0 and co <= 32:
-                _lib.check(_L().pmx_conv3d_first(_p(x_u8), dims, s, co, _p(w[2]), _p(bias), _p(a), _device.stream()))
+            if kinds[i] == "first":
+                _lib.check(_L().pmx_conv3d_first(_p(x_u8), dims, s, co, _p(w[2]), _p(bias), _p(a), a_pad,
+                                                 _device.stream()))
             else:
                 if h is None:
                     h = torch.empty(tuple(shape), dtype=BF16, device=dev)
                     _lib.check(_L().pmx_u8_to_bf16(_p(x_u8), x_u8.numel(), _p(h), _device.stream()))
-                if shape[4] % 8 == 0:
-                    _lib.check(_L().pmx_conv3d_bf16(_p(h), dims, s, co, _p(w[2]), _p(bias), 1, _p(a),
+                if kinds[i] == "igemm":
+                    _lib.check(_L().pmx_conv3d_bf16(_p(h), dims, h_pad, s, co, _p(w[2]), _p(bias), 1, _p(a), a_pad,
                                                     _device.stream()))
                 else:
                     kk = 27 * shape[4]
@@ -257,7 +281,7 @@ class SurrogateNet:
                     del cols
                     _lib.check(_L().pmx_bias_act(_p(y), _p(bias), m, co, 1, _p(a), None, _device.stream()))
                     del y
-            h = a
+            h, h_pad = a, a_pad
             shape = [b, od, oh, ow, co]
         return self._head(h, shape)
 

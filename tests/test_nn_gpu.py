@@ -59,10 +59,18 @@ def test_im2col_col2im_match_oracle(s):
     np.testing.assert_allclose(dx.cpu().numpy(), N.col2im(g.astype(np.float64), x.shape, s), rtol=1e-5, atol=1e-5)
 
 
+def _pad(a):
+    """[N,D,H,W,C] -> zero-bordered [N,D+2,H+2,W+2,C]."""
+    out = np.zeros((a.shape[0],) + tuple(v + 2 for v in a.shape[1:4]) + (a.shape[4],), dtype=a.dtype)
+    out[:, 1:-1, 1:-1, 1:-1] = a
+    return out
+
+
 @pytest.mark.parametrize("co,s,shape", [(8, 2, (2, 9, 7, 10)), (16, 1, (1, 5, 6, 7)), (32, 2, (3, 4, 4, 4)),
                                         (8, 2, (2, 6, 19, 16)), (16, 2, (1, 3, 5, 136)), (8, 1, (1, 3, 4, 130)),
                                         (12, 2, (1, 4, 6, 8)), (24, 1, (2, 3, 3, 5))])
-def test_first_layer_conv_matches_oracle(co, s, shape):
+@pytest.mark.parametrize("out_pad", [0, 1])
+def test_first_layer_conv_matches_oracle(co, s, shape, out_pad):
     """uint8 first-layer convolution on the matrix cores, bias + ReLU + BF16 fused."""
     import ctypes as C
 
@@ -71,25 +79,31 @@ def test_first_layer_conv_matches_oracle(co, s, shape):
     from pmmoto_amd import _device, _lib
     rng = np.random.default_rng(co + s)
     x = (rng.random(shape) < 0.5).astype(np.uint8)
+    x[0, 0, 0, :3] = (7, 200, 255)  # bytes other than 0/1 convert exactly too
     w = N.bf16(rng.normal(size=(co, 3, 3, 3, 1)))
     b = rng.normal(size=co).astype(np.float32).astype(np.float64)
     want = N.bf16(np.maximum(N.conv_direct(x[..., None].astype(np.float64), w, b, s), 0.0))
-    out = torch.empty(want.shape, dtype=torch.bfloat16, device="cuda")
+    if out_pad:
+        want = _pad(want)
+    out = torch.zeros(want.shape, dtype=torch.bfloat16, device="cuda")
     dims = (C.c_int32 * 5)(*shape, 1)
-    print(f"[conv_first] co={co} s={s} shape={shape}", flush=True)
+    print(f"[conv_first] co={co} s={s} shape={shape} out_pad={out_pad}", flush=True)
     # operands held in names: a temporary freed before the launch would hand its memory to the next one
     xd, wd, bd = torch.from_numpy(x).cuda(), tbf(w), torch.from_numpy(b.astype(np.float32)).cuda()
     _lib.check(_lib.lib().pmx_conv3d_first(_device.ptr(xd), dims, s, co, _device.ptr(wd), _device.ptr(bd),
-                                           _device.ptr(out), _device.stream()))
-    # FP32 vs float64 sums may round to neighbouring BF16 values
+                                           _device.ptr(out), out_pad, _device.stream()))
+    # FP32 vs float64 sums may round to neighbouring BF16 values; the border stays zero
     np.testing.assert_allclose(out.float().cpu().numpy(), want, rtol=2 ** -7, atol=1e-6)
 
 
 @pytest.mark.parametrize("c,co,s,shape,relu", [(8, 16, 2, (2, 9, 7, 10), 1), (16, 48, 2, (1, 6, 5, 8), 1),
                                                (8, 24, 1, (1, 5, 4, 6), 0), (48, 96, 2, (2, 4, 4, 4), 1),
-                                               (16, 130, 1, (1, 3, 3, 3), 1)])
-def test_implicit_gemm_conv_matches_oracle(c, co, s, shape, relu):
-    """Matrix-core convolution without a patch matrix, bias/ReLU/BF16 fused."""
+                                               (16, 130, 1, (1, 3, 3, 3), 1), (24, 20, 2, (1, 5, 5, 5), 1),
+                                               (160, 256, 2, (3, 2, 2, 2), 1)])
+@pytest.mark.parametrize("in_pad,out_pad", [(0, 0), (1, 1), (1, 0)])
+def test_implicit_gemm_conv_matches_oracle(c, co, s, shape, relu, in_pad, out_pad):
+    """Matrix-core convolution without a patch matrix, bias/ReLU/BF16 fused,
+    on plain and zero-bordered tensors."""
     import ctypes as C
 
     import torch
@@ -101,12 +115,14 @@ def test_implicit_gemm_conv_matches_oracle(c, co, s, shape, relu):
     b = rng.normal(size=co).astype(np.float32).astype(np.float64)
     y = N.conv_fwd(x, w, b, s)
     want = N.bf16(np.maximum(y, 0.0) if relu else y)
-    out = torch.empty(want.shape, dtype=torch.bfloat16, device="cuda")
+    if out_pad:
+        want = _pad(want)
+    out = torch.zeros(want.shape, dtype=torch.bfloat16, device="cuda")
     dims = (C.c_int32 * 5)(*shape, c)
-    print(f"[conv_igemm] c={c} co={co} s={s} shape={shape}", flush=True)
-    xd, wd, bd = tbf(x), tbf(w), torch.from_numpy(b.astype(np.float32)).cuda()
-    _lib.check(_lib.lib().pmx_conv3d_bf16(_device.ptr(xd), dims, s, co, _device.ptr(wd), _device.ptr(bd), relu,
-                                          _device.ptr(out), _device.stream()))
+    print(f"[conv_igemm] c={c} co={co} s={s} shape={shape} pads={in_pad},{out_pad}", flush=True)
+    xd, wd, bd = tbf(_pad(x) if in_pad else x), tbf(w), torch.from_numpy(b.astype(np.float32)).cuda()
+    _lib.check(_lib.lib().pmx_conv3d_bf16(_device.ptr(xd), dims, in_pad, s, co, _device.ptr(wd), _device.ptr(bd),
+                                          relu, _device.ptr(out), out_pad, _device.stream()))
     np.testing.assert_allclose(out.float().cpu().numpy(), want, rtol=2 ** -7, atol=1e-5)
 
 
@@ -121,6 +137,11 @@ def test_fused_inference_matches_training_forward():
     want, _ = net.forward(x, keep=False)
     got = net.infer(x)[0]
     np.testing.assert_allclose(got.cpu().numpy(), want.cpu().numpy(), rtol=1e-2, atol=1e-2)
+    # the cached zero-bordered buffers serve a second call, and a new batch size
+    again = net.infer(x)[0]
+    np.testing.assert_array_equal(again.cpu().numpy(), got.cpu().numpy())
+    part = net.infer(x[:2].contiguous())[0]
+    np.testing.assert_allclose(part.cpu().numpy(), want[:2].cpu().numpy(), rtol=1e-2, atol=1e-2)
 
 
 def emulated(net_state, channels, x, target, q):

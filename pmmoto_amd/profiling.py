@@ -5,6 +5,9 @@
     rocprofv3 --pmc WRITE_SIZE -d DIR_WRITE -o run -- python3 bench.py --no-cpu --no-psd --steps 2 --warmup 1 --calibrate-copy
     python -m pmmoto_amd.profiling DIR_STATS DIR_FETCH DIR_WRITE OUT.json OUT.md
 
+(or the same three runs of ``python3 -m pmmoto_amd.surrogate.infer --calibrate-copy``
+for the surrogate's inference kernels).
+
 FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what the same
 counter reads for the copy of a 1 GiB buffer that ``--calibrate-copy``
 runs (1 GiB read, 1 GiB written), so the traffic is in bytes of HBM.
@@ -29,6 +32,8 @@ BENCH_NAMES = (
     (re.compile(r"k_window<.*int>"), "edt_pass_x"),
     (re.compile(r"k_fallback_y"), "edt_fallback_y"),
     (re.compile(r"k_fallback_x"), "edt_fallback_x"),
+    (re.compile(r"k_conv_first<"), None),  # surrogate inference kernels: reported under their symbol
+    (re.compile(r"k_conv_igemm<"), None),
 )
 COPY = re.compile(r"copyBuffer")
 
@@ -54,30 +59,34 @@ def kernel_times(stats_dir: str) -> dict:
     return out
 
 
-def counter(pmc_dir: str, counter_name: str) -> dict:
+def counter(pmc_dir: str, counter_name: str, reduce=None) -> dict:
+    """Per kernel symbol: the counter's mean over its dispatches (or reduce(values))."""
     con = sqlite3.connect(_db(pmc_dir))
     acc = defaultdict(list)
     for k, v in con.execute("select kernel_name, value from counters_collection where counter_name = ?",
                             (counter_name,)):
         acc[_short(k)].append(v)
-    return {k: sum(v) / len(v) for k, v in acc.items()}
+    reduce = reduce or (lambda v: sum(v) / len(v))
+    return {k: reduce(v) for k, v in acc.items()}
 
 
 def summarize(stats_dir: str, fetch_dir: str, write_dir: str) -> dict:
     times = kernel_times(stats_dir)
     fetch = counter(fetch_dir, "FETCH_SIZE")
     write = counter(write_dir, "WRITE_SIZE")
-    copy_f = [v for k, v in fetch.items() if COPY.search(k)]
-    copy_w = [v for k, v in write.items() if COPY.search(k)]
+    # the 1 GiB copy is the largest copy dispatch (small host/device copies share its symbol)
+    copy_f = [v for k, v in counter(fetch_dir, "FETCH_SIZE", max).items() if COPY.search(k)]
+    copy_w = [v for k, v in counter(write_dir, "WRITE_SIZE", max).items() if COPY.search(k)]
     if not copy_f or not copy_w:
         raise RuntimeError("the 1 GiB copy kernel is missing: run bench.py with --calibrate-copy")
     f_fetch = GIB / (copy_f[0] * 1024.0)  # bytes of HBM per KiB the counter reports
     f_write = GIB / (copy_w[0] * 1024.0)
     kernels = {}
     for sym in sorted(set(times) | set(fetch) | set(write)):
-        bench = next((b for rx, b in BENCH_NAMES if rx.search(sym)), None)
-        if bench is None:
+        hit = next(((rx, b) for rx, b in BENCH_NAMES if rx.search(sym)), None)
+        if hit is None:
             continue
+        bench = hit[1] or sym
         rd = fetch.get(sym, 0.0) * 1024.0 * f_fetch
         wr = write.get(sym, 0.0) * 1024.0 * f_write
         kernels[bench] = {

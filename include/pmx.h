@@ -194,12 +194,16 @@ int pmx_both_one(const uint8_t *a, const uint8_t *b, int64_t n, uint8_t *out, hi
  * op 0 fills it with value (update_reservoir); op 1 writes the number of
  * voxels equal to value to *count (device uint64; volume fractions).
  * pmx_max_f32: *out (device) = max of n non-negative float32 values
- * (determine_maximum, utils.py:148-162). */
+ * (determine_maximum, utils.py:148-162).
+ * pmx_max_u8: *out (device uint32) = max of n bytes (a 16-byte aligned); the
+ * EDT drop-in uses it to find multi-label images, whose first pass resets
+ * at label changes (_distance.hpp:155-165). */
 int pmx_drain_update(const uint8_t *morph, const uint8_t *wet, int64_t n, uint8_t *img,
                      hipStream_t stream);
 int pmx_box_op(uint8_t *arr, const int64_t *shape, const int64_t *lo, const int64_t *extent, int op,
                uint8_t value, uint64_t *count, hipStream_t stream);
 int pmx_max_f32(const float *a, int64_t n, float *out, hipStream_t stream);
+int pmx_max_u8(const uint8_t *a, int64_t n, uint32_t *out, hipStream_t stream);
 
 /* The pore-size-distribution sweep of pore_size_distribution (porosimetry.py:
  * 163-227) in one call, on a contiguous box (in and out strides equal the
@@ -318,6 +322,17 @@ int pmx_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, 
 int pmx_sphere_first_cover(const double *spheres, int64_t n_spheres, const int64_t *shape,
                            const int32_t *periodic, int32_t *first, int64_t *counts,
                            hipStream_t stream);
+/* pmx_spheres_on_grid: the reference's gen_pm_shape (_domain_generation.pyx
+ * :30-70 with Sphere::contains, particles/spheres.hpp:42-48) for spheres
+ * (n x (cx, cy, cz, r) float64, device) in the domain's own coordinates:
+ * img (shape[0] x shape[1] x shape[2] uint8) = 1, and 0 at every voxel whose
+ * centroid (x[i], y[j], z[k]) (device float64 coordinate arrays) satisfies
+ * (cx-x)^2 + (cy-y)^2 + (cz-z)^2 <= r*r; invert swaps 0 and 1.  grid_lo and
+ * spacing (host, 3 each) describe the nominal uniform grid (centroid i near
+ * grid_lo + (i + 0.5) spacing) used only to bound each sphere's voxels. */
+int pmx_spheres_on_grid(const double *spheres, int64_t n_spheres, const double *x, const double *y,
+                        const double *z, const int64_t *shape, const double *grid_lo, const double *spacing,
+                        int invert, uint8_t *img, hipStream_t stream);
 int pmx_sphere_image(const int32_t *first, const int64_t *shape, int32_t k, uint8_t *img,
                      hipStream_t stream);
 

@@ -248,6 +248,42 @@ def rasterize_spheres(shape, periodic, spheres, target):
     return img, int(n)
 
 
+def spheres_on_grid(spheres, coords, invert=False):
+    """gen_pm_shape (_domain_generation.pyx:30-70) with Sphere::contains
+    (particles/spheres.hpp:42-48): 1 everywhere, 0 where a voxel centroid
+    (coords[0][i], coords[1][j], coords[2][k]) satisfies
+    (cx-x)^2 + (cy-y)^2 + (cz-z)^2 <= r*r, in float64 and in that order;
+    ``invert`` swaps 0 and 1 (domain_generation.py:148-149)."""
+    x, y, z = (np.asarray(c, dtype=np.float64) for c in coords)
+    img = np.ones((x.size, y.size, z.size), dtype=np.uint8)
+    for cx, cy, cz, r in np.asarray(spheres, dtype=np.float64):
+        dx = (cx - x)[:, None, None]
+        dy = (cy - y)[None, :, None]
+        dz = (cz - z)[None, None, :]
+        img[(dx * dx + dy * dy) + dz * dz <= r * r] = 0
+    return (1 - img).astype(np.uint8) if invert else img
+
+
+def periodic_pads(img, pads):
+    """update_buffer (communication.py:101-118) for one rank: each periodic
+    pad holds the own voxels at the opposite side, axis by axis."""
+    img = img.copy()
+    for ax in range(3):
+        lo, hi = pads[ax]
+        n = img.shape[ax]
+        if lo:
+            src = np.take(img, np.arange(n - hi - lo, n - hi), axis=ax)
+            idx = [slice(None)] * 3
+            idx[ax] = slice(0, lo)
+            img[tuple(idx)] = src
+        if hi:
+            src = np.take(img, np.arange(lo, lo + hi), axis=ax)
+            idx = [slice(None)] * 3
+            idx[ax] = slice(n - hi, n)
+            img[tuple(idx)] = src
+    return img
+
+
 # --------------------------------------------------------------------------
 # Padded-subdomain model and the Python-level reference drivers
 # --------------------------------------------------------------------------

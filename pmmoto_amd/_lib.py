@@ -79,6 +79,7 @@ SIGNATURES = {
     "pmx_drain_update": (C.c_int, [P, P, I64, P, P]),
     "pmx_box_op": (C.c_int, [P, P, P, P, C.c_int, U8, P, P]),
     "pmx_max_f32": (C.c_int, [P, I64, P, P]),
+    "pmx_max_u8": (C.c_int, [P, I64, P, P]),
     "pmx_gemm_bf16": (C.c_int, [I64, C.c_int, C.c_int, P, I64, C.c_int, P, I64, C.c_int, P, I64, C.c_int, C.c_int, P]),
     "pmx_conv3d_first": (C.c_int, [P, P, C.c_int, C.c_int, P, P, P, C.c_int, P]),
     "pmx_conv3d_bf16": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, P, C.c_int, P, C.c_int, P]),
@@ -94,6 +95,7 @@ SIGNATURES = {
     "pmx_gnll_loss": (C.c_int, [P, P, C.c_int, C.c_int, P, P, P]),
     "pmx_adam": (C.c_int, [P, P, P, P, I64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, P, P]),
     "pmx_sphere_first_cover": (C.c_int, [P, I64, P, P, P, P, P]),
+    "pmx_spheres_on_grid": (C.c_int, [P, I64, P, P, P, P, P, P, C.c_int, P, P]),
     "pmx_sphere_image": (C.c_int, [P, P, I32, P, P]),
 }
 

@@ -81,6 +81,12 @@ class PaddedSubdomain:
                      for v, p, r in zip(self.domain.voxels, self.pad, self.reservoir_pad))
 
     @property
+    def start(self) -> tuple:
+        """Global index of the first (padded) voxel: minus the low pads and
+        reservoir (subdomain_padded.py:242-269, the image extent in VTK files)."""
+        return tuple(-(p[0] + r[0]) for p, r in zip(self.pad, self.reservoir_pad))
+
+    @property
     def own_voxels(self) -> np.ndarray:
         return np.array(self.domain.voxels, dtype=np.int64)
 

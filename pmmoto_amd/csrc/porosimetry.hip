@@ -112,6 +112,26 @@ __global__ __launch_bounds__(256) void k_max_f32(const float *__restrict__ a, in
     if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
 }
 
+// max of n bytes: 16 per lane per step (the tail byte by byte), wave max, one atomic per wave
+__global__ __launch_bounds__(256) void k_max_u8(const uint8_t *__restrict__ a, int64_t n, unsigned int *__restrict__ out)
+{
+    unsigned int m = 0u;
+    const int64_t nv = n / 16;
+    const uint4 *v = reinterpret_cast<const uint4 *>(a);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint4 q = v[i];
+        const unsigned int w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            m = max(m, max(max(w[k] & 0xffu, (w[k] >> 8) & 0xffu), max((w[k] >> 16) & 0xffu, w[k] >> 24)));
+    }
+    for (int64_t i = 16 * nv + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        m = max(m, (unsigned int)a[i]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, o, 64));
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
 __global__ __launch_bounds__(64) void k_fill_u32(unsigned int *a, int64_t n)
 {
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0;
@@ -210,6 +230,18 @@ int pmx_max_f32(const float *a, int64_t n, float *out, hipStream_t s)
     if (int e = pmx::check_launch("max clear")) return e;
     hipLaunchKernelGGL(pmx::k_max_f32, dim3(std::min<unsigned>(pmx::grid_for(n), 4096)), dim3(256), 0, s, a, n, o);
     return pmx::check_launch("max");
+}
+
+int pmx_max_u8(const uint8_t *a, int64_t n, uint32_t *out, hipStream_t s)
+{
+    pmx::clear_error();
+    if (!a || !out || n < 1) return pmx::fail(PMX_ERR_INVALID, "bad argument");
+    if ((uintptr_t)a % 16) return pmx::fail(PMX_ERR_INVALID, "a must be 16-byte aligned");
+    hipLaunchKernelGGL(pmx::k_fill_u32, dim3(1), dim3(64), 0, s, out, (int64_t)1);
+    if (int e = pmx::check_launch("max clear")) return e;
+    hipLaunchKernelGGL(pmx::k_max_u8, dim3(std::min<unsigned>(pmx::grid_for(n / 16 + 1), 2048)), dim3(256), 0, s, a, n,
+                       out);
+    return pmx::check_launch("max u8");
 }
 
 int pmx_both_one(const uint8_t *a, const uint8_t *b, int64_t n, uint8_t *out, hipStream_t s)

@@ -22,7 +22,8 @@ import torch
 
 from .. import _device, _lib
 
-__all__ = ["gen_img_random_binary", "sphere_pack", "pack_radii", "gen_pm_inkbottle", "gen_mp_constant"]
+__all__ = ["gen_img_random_binary", "sphere_pack", "pack_radii", "gen_pm_spheres_domain", "gen_pm_inkbottle",
+           "gen_mp_constant"]
 
 
 def gen_img_random_binary(shape, p_zero=0.5, seed=None):
@@ -93,6 +94,41 @@ def sphere_pack(shape, porosity, radii, seed, periodic=(False, False, False), de
     _lib.check(L.pmx_sphere_image(_device.ptr(first), sh, k, _device.ptr(img), _device.stream()))
     del first
     return (img if device_out else img.cpu().numpy()), spheres[:k]
+
+
+def gen_pm_spheres_domain(subdomain, spheres, kd: bool = False, invert: bool = False):
+    """gen_pm_spheres_domain (domain_generation.py:125-155): spheres
+    ``(n, 4)`` = (x, y, z, r) in the domain's coordinates are solid (0) on
+    the subdomain's padded grid of voxel centroids (``subdomain.coords``),
+    tested with the reference's float64 predicate (spheres.hpp:42-48); the
+    periodic pads then hold periodic copies (update_buffer), ``invert``
+    swaps pore and solid, and WALL faces get their solid layer
+    (set_wall_bcs).  No periodic images of the spheres are added, as in the
+    reference.  ``kd`` (a CPU search structure there) has no effect."""
+    from ..filters.distance.euclidean_distance_transform import compute_box, fill_periodic_pads
+    from .porousmedia import gen_pm
+
+    sp = np.ascontiguousarray(np.asarray(spheres, dtype=np.float64))
+    if sp.ndim != 2 or sp.shape[1] != 4:
+        raise ValueError(f"spheres must be an (n, 4) array of x, y, z, r; got shape {sp.shape}")
+    if not np.all(np.isfinite(sp)):
+        raise ValueError("sphere data must be finite")
+    dev = _device.require_gpu()
+    shape = tuple(int(v) for v in subdomain.voxels)
+    coords = [torch.from_numpy(np.ascontiguousarray(c, dtype=np.float64)).to(dev) for c in subdomain.coords]
+    res = subdomain.domain.resolution
+    lo = [float(b[0]) for b in subdomain.box]
+    img = torch.empty(shape, dtype=torch.uint8, device=dev)
+    sp_d = torch.from_numpy(sp).to(dev) if sp.shape[0] else None
+    _lib.check(_lib.lib().pmx_spheres_on_grid(_device.ptr(sp_d) if sp_d is not None else None, sp.shape[0],
+                                              _device.ptr(coords[0]), _device.ptr(coords[1]),
+                                              _device.ptr(coords[2]), (C.c_int64 * 3)(*shape),
+                                              (C.c_double * 3)(*lo), (C.c_double * 3)(*res), int(bool(invert)),
+                                              _device.ptr(img), _device.stream()))
+    _, _, _, pads = compute_box(shape, subdomain=subdomain)
+    fill_periodic_pads([img], shape, pads)
+    img = subdomain.set_wall_bcs(img)
+    return gen_pm(subdomain, img)
 
 
 def gen_pm_inkbottle(subdomain, r_y: float = 1.0, r_z: float = 1.0):

@@ -135,13 +135,25 @@ def _finish(t, squeeze, as_numpy):
     return _device.to_host(t, as_numpy)
 
 
-def _use_reference(mode, resolution, periodic) -> bool:
+def _multilabel(t: torch.Tensor) -> bool:
+    """True when the image holds a value above 1 (one device reduction)."""
+    if t.data_ptr() % 16 or not t.is_contiguous():
+        t = t.contiguous().clone()
+    out = torch.empty(1, dtype=torch.int32, device=t.device)
+    _lib.check(_lib.lib().pmx_max_u8(_device.ptr(t), t.numel(), _device.ptr(out), _device.stream()))
+    return int(out.item()) > 1
+
+
+def _use_reference(mode, resolution, periodic, img_t=None) -> bool:
     """mode "auto": the exact integer kernels where they equal the reference
-    bit for bit (unit resolution, no periodic axis); the reference's float32
-    arithmetic otherwise -- at periodic faces the reference keeps only five
-    boundary parabolas per line, which differs from the exact torus distance
-    near the face on some images.  "exact": the integer kernels (exact torus
-    distance on periodic axes); "reference": always the reference arithmetic."""
+    bit for bit (unit resolution, no periodic axis, a 0/1 image); the
+    reference's float32 arithmetic otherwise -- at periodic faces the
+    reference keeps only five boundary parabolas per line, which differs from
+    the exact torus distance near the face on some images, and on a
+    multi-label image its first pass restarts at every label change
+    (_distance.hpp:155-165) while the later passes ignore labels.  "exact":
+    the integer kernels (exact torus distance on periodic axes; every nonzero
+    voxel is pore); "reference": always the reference arithmetic."""
     if mode == "reference":
         return True
     if mode == "exact":
@@ -150,7 +162,9 @@ def _use_reference(mode, resolution, periodic) -> bool:
         return False
     if mode != "auto":
         raise ValueError(f"mode {mode!r} must be 'auto', 'exact' or 'reference'")
-    return not _unit(resolution) or any(bool(p) for p in periodic)
+    if not _unit(resolution) or any(bool(p) for p in periodic):
+        return True
+    return img_t is not None and img_t.numel() > 0 and _multilabel(img_t)
 
 
 def edt3d(img, periodic=(False, False, False), resolution=(1.0, 1.0, 1.0), squared=False, mode="auto"):
@@ -159,7 +173,7 @@ def edt3d(img, periodic=(False, False, False), resolution=(1.0, 1.0, 1.0), squar
     t, as_np = _device.as_device_u8(img)
     if t.dim() != 3:
         raise ValueError(f"edt3d needs a 3-D image, got {tuple(t.shape)}")
-    if _use_reference(mode, resolution, periodic):
+    if _use_reference(mode, resolution, periodic, t):
         out = run_edt_ref(t, (0, 0, 0), t.shape, 0, periodic, resolution, not squared)
         return _device.to_host(out, as_np)
     origin, n, bc, pads = compute_box(tuple(t.shape), periodic=periodic)
@@ -173,7 +187,7 @@ def edt2d(img, periodic=(False, False), resolution=(1.0, 1.0), mode="auto"):
     if t.dim() != 2:
         raise ValueError(f"edt2d needs a 2-D image, got {tuple(t.shape)}")
     t3, _ = _as3d(t)
-    if _use_reference(mode, resolution, periodic):
+    if _use_reference(mode, resolution, periodic, t):
         # the reference sweeps axis 1 first, then axis 0: box (1, X, Y), axis 2 first
         out = run_edt_ref(t3, (0, 0, 0), t3.shape, 2, (False,) + tuple(periodic),
                           (1.0,) + tuple(resolution), True)
@@ -190,9 +204,9 @@ def edt(img, subdomain=None, mode="auto"):
     if subdomain is None:
         nd = img.dim() if isinstance(img, torch.Tensor) else np.ndim(img)
         if nd == 3:
-            return edt3d(img)
+            return edt3d(img, mode=mode)
         if nd == 2:
-            return edt2d(img)
+            return edt2d(img, mode=mode)
         raise ValueError(f"Wrong img dimension {np.shape(img)}")
     t, as_np = _device.as_device_u8(img)
     if t.dim() != 3:
@@ -200,7 +214,7 @@ def edt(img, subdomain=None, mode="auto"):
     origin, n, bc, pads = compute_box(tuple(t.shape), subdomain=subdomain)
     res = subdomain.domain.resolution
     per = [bc[2 * a] == _lib.BC_PERIODIC for a in range(3)]
-    if _use_reference(mode, res, per):
+    if _use_reference(mode, res, per, t):
         # non-periodic: edt3d of the padded image, as the reference; periodic:
         # the reference arithmetic on the own box with wrap, pads copied
         out = run_edt_ref(t, origin, n, 0, per, res, True)

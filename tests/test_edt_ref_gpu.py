@@ -40,6 +40,36 @@ def test_multilabel_pass0():
     np.testing.assert_array_equal(got, O.ref_edt_sq(img))
 
 
+def test_multilabel_auto_mode_follows_reference():
+    """A multi-label image at unit resolution (the default path) gets the
+    reference's label-aware first pass, not the binary exact EDT."""
+    from pmmoto_amd.filters import distance
+    rng = np.random.default_rng(7)
+    img = rng.choice([0, 1, 2], size=(19, 16, 13), p=[0.1, 0.6, 0.3]).astype(np.uint8)
+    want = O.ref_edt_sq(img)
+    got = distance.edt3d(img, squared=True)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(distance.edt(img).view(np.uint32), np.sqrt(want).view(np.uint32))
+    # a 0/1 image keeps the exact kernels, which agree with the reference there
+    b = (img > 0).astype(np.uint8)
+    np.testing.assert_array_equal(distance.edt3d(b, squared=True), O.ref_edt_sq(b))
+
+
+@pytest.mark.parametrize("n", [1, 15, 16, 17, 4099, 1 << 20])
+def test_max_u8(n):
+    import torch
+
+    from pmmoto_amd import _device, _lib
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 3, size=n, dtype=np.uint8)
+    a[rng.integers(0, n)] = 200 if n > 16 else 9
+    t = torch.from_numpy(a).cuda()
+    out = torch.empty(1, dtype=torch.int32, device="cuda")
+    print(f"[max_u8] n={n}", flush=True)
+    _lib.check(_lib.lib().pmx_max_u8(_device.ptr(t), n, _device.ptr(out), _device.stream()))
+    assert int(out.item()) == int(a.max())
+
+
 @pytest.mark.parametrize("periodic,res", [((False, False), (1.0, 1.0)), ((True, True), (1.0, 1.0)),
                                           ((True, False), (0.5, 0.25))])
 def test_edt2d_reference_mode(periodic, res):

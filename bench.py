@@ -43,26 +43,45 @@ def traffic_for(kernel: str, n: int):
     return None if k is None else k["hbm_bytes_per_launch"]
 
 
-def cpu_baseline(img_t, edge: int):
-    """The reference EDT on the host, single thread, on an edge^3 crop of
-    the same image: the reference's own per-line kernels (oracle/_ref) when
-    built, else the oracle's restatement of them."""
+def host_threads() -> int:
+    """Host cores this process may use, at most 16 (one GPU's share of the box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(img_t, edge: int, threads: int):
+    """The reference EDT on the host, on a corner crop of the same image:
+    the reference's own per-line kernels (oracle/_ref) when built, else the
+    oracle's restatement of them, in the reference's pass order.  Timed on
+    `threads` host threads (each pass's lines split over them, standing in
+    for the reference's MPI ranks) on an edge^3 crop, and on one core on the
+    smaller (edge/2)^3 crop, so the whole leg stays a few seconds."""
     import numpy as np
 
     from oracle import oracle as O
 
-    crop = np.ascontiguousarray(img_t[:edge, :edge, :edge].cpu().numpy())
     use_ref = O.ref_lib() is not None
+    crop = np.ascontiguousarray(img_t[:edge, :edge, :edge].cpu().numpy())
     t0 = time.perf_counter()
-    O.ref_edt_sq(crop, use_ref=use_ref)
+    O.ref_edt_sq_threads(crop, threads, use_ref=use_ref)
     dt = time.perf_counter() - t0
+    e1 = max(1, edge // 2)
+    crop1 = np.ascontiguousarray(crop[:e1, :e1, :e1])
+    t0 = time.perf_counter()
+    O.ref_edt_sq(crop1, use_ref=use_ref)
+    dt1 = time.perf_counter() - t0
     return {
         "value": round(crop.size / dt / 1e9, 6),
         "unit": "Gvoxel/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "reference" if use_ref else "port",
         "sample": f"squared EDT (reference pass order, float32) of the {edge}^3 corner crop of the "
-                  f"same pack, non-periodic, {dt:.2f} s on one host core",
+                  f"same pack, non-periodic, {dt:.2f} s on {threads} host threads",
+        "single_core_value": round(crop1.size / dt1 / 1e9, 6),
+        "single_core_sample": f"{e1}^3 corner crop, {dt1:.2f} s on one host core",
     }
 
 
@@ -231,7 +250,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=1024)
-    ap.add_argument("--cpu-edge", type=int, default=320)
+    ap.add_argument("--cpu-edge", type=int, default=640)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-psd", action="store_true")
     ap.add_argument("--psd-reps", type=int, default=2)
@@ -328,7 +347,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(img, min(args.cpu_edge, n))
+        cpu = cpu_baseline(img, min(args.cpu_edge, n), host_threads())
 
     line = {
         "metric": "EDT Gvoxel/s @1024³ (% HBM peak); porosimetry curves/s surrogate vs exact",

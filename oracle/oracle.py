@@ -68,6 +68,7 @@ def lib() -> C.CDLL:
         L.orc_ref_nearest_index_1d.restype = C.c_long
         L.orc_ref_edt3d_sq.argtypes = [P, C.c_long, C.c_long, C.c_long, P, P, P, P]
         L.orc_ref_edt2d_sq.argtypes = [P, C.c_long, C.c_long, P, P, P, P]
+        L.orc_ref_edt3d_sq_mt.argtypes = [P, C.c_long, C.c_long, C.c_long, P, P, P, C.c_int]
         L.orc_exact_edt_sq.argtypes = [P, C.c_int, P, P, P]
         L.orc_exact_ccl26.argtypes = [P, C.c_long, C.c_long, C.c_long, P, P]
         L.orc_exact_ccl26.restype = C.c_int64
@@ -183,6 +184,22 @@ def ref_edt_sq(img, periodic=None, resolution=None, use_ref=False):
         lib().orc_ref_edt2d_sq(_p(img), *img.shape, _p(per), _p(res), _p(out), opp)
     else:
         raise ValueError(f"Wrong img dimension {img.shape}")
+    return out
+
+
+def ref_edt_sq_threads(img, threads, resolution=None, use_ref=False):
+    """ref_edt_sq of a non-periodic 3-D image with each pass's lines split
+    over `threads` host threads (orc_ref_edt3d_sq_mt; identical output)."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    if img.ndim != 3:
+        raise ValueError("3-D images only")
+    res = np.array(resolution or (1.0, 1.0, 1.0), dtype=np.float32)
+    out = np.empty(img.shape, dtype=np.float32)
+    ops = ref_ops() if use_ref else None
+    if use_ref and ops is None:
+        raise RuntimeError("reference kernels not built (make -C oracle ref)")
+    lib().orc_ref_edt3d_sq_mt(_p(img), *img.shape, _p(res), _p(out), C.byref(ops) if ops is not None else None,
+                              int(threads))
     return out
 
 

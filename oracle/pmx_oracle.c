@@ -21,6 +21,7 @@
  * How each function is pinned is stated in tests/test_oracle_*.py.
  */
 #include <float.h>
+#include <pthread.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -398,6 +399,96 @@ ORC_EXPORT void orc_ref_edt3d_sq(const uint8_t *img, long nx, long ny, long nz,
     ref_initial_envelope(img, out, g, 0, res[0], periodic[0], ops);
     for (int dim = 1; dim <= 2; dim++)
         ref_parabolic_pass(img, out, g, dim, res[dim], periodic[dim], ops);
+}
+
+/* The same non-periodic 3-D EDT with each pass's lines split over nthreads
+ * POSIX threads (ranges of the first other axis; a join between passes),
+ * standing in for the reference's MPI ranks when it is timed as a CPU
+ * baseline.  Line arithmetic and pass order are those of orc_ref_edt3d_sq,
+ * so the output is identical. */
+typedef struct {
+    const uint8_t *img;
+    float *out;
+    geom g;
+    int dim;
+    float res;
+    const orc_line_ops *ops;
+    long a0, a1;      /* lines with first-other-axis index in [a0, a1) */
+    long e0, e1;      /* elements [e0, e1) for to_finite */
+    int stage;        /* 0: initial envelope, 1: to_finite, 2: parabolic */
+} orc_job;
+
+static void *orc_worker(void *arg)
+{
+    orc_job *j = (orc_job *)arg;
+    if (j->stage == 1) {
+        to_finite(j->out + j->e0, (size_t)(j->e1 - j->e0));
+        return NULL;
+    }
+    int d1, d2;
+    other_dims(j->g.nd, j->dim, &d1, &d2);
+    long s2 = d2 >= 0 ? j->g.shape[d2] : 1;
+    long st1 = j->g.stride[d1], st2 = d2 >= 0 ? j->g.stride[d2] : 0;
+    int n = (int)j->g.shape[j->dim];
+    long st = j->g.stride[j->dim];
+    for (long a = j->a0; a < j->a1; a++)
+        for (long b = 0; b < s2; b++) {
+            long base = a * st1 + b * st2;
+            if (j->stage == 0)
+                j->ops->sq_edt_1d(j->img + base, j->out + base, n, st, j->res, INFINITY, INFINITY);
+            else
+                j->ops->parabolic(j->out + base, n, j->res, st, NULL, 0, NULL, 0);
+        }
+    return NULL;
+}
+
+static void orc_run_stage(orc_job proto, int nthreads)
+{
+    pthread_t th[256];
+    orc_job jobs[256];
+    int d1, d2;
+    other_dims(proto.g.nd, proto.dim, &d1, &d2);
+    long s1 = proto.g.shape[d1];
+    long total = proto.g.shape[0] * proto.g.shape[1] * proto.g.shape[2];
+    int created[256];
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t] = proto;
+        jobs[t].a0 = s1 * t / nthreads;
+        jobs[t].a1 = s1 * (t + 1) / nthreads;
+        jobs[t].e0 = total * t / nthreads;
+        jobs[t].e1 = total * (t + 1) / nthreads;
+        created[t] = pthread_create(&th[t], NULL, orc_worker, &jobs[t]) == 0;
+        if (!created[t]) orc_worker(&jobs[t]); /* no thread: run the share here */
+    }
+    for (int t = 0; t < nthreads; t++)
+        if (created[t]) pthread_join(th[t], NULL);
+}
+
+ORC_EXPORT void orc_ref_edt3d_sq_mt(const uint8_t *img, long nx, long ny, long nz, const float *res,
+                                    float *out, const orc_line_ops *ops, int nthreads)
+{
+    if (!ops) ops = &restated_ops;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    long shape[3] = {nx, ny, nz};
+    orc_job j;
+    memset(&j, 0, sizeof(j));
+    j.img = img;
+    j.out = out;
+    j.g = make_geom(3, shape);
+    j.ops = ops;
+    j.dim = 0;
+    j.res = res[0];
+    j.stage = 0;
+    orc_run_stage(j, nthreads);
+    j.stage = 1;
+    orc_run_stage(j, nthreads);
+    for (int dim = 1; dim <= 2; dim++) {
+        j.dim = dim;
+        j.res = res[dim];
+        j.stage = 2;
+        orc_run_stage(j, nthreads);
+    }
 }
 
 /* edt2d(img, periodic, resolution): euclidean_distance_transform.py:104-205

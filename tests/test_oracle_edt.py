@@ -234,3 +234,16 @@ def test_reference_periodic_vs_tiled():
     want = ndimage.distance_transform_edt(tiled)[40:80, 40:80, 40:80]
     got = np.sqrt(O.ref_edt_sq(img, (True, True, True)))
     np.testing.assert_array_almost_equal(got, want)
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+@pytest.mark.parametrize("res", [(1.0, 1.0, 1.0), (0.5, 0.25, 0.125)])
+def test_threaded_driver_is_identical(threads, res):
+    """The threaded CPU-baseline driver splits each pass's lines over host
+    threads; its output equals the single-threaded driver bit for bit."""
+    rng = np.random.default_rng(threads)
+    img = (rng.random((23, 19, 17)) < 0.75).astype(np.uint8)
+    use_ref = O.ref_lib() is not None
+    want = O.ref_edt_sq(img, resolution=res, use_ref=use_ref)
+    got = O.ref_edt_sq_threads(img, threads, resolution=res, use_ref=use_ref)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))

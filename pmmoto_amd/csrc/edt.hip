@@ -529,8 +529,10 @@ __device__ __forceinline__ void commit_rows(int *tile, const WinGeom &g, int dst
 //
 // The tile is a sliding window: after tile i is scanned its last 2W rows
 // move to the top and only the T new rows are loaded (issued while tile i
-// is scanned), so each row is loaded once per block.
-template <int T, int W, class E>
+// is scanned), so each row is loaded once per block.  D2ONLY (pass X):
+// the int32 squared distance is the only output, written without the
+// per-row tests of the other outputs.
+template <int T, int W, class E, bool D2ONLY = false>
 __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in, int32_t *__restrict__ out,
                                                         PassXOut o, WinGeom g, int nx, int ny,
                                                         uint32_t *__restrict__ line_bits,
@@ -607,8 +609,12 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
             if (y >= g.n) break;
             const int *c = col + (W + j) * kTileZ;
             int best = c[0];
+            // blocks of four offsets while any lane of the wave can still improve; a lane that
+            // is done only folds in candidates >= k^2 >= best, which leave it unchanged, so the
+            // lanes need no exec mask and the loop exit is one wave-wide ballot
+#pragma unroll
             for (int k = 1; k <= W; k += 4) {
-                if (cm + k * k >= best) break;
+                if (__builtin_amdgcn_ballot_w64(cm + k * k < best) == 0) break;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int kk = k + u;
@@ -637,6 +643,8 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
             if (!ok || z >= g.nz) continue;
             if constexpr (kY) {
                 out[(int64_t)outer * g.out_os + (int64_t)y * g.out_rs + z] = best >= kBig ? kInf : best;
+            } else if constexpr (D2ONLY) {
+                o.d2[(int64_t)y * o.sx + (int64_t)outer * o.sy + z] = best >= kBig ? kInf : best;
             } else {
                 emit_x(o, y, outer, z, nx, ny, g.nz, best >= kBig ? kInf : best);
             }
@@ -738,8 +746,13 @@ void launch_window(const E *in, int32_t *out, const PassXOut &o, const WinGeom &
 {
     const int nzt = (wg.nz + kTileZ - 1) / kTileZ;
     const int64_t blocks = (int64_t)wg.nouter * nzt * (((wg.n + T - 1) / T + kTilesPerBlock - 1) / kTilesPerBlock);
-    hipLaunchKernelGGL((k_window<T, W, E>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, in, out, o, wg, nx,
-                       ny, bits, count, list, proceed);
+    const bool d2only = sizeof(E) == 4 && o.d2 && !o.f32sq && !o.f32 && !o.f64 && !o.u8 && !o.psd_map;
+    if (d2only)
+        hipLaunchKernelGGL((k_window<T, W, E, true>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, in, out, o, wg,
+                           nx, ny, bits, count, list, proceed);
+    else
+        hipLaunchKernelGGL((k_window<T, W, E, false>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, in, out, o,
+                           wg, nx, ny, bits, count, list, proceed);
 }
 
 template <class E>

@@ -604,9 +604,9 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
         const bool more = rt + 1 < rt_end;
         if (more) issue_rows<T / 16>(base, g, r0 + T + W, row0, c4, xm, xr);
 
-        for (int j = wv; j < T; j += kWaves) {
+        const int jend = min(T, g.n - r0);  // wave-uniform: the row loop needs no exec mask
+        for (int j = wv; j < jend; j += kWaves) {
             const int y = r0 + j;
-            if (y >= g.n) break;
             const int *c = col + (W + j) * kTileZ;
             int best = c[0];
             // blocks of four offsets while any lane of the wave can still improve; a lane that
@@ -624,7 +624,8 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
             // the tile is done; rows beyond it are at least W + 1 away
             int k = W + 1;
             bool ok = k * k >= best;
-            if (!ok && z < g.nz) {
+            // rare: one ballot keeps the common case free of exec masks
+            if (__builtin_amdgcn_ballot_w64(!ok && z < g.nz) != 0 && !ok && z < g.nz) {
                 // beyond the tile: scan the line in global memory, unless the
                 // line is already deferred
                 const uint32_t bit = 1u << (line & 31);

@@ -45,24 +45,35 @@ constexpr int kNone = 1 << 29;  // "no solid on this side" position marker
 // pass Z
 // ---------------------------------------------------------------------------
 
+// Wave scans on the data-parallel-primitive paths: Hillis-Steele steps
+// within each row of 16 lanes (row_shr / row_shl), then the row totals
+// (three readlanes) folded into the later (or earlier) rows.
 __device__ __forceinline__ int wave_incl_max(int v, int lane)
 {
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        int t = __shfl_up(v, o, kWave);
-        if (lane >= o) v = max(v, t);
-    }
-    return v;
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+    const int r0 = __builtin_amdgcn_readlane(v, 15), r1 = __builtin_amdgcn_readlane(v, 31),
+              r2 = __builtin_amdgcn_readlane(v, 47);
+    const int row = lane >> 4;
+    const int s01 = max(r0, r1), s012 = max(s01, r2);
+    const int add = row == 0 ? -kNone : row == 1 ? r0 : row == 2 ? s01 : s012;
+    return max(v, add);
 }
 
 __device__ __forceinline__ int wave_incl_min_rev(int v, int lane)
 {
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        int t = __shfl_down(v, o, kWave);
-        if (lane + o < kWave) v = min(v, t);
-    }
-    return v;
+    v = min(v, __builtin_amdgcn_update_dpp(kNone, v, 0x101, 0xf, 0xf, false));  // row_shl:1
+    v = min(v, __builtin_amdgcn_update_dpp(kNone, v, 0x102, 0xf, 0xf, false));  // row_shl:2
+    v = min(v, __builtin_amdgcn_update_dpp(kNone, v, 0x104, 0xf, 0xf, false));  // row_shl:4
+    v = min(v, __builtin_amdgcn_update_dpp(kNone, v, 0x108, 0xf, 0xf, false));  // row_shl:8
+    const int r1 = __builtin_amdgcn_readlane(v, 16), r2 = __builtin_amdgcn_readlane(v, 32),
+              r3 = __builtin_amdgcn_readlane(v, 48);
+    const int row = lane >> 4;
+    const int s23 = min(r2, r3), s123 = min(r1, s23);
+    const int add = row == 0 ? s123 : row == 1 ? s23 : row == 2 ? r3 : kNone;
+    return min(v, add);
 }
 
 // One wavefront per row of up to 16 * 1024 voxels.  Lane l owns voxels
@@ -152,17 +163,17 @@ __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img,
     const int nsup = (nz + 1023) / 1024;
     const bool vin = vec_in != 0;
 
-    // sweep 1: each super-chunk's solid masks, last and first solid
+    // sweep 1: each super-chunk's solid masks, last and first solid (the
+    // totals only matter across super-chunks or periodic faces)
+    const bool totals = nsup > 1 || bc_lo == PMX_BC_PERIODIC;
     for (int s = 0; s < nsup; ++s) {
         const int e0 = s * 1024 + lane * 16;
         const uint32_t m = solid_mask16<kInvert>(src, e0, nz, vin);
         smask[w][s][lane] = m;
-        int l = m ? e0 + 31 - __clz(m) : -kNone;
-        int f = m ? e0 + __ffs(m) - 1 : kNone;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            l = max(l, __shfl_xor(l, o, kWave));
-            f = min(f, __shfl_xor(f, o, kWave));
+        int l = -kNone, f = kNone;
+        if (totals) {
+            l = __builtin_amdgcn_readlane(wave_incl_max(m ? e0 + 31 - __clz(m) : -kNone, lane), 63);
+            f = __builtin_amdgcn_readlane(wave_incl_min_rev(m ? e0 + __ffs(m) - 1 : kNone, lane), 0);
         }
         if (lane == 0) {
             sc[w][0][s] = l;
@@ -194,27 +205,36 @@ __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img,
         const int l = m ? e0 + 31 - __clz(m) : -kNone;
         const int f = m ? e0 + __ffs(m) - 1 : kNone;
         const int inc_l = wave_incl_max(l, lane);
-        int pc = __shfl_up(inc_l, 1, kWave);
-        if (lane == 0) pc = -kNone;
+        int pc = __builtin_amdgcn_update_dpp(-kNone, inc_l, 0x138, 0xf, 0xf, false);  // wave_shr:1
         pc = max(pc, pcar);
         const int inc_f = wave_incl_min_rev(f, lane);
-        int nc = __shfl_down(inc_f, 1, kWave);
-        if (lane == kWave - 1) nc = kNone;
+        int nc = __builtin_amdgcn_update_dpp(kNone, inc_f, 0x130, 0xf, 0xf, false);  // wave_shl:1
         nc = min(nc, ncar);
-        uint32_t dv[8];  // two distances per word
+        // running distances, no bit searches: bm[b] = -1 where element b is solid; the
+        // distance to the previous solid restarts at 0 there and otherwise grows by one
+        // (forward from element -1, pc < e0), likewise to the next solid (backward from
+        // element 16, nc >= e0 + 16 except past a partial row's end, where nothing is
+        // stored); min(.., kNoSolid) turns the "no solid on either side" sentinels
+        // (about kNone) into kNoSolid, real distances being below 16 * 1024
+        int bm[16], dl[16];
+#pragma unroll
+        for (int b = 0; b < 16; ++b) bm[b] = (int)(m << (31 - b)) >> 31;
+        int cur = e0 - 1 - pc;
 #pragma unroll
         for (int b = 0; b < 16; ++b) {
-            const uint32_t below = m & ((2u << b) - 1u);
-            const uint32_t above = m & ~((1u << b) - 1u);
-            const int e = e0 + b;
-            const int prev = below ? e0 + 31 - __clz(below) : pc;
-            const int next = above ? e0 + __ffs(above) - 1 : nc;
-            const int d = min(e - prev, next - e);
-            const uint32_t v = d >= kNone / 2 ? kNoSolid : (uint32_t)d;
+            cur = ~bm[b] & (cur + 1);
+            dl[b] = cur;
+        }
+        uint32_t dv[8];  // two distances per word
+        int curr = nc - (e0 + 16);
+#pragma unroll
+        for (int b = 15; b >= 0; --b) {
+            curr = ~bm[b] & (curr + 1);
+            const uint32_t v = (uint32_t)min(min(dl[b], curr), (int)kNoSolid);
             if (b & 1)
-                dv[b >> 1] |= v << 16;
+                dv[b >> 1] = v << 16;
             else
-                dv[b >> 1] = v;
+                dv[b >> 1] |= v;
         }
         if (e0 + 16 <= nz) {
             uint4 *q = reinterpret_cast<uint4 *>(dst + e0);  // rows are 16-byte aligned

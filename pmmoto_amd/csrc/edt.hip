@@ -477,7 +477,9 @@ struct WinGeom {
 // distances (kNoSolid: none), pass X the int32 squared distances of pass Y.
 __device__ __forceinline__ int decode(uint16_t d, int cap)
 {
-    return d == kNoSolid ? cap : min((int)d * (int)d, cap);
+    // kNoSolid^2 = 0xfffe0001 fits in 32 bits and exceeds every cap (<= kBig), so one
+    // unsigned minimum also maps "no solid" to cap
+    return (int)min((uint32_t)d * (uint32_t)d, (uint32_t)cap);
 }
 __device__ __forceinline__ int decode(int32_t v, int cap) { return min(v, cap); }
 
@@ -524,8 +526,22 @@ __device__ __forceinline__ int4 unpack(int4 r, int cap)
 // per-column minimum
 template <int N, class V>
 __device__ __forceinline__ void commit_rows(int *tile, const WinGeom &g, int dst, int row0, int q, int zl,
-                                            const int (&mrow)[N], const V (&raw)[N], int4 &acc)
+                                            const int (&mrow)[N], const V (&raw)[N], int4 &acc, bool inside)
 {
+    // inside (wave-uniform): every staged row lies in the line; with a full quad of columns
+    // the values go to the tile as loaded
+    if (inside && zl >= 3) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const int4 v = unpack(raw[i], g.cap);
+            acc.x = min(acc.x, v.x);
+            acc.y = min(acc.y, v.y);
+            acc.z = min(acc.z, v.z);
+            acc.w = min(acc.w, v.w);
+            *reinterpret_cast<int4 *>(tile + (dst + row0 + 16 * i) * kTileZ + 4 * q) = v;
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const bool in_line = mrow[i] >= 0;
@@ -605,7 +621,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
         int fm[kRows / 16];
         V fr[kRows / 16];
         issue_rows<kRows / 16>(base, g, r0 - W, row0, c4, fm, fr);
-        commit_rows<kRows / 16>(tile, g, 0, row0, q, zl, fm, fr, acc);
+        commit_rows<kRows / 16>(tile, g, 0, row0, q, zl, fm, fr, acc, r0 - W >= 0 && r0 - W + kRows <= g.n);
     }
     for (int rt = chunk * kTilesPerBlock; rt < rt_end; ++rt, r0 += T) {
 #pragma unroll
@@ -685,7 +701,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
             acc.z = min(acc.z, h[i].z);
             acc.w = min(acc.w, h[i].w);
         }
-        commit_rows<T / 16>(tile, g, 2 * W, row0, q, zl, xm, xr, acc);
+        commit_rows<T / 16>(tile, g, 2 * W, row0, q, zl, xm, xr, acc, r0 + T + W >= 0 && r0 + 2 * T + W <= g.n);
     }
 }
 

@@ -249,6 +249,128 @@ __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img,
     }
 }
 
+// Pass Z for short rows (nz <= 16 L): L lanes per row, 64 / L rows per
+// wave, the scans segmented at the 16-lane DPP rows (L = 16: no cross-row
+// step; L = 32: rows 1 and 3 take rows 0 and 2).  Same arithmetic as
+// k_pass_z with one super-chunk.
+template <int L>
+__device__ __forceinline__ int seg_incl_max(int v, int lane)
+{
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+    if constexpr (L == 32) {
+        const int r0 = __builtin_amdgcn_readlane(v, 15), r2 = __builtin_amdgcn_readlane(v, 47);
+        const int row = lane >> 4;
+        v = max(v, row == 1 ? r0 : row == 3 ? r2 : -kNone);
+    }
+    return v;
+}
+
+template <int L>
+__device__ __forceinline__ int seg_incl_min_rev(int v, int lane)
+{
+    v = min(v, __builtin_amdgcn_update_dpp(kNone, v, 0x101, 0xf, 0xf, false));  // row_shl:1
+    v = min(v, __builtin_amdgcn_update_dpp(kNone, v, 0x102, 0xf, 0xf, false));  // row_shl:2
+    v = min(v, __builtin_amdgcn_update_dpp(kNone, v, 0x104, 0xf, 0xf, false));  // row_shl:4
+    v = min(v, __builtin_amdgcn_update_dpp(kNone, v, 0x108, 0xf, 0xf, false));  // row_shl:8
+    if constexpr (L == 32) {
+        const int r1 = __builtin_amdgcn_readlane(v, 16), r3 = __builtin_amdgcn_readlane(v, 48);
+        const int row = lane >> 4;
+        v = min(v, row == 0 ? r1 : row == 2 ? r3 : kNone);
+    }
+    return v;
+}
+
+// the value of lane seg * L + pos, for every lane of segment seg
+template <int L>
+__device__ __forceinline__ int seg_lane(int v, int seg, int pos)
+{
+    if constexpr (L == 16) {
+        const int a = __builtin_amdgcn_readlane(v, pos), b = __builtin_amdgcn_readlane(v, 16 + pos),
+                  c = __builtin_amdgcn_readlane(v, 32 + pos), d = __builtin_amdgcn_readlane(v, 48 + pos);
+        return seg == 0 ? a : seg == 1 ? b : seg == 2 ? c : d;
+    } else {
+        const int a = __builtin_amdgcn_readlane(v, pos), b = __builtin_amdgcn_readlane(v, 32 + pos);
+        return seg == 0 ? a : b;
+    }
+}
+
+template <bool kInvert, int L>
+__global__ __launch_bounds__(256) void k_pass_z_short(const uint8_t *__restrict__ img, int64_t in_sx,
+                                                      int64_t in_sy, int ny, int nz, int64_t rows, int bc_lo,
+                                                      int bc_hi, int vec_in, int64_t pitch,
+                                                      uint16_t *__restrict__ out, uint32_t *__restrict__ counters,
+                                                      const unsigned int *__restrict__ proceed)
+{
+    if (proceed && *proceed == 0u) return;
+    constexpr int R = kWave / L;  // rows per wave
+    const int lane = threadIdx.x & (kWave - 1);
+    const int seg = lane / L, sl = lane % L;
+    const int64_t row0 = ((int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * R;
+    if (blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0;  // deferred-line counts
+    if (row0 >= rows) return;  // wave-uniform: the scans below need every lane of the wave
+    const int64_t row = row0 + seg;
+    const bool valid = row < rows;
+    const int64_t rr = valid ? row : row0;
+    const int64_t x = rr / ny, y = rr - x * ny;
+    const uint8_t *src = img + x * in_sx + y * in_sy;
+    const int e0 = sl * 16;
+    const uint32_t m = valid && e0 < nz ? solid_mask16<kInvert>(src, e0, nz, vec_in != 0) : 0u;
+    const int l = m ? e0 + 31 - __clz(m) : -kNone;
+    const int f = m ? e0 + __ffs(m) - 1 : kNone;
+    const int inc_l = seg_incl_max<L>(l, lane);
+    const int inc_f = seg_incl_min_rev<L>(f, lane);
+    int prev0, next0;  // the solids beyond the row's ends
+    if (bc_lo == PMX_BC_PERIODIC) {
+        const int row_last = seg_lane<L>(inc_l, seg, L - 1), row_first = seg_lane<L>(inc_f, seg, 0);
+        prev0 = row_last > -kNone ? row_last - nz : -kNone;
+        next0 = row_first < kNone ? row_first + nz : kNone;
+    } else {
+        prev0 = bc_lo == PMX_BC_WALL ? -1 : -kNone;
+        next0 = bc_hi == PMX_BC_WALL ? nz : kNone;
+    }
+    int pc = __builtin_amdgcn_update_dpp(-kNone, inc_l, 0x138, 0xf, 0xf, false);  // wave_shr:1
+    if (sl == 0) pc = -kNone;
+    pc = max(pc, prev0);
+    int nc = __builtin_amdgcn_update_dpp(kNone, inc_f, 0x130, 0xf, 0xf, false);  // wave_shl:1
+    if (sl == L - 1) nc = kNone;
+    nc = min(nc, next0);
+    // running distances as in k_pass_z
+    int bm[16], dl[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) bm[b] = (int)(m << (31 - b)) >> 31;
+    int cur = e0 - 1 - pc;
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+        cur = ~bm[b] & (cur + 1);
+        dl[b] = cur;
+    }
+    uint32_t dv[8];
+    int curr = nc - (e0 + 16);
+#pragma unroll
+    for (int b = 15; b >= 0; --b) {
+        curr = ~bm[b] & (curr + 1);
+        const uint32_t v = (uint32_t)min(min(dl[b], curr), (int)kNoSolid);
+        if (b & 1)
+            dv[b >> 1] = v << 16;
+        else
+            dv[b >> 1] |= v;
+    }
+    if (!valid || e0 >= nz) return;
+    uint16_t *dst = out + row * pitch;
+    if (e0 + 16 <= nz) {
+        uint4 *q = reinterpret_cast<uint4 *>(dst + e0);  // rows are 16-byte aligned
+        q[0] = make_uint4(dv[0], dv[1], dv[2], dv[3]);
+        q[1] = make_uint4(dv[4], dv[5], dv[6], dv[7]);
+    } else {
+#pragma unroll
+        for (int b = 0; b < 16; ++b)
+            if (e0 + b < nz) dst[e0 + b] = (uint16_t)(dv[b >> 1] >> (16 * (b & 1)));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // passes Y and X: lower envelope of parabolas along strided lines
 // ---------------------------------------------------------------------------
@@ -862,7 +984,22 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     const dim3 bz(256);
     const dim3 gz((unsigned)((rows + 3) / 4));
     const int vec_in = ((uintptr_t)img % 16 == 0 && g->in_s[0] % 16 == 0 && g->in_s[1] % 16 == 0) ? 1 : 0;
-    if (invert)
+    // short rows: several rows per wave
+    const int lpr = nz <= 256 ? 16 : nz <= 512 ? 32 : 64;
+    const dim3 gs((unsigned)((rows + 4 * (kWave / lpr) - 1) / (4 * (kWave / lpr))));
+    if (lpr == 16 && invert)
+        hipLaunchKernelGGL((k_pass_z_short<true, 16>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
+                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);
+    else if (lpr == 16)
+        hipLaunchKernelGGL((k_pass_z_short<false, 16>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
+                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);
+    else if (lpr == 32 && invert)
+        hipLaunchKernelGGL((k_pass_z_short<true, 32>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
+                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);
+    else if (lpr == 32)
+        hipLaunchKernelGGL((k_pass_z_short<false, 32>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
+                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);
+    else if (invert)
         hipLaunchKernelGGL(k_pass_z<true>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
                            g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);
     else

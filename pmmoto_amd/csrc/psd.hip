@@ -244,12 +244,31 @@ __global__ void k_store_thresholds(Thresholds th, int n_levels, int64_t *te)
 
 // Without an inlet requirement the eroded sets themselves are the centres:
 // level lv's new centres are the voxels whose erosion level is lv.
+// 1 in each byte of w equal to the byte in pat, else 0 (exact: no carry
+// crosses a byte, (b & 0x7f) + 0x7f <= 0xfe)
+__device__ __forceinline__ uint32_t eq_bytes(uint32_t w, uint32_t pat)
+{
+    const uint32_t x = w ^ pat;
+    return ~(((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x | 0x7f7f7f7fu) >> 7;
+}
+
+// 16 voxels per lane step (level and mask are 256-byte aligned workspace slices)
 __global__ __launch_bounds__(256) void k_mask_level(int64_t n, const uint8_t *__restrict__ level, int lv,
                                                     uint8_t *__restrict__ mask, unsigned int *__restrict__ any)
 {
     bool seen = false;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t pat = 0x01010101u * (uint32_t)(lv & 0xff);
+    const int64_t nv = n / 16, stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint4 *lv4 = reinterpret_cast<const uint4 *>(level);
+    uint4 *m4 = reinterpret_cast<uint4 *>(mask);
+    for (int64_t i = tid; i < nv; i += stride) {
+        const uint4 q = lv4[i];
+        const uint4 r = make_uint4(eq_bytes(q.x, pat), eq_bytes(q.y, pat), eq_bytes(q.z, pat), eq_bytes(q.w, pat));
+        m4[i] = r;
+        seen |= (r.x | r.y | r.z | r.w) != 0u;
+    }
+    for (int64_t i = 16 * nv + tid; i < n; i += stride) {
         const uint8_t m = level[i] == lv;
         mask[i] = m;
         seen |= m != 0;
@@ -401,7 +420,8 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
     for (int lv = 0; lv < n_levels; ++lv) {
         if (!has_inlet) {
             ProfScope ps("psd_mask", s);
-            hipLaunchKernelGGL(k_mask_level, dim3(gr), dim3(256), 0, s, n, p.level, lv, p.mask, p.any + lv);
+            hipLaunchKernelGGL(k_mask_level, dim3(grid_for(n / 16 + 1)), dim3(256), 0, s, n, p.level, lv, p.mask,
+                               p.any + lv);
         } else {
             {
                 ProfScope ps("psd_union", s);

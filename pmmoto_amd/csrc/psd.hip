@@ -217,6 +217,9 @@ __global__ __launch_bounds__(256) void k_mask(int64_t n, int *__restrict__ paren
                                               uint8_t *__restrict__ in_c, uint8_t *__restrict__ mask,
                                               unsigned int *__restrict__ any)
 {
+    __shared__ int blk_seen;
+    if (threadIdx.x == 0) blk_seen = 0;
+    __syncthreads();
     bool seen = false;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -227,8 +230,11 @@ __global__ __launch_bounds__(256) void k_mask(int64_t n, int *__restrict__ paren
         mask[i] = fresh;
         seen |= fresh != 0;
     }
-    // one lane per wave, and only while the flag still reads 0 (one shared word)
-    if (__any(seen) && (threadIdx.x & 63) == 0 && ld(reinterpret_cast<const int *>(any)) == 0) atomicOr(any, 1u);
+    if (__any(seen) && (threadIdx.x & 63) == 0) blk_seen = 1;
+    __syncthreads();
+    // one lane per block, and only while the flag still reads 0: the shared word is
+    // read at device scope (past the per-XCD L2), so few reads and fewer atomics
+    if (threadIdx.x == 0 && blk_seen && ld(reinterpret_cast<const int *>(any)) == 0) atomicOr(any, 1u);
 }
 
 // the erosion thresholds travel as a kernel argument (a graph-capturable
@@ -252,11 +258,13 @@ __device__ __forceinline__ uint32_t eq_bytes(uint32_t w, uint32_t pat)
     return ~(((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x | 0x7f7f7f7fu) >> 7;
 }
 
-// 16 voxels per lane step (level and mask are 256-byte aligned workspace slices)
+// 16 voxels per lane step (level and mask are 256-byte aligned workspace
+// slices).  Whether the level has any voxel comes from the level histogram
+// (k_any_from_counts), not from here: a flag word shared by every wave is
+// read at device scope, past the per-XCD L2, and those reads serialise.
 __global__ __launch_bounds__(256) void k_mask_level(int64_t n, const uint8_t *__restrict__ level, int lv,
-                                                    uint8_t *__restrict__ mask, unsigned int *__restrict__ any)
+                                                    uint8_t *__restrict__ mask)
 {
-    bool seen = false;
     const uint32_t pat = 0x01010101u * (uint32_t)(lv & 0xff);
     const int64_t nv = n / 16, stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,17 +272,15 @@ __global__ __launch_bounds__(256) void k_mask_level(int64_t n, const uint8_t *__
     uint4 *m4 = reinterpret_cast<uint4 *>(mask);
     for (int64_t i = tid; i < nv; i += stride) {
         const uint4 q = lv4[i];
-        const uint4 r = make_uint4(eq_bytes(q.x, pat), eq_bytes(q.y, pat), eq_bytes(q.z, pat), eq_bytes(q.w, pat));
-        m4[i] = r;
-        seen |= (r.x | r.y | r.z | r.w) != 0u;
+        m4[i] = make_uint4(eq_bytes(q.x, pat), eq_bytes(q.y, pat), eq_bytes(q.z, pat), eq_bytes(q.w, pat));
     }
-    for (int64_t i = 16 * nv + tid; i < n; i += stride) {
-        const uint8_t m = level[i] == lv;
-        mask[i] = m;
-        seen |= m != 0;
-    }
-    // one lane per wave, and only while the flag still reads 0 (one shared word)
-    if (__any(seen) && (threadIdx.x & 63) == 0 && ld(reinterpret_cast<const int *>(any)) == 0) atomicOr(any, 1u);
+    for (int64_t i = 16 * nv + tid; i < n; i += stride) mask[i] = level[i] == lv;
+}
+
+// any[k] = 1 where level k has voxels (its new centres without an inlet requirement)
+__global__ void k_any_from_counts(const unsigned int *__restrict__ counts, int n_levels, unsigned int *__restrict__ any)
+{
+    for (int k = threadIdx.x; k < n_levels; k += blockDim.x) any[k] = counts[k] != 0u;
 }
 
 __global__ void k_clear_u32(unsigned int *a, int n)
@@ -407,6 +413,7 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
         hipLaunchKernelGGL(k_levels, dim3(std::min<unsigned>(gr, 2048u)), dim3(256), 0, s, p.d2, n, p.te, n_levels,
                            p.level, p.counts,
                            p.parent, p.flag, p.in_c);
+        if (!has_inlet) hipLaunchKernelGGL(k_any_from_counts, dim3(1), dim3(256), 0, s, p.counts, n_levels, p.any);
         if (has_inlet) {  // the union-find absorbs each level's voxels from these buckets
             hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, s, p.counts, n_levels, p.offsets, p.cursor);
             hipLaunchKernelGGL(k_bucket, dim3(grid_for((n + kBucketItems - 1) / kBucketItems)), dim3(256), 0, s,
@@ -420,8 +427,7 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
     for (int lv = 0; lv < n_levels; ++lv) {
         if (!has_inlet) {
             ProfScope ps("psd_mask", s);
-            hipLaunchKernelGGL(k_mask_level, dim3(grid_for(n / 16 + 1)), dim3(256), 0, s, n, p.level, lv, p.mask,
-                               p.any + lv);
+            hipLaunchKernelGGL(k_mask_level, dim3(grid_for(n / 16 + 1)), dim3(256), 0, s, n, p.level, lv, p.mask);
         } else {
             {
                 ProfScope ps("psd_union", s);

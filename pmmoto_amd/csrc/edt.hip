@@ -141,6 +141,17 @@ constexpr uint16_t kNoSolid = 0xffff;  // pass-Z output where no solid is reacha
 // solids; wave prefix-max / suffix-min scans over the lanes' last / first
 // solid give the carries from other lanes, and sweep 1's per-super-chunk
 // summaries the carries from other super-chunks.
+// the deferred-line bitmaps of passes Y and X start each EDT empty; the
+// short-row pass Z clears them on the way (small domains, where a launch
+// is a large share of the EDT); long rows keep a separate clear (folded
+// into the long-row kernel it cost more than the launch at 1024^3)
+__device__ __forceinline__ void clear_bits(uint32_t *bits, int64_t n)
+{
+    const int64_t base = (int64_t)blockIdx.x * blockDim.x;
+    if (base >= n) return;  // block-uniform: most blocks of a large grid skip at once
+    for (int64_t i = base + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) bits[i] = 0u;
+}
+
 template <bool kInvert>
 __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img, int64_t in_sx,
                                                 int64_t in_sy, int ny, int nz, int64_t rows,
@@ -302,9 +313,11 @@ __global__ __launch_bounds__(256) void k_pass_z_short(const uint8_t *__restrict_
                                                       int64_t in_sy, int ny, int nz, int64_t rows, int bc_lo,
                                                       int bc_hi, int vec_in, int64_t pitch,
                                                       uint16_t *__restrict__ out, uint32_t *__restrict__ counters,
-                                                      const unsigned int *__restrict__ proceed)
+                                                      const unsigned int *__restrict__ proceed,
+                                                      uint32_t *__restrict__ bits, int64_t n_bits)
 {
     if (proceed && *proceed == 0u) return;
+    clear_bits(bits, n_bits);
     constexpr int R = kWave / L;  // rows per wave
     const int lane = threadIdx.x & (kWave - 1);
     const int seg = lane / L, sl = lane % L;
@@ -977,28 +990,34 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     const int nx = (int)g->n[0], ny = (int)g->n[1], nz = (int)g->n[2];
     const int64_t rows = (int64_t)nx * ny;
     const int64_t pitch = p.pitch;
-    hipLaunchKernelGGL(k_clear_words, dim3((unsigned)std::min<int64_t>((p.n_bits + 255) / 256 + 1, 4096)),
-                       dim3(256), 0, s, p.bits_y, (int64_t)p.n_bits);
+
+    // short rows: several rows per wave (that kernel also clears the deferred-line bitmaps)
+    const int lpr = nz <= 256 ? 16 : nz <= 512 ? 32 : 64;
+    if (lpr == 64)
+        hipLaunchKernelGGL(k_clear_words, dim3((unsigned)std::min<int64_t>((p.n_bits + 255) / 256 + 1, 4096)),
+                           dim3(256), 0, s, p.bits_y, (int64_t)p.n_bits);
     {
     ProfScope ps("edt_pass_z", s);
     const dim3 bz(256);
     const dim3 gz((unsigned)((rows + 3) / 4));
     const int vec_in = ((uintptr_t)img % 16 == 0 && g->in_s[0] % 16 == 0 && g->in_s[1] % 16 == 0) ? 1 : 0;
-    // short rows: several rows per wave
-    const int lpr = nz <= 256 ? 16 : nz <= 512 ? 32 : 64;
     const dim3 gs((unsigned)((rows + 4 * (kWave / lpr) - 1) / (4 * (kWave / lpr))));
     if (lpr == 16 && invert)
         hipLaunchKernelGGL((k_pass_z_short<true, 16>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);
+                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed, p.bits_y,
+                           (int64_t)p.n_bits);
     else if (lpr == 16)
         hipLaunchKernelGGL((k_pass_z_short<false, 16>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);
+                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed, p.bits_y,
+                           (int64_t)p.n_bits);
     else if (lpr == 32 && invert)
         hipLaunchKernelGGL((k_pass_z_short<true, 32>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);
+                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed, p.bits_y,
+                           (int64_t)p.n_bits);
     else if (lpr == 32)
         hipLaunchKernelGGL((k_pass_z_short<false, 32>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);
+                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed, p.bits_y,
+                           (int64_t)p.n_bits);
     else if (invert)
         hipLaunchKernelGGL(k_pass_z<true>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
                            g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);

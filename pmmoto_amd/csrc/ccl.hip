@@ -187,44 +187,69 @@ __global__ __launch_bounds__(256) void k_local(const uint8_t *__restrict__ img, 
 __global__ __launch_bounds__(256) void k_border(const uint8_t *__restrict__ img, Box b, TileGeom tg,
                                                 int *__restrict__ parent)
 {
+    // the (parent of i, parent of j) pairs this block has joined: a pair already in the set adds
+    // nothing (the lane that inserted it joins them); a full probe falls back to joining
+    constexpr int kHash = 1024, kProbe = 8;
+    constexpr unsigned long long kEmpty = ~0ull;
+    __shared__ unsigned long long seen[kHash];
+    for (int h = threadIdx.x; h < kHash; h += blockDim.x) seen[h] = kEmpty;
+    __syncthreads();
     int x0, y0, z0, ex, ey, ez;
     tile_of(b, tg, blockIdx.x, x0, y0, z0, ex, ey, ez);
+    const int lane = threadIdx.x & 63;
+    // every lane runs every step (kTVox is a multiple of the block), so the shuffles below see
+    // the whole wave: lanes hold z-consecutive voxels, and a lane whose (parent of i, parent of
+    // j) pair equals the previous lane's adds nothing (that lane, or one before it, joins them)
     for (int l = threadIdx.x; l < kTVox; l += blockDim.x) {
         const int lx = l / (kTY * kTZ), ly = (l / kTZ) % kTY, lz = l % kTZ;
-        if (lx >= ex || ly >= ey || lz >= ez) continue;
+        const bool in_tile = lx < ex && ly < ey && lz < ez;
         // interior voxels have every backward neighbour inside the tile
-        if (lx >= 1 && ly >= 1 && ly + 1 < ey && lz >= 1 && lz + 1 < ez) continue;
+        const bool border = in_tile && !(lx >= 1 && ly >= 1 && ly + 1 < ey && lz >= 1 && lz + 1 < ez);
         const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
-        const uint8_t v = img[(int64_t)x * b.is0 + (int64_t)y * b.is1 + z];
-        if (!v) continue;
-        const int i = (int)(((int64_t)x * b.n1 + y) * b.n2 + z);
+        const uint8_t v = border ? img[(int64_t)x * b.is0 + (int64_t)y * b.is1 + z] : (uint8_t)0;
+        const bool act = v != 0;
+        const int i = act ? (int)(((int64_t)x * b.n1 + y) * b.n2 + z) : 0;
+        const int pi = act ? ld(parent + i) : -1;
         int last = -1;
 #pragma unroll
         for (int k = 0; k < 13; ++k) {
             int dx, dy, dz;
             back_nb(k, dx, dy, dz);
             const int tlx = lx + dx, tly = ly + dy, tlz = lz + dz;
-            if (tlx >= 0 && tly >= 0 && tly < ey && tlz >= 0 && tlz < ez) continue;  // phase 1 had it
+            bool valid = act && !(tlx >= 0 && tly >= 0 && tly < ey && tlz >= 0 && tlz < ez);  // else phase 1 had it
             int nx = x + dx, ny = y + dy, nz = z + dz;
             if (nx < 0) {
-                if (!b.per0) continue;
+                valid = valid && b.per0;
                 nx += b.n0;
             }
             if (ny < 0 || ny >= b.n1) {
-                if (!b.per1) continue;
+                valid = valid && b.per1;
                 ny = ny < 0 ? ny + b.n1 : ny - b.n1;
             }
             if (nz < 0 || nz >= b.n2) {
-                if (!b.per2) continue;
+                valid = valid && b.per2;
                 nz = nz < 0 ? nz + b.n2 : nz - b.n2;
             }
-            if (img[(int64_t)nx * b.is0 + (int64_t)ny * b.is1 + nz] != v) continue;
-            const int j = (int)(((int64_t)nx * b.n1 + ny) * b.n2 + nz);
-            // a neighbour whose parent is the node the last union reached is already joined
-            const int pj = ld(parent + j);
-            if (pj == last) continue;
-            unite(parent, i, j);
-            last = pj;
+            if (valid) valid = img[(int64_t)nx * b.is0 + (int64_t)ny * b.is1 + nz] == v;
+            const int j = valid ? (int)(((int64_t)nx * b.n1 + ny) * b.n2 + nz) : 0;
+            const int pj = valid ? ld(parent + j) : -2;
+            const int ppi = __shfl_up(pi, 1), ppj = __shfl_up(pj, 1);
+            const bool dup = lane > 0 && ppj >= 0 && ppi == pi && ppj == pj;
+            if (valid && !dup && pj != last) {
+                last = pj;
+                const unsigned long long key = ((unsigned long long)(uint32_t)pi << 32) | (uint32_t)pj;
+                uint32_t h = ((uint32_t)pi * 0x9e3779b1u) ^ ((uint32_t)pj * 0x85ebca6bu);
+                bool fresh = true;
+                for (int q = 0; q < kProbe; ++q) {
+                    const unsigned long long old = atomicCAS(&seen[(h + q) & (kHash - 1)], kEmpty, key);
+                    if (old == kEmpty) break;   // inserted: this lane joins the pair
+                    if (old == key) {           // inserted before: already joined
+                        fresh = false;
+                        break;
+                    }
+                }
+                if (fresh) unite(parent, i, j);
+            }
         }
     }
 }

@@ -608,8 +608,16 @@ struct WinGeom {
     int64_t out_os, out_rs;  // output strides of the outer index and of a row (pass Y)
 };
 
+// A squared distance already clamped to a cap <= 65535, stored in 16 bits:
+// the pass-Y output of a clamped EDT (morphology, porosimetry), which
+// halves the traffic between passes Y and X.
+struct Sq16 {
+    uint16_t v;
+};
+
 // Input element -> clamped squared distance: pass Y reads pass Z's uint16
-// distances (kNoSolid: none), pass X the int32 squared distances of pass Y.
+// distances (kNoSolid: none), pass X the squared distances of pass Y (int32,
+// or Sq16 when the cap fits in 16 bits).
 __device__ __forceinline__ int decode(uint16_t d, int cap)
 {
     // kNoSolid^2 = 0xfffe0001 fits in 32 bits and exceeds every cap (<= kBig), so one
@@ -617,6 +625,7 @@ __device__ __forceinline__ int decode(uint16_t d, int cap)
     return (int)min((uint32_t)d * (uint32_t)d, (uint32_t)cap);
 }
 __device__ __forceinline__ int decode(int32_t v, int cap) { return min(v, cap); }
+__device__ __forceinline__ int decode(Sq16 v, int cap) { return min((int)v.v, cap); }
 
 // input at row r of the line starting at `line`, under the boundary rule
 template <class E>
@@ -634,6 +643,7 @@ __device__ __forceinline__ int fetch(const E *line, const WinGeom &g, int r)
 template <class E> struct Quad;
 template <> struct Quad<uint16_t> { using type = uint2; };
 template <> struct Quad<int32_t> { using type = int4; };
+template <> struct Quad<Sq16> { using type = uint2; };
 
 template <int N, class E>
 __device__ __forceinline__ void issue_rows(const E *__restrict__ base, const WinGeom &g, int first, int row0,
@@ -646,12 +656,19 @@ __device__ __forceinline__ void issue_rows(const E *__restrict__ base, const Win
     for (int i = 0; i < N; ++i) raw[i] = *reinterpret_cast<const V *>(base + (int64_t)max(mrow[i], 0) * g.rs + c4);
 }
 
-__device__ __forceinline__ int4 unpack(uint2 r, int cap)
+// four loaded elements -> four clamped squared distances
+template <class E> __device__ __forceinline__ int4 unpack(typename Quad<E>::type r, int cap);
+template <> __device__ __forceinline__ int4 unpack<uint16_t>(uint2 r, int cap)
 {
     return make_int4(decode((uint16_t)(r.x & 0xffffu), cap), decode((uint16_t)(r.x >> 16), cap),
                      decode((uint16_t)(r.y & 0xffffu), cap), decode((uint16_t)(r.y >> 16), cap));
 }
-__device__ __forceinline__ int4 unpack(int4 r, int cap)
+template <> __device__ __forceinline__ int4 unpack<Sq16>(uint2 r, int cap)
+{
+    return make_int4(min((int)(r.x & 0xffffu), cap), min((int)(r.x >> 16), cap), min((int)(r.y & 0xffffu), cap),
+                     min((int)(r.y >> 16), cap));
+}
+template <> __device__ __forceinline__ int4 unpack<int32_t>(int4 r, int cap)
 {
     return make_int4(min(r.x, cap), min(r.y, cap), min(r.z, cap), min(r.w, cap));
 }
@@ -659,7 +676,7 @@ __device__ __forceinline__ int4 unpack(int4 r, int cap)
 // clamp (columns past nz hold 0, settled at once; rows beyond the line the
 // boundary value) and store at tile rows dst + row0 + 16 i; acc keeps the
 // per-column minimum
-template <int N, class V>
+template <class E, int N, class V>
 __device__ __forceinline__ void commit_rows(int *tile, const WinGeom &g, int dst, int row0, int q, int zl,
                                             const int (&mrow)[N], const V (&raw)[N], int4 &acc, bool inside)
 {
@@ -668,7 +685,7 @@ __device__ __forceinline__ void commit_rows(int *tile, const WinGeom &g, int dst
     if (inside && zl >= 3) {
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            const int4 v = unpack(raw[i], g.cap);
+            const int4 v = unpack<E>(raw[i], g.cap);
             acc.x = min(acc.x, v.x);
             acc.y = min(acc.y, v.y);
             acc.z = min(acc.z, v.z);
@@ -681,7 +698,7 @@ __device__ __forceinline__ void commit_rows(int *tile, const WinGeom &g, int dst
     for (int i = 0; i < N; ++i) {
         const bool in_line = mrow[i] >= 0;
         const int c = mrow[i] == kRowWall ? 0 : g.cap;
-        const int4 u = unpack(raw[i], g.cap);
+        const int4 u = unpack<E>(raw[i], g.cap);
         int4 v;
         v.x = in_line ? (zl >= 0 ? u.x : 0) : c;
         v.y = in_line ? (zl >= 1 ? u.y : 0) : c;
@@ -695,16 +712,17 @@ __device__ __forceinline__ void commit_rows(int *tile, const WinGeom &g, int dst
     }
 }
 
-// E = uint16_t: pass Y (input: pass Z's distances, output: the int32 array
-// `out`); E = int32_t: pass X (input: that array, output: PassXOut).
+// E = uint16_t: pass Y (input: pass Z's distances, output: the array `out`
+// of O = int32_t, or Sq16 under a cap <= 65535); E = int32_t or Sq16: pass X
+// (input: that array, output: PassXOut).
 //
 // The tile is a sliding window: after tile i is scanned its last 2W rows
 // move to the top and only the T new rows are loaded (issued while tile i
 // is scanned), so each row is loaded once per block.  D2ONLY (pass X):
 // the int32 squared distance is the only output, written without the
 // per-row tests of the other outputs.
-template <int T, int W, class E, bool D2ONLY = false>
-__global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in, int32_t *__restrict__ out,
+template <int T, int W, class E, class O, bool D2ONLY = false>
+__global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in, O *__restrict__ out,
                                                         PassXOut o, WinGeom g, int nx, int ny,
                                                         uint32_t *__restrict__ line_bits,
                                                         uint32_t *__restrict__ ovf_count,
@@ -712,7 +730,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
                                                         const unsigned int *__restrict__ proceed)
 {
     if (proceed && *proceed == 0u) return;
-    constexpr bool kY = sizeof(E) == 2;
+    constexpr bool kY = std::is_same<E, uint16_t>::value;
     using V = typename Quad<E>::type;
     static_assert(W % 8 == 0 && T % 16 == 0, "tile layout");
     constexpr int kRows = T + 2 * W;
@@ -756,7 +774,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
         int fm[kRows / 16];
         V fr[kRows / 16];
         issue_rows<kRows / 16>(base, g, r0 - W, row0, c4, fm, fr);
-        commit_rows<kRows / 16>(tile, g, 0, row0, q, zl, fm, fr, acc, r0 - W >= 0 && r0 - W + kRows <= g.n);
+        commit_rows<E, kRows / 16>(tile, g, 0, row0, q, zl, fm, fr, acc, r0 - W >= 0 && r0 - W + kRows <= g.n);
     }
     for (int rt = chunk * kTilesPerBlock; rt < rt_end; ++rt, r0 += T) {
 #pragma unroll
@@ -814,7 +832,11 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
             }
             if (!ok || z >= g.nz) continue;
             if constexpr (kY) {
-                out[(int64_t)outer * g.out_os + (int64_t)y * g.out_rs + z] = best >= kBig ? kInf : best;
+                const int32_t v = best >= kBig ? kInf : best;
+                if constexpr (std::is_same<O, Sq16>::value)
+                    out[(int64_t)outer * g.out_os + (int64_t)y * g.out_rs + z] = Sq16{(uint16_t)v};
+                else
+                    out[(int64_t)outer * g.out_os + (int64_t)y * g.out_rs + z] = v;
             } else if constexpr (D2ONLY) {
                 o.d2[(int64_t)y * o.sx + (int64_t)outer * o.sy + z] = best >= kBig ? kInf : best;
             } else {
@@ -836,7 +858,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
             acc.z = min(acc.z, h[i].z);
             acc.w = min(acc.w, h[i].w);
         }
-        commit_rows<T / 16>(tile, g, 2 * W, row0, q, zl, xm, xr, acc, r0 + T + W >= 0 && r0 + 2 * T + W <= g.n);
+        commit_rows<E, T / 16>(tile, g, 2 * W, row0, q, zl, xm, xr, acc, r0 + T + W >= 0 && r0 + 2 * T + W <= g.n);
     }
 }
 
@@ -848,7 +870,8 @@ __device__ __forceinline__ int finish(int v, int cap)
     return v >= kBig ? kInf : v;
 }
 
-__global__ __launch_bounds__(256) void k_fallback_y(const uint16_t *__restrict__ in, int32_t *__restrict__ out,
+template <class O>
+__global__ __launch_bounds__(256) void k_fallback_y(const uint16_t *__restrict__ in, O *__restrict__ out,
                                                     int nx, int ny, int nz, int64_t pitch, int64_t out_sx,
                                                     int64_t out_sy, int bc_lo, int bc_hi, int cap,
                                                     const uint32_t *__restrict__ ovf_count,
@@ -866,18 +889,24 @@ __global__ __launch_bounds__(256) void k_fallback_y(const uint16_t *__restrict__
         const int64_t line = ovf_list[i];
         const int64_t x = line / nz, z = line - x * nz;
         const uint16_t *src = in + x * ny * pitch + z;
-        int32_t *dst = out + x * out_sx + z;
+        O *dst = out + x * out_sx + z;
         envelope_line(
             ny, bc_lo, bc_hi, st, true,
             [&](int j) {
                 const uint16_t d = src[(int64_t)j * pitch];
                 return d == kNoSolid ? kInf : (int)d * (int)d;
             },
-            [&](int j, int v) { dst[(int64_t)j * out_sy] = finish(v, cap); });
+            [&](int j, int v) {
+                if constexpr (std::is_same<O, Sq16>::value)
+                    dst[(int64_t)j * out_sy] = Sq16{(uint16_t)finish(v, cap)};
+                else
+                    dst[(int64_t)j * out_sy] = finish(v, cap);
+            });
     }
 }
 
-__global__ __launch_bounds__(256) void k_fallback_x(const int32_t *__restrict__ in, PassXOut o,
+template <class E>
+__global__ __launch_bounds__(256) void k_fallback_x(const E *__restrict__ in, PassXOut o,
                                                     int nx, int ny, int nz, int64_t in_sx, int64_t in_sy,
                                                     int bc_lo, int bc_hi, int cap,
                                                     const uint32_t *__restrict__ ovf_count,
@@ -894,9 +923,9 @@ __global__ __launch_bounds__(256) void k_fallback_x(const int32_t *__restrict__ 
     for (int64_t i = tid; i < (int64_t)count; i += nthreads) {
         const int64_t line = ovf_list[i];
         const int64_t y = line / nz, z = line - y * nz;
-        const int32_t *src = in + y * in_sy + z;
+        const E *src = in + y * in_sy + z;
         envelope_line(
-            nx, bc_lo, bc_hi, st, true, [&](int j) { return src[(int64_t)j * in_sx]; },
+            nx, bc_lo, bc_hi, st, true, [&](int j) { return decode(src[(int64_t)j * in_sx], kInf); },
             [&](int j, int v) { emit_x(o, j, y, z, nx, ny, nz, finish(v, cap)); });
     }
 }
@@ -912,23 +941,24 @@ constexpr int kDefaultTileRows = 64;  // rows per tile of the windowed passes (3
 constexpr size_t kFallbackStackBytes = size_t(256) << 20;
 
 // launches one windowed pass with tile rows T and halo W
-template <int T, int W, class E>
-void launch_window(const E *in, int32_t *out, const PassXOut &o, const WinGeom &wg, int nx, int ny,
+template <int T, int W, class E, class O>
+void launch_window(const E *in, O *out, const PassXOut &o, const WinGeom &wg, int nx, int ny,
                    uint32_t *bits, uint32_t *count, uint32_t *list, const unsigned int *proceed, hipStream_t s)
 {
     const int nzt = (wg.nz + kTileZ - 1) / kTileZ;
     const int64_t blocks = (int64_t)wg.nouter * nzt * (((wg.n + T - 1) / T + kTilesPerBlock - 1) / kTilesPerBlock);
-    const bool d2only = sizeof(E) == 4 && o.d2 && !o.f32sq && !o.f32 && !o.f64 && !o.u8 && !o.psd_map;
+    const bool d2only = !std::is_same<E, uint16_t>::value && o.d2 && !o.f32sq && !o.f32 && !o.f64 && !o.u8 &&
+                        !o.psd_map;
     if (d2only)
-        hipLaunchKernelGGL((k_window<T, W, E, true>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, in, out, o, wg,
-                           nx, ny, bits, count, list, proceed);
-    else
-        hipLaunchKernelGGL((k_window<T, W, E, false>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, in, out, o,
+        hipLaunchKernelGGL((k_window<T, W, E, O, true>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, in, out, o,
                            wg, nx, ny, bits, count, list, proceed);
+    else
+        hipLaunchKernelGGL((k_window<T, W, E, O, false>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, in, out,
+                           o, wg, nx, ny, bits, count, list, proceed);
 }
 
-template <class E>
-void dispatch_window(int tile_rows, bool wide, const E *in, int32_t *out, const PassXOut &o, const WinGeom &wg,
+template <class E, class O>
+void dispatch_window(int tile_rows, bool wide, const E *in, O *out, const PassXOut &o, const WinGeom &wg,
                      int nx, int ny, uint32_t *bits, uint32_t *count, uint32_t *list, const unsigned int *proceed,
                      hipStream_t s)
 {
@@ -972,6 +1002,47 @@ size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p)
     p->stk = c.take<int>((size_t)(2 * p->cap + kChunk) * lanes);
     return c.need();
 }
+
+namespace {
+
+// passes Y and X (and their fallbacks) with the intermediate array `b` of
+// type B, stored [y][x][z] with the row pitch of pass Z's array: pass X then
+// reads rows pitch * sizeof(B) bytes apart
+template <class B>
+int run_yx(const EdtPlan &p, const pmx_geom *g, const PassXOut &o, int ccap, bool wide, int tr, B *b,
+           const unsigned int *proceed, hipStream_t s)
+{
+    const int nx = (int)g->n[0], ny = (int)g->n[1], nz = (int)g->n[2];
+    const int64_t pitch = p.pitch;
+    const int64_t bsx = pitch, bsy = (int64_t)nx * pitch;
+    {
+    ProfScope ps("edt_pass_y", s);
+    const WinGeom wg{ny, pitch, ny * pitch, nx, nz, g->bc[2], g->bc[3], ccap, pitch, bsx, bsy};
+    dispatch_window(tr, wide, p.dz, b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, proceed, s);
+    }
+    if (int e = check_launch("pass y")) return e;
+    {
+    ProfScope ps("edt_fallback_y", s);
+    hipLaunchKernelGGL(k_fallback_y<B>, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.dz, b, nx, ny, nz, pitch,
+                       bsx, bsy, g->bc[2], g->bc[3], ccap, p.counters + 0, p.ovf_list, p.stk, p.cap, proceed);
+    }
+    if (int e = check_launch("pass y fallback")) return e;
+    {
+    ProfScope ps("edt_pass_x", s);
+    const WinGeom wg{nx, bsx, bsy, ny, nz, g->bc[0], g->bc[1], ccap, pitch, 0, 0};
+    dispatch_window(tr, wide, (const B *)b, (int32_t *)nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1,
+                    p.ovf_list, proceed, s);
+    }
+    if (int e = check_launch("pass x")) return e;
+    {
+    ProfScope ps("edt_fallback_x", s);
+    hipLaunchKernelGGL(k_fallback_x<B>, dim3(p.fallback_lanes / 256), dim3(256), 0, s, (const B *)b, o, nx, ny, nz,
+                       bsx, bsy, g->bc[0], g->bc[1], ccap, p.counters + 1, p.ovf_list, p.stk, p.cap, proceed);
+    }
+    return check_launch("pass x fallback");
+}
+
+}  // namespace
 
 int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &o, int64_t cap,
             void *ws, size_t ws_bytes, hipStream_t s, const unsigned int *proceed, int tile_rows)
@@ -1027,37 +1098,16 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     }
     if (int e = check_launch("pass z")) return e;
 
-    // pass-Y output stored [y][x][z]: pass X then reads rows 4*pitch bytes apart
-    const int64_t bsx = pitch, bsy = (int64_t)nx * pitch;
     // halo: 16 rows settle nearly every voxel of a porous medium; a clamped
     // EDT whose threshold reaches past 17^2 scans up to 32 rows in LDS
     const bool wide = ccap < kBig && ccap > 17 * 17;
-    {
-    ProfScope ps("edt_pass_y", s);
-    const WinGeom wg{ny, pitch, ny * pitch, nx, nz, g->bc[2], g->bc[3], ccap, pitch, bsx, bsy};
-    dispatch_window(tr, wide, p.dz, p.b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, proceed, s);
-    }
-    if (int e = check_launch("pass y")) return e;
-    {
-    ProfScope ps("edt_fallback_y", s);
-    hipLaunchKernelGGL(k_fallback_y, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.dz, p.b, nx, ny, nz, pitch,
-                       bsx, bsy, g->bc[2], g->bc[3], ccap, p.counters + 0, p.ovf_list, p.stk, p.cap, proceed);
-    }
-    if (int e = check_launch("pass y fallback")) return e;
-    {
-    ProfScope ps("edt_pass_x", s);
-    const WinGeom wg{nx, bsx, bsy, ny, nz, g->bc[0], g->bc[1], ccap, pitch, 0, 0};
-    dispatch_window(tr, wide, (const int32_t *)p.b, (int32_t *)nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1,
-                    p.ovf_list, proceed, s);
-    }
-    if (int e = check_launch("pass x")) return e;
-    {
-    ProfScope ps("edt_fallback_x", s);
-    hipLaunchKernelGGL(k_fallback_x, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.b, o, nx, ny, nz,
-                       bsx, bsy, g->bc[0], g->bc[1], ccap, p.counters + 1, p.ovf_list, p.stk, p.cap, proceed);
-    }
-    return check_launch("pass x fallback");
+    // a cap that fits in 16 bits: pass Y hands pass X 16-bit squared distances
+    if (ccap <= 65535)
+        return run_yx(p, g, o, ccap, wide, tr, reinterpret_cast<Sq16 *>(p.b), proceed, s);
+    return run_yx(p, g, o, ccap, wide, tr, p.b, proceed, s);
 }
+
+
 
 
 int validate_geom(const pmx_geom *g)

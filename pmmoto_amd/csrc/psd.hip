@@ -36,15 +36,18 @@ __device__ __forceinline__ int find_root(int *parent, int x)
     return x;
 }
 
-__device__ __forceinline__ void unite(int *parent, int a, int b)
+// joins the sets of a and b; returns the root of the joined set as it was
+// at the join (a later join may hook it under a smaller root, so it is an
+// element of the set, a short walk from the root)
+__device__ __forceinline__ int unite(int *parent, int a, int b)
 {
     while (true) {
         a = find_root(parent, a);
         b = find_root(parent, b);
-        if (a == b) return;
+        if (a == b) return a;
         const int hi = max(a, b), lo = min(a, b);
         const int old = atomicCAS(parent + hi, hi, lo);
-        if (old == hi) return;
+        if (old == hi) return lo;
         a = old;
         b = lo;
     }
@@ -154,14 +157,21 @@ __global__ __launch_bounds__(256) void k_activate(const int *__restrict__ order,
     }
 }
 
-// each new voxel unions with every active 26-neighbour (wrapping on periodic axes)
+// each new voxel unions with its active 26-neighbours (wrapping on periodic
+// axes).  Activity comes from the level map (active at level lv <=> level
+// <= lv; k_activate has run for this level), a read-only byte array that
+// stays in L2, rather than from the parent array.  A pair of new voxels is
+// joined once, by the one with the larger index.  Each union starts from the
+// root the previous one returned (an element of v's set), not from v.
 __global__ __launch_bounds__(256) void k_merge_new(const int *__restrict__ order, const unsigned int *__restrict__ offsets,
-                                                   int lv, Dims d, int *__restrict__ parent)
+                                                   int lv, Dims d, const uint8_t *__restrict__ level,
+                                                   int *__restrict__ parent)
 {
     const unsigned int begin = offsets[lv], end = offsets[lv + 1];
     for (int64_t k = (int64_t)begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < end;
          k += (int64_t)gridDim.x * blockDim.x) {
         const int v = order[k];
+        int rv = v;
         const int z = v % d.n2, t = v / d.n2, y = t % d.n1, x = t / d.n1;
         for (int dx = -1; dx <= 1; ++dx) {
             int nx = x + dx;
@@ -183,7 +193,8 @@ __global__ __launch_bounds__(256) void k_merge_new(const int *__restrict__ order
                         nz = nz < 0 ? nz + d.n2 : nz - d.n2;
                     }
                     const int u = (nx * d.n1 + ny) * d.n2 + nz;
-                    if (ld(parent + u) != kInactive) unite(parent, v, u);
+                    const int lu = level[u];
+                    if (lu < lv || (lu == lv && u < v)) rv = unite(parent, rv, u);
                 }
             }
         }
@@ -213,7 +224,20 @@ __global__ __launch_bounds__(256) void k_flag_inlet(Dims d, int64_t lo0, int64_t
 // from every older member, so only the new members can invade it at level
 // i: they are the only centres the level's dilation needs.  any: set when
 // the level has a new member (otherwise its dilation is skipped on the GPU).
-__global__ __launch_bounds__(256) void k_mask(int64_t n, int *__restrict__ parent, const uint8_t *__restrict__ flag,
+//
+// 16 voxels per lane step (in_c, level and mask are 256-byte aligned
+// workspace slices).  A member stays a member (its root stays flagged), so
+// only active voxels outside in_c (level <= lv) need a root lookup; a group
+// of 16 members skips the level read.
+__device__ __forceinline__ uint32_t join_byte(int64_t i, uint32_t c, uint32_t l, int lv, int *parent,
+                                              const uint8_t *flag)
+{
+    if (c || (int)l > lv) return 0u;
+    return flag[find_root(parent, (int)i)] ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_mask(int64_t n, const uint8_t *__restrict__ level, int lv,
+                                              int *__restrict__ parent, const uint8_t *__restrict__ flag,
                                               uint8_t *__restrict__ in_c, uint8_t *__restrict__ mask,
                                               unsigned int *__restrict__ any)
 {
@@ -221,14 +245,35 @@ __global__ __launch_bounds__(256) void k_mask(int64_t n, int *__restrict__ paren
     if (threadIdx.x == 0) blk_seen = 0;
     __syncthreads();
     bool seen = false;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        uint8_t m = 0;
-        if (ld(parent + i) != kInactive && flag[find_root(parent, (int)i)]) m = 1;
-        const uint8_t fresh = m && !in_c[i];
-        if (m) in_c[i] = 1;
-        mask[i] = fresh;
-        seen |= fresh != 0;
+    const int64_t nv = n / 16, stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint4 *c4 = reinterpret_cast<uint4 *>(in_c);
+    const uint4 *l4 = reinterpret_cast<const uint4 *>(level);
+    uint4 *m4 = reinterpret_cast<uint4 *>(mask);
+    for (int64_t q = tid; q < nv; q += stride) {
+        const uint4 c = c4[q];
+        uint32_t m[4] = {0u, 0u, 0u, 0u};
+        if ((c.x & c.y & c.z & c.w) != 0x01010101u) {
+            const uint4 l = l4[q];
+            const uint32_t cw[4] = {c.x, c.y, c.z, c.w}, lw[4] = {l.x, l.y, l.z, l.w};
+#pragma unroll
+            for (int w = 0; w < 4; ++w)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    m[w] |= join_byte(16 * q + 4 * w + b, (cw[w] >> (8 * b)) & 0xffu, (lw[w] >> (8 * b)) & 0xffu, lv,
+                                      parent, flag) << (8 * b);
+            if (m[0] | m[1] | m[2] | m[3]) {
+                c4[q] = make_uint4(c.x | m[0], c.y | m[1], c.z | m[2], c.w | m[3]);
+                seen = true;
+            }
+        }
+        m4[q] = make_uint4(m[0], m[1], m[2], m[3]);
+    }
+    for (int64_t i = 16 * nv + tid; i < n; i += stride) {
+        const uint32_t f = join_byte(i, in_c[i], level[i], lv, parent, flag);
+        if (f) in_c[i] = 1;
+        mask[i] = (uint8_t)f;
+        seen |= f != 0u;
     }
     if (__any(seen) && (threadIdx.x & 63) == 0) blk_seen = 1;
     __syncthreads();
@@ -432,13 +477,15 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
             {
                 ProfScope ps("psd_union", s);
                 hipLaunchKernelGGL(k_activate, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, p.parent);
-                hipLaunchKernelGGL(k_merge_new, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, d, p.parent);
+                hipLaunchKernelGGL(k_merge_new, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, d, p.level,
+                                   p.parent);
             }
             ProfScope ps("psd_mask", s);
             const int64_t ni = inlet_ext[0] * inlet_ext[1] * inlet_ext[2];
             hipLaunchKernelGGL(k_flag_inlet, dim3(grid_for(ni)), dim3(256), 0, s, d, inlet_lo[0], inlet_lo[1],
                                inlet_lo[2], inlet_ext[0], inlet_ext[1], inlet_ext[2], p.parent, p.flag);
-            hipLaunchKernelGGL(k_mask, dim3(gr), dim3(256), 0, s, n, p.parent, p.flag, p.in_c, p.mask, p.any + lv);
+            hipLaunchKernelGGL(k_mask, dim3(grid_for(n / 16 + 1)), dim3(256), 0, s, n, p.level, lv, p.parent, p.flag,
+                               p.in_c, p.mask, p.any + lv);
         }
         if (int rc = check_launch("psd level")) return rc;
         // dilation of the level's new centres with the first-invasion update

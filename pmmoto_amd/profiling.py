@@ -28,8 +28,9 @@ GIB = 1 << 30
 # kernel symbol -> the name bench.py reports it under
 BENCH_NAMES = (
     (re.compile(r"k_pass_z<"), "edt_pass_z"),
-    (re.compile(r"k_window<.*unsigned short[,>]"), "edt_pass_y"),
-    (re.compile(r"k_window<.*, int[,>]"), "edt_pass_x"),
+    # k_window<tile rows, halo, input element, pass-Y output element, d2 only>
+    (re.compile(r"k_window<\d+, \d+, unsigned short,"), "edt_pass_y"),
+    (re.compile(r"k_window<\d+, \d+, (int|[\w:() ]*Sq16),"), "edt_pass_x"),
     (re.compile(r"k_fallback_y"), "edt_fallback_y"),
     (re.compile(r"k_fallback_x"), "edt_fallback_x"),
     (re.compile(r"k_conv_first<"), None),  # surrogate inference kernels: reported under their symbol

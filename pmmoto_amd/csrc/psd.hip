@@ -24,11 +24,24 @@ constexpr int kInactive = INT32_MIN;
 
 __device__ __forceinline__ int ld(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
+// Parent reads at device scope (past the per-XCD L2) by default.  The
+// merge of one level reads through L2 (kCached): within one kernel a stale
+// parent is still an ancestor (pointers only move to smaller indices, so
+// every walk ends), and a union acting on a stale root fails its CAS, which
+// is atomic at device scope, and retries from the value it returns.
+template <bool kCached = false>
+__device__ __forceinline__ int ld_parent(const int *p)
+{
+    if constexpr (kCached) return *p;
+    return ld(p);
+}
+
+template <bool kCached = false>
 __device__ __forceinline__ int find_root(int *parent, int x)
 {
-    int p = ld(parent + x);
+    int p = ld_parent<kCached>(parent + x);
     while (p != x) {
-        const int gp = ld(parent + p);
+        const int gp = ld_parent<kCached>(parent + p);
         if (gp != p) parent[x] = gp;
         x = p;
         p = gp;
@@ -39,11 +52,12 @@ __device__ __forceinline__ int find_root(int *parent, int x)
 // joins the sets of a and b; returns the root of the joined set as it was
 // at the join (a later join may hook it under a smaller root, so it is an
 // element of the set, a short walk from the root)
+template <bool kCached = false>
 __device__ __forceinline__ int unite(int *parent, int a, int b)
 {
     while (true) {
-        a = find_root(parent, a);
-        b = find_root(parent, b);
+        a = find_root<kCached>(parent, a);
+        b = find_root<kCached>(parent, b);
         if (a == b) return a;
         const int hi = max(a, b), lo = min(a, b);
         const int old = atomicCAS(parent + hi, hi, lo);
@@ -162,7 +176,9 @@ __global__ __launch_bounds__(256) void k_activate(const int *__restrict__ order,
 // <= lv; k_activate has run for this level), a read-only byte array that
 // stays in L2, rather than from the parent array.  A pair of new voxels is
 // joined once, by the one with the larger index.  Each union starts from the
-// root the previous one returned (an element of v's set), not from v.
+// root the previous one returned (an element of v's set), not from v, and
+// from u's parent; a neighbour whose parent is that root, or the parent just
+// joined, is already in v's set.
 __global__ __launch_bounds__(256) void k_merge_new(const int *__restrict__ order, const unsigned int *__restrict__ offsets,
                                                    int lv, Dims d, const uint8_t *__restrict__ level,
                                                    int *__restrict__ parent)
@@ -171,7 +187,7 @@ __global__ __launch_bounds__(256) void k_merge_new(const int *__restrict__ order
     for (int64_t k = (int64_t)begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < end;
          k += (int64_t)gridDim.x * blockDim.x) {
         const int v = order[k];
-        int rv = v;
+        int rv = v, last = -1;
         const int z = v % d.n2, t = v / d.n2, y = t % d.n1, x = t / d.n1;
         for (int dx = -1; dx <= 1; ++dx) {
             int nx = x + dx;
@@ -194,14 +210,20 @@ __global__ __launch_bounds__(256) void k_merge_new(const int *__restrict__ order
                     }
                     const int u = (nx * d.n1 + ny) * d.n2 + nz;
                     const int lu = level[u];
-                    if (lu < lv || (lu == lv && u < v)) rv = unite(parent, rv, u);
+                    if (lu > lv || (lu == lv && u > v)) continue;
+                    const int pu = parent[u];
+                    if (pu == rv || pu == last) continue;
+                    last = pu;
+                    rv = unite<true>(parent, rv, pu);
                 }
             }
         }
     }
 }
 
-// active voxels of the inlet slab flag their current roots
+// active voxels of the inlet slab flag their current roots.  Here and in
+// k_mask no union runs, so the forest only changes by path halving (a
+// pointer moves to an ancestor) and its reads go through L2.
 __global__ __launch_bounds__(256) void k_flag_inlet(Dims d, int64_t lo0, int64_t lo1, int64_t lo2, int64_t e0,
                                                     int64_t e1, int64_t e2, int *__restrict__ parent,
                                                     uint8_t *__restrict__ flag)
@@ -211,8 +233,8 @@ __global__ __launch_bounds__(256) void k_flag_inlet(Dims d, int64_t lo0, int64_t
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t z = i % e2, t = i / e2, y = t % e1, x = t / e1;
         const int v = (int)(((lo0 + x) * d.n1 + lo1 + y) * d.n2 + lo2 + z);
-        if (ld(parent + v) != kInactive) {
-            const int r = find_root(parent, v);
+        if (parent[v] != kInactive) {
+            const int r = find_root<true>(parent, v);
             if (!flag[r]) flag[r] = 1;  // read first: a large component's voxels share one root
         }
     }
@@ -233,7 +255,7 @@ __device__ __forceinline__ uint32_t join_byte(int64_t i, uint32_t c, uint32_t l,
                                               const uint8_t *flag)
 {
     if (c || (int)l > lv) return 0u;
-    return flag[find_root(parent, (int)i)] ? 1u : 0u;
+    return flag[find_root<true>(parent, (int)i)] ? 1u : 0u;
 }
 
 __global__ __launch_bounds__(256) void k_mask(int64_t n, const uint8_t *__restrict__ level, int lv,

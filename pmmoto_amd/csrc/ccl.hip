@@ -70,6 +70,7 @@ __device__ __forceinline__ void unite(int *parent, int a, int b)
 // globally, only the neighbour pairs phase 1 did not see: those that leave
 // the tile or wrap.  Together they cover every backward pair once.
 constexpr int kTX = 8, kTY = 8, kTZ = 64, kTVox = kTX * kTY * kTZ;  // 16 x 16 x 64 and 8 x 16 x 64 measured slower
+constexpr int kBorderHash = 1024;  // entries of a border block's set of joined parent pairs
 
 struct TileGeom {
     int tx, ty, tz;     // tiles per axis
@@ -184,72 +185,120 @@ __global__ __launch_bounds__(256) void k_local(const uint8_t *__restrict__ img, 
     }
 }
 
+// coordinate c + d on an axis of n voxels: wrapped on a periodic axis,
+// else -1 outside the box
+__device__ __forceinline__ int step(int c, int d, int n, int per)
+{
+    int r = c + d;
+    if (r < 0 || r >= n) {
+        if (!per) return -1;
+        r = r < 0 ? r + n : r - n;
+    }
+    return r;
+}
+
+// One candidate pair per lane (pj < 0: none).  A lane whose (parent of i,
+// parent of j) pair equals the previous lane's adds nothing (that lane, or
+// one before it, joins them), nor does one whose pj it has just joined;
+// the rest go through the block's set of joined parent pairs: a pair
+// already there was joined by the lane that inserted it, and a full probe
+// falls back to joining.  Every lane of the wave calls this (shuffles).
+__device__ __forceinline__ void join_pair(int *parent, unsigned long long *seen, int pi, int pj, int &last, int lane)
+{
+    constexpr int kProbe = 8;
+    constexpr unsigned long long kEmpty = ~0ull;
+    const int ppi = __shfl_up(pi, 1), ppj = __shfl_up(pj, 1);
+    const bool dup = lane > 0 && ppj >= 0 && ppi == pi && ppj == pj;
+    if (pj < 0 || dup || pj == last) return;
+    last = pj;
+    const unsigned long long key = ((unsigned long long)(uint32_t)pi << 32) | (uint32_t)pj;
+    const uint32_t h = ((uint32_t)pi * 0x9e3779b1u) ^ ((uint32_t)pj * 0x85ebca6bu);
+    for (int q = 0; q < kProbe; ++q) {
+        const unsigned long long old = atomicCAS(&seen[(h + q) & (kBorderHash - 1)], kEmpty, key);
+        if (old == kEmpty) break;  // inserted: this lane joins the pair
+        if (old == key) return;    // inserted before: already joined
+    }
+    unite(parent, pi, pj);
+}
+
+// Phase 2: the backward pairs k_local did not see, in two disjoint groups.
+//  (a) Face rows: a z-row of the tile with a backward row R (offsets
+//      (-1,-1), (-1,0), (-1,+1), (0,-1) in x, y) outside the tile's x/y
+//      footprint takes every dz of that R (wrapped on periodic axes).  A
+//      wave holds the row, lane = z: R is loaded once, with one halo voxel
+//      at each end, and dz = -1 / +1 come from lane shuffles.
+//  (b) Z faces: for backward rows inside the footprint (and the voxel's own
+//      row for dz = -1), the pairs whose dz leaves the tile's z range: the
+//      lz = 0 voxels with dz = -1 and the lz = ez-1 voxels with dz = +1, one
+//      lane per row.
+// Parent reads go through L2: a stale parent is still an ancestor, and a
+// union acting on a stale root fails its CAS and retries.
 __global__ __launch_bounds__(256) void k_border(const uint8_t *__restrict__ img, Box b, TileGeom tg,
                                                 int *__restrict__ parent)
 {
-    // the (parent of i, parent of j) pairs this block has joined: a pair already in the set adds
-    // nothing (the lane that inserted it joins them); a full probe falls back to joining
-    constexpr int kHash = 1024, kProbe = 8;
+    static_assert(kTX * kTY == 64 && kTZ == 64, "a z-row per wave, a z face per wave");
     constexpr unsigned long long kEmpty = ~0ull;
-    __shared__ unsigned long long seen[kHash];
-    for (int h = threadIdx.x; h < kHash; h += blockDim.x) seen[h] = kEmpty;
+    __shared__ unsigned long long seen[kBorderHash];
+    for (int h = threadIdx.x; h < kBorderHash; h += blockDim.x) seen[h] = kEmpty;
     __syncthreads();
     int x0, y0, z0, ex, ey, ez;
     tile_of(b, tg, blockIdx.x, x0, y0, z0, ex, ey, ez);
-    const int lane = threadIdx.x & 63;
-    // every lane runs every step (kTVox is a multiple of the block), so the shuffles below see
-    // the whole wave: lanes hold z-consecutive voxels, and a lane whose (parent of i, parent of
-    // j) pair equals the previous lane's adds nothing (that lane, or one before it, joins them)
-    for (int l = threadIdx.x; l < kTVox; l += blockDim.x) {
-        const int lx = l / (kTY * kTZ), ly = (l / kTZ) % kTY, lz = l % kTZ;
-        const bool in_tile = lx < ex && ly < ey && lz < ez;
-        // interior voxels have every backward neighbour inside the tile
-        const bool border = in_tile && !(lx >= 1 && ly >= 1 && ly + 1 < ey && lz >= 1 && lz + 1 < ez);
-        const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
-        const uint8_t v = border ? img[(int64_t)x * b.is0 + (int64_t)y * b.is1 + z] : (uint8_t)0;
-        const bool act = v != 0;
-        const int i = act ? (int)(((int64_t)x * b.n1 + y) * b.n2 + z) : 0;
-        const int pi = act ? ld(parent + i) : -1;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+
+    // (a) face rows
+    for (int row = wv; row < ex * ey; row += 4) {
+        const int lx = row / ey, ly = row - lx * ey;
+        if (lx != 0 && ly != 0 && ly != ey - 1) continue;  // every backward row inside the footprint
+        const int x = x0 + lx, y = y0 + ly, z = z0 + lane;
+        const bool in_z = lane < ez;
+        const uint8_t v = in_z ? img[(int64_t)x * b.is0 + (int64_t)y * b.is1 + z] : (uint8_t)0;
+        const int pi = v ? parent[((int64_t)x * b.n1 + y) * b.n2 + z] : -1;
+        const int zlo = lane == 0 ? step(z0, -1, b.n2, b.per2) : -1;
+        const int zhi = lane == ez - 1 ? step(z0 + ez - 1, 1, b.n2, b.per2) : -1;
         int last = -1;
-#pragma unroll
-        for (int k = 0; k < 13; ++k) {
-            int dx, dy, dz;
-            back_nb(k, dx, dy, dz);
-            const int tlx = lx + dx, tly = ly + dy, tlz = lz + dz;
-            bool valid = act && !(tlx >= 0 && tly >= 0 && tly < ey && tlz >= 0 && tlz < ez);  // else phase 1 had it
-            int nx = x + dx, ny = y + dy, nz = z + dz;
-            if (nx < 0) {
-                valid = valid && b.per0;
-                nx += b.n0;
+        for (int r = 0; r < 4; ++r) {
+            const int dx = r < 3 ? -1 : 0, dy = r < 3 ? r - 1 : -1;
+            if (lx + dx >= 0 && ly + dy >= 0 && ly + dy < ey) continue;  // k_local or the z faces
+            const int nx = step(x, dx, b.n0, b.per0), ny = step(y, dy, b.n1, b.per1);
+            if (nx < 0 || ny < 0) continue;
+            const uint8_t *irow = img + (int64_t)nx * b.is0 + (int64_t)ny * b.is1;
+            const int *prow = parent + ((int64_t)nx * b.n1 + ny) * b.n2;
+            const uint8_t cv = in_z ? irow[z] : (uint8_t)0;
+            const int cp = cv ? prow[z] : -1;
+            int lov = __shfl_up((int)cv, 1), lop = __shfl_up(cp, 1);
+            int hiv = __shfl_down((int)cv, 1), hip = __shfl_down(cp, 1);
+            if (lane == 0) {
+                lov = zlo >= 0 ? irow[zlo] : 0;
+                lop = lov ? prow[zlo] : -1;
             }
-            if (ny < 0 || ny >= b.n1) {
-                valid = valid && b.per1;
-                ny = ny < 0 ? ny + b.n1 : ny - b.n1;
+            if (lane == ez - 1) {
+                hiv = zhi >= 0 ? irow[zhi] : 0;
+                hip = hiv ? prow[zhi] : -1;
             }
-            if (nz < 0 || nz >= b.n2) {
-                valid = valid && b.per2;
-                nz = nz < 0 ? nz + b.n2 : nz - b.n2;
-            }
-            if (valid) valid = img[(int64_t)nx * b.is0 + (int64_t)ny * b.is1 + nz] == v;
-            const int j = valid ? (int)(((int64_t)nx * b.n1 + ny) * b.n2 + nz) : 0;
-            const int pj = valid ? ld(parent + j) : -2;
-            const int ppi = __shfl_up(pi, 1), ppj = __shfl_up(pj, 1);
-            const bool dup = lane > 0 && ppj >= 0 && ppi == pi && ppj == pj;
-            if (valid && !dup && pj != last) {
-                last = pj;
-                const unsigned long long key = ((unsigned long long)(uint32_t)pi << 32) | (uint32_t)pj;
-                uint32_t h = ((uint32_t)pi * 0x9e3779b1u) ^ ((uint32_t)pj * 0x85ebca6bu);
-                bool fresh = true;
-                for (int q = 0; q < kProbe; ++q) {
-                    const unsigned long long old = atomicCAS(&seen[(h + q) & (kHash - 1)], kEmpty, key);
-                    if (old == kEmpty) break;   // inserted: this lane joins the pair
-                    if (old == key) {           // inserted before: already joined
-                        fresh = false;
-                        break;
-                    }
-                }
-                if (fresh) unite(parent, i, j);
-            }
+            join_pair(parent, seen, pi, v && lov == v ? lop : -2, last, lane);
+            join_pair(parent, seen, pi, v && cv == v ? cp : -2, last, lane);
+            join_pair(parent, seen, pi, v && hiv == v ? hip : -2, last, lane);
+        }
+    }
+
+    // (b) z faces: wave 2 the lz = 0 face (dz = -1), wave 3 the lz = ez-1 face (dz = +1)
+    if (wv >= 2) {
+        const bool lo = wv == 2;
+        const int lx = lane / kTY, ly = lane % kTY;
+        const int x = x0 + lx, y = y0 + ly, z = z0 + (lo ? 0 : ez - 1);
+        const int nz = step(z, lo ? -1 : 1, b.n2, b.per2);  // the same for every lane
+        if (nz < 0) return;
+        const uint8_t v = lx < ex && ly < ey ? img[(int64_t)x * b.is0 + (int64_t)y * b.is1 + z] : (uint8_t)0;
+        const int pi = v ? parent[((int64_t)x * b.n1 + y) * b.n2 + z] : -1;
+        int last = -1;
+        for (int r = 0; r < 5; ++r) {
+            const int dx = r < 3 ? -1 : 0, dy = r < 3 ? r - 1 : (r == 3 ? -1 : 0);
+            // rows outside the footprint are the face rows' (a); the own row only for dz = -1
+            const bool inside = lx + dx >= 0 && ly + dy >= 0 && ly + dy < ey && (r < 4 || lo);
+            const int64_t m = (int64_t)(x + dx) * b.is0 + (int64_t)(y + dy) * b.is1 + nz;
+            const uint8_t nv = v && inside ? img[m] : (uint8_t)0;
+            const int pj = v && nv == v ? parent[((int64_t)(x + dx) * b.n1 + (y + dy)) * b.n2 + nz] : -2;
+            join_pair(parent, seen, pi, pj, last, lane);
         }
     }
 }

@@ -185,3 +185,40 @@ def test_porosimetry_modes_and_inlet_kat():
     assert np.array_equal(PO.porosimetry(sd, pm, 2, mode="hybrid"), PO.porosimetry(sd, pm, [2, 2], mode="distance"))
     with pytest.raises(ValueError):
         PO.porosimetry(sd, pm, [2, 2, 2])
+
+
+@pytest.mark.parametrize("shape,per", [
+    ((20, 19, 150), (True, True, True)),     # three z tiles, the last partial; wrap on every axis
+    ((17, 9, 130), (False, False, False)),
+    ((9, 70, 65), (False, True, True)),      # a one-voxel z tile
+    ((3, 2, 200), (True, False, True)),      # one x/y tile, wrap inside it
+    ((1, 1, 64), (False, False, True)),      # one full z tile that wraps onto itself
+    ((26, 17, 2), (True, True, True)),
+])
+def test_ccl26_tiles_match_oracle(shape, per):
+    """pmx_ccl26 on boxes of several (and partial) 8 x 8 x 64 labelling
+    tiles per axis, multi-label values, periodic and END faces."""
+    import ctypes as C
+
+    import torch
+
+    from pmmoto_amd import _device, _lib
+    from pmmoto_amd.filters.distance import euclidean_distance_transform as EDT
+    rng = np.random.default_rng(sum(shape))
+    img = rng.choice([0, 1, 2], size=shape, p=[0.42, 0.46, 0.12]).astype(np.uint8)
+    bc = [_lib.BC_PERIODIC if per[a // 2] else _lib.BC_END for a in range(6)]
+    g = EDT._geom(shape, (0, 0, 0), shape, bc)
+    L = _lib.lib()
+    nb = C.c_size_t()
+    _lib.check(L.pmx_ccl_workspace_size(C.byref(g), C.byref(nb)))
+    ws = _device.workspace(nb.value)
+    dimg = torch.from_numpy(img).cuda()
+    labels = torch.empty(shape, dtype=torch.int32, device="cuda")
+    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    print(f"[pmx_ccl26] shape={shape} periodic={per}", flush=True)
+    _lib.check(L.pmx_ccl26(_device.ptr(dimg), C.byref(g), _device.ptr(labels), None, _device.ptr(count),
+                           _device.ptr(ws), nb.value, _device.stream()))
+    torch.cuda.synchronize()
+    want, n = O.ccl26(img, per)
+    assert int(count.item()) == n
+    np.testing.assert_array_equal(labels.cpu().numpy().astype(np.int64), want)

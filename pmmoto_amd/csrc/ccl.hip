@@ -13,9 +13,10 @@
 //             axes, unioned globally; lock-free union by atomicCAS hooking
 //             the larger root under the smaller, so every component's root
 //             is its smallest C-order index
-//   compress  parent[i] = root(i)
-//   rank      roots counted per 4096-voxel tile, tile counts scanned, each
-//             root's rank written over its parent entry as -(rank+1)
+//   compress  parent[i] = root(i); per 4096-voxel rank tile, its roots as a
+//             bitmap and their count
+//   rank      tile counts scanned; from the bitmap, each root's rank is
+//             written over its parent entry as -(rank+1)
 //   label     label[i] = rank of root(i) + 1: components numbered 1..N in
 //             C-order of their first voxel (the canonical numbering)
 #include "common.hpp"
@@ -24,8 +25,7 @@ namespace pmx {
 namespace {
 
 constexpr int kBg = INT32_MIN;
-constexpr int kTile = 4096;  // voxels per rank tile: 256 lanes x 16
-constexpr int kPer = 16;
+constexpr int kTile = 4096;  // voxels per rank tile: 16 chunks of 256, 64 root words
 
 struct Box {
     int n0, n1, n2;
@@ -306,62 +306,51 @@ __global__ __launch_bounds__(256) void k_border(const uint8_t *__restrict__ img,
 // Read-only walk to the root.  Compression must not halve other voxels'
 // paths: a halving write computed from a stale read could overwrite a
 // pointer another lane has just set to the root with a non-root ancestor,
-// and the label gather relies on every parent being a root.
+// and the label gather relies on every parent being a root.  No union runs
+// any more, so plain (L2) reads are enough: an entry only ever changes to
+// its root.
 __device__ __forceinline__ int root_of(const int *parent, int x)
 {
-    int p = ld(parent + x);
+    int p = parent[x];
     while (p != x) {
         x = p;
-        p = ld(parent + x);
+        p = parent[x];
     }
     return x;
 }
 
-__global__ __launch_bounds__(256) void k_compress(int *__restrict__ parent, int64_t n)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int p = ld(parent + i);
-        if (p == kBg || p == (int)i) continue;
-        // each lane writes only its own entry, so roots (and every pointer
-        // already set to a root) stay as written
-        __hip_atomic_store(parent + i, root_of(parent, (int)i), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
-
-__device__ __forceinline__ int block_excl_scan(int v, int *sh, int *total)
-{
-    // 256 lanes: wave inclusive scans then wave offsets
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int inc = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += t;
-    }
-    if (lane == 63) sh[w] = inc;
-    __syncthreads();
-    int off = 0;
-    for (int k = 0; k < w; ++k) off += sh[k];
-    *total = sh[0] + sh[1] + sh[2] + sh[3];
-    __syncthreads();
-    return off + inc - v;
-}
-
-__global__ __launch_bounds__(256) void k_count_roots(const int *__restrict__ parent, int64_t n,
-                                                     int *__restrict__ tile_counts)
+// One block per rank tile of kTile voxels, in chunks of 256 consecutive
+// voxels: parent[i] = root(i) (each lane writes only its own entry, so roots
+// and pointers already set to a root stay as written), the tile's roots as
+// one bit per voxel (a ballot per 64 voxels), and its root count.
+__global__ __launch_bounds__(256) void k_compress(int *__restrict__ parent, int64_t n,
+                                                  uint64_t *__restrict__ root_bits, int *__restrict__ tile_counts)
 {
     __shared__ int sh[4];
-    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kPer;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kTile;
     int c = 0;
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const int64_t i = base + k;
-        if (i < n && parent[i] == (int)i) ++c;
+    for (int k = 0; k < kTile / 256; ++k) {
+        const int64_t i = base + 256 * k + threadIdx.x;
+        bool root = false;
+        if (i < n) {
+            const int p = parent[i];
+            if (p == (int)i) {
+                root = true;
+            } else if (p != kBg) {
+                const int r = root_of(parent, p);
+                if (r != p) parent[i] = r;
+            }
+        }
+        const uint64_t m = __ballot(root);
+        if (lane == 0) root_bits[(base + 256 * k) / 64 + wv] = m;
+        c += root;
     }
-    int total;
-    block_excl_scan(c, sh, &total);
-    if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (lane == 0) sh[wv] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_counts[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
 // single block: exclusive scan of tile counts in place, total to *count
@@ -392,23 +381,31 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(int *__restrict__ tile_coun
     if (threadIdx.x == 1023) *count = sh[1023];
 }
 
-__global__ __launch_bounds__(256) void k_rank_roots(int *__restrict__ parent, int64_t n,
+// One wave per rank tile (its kTile / 64 root words, one per lane): each
+// root's entry becomes -(rank + 1), rank = its position among all roots in
+// C-order.
+__global__ __launch_bounds__(256) void k_rank_roots(int *__restrict__ parent, int64_t ntiles,
+                                                    const uint64_t *__restrict__ root_bits,
                                                     const int *__restrict__ tile_offsets)
 {
-    __shared__ int sh[4];
-    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kPer;
-    int c = 0;
+    static_assert(kTile == 64 * 64, "one root word per lane");
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= ntiles) return;
+    uint64_t m = root_bits[t * 64 + lane];
+    const int c = __popcll(m);
+    int inc = c;
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const int64_t i = base + k;
-        if (i < n && parent[i] == (int)i) ++c;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(inc, o);
+        if (lane >= o) inc += u;
     }
-    int total;
-    int r = block_excl_scan(c, sh, &total) + tile_offsets[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const int64_t i = base + k;
-        if (i < n && parent[i] == (int)i) parent[i] = -(++r);
+    int r = tile_offsets[t] + inc - c;
+    const int64_t i0 = t * kTile + 64 * lane;
+    while (m) {
+        const int b = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        parent[i0 + b] = -(++r);
     }
 }
 
@@ -417,13 +414,18 @@ __global__ __launch_bounds__(256) void k_labels(const int *__restrict__ parent, 
                                                 int64_t os1, T *__restrict__ out)
 {
     const int64_t n = (int64_t)b.n0 * b.n1 * b.n2;
+    const bool dense = os1 == b.n2 && os0 == (int64_t)b.n1 * b.n2;  // C-contiguous labels: no index split
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int p = parent[i];
         T lab = 0;
         if (p != kBg) lab = (T)(p < 0 ? -(int64_t)p : -(int64_t)parent[p]);
-        const int64_t z = i % b.n2, t = i / b.n2, y = t % b.n1, x = t / b.n1;
-        out[x * os0 + y * os1 + z] = lab;
+        if (dense) {
+            out[i] = lab;
+        } else {
+            const int64_t z = i % b.n2, t = i / b.n2, y = t % b.n1, x = t / b.n1;
+            out[x * os0 + y * os1 + z] = lab;
+        }
     }
 }
 
@@ -469,6 +471,7 @@ __global__ __launch_bounds__(256) void k_keep(const uint8_t *__restrict__ img, c
 
 struct CclPlan {
     int *parent = nullptr, *tiles = nullptr;
+    uint64_t *root_bits = nullptr;  // one bit per voxel, kTile / 64 words per rank tile
     int64_t ntiles = 0;
 };
 
@@ -479,6 +482,7 @@ size_t plan_ccl(const pmx_geom *g, void *ws, CclPlan *p)
     p->parent = c.take<int>((size_t)n);
     p->ntiles = (n + kTile - 1) / kTile;
     p->tiles = c.take<int>((size_t)p->ntiles);
+    p->root_bits = c.take<uint64_t>((size_t)p->ntiles * (kTile / 64));
     return c.need();
 }
 
@@ -535,14 +539,14 @@ int pmx_ccl26(const uint8_t *img, const pmx_geom *g, int32_t *labels32, uint64_t
     if (int e = check_launch("ccl merge")) return e;
     {
         ProfScope ps("ccl_compress", s);
-        hipLaunchKernelGGL(k_compress, dim3(gr), dim3(256), 0, s, p.parent, n);
+        hipLaunchKernelGGL(k_compress, dim3((unsigned)p.ntiles), dim3(256), 0, s, p.parent, n, p.root_bits, p.tiles);
     }
     if (int e = check_launch("ccl compress")) return e;
     {
         ProfScope ps("ccl_rank", s);
-        hipLaunchKernelGGL(k_count_roots, dim3((unsigned)p.ntiles), dim3(256), 0, s, p.parent, n, p.tiles);
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, p.tiles, p.ntiles, count);
-        hipLaunchKernelGGL(k_rank_roots, dim3((unsigned)p.ntiles), dim3(256), 0, s, p.parent, n, p.tiles);
+        hipLaunchKernelGGL(k_rank_roots, dim3((unsigned)((p.ntiles + 3) / 4)), dim3(256), 0, s, p.parent, p.ntiles,
+                           p.root_bits, p.tiles);
     }
     if (int e = check_launch("ccl rank")) return e;
     {

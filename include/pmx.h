@@ -206,10 +206,12 @@ int pmx_max_f32(const float *a, int64_t n, float *out, hipStream_t stream);
 int pmx_max_u8(const uint8_t *a, int64_t n, uint32_t *out, hipStream_t stream);
 
 /* The pore-size-distribution sweep of pore_size_distribution (porosimetry.py:
- * 163-227) in one call, on a contiguous box (in and out strides equal the
- * box).  A padded PMMoTo image with END faces, or the own region with
- * WALL / PERIODIC faces (the torus for the erosions, dilations and the
- * inlet connectivity) give the same codes on the own voxels.  Level i (radii
+ * 163-227) in one call, on a box of img and code_map that share their
+ * strides (a view into a larger array).  A padded PMMoTo image with END
+ * faces, the same box with an all-solid wall pad layer cut off and that
+ * face marked WALL, or the own region with WALL / PERIODIC faces (the torus
+ * for the erosions, dilations and the inlet connectivity) give the same
+ * codes on the own voxels.  Level i (radii
  * in descending order) erodes to d2 > t_erode[i] (non-increasing), keeps
  * the part 26-connected to the inlet slab inlet_lo + [0, inlet_ext) (all of
  * it when inlet_lo is NULL), dilates it to d2 <= t_dilate[i] and writes

@@ -416,9 +416,9 @@ int check_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, co
     if (n_levels < 1 || n_levels > kMaxLevels) return fail(PMX_ERR_INVALID, "n_levels must be in [1, %d]", kMaxLevels);
     const int64_t n = g->n[0] * g->n[1] * g->n[2];
     if (n >= INT32_MAX) return fail(PMX_ERR_INVALID, "box has %lld voxels (limit 2^31-1)", (long long)n);
-    if (g->in_s[0] != g->n[1] * g->n[2] || g->in_s[1] != g->n[2] || g->out_s[0] != g->in_s[0] ||
-        g->out_s[1] != g->in_s[1])
-        return fail(PMX_ERR_INVALID, "the sweep needs contiguous input and output boxes");
+    // the first-invasion update reads the image at the code map's index
+    if (g->out_s[0] != g->in_s[0] || g->out_s[1] != g->in_s[1])
+        return fail(PMX_ERR_INVALID, "the sweep needs the same strides for the image and the code map");
     for (int k = 1; k < n_levels; ++k)
         if (t_erode[k] > t_erode[k - 1])
             return fail(PMX_ERR_INVALID, "erosion thresholds must not increase (radii in descending order)");
@@ -444,15 +444,18 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
     Dims d{(int)g->n[0], (int)g->n[1], (int)g->n[2], g->bc[0] == PMX_BC_PERIODIC, g->bc[2] == PMX_BC_PERIODIC,
            g->bc[4] == PMX_BC_PERIODIC};
     const unsigned gr = grid_for(n);
-    // contiguous box geometry for the internal EDTs
+    // the erosion EDT reads the image with its strides and writes the
+    // internal contiguous d2; the dilations read the internal contiguous mask
     pmx_geom gc = *g;
-    gc.in_s[0] = gc.out_s[0] = g->n[1] * g->n[2];
-    gc.in_s[1] = gc.out_s[1] = g->n[2];
+    gc.out_s[0] = g->n[1] * g->n[2];
+    gc.out_s[1] = g->n[2];
     // A WALL face's layer is solid: a feature of the erosion EDT (distance
     // to solid) but not of the dilations (distance to the centres), which
     // see nothing beyond the face (morphological_operators.py:103-172 pads
     // with solid and dilates from the pore-phase centres only).
     pmx_geom gd = gc;
+    gd.in_s[0] = gc.out_s[0];
+    gd.in_s[1] = gc.out_s[1];
     for (int f = 0; f < 6; ++f)
         if (gd.bc[f] == PMX_BC_WALL) gd.bc[f] = PMX_BC_END;
 

@@ -125,13 +125,41 @@ def fused_sweep_ok(subdomain, mode) -> bool:
     return _unit(subdomain.domain.resolution) or mode == "morph"
 
 
+def _sweep_box(subdomain, pm_t: torch.Tensor):
+    """(origin, extent, faces) of the box the fused sweep runs on.  The
+    padded image with END faces gives the reference's codes; a WALL face's
+    pad layer that holds no pore voxel is cut off and the face marked WALL
+    (a solid layer just outside the box), which gives the same codes: the
+    erosion sees the same solid, the layer holds no dilation centre and is
+    never invaded (its codes stay 0).  The dilations and the internal arrays
+    then run on the own extent (1024 rather than 1026 voxels per row for a
+    1024^3 domain with y/z walls)."""
+    from ..core import BoundaryType
+    shape = list(pm_t.shape)
+    origin, n, bc = [0, 0, 0], list(shape), [_lib.BC_END] * 6
+    for a in range(3):
+        for s in (0, 1):
+            if subdomain.boundary_types[a][s] != BoundaryType.WALL or not subdomain.pad[a][s] or n[a] < 3:
+                continue
+            layer = origin[a] if s == 0 else origin[a] + n[a] - 1
+            if bool(pm_t.select(a, layer).any()):
+                continue
+            if s == 0:
+                origin[a] += 1
+            n[a] -= 1
+            bc[2 * a + s] = _lib.BC_WALL
+    return origin, n, bc
+
+
 def psd_sweep_fused(subdomain, pm_t: torch.Tensor, radii, inlet=False, mode="hybrid") -> torch.Tensor:
     """The whole sweep in one pmx_psd_sweep call (same codes as psd_sweep)."""
     shape = tuple(pm_t.shape)
     te = [_thresholds(subdomain, r, mode == "morph") for r in radii]
     td = [_thresholds(subdomain, r, mode in ("morph", "hybrid")) for r in radii]
-    g = _geom(shape, (0, 0, 0), shape, [_lib.BC_END] * 6)
-    code_map = torch.empty(shape, dtype=torch.int32, device=pm_t.device)
+    origin, n, bc = _sweep_box(subdomain, pm_t)
+    g = _geom(shape, origin, n, bc)
+    off = (origin[0] * shape[1] + origin[1]) * shape[2] + origin[2]
+    code_map = torch.zeros(shape, dtype=torch.int32, device=pm_t.device)
     L = _lib.lib()
     nbytes = C.c_size_t()
     _lib.check(L.pmx_psd_workspace_size(C.byref(g), C.byref(nbytes)))
@@ -143,13 +171,16 @@ def psd_sweep_fused(subdomain, pm_t: torch.Tensor, radii, inlet=False, mode="hyb
             raise NotImplementedError("the fused sweep takes one inlet face")
         if slabs:
             sl = slabs[0]
-            lo = (C.c_int64 * 3)(*[x.start for x in sl])
-            ext = (C.c_int64 * 3)(*[x.stop - x.start for x in sl])
+            # in box coordinates, clipped to the box (a cut pad layer holds no pore voxel)
+            b0 = [min(max(sl[a].start - origin[a], 0), n[a]) for a in range(3)]
+            b1 = [min(max(sl[a].stop - origin[a], 0), n[a]) for a in range(3)]
+            lo = (C.c_int64 * 3)(*b0)
+            ext = (C.c_int64 * 3)(*[b1[a] - b0[a] for a in range(3)])
         else:
             lo = (C.c_int64 * 3)(0, 0, 0)  # inlet requested but no inlet face: nothing connects
             ext = (C.c_int64 * 3)(0, 0, 0)
-    _lib.check(L.pmx_psd_sweep(_device.ptr(pm_t), C.byref(g), lo, ext, len(radii), (C.c_int64 * len(te))(*te),
-                               (C.c_int64 * len(td))(*td), morph._wall_mask(subdomain), _device.ptr(code_map),
+    _lib.check(L.pmx_psd_sweep(_device.ptr(pm_t, off), C.byref(g), lo, ext, len(radii), (C.c_int64 * len(te))(*te),
+                               (C.c_int64 * len(td))(*td), morph._wall_mask(subdomain), _device.ptr(code_map, off),
                                _device.ptr(ws), nbytes.value, _device.stream()))
     return code_map
 

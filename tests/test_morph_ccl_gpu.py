@@ -169,6 +169,26 @@ def test_fused_sweep_equals_per_radius_sweep_and_oracle(mode):
     np.testing.assert_array_equal(got, want)
 
 
+def test_fused_sweep_pad_with_pore_voxel():
+    """A wall pad layer holding a pore voxel is not cut off (only all-solid
+    pad layers are): the fused sweep still equals the per-radius path."""
+    from pmmoto_amd.domain_generation import pack_radii, sphere_pack
+    from pmmoto_amd.filters import porosimetry as PO
+    img_t, _ = sphere_pack((30, 26, 28), 0.4, pack_radii("lognormal", mean=4.0, sigma=0.3), seed=53)
+    img, sd, _ = padded_case(img_t.cpu().numpy(), BT_CASES["cfg3"],
+                             inlet=((True, False), (False, False), (False, False)))
+    img = img.copy()
+    img[5:9, 0, 6:12] = 1  # pore voxels in the y = 0 pad, next to the own region
+    import torch
+    pm_t = torch.from_numpy(img).cuda()
+    origin, n, bc = PO._sweep_box(sd, pm_t)
+    assert origin[1] == 0 and origin[2] == 1 and n[2] == img.shape[2] - 2
+    radii = [np.float64(r) for r in (5.0, 3.5, 2.0, 1.0)]
+    fused = PO.psd_sweep_fused(sd, pm_t, radii, inlet=True, mode="distance").cpu().numpy()
+    levels = PO.psd_sweep_levels(sd, pm_t, radii, inlet=True, mode="distance").cpu().numpy()
+    np.testing.assert_array_equal(fused, levels)
+
+
 def test_porosimetry_modes_and_inlet_kat():
     import pmmoto_amd
     from pmmoto_amd.domain_generation import gen_pm

@@ -84,8 +84,18 @@ __device__ __forceinline__ int wave_incl_min_rev(int v, int lane)
 // solid give the carries from other blocks.
 constexpr int kSup = 16;
 
-template <bool kInvert>
-__device__ __forceinline__ uint32_t bytes_mask(const uint32_t w[4])
+// The voxels the distance is measured to: zeros (the solid of a pore
+// image), nonzeros (kInvert: the centres of a dilation), or, with kMatch,
+// the bytes equal to `match` (a dilation from one level of a level map).
+template <bool kInvert, bool kMatch>
+__device__ __forceinline__ bool is_site(uint32_t v, uint32_t match)
+{
+    if constexpr (kMatch) return v == match;
+    return kInvert ? (v != 0) : (v == 0);
+}
+
+template <bool kInvert, bool kMatch>
+__device__ __forceinline__ uint32_t bytes_mask(const uint32_t w[4], uint32_t match)
 {
     uint32_t m = 0;
 #pragma unroll
@@ -93,7 +103,7 @@ __device__ __forceinline__ uint32_t bytes_mask(const uint32_t w[4])
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const uint32_t v = (w[k] >> (8 * b)) & 0xffu;
-            if (kInvert ? (v != 0) : (v == 0)) m |= 1u << (4 * k + b);
+            if (is_site<kInvert, kMatch>(v, match)) m |= 1u << (4 * k + b);
         }
     return m;
 }
@@ -102,13 +112,13 @@ __device__ __forceinline__ uint32_t bytes_mask(const uint32_t w[4])
 // take one 16-byte load; rows at any other byte offset read the five
 // aligned dwords around the chunk and shift them into place, as long as
 // those dwords lie inside the row; the ends of such rows go byte by byte.
-template <bool kInvert>
-__device__ __forceinline__ uint32_t solid_mask16(const uint8_t *src, int e0, int nz, bool vec)
+template <bool kInvert, bool kMatch>
+__device__ __forceinline__ uint32_t solid_mask16(const uint8_t *src, int e0, int nz, bool vec, uint32_t match)
 {
     if (vec && e0 + 16 <= nz) {
         const uint4 q = *reinterpret_cast<const uint4 *>(src + e0);
         const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-        return bytes_mask<kInvert>(w);
+        return bytes_mask<kInvert, kMatch>(w, match);
     }
     const int sh = (int)((uintptr_t)(src + e0) & 3u);
     if (!vec && e0 - sh >= 0 && e0 - sh + 20 <= nz) {
@@ -116,7 +126,7 @@ __device__ __forceinline__ uint32_t solid_mask16(const uint8_t *src, int e0, int
         const uint32_t d0 = a[0], d1 = a[1], d2 = a[2], d3 = a[3], d4 = a[4];
         const uint32_t w[4] = {__builtin_amdgcn_alignbyte(d1, d0, sh), __builtin_amdgcn_alignbyte(d2, d1, sh),
                                __builtin_amdgcn_alignbyte(d3, d2, sh), __builtin_amdgcn_alignbyte(d4, d3, sh)};
-        return bytes_mask<kInvert>(w);
+        return bytes_mask<kInvert, kMatch>(w, match);
     }
     uint32_t m = 0;
 #pragma unroll
@@ -124,7 +134,7 @@ __device__ __forceinline__ uint32_t solid_mask16(const uint8_t *src, int e0, int
         const int e = e0 + b;
         if (e < nz) {
             const uint8_t v = src[e];
-            if (kInvert ? (v != 0) : (v == 0)) m |= 1u << b;
+            if (is_site<kInvert, kMatch>(v, match)) m |= 1u << b;
         }
     }
     return m;
@@ -152,10 +162,10 @@ __device__ __forceinline__ void clear_bits(uint32_t *bits, int64_t n)
     for (int64_t i = base + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) bits[i] = 0u;
 }
 
-template <bool kInvert>
+template <bool kInvert, bool kMatch>
 __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img, int64_t in_sx,
                                                 int64_t in_sy, int ny, int nz, int64_t rows,
-                                                int bc_lo, int bc_hi, int vec_in, int64_t pitch,
+                                                int bc_lo, int bc_hi, int vec_in, uint32_t match, int64_t pitch,
                                                 uint16_t *__restrict__ out,
                                                 uint32_t *__restrict__ counters,
                                                 const unsigned int *__restrict__ proceed)
@@ -179,7 +189,7 @@ __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img,
     const bool totals = nsup > 1 || bc_lo == PMX_BC_PERIODIC;
     for (int s = 0; s < nsup; ++s) {
         const int e0 = s * 1024 + lane * 16;
-        const uint32_t m = solid_mask16<kInvert>(src, e0, nz, vin);
+        const uint32_t m = solid_mask16<kInvert, kMatch>(src, e0, nz, vin, match);
         smask[w][s][lane] = m;
         int l = -kNone, f = kNone;
         if (totals) {
@@ -308,10 +318,10 @@ __device__ __forceinline__ int seg_lane(int v, int seg, int pos)
     }
 }
 
-template <bool kInvert, int L>
+template <bool kInvert, bool kMatch, int L>
 __global__ __launch_bounds__(256) void k_pass_z_short(const uint8_t *__restrict__ img, int64_t in_sx,
                                                       int64_t in_sy, int ny, int nz, int64_t rows, int bc_lo,
-                                                      int bc_hi, int vec_in, int64_t pitch,
+                                                      int bc_hi, int vec_in, uint32_t match, int64_t pitch,
                                                       uint16_t *__restrict__ out, uint32_t *__restrict__ counters,
                                                       const unsigned int *__restrict__ proceed,
                                                       uint32_t *__restrict__ bits, int64_t n_bits)
@@ -330,7 +340,7 @@ __global__ __launch_bounds__(256) void k_pass_z_short(const uint8_t *__restrict_
     const int64_t x = rr / ny, y = rr - x * ny;
     const uint8_t *src = img + x * in_sx + y * in_sy;
     const int e0 = sl * 16;
-    const uint32_t m = valid && e0 < nz ? solid_mask16<kInvert>(src, e0, nz, vec_in != 0) : 0u;
+    const uint32_t m = valid && e0 < nz ? solid_mask16<kInvert, kMatch>(src, e0, nz, vec_in != 0, match) : 0u;
     const int l = m ? e0 + 31 - __clz(m) : -kNone;
     const int f = m ? e0 + __ffs(m) - 1 : kNone;
     const int inc_l = seg_incl_max<L>(l, lane);
@@ -1045,7 +1055,7 @@ int run_yx(const EdtPlan &p, const pmx_geom *g, const PassXOut &o, int ccap, boo
 }  // namespace
 
 int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &o, int64_t cap,
-            void *ws, size_t ws_bytes, hipStream_t s, const unsigned int *proceed, int tile_rows)
+            void *ws, size_t ws_bytes, hipStream_t s, const unsigned int *proceed, int tile_rows, int match)
 {
     if (tile_rows != 0 && tile_rows != 32 && tile_rows != 64)
         return fail(PMX_ERR_INVALID, "tile_rows %d: use 0 (default), 32 or 64", tile_rows);
@@ -1073,28 +1083,29 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     const dim3 gz((unsigned)((rows + 3) / 4));
     const int vec_in = ((uintptr_t)img % 16 == 0 && g->in_s[0] % 16 == 0 && g->in_s[1] % 16 == 0) ? 1 : 0;
     const dim3 gs((unsigned)((rows + 4 * (kWave / lpr) - 1) / (4 * (kWave / lpr))));
-    if (lpr == 16 && invert)
-        hipLaunchKernelGGL((k_pass_z_short<true, 16>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed, p.bits_y,
-                           (int64_t)p.n_bits);
-    else if (lpr == 16)
-        hipLaunchKernelGGL((k_pass_z_short<false, 16>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed, p.bits_y,
-                           (int64_t)p.n_bits);
-    else if (lpr == 32 && invert)
-        hipLaunchKernelGGL((k_pass_z_short<true, 32>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed, p.bits_y,
-                           (int64_t)p.n_bits);
-    else if (lpr == 32)
-        hipLaunchKernelGGL((k_pass_z_short<false, 32>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed, p.bits_y,
-                           (int64_t)p.n_bits);
-    else if (invert)
-        hipLaunchKernelGGL(k_pass_z<true>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);
-    else
-        hipLaunchKernelGGL(k_pass_z<false>, gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,
-                           g->bc[4], g->bc[5], vec_in, pitch, p.dz, p.counters, proceed);
+    const uint32_t mv = match < 0 ? 0u : (uint32_t)match;
+    const bool mt = match >= 0;
+#define PMX_PASS_Z_SHORT(INV, MT, L)                                                                            \
+    hipLaunchKernelGGL((k_pass_z_short<INV, MT, L>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,  \
+                       g->bc[4], g->bc[5], vec_in, mv, pitch, p.dz, p.counters, proceed, p.bits_y, (int64_t)p.n_bits)
+#define PMX_PASS_Z_LONG(INV, MT)                                                                                \
+    hipLaunchKernelGGL((k_pass_z<INV, MT>), gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows, g->bc[4], \
+                       g->bc[5], vec_in, mv, pitch, p.dz, p.counters, proceed)
+    if (lpr == 16) {
+        if (mt) PMX_PASS_Z_SHORT(true, true, 16);
+        else if (invert) PMX_PASS_Z_SHORT(true, false, 16);
+        else PMX_PASS_Z_SHORT(false, false, 16);
+    } else if (lpr == 32) {
+        if (mt) PMX_PASS_Z_SHORT(true, true, 32);
+        else if (invert) PMX_PASS_Z_SHORT(true, false, 32);
+        else PMX_PASS_Z_SHORT(false, false, 32);
+    } else {
+        if (mt) PMX_PASS_Z_LONG(true, true);
+        else if (invert) PMX_PASS_Z_LONG(true, false);
+        else PMX_PASS_Z_LONG(false, false);
+    }
+#undef PMX_PASS_Z_SHORT
+#undef PMX_PASS_Z_LONG
     }
     if (int e = check_launch("pass z")) return e;
 

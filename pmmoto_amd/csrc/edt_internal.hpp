@@ -43,9 +43,10 @@ size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p);
 // delivered through `o`; stream-ordered on s.  A caller that only tests
 // d2 <= t passes cap = t + 1.  With `proceed` (device memory) set, every
 // kernel does nothing when *proceed == 0 at run time.  tile_rows: rows per
-// tile of the windowed passes, 0 (default 64), 32 or 64.
+// tile of the windowed passes, 0 (default 64), 32 or 64.  match >= 0: the
+// distance is to the voxels whose byte equals match (invert is ignored).
 int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &o, int64_t cap,
             void *ws, size_t ws_bytes, hipStream_t s, const unsigned int *proceed = nullptr,
-            int tile_rows = 0);
+            int tile_rows = 0, int match = -1);
 
 }  // namespace pmx

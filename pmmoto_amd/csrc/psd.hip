@@ -21,6 +21,7 @@ namespace pmx {
 namespace {
 
 constexpr int kInactive = INT32_MIN;
+constexpr uint32_t kNotJoined = 0xff;  // in_c: not (yet) in the inlet-connected set; levels are < 250
 
 __device__ __forceinline__ int ld(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(256) void k_levels(const int32_t *__restrict__ d2, 
         level[i] = (uint8_t)lo;
         parent[i] = kInactive;
         flag[i] = 0;
-        in_c[i] = 0;
+        in_c[i] = kNotJoined;
         if (lo < n_levels) atomicAdd(hist + lo, 1u);
     }
     __syncthreads();
@@ -240,28 +241,37 @@ __global__ __launch_bounds__(256) void k_flag_inlet(Dims d, int64_t lo0, int64_t
     }
 }
 
-// mask = the voxels that joined the inlet-connected eroded set at this
-// level (in_c remembers the set of the previous level; the sets only grow).
-// A pore voxel left unassigned by level i-1 is farther than r_{i-1} > r_i
-// from every older member, so only the new members can invade it at level
-// i: they are the only centres the level's dilation needs.  any: set when
-// the level has a new member (otherwise its dilation is skipped on the GPU).
+// 1 in each byte of w equal to the byte in pat, else 0 (exact: no carry
+// crosses a byte, (b & 0x7f) + 0x7f <= 0xfe)
+__device__ __forceinline__ uint32_t eq_bytes(uint32_t w, uint32_t pat)
+{
+    const uint32_t x = w ^ pat;
+    return ~(((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x | 0x7f7f7f7fu) >> 7;
+}
+
+// in_c[v] = the level at which v joined the inlet-connected eroded set
+// (kNotJoined: not yet; the sets only grow).  A pore voxel left unassigned
+// by level i-1 is farther than r_{i-1} > r_i from every older member, so
+// only the members that join at level i can invade it at level i: the
+// level's dilation takes the voxels with in_c == i as its centres (a byte
+// match in its pass Z), and no per-level mask is written.  any: set when the
+// level has a new member (otherwise its dilation is skipped on the GPU).
 //
-// 16 voxels per lane step (in_c, level and mask are 256-byte aligned
-// workspace slices).  A member stays a member (its root stays flagged), so
-// only active voxels outside in_c (level <= lv) need a root lookup; a group
-// of 16 members skips the level read.
+// 16 voxels per lane step (in_c and level are 256-byte aligned workspace
+// slices).  A member stays a member (its root stays flagged), so only
+// active voxels not yet joined (level <= lv) need a root lookup; a group of
+// 16 members skips the level read, and only groups with a new member are
+// written.
 __device__ __forceinline__ uint32_t join_byte(int64_t i, uint32_t c, uint32_t l, int lv, int *parent,
                                               const uint8_t *flag)
 {
-    if (c || (int)l > lv) return 0u;
-    return flag[find_root<true>(parent, (int)i)] ? 1u : 0u;
+    if (c != kNotJoined || (int)l > lv) return c;
+    return flag[find_root<true>(parent, (int)i)] ? (uint32_t)lv : c;
 }
 
 __global__ __launch_bounds__(256) void k_mask(int64_t n, const uint8_t *__restrict__ level, int lv,
                                               int *__restrict__ parent, const uint8_t *__restrict__ flag,
-                                              uint8_t *__restrict__ in_c, uint8_t *__restrict__ mask,
-                                              unsigned int *__restrict__ any)
+                                              uint8_t *__restrict__ in_c, unsigned int *__restrict__ any)
 {
     __shared__ int blk_seen;
     if (threadIdx.x == 0) blk_seen = 0;
@@ -271,31 +281,33 @@ __global__ __launch_bounds__(256) void k_mask(int64_t n, const uint8_t *__restri
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint4 *c4 = reinterpret_cast<uint4 *>(in_c);
     const uint4 *l4 = reinterpret_cast<const uint4 *>(level);
-    uint4 *m4 = reinterpret_cast<uint4 *>(mask);
     for (int64_t q = tid; q < nv; q += stride) {
         const uint4 c = c4[q];
-        uint32_t m[4] = {0u, 0u, 0u, 0u};
-        if ((c.x & c.y & c.z & c.w) != 0x01010101u) {
-            const uint4 l = l4[q];
-            const uint32_t cw[4] = {c.x, c.y, c.z, c.w}, lw[4] = {l.x, l.y, l.z, l.w};
+        const uint32_t open = eq_bytes(c.x, 0xffffffffu) | eq_bytes(c.y, 0xffffffffu) | eq_bytes(c.z, 0xffffffffu) |
+                              eq_bytes(c.w, 0xffffffffu);
+        if (!open) continue;  // all 16 joined at earlier levels
+        const uint4 l = l4[q];
+        const uint32_t cw[4] = {c.x, c.y, c.z, c.w}, lw[4] = {l.x, l.y, l.z, l.w};
+        uint32_t nw[4];
 #pragma unroll
-            for (int w = 0; w < 4; ++w)
+        for (int w = 0; w < 4; ++w) {
+            nw[w] = 0;
 #pragma unroll
-                for (int b = 0; b < 4; ++b)
-                    m[w] |= join_byte(16 * q + 4 * w + b, (cw[w] >> (8 * b)) & 0xffu, (lw[w] >> (8 * b)) & 0xffu, lv,
-                                      parent, flag) << (8 * b);
-            if (m[0] | m[1] | m[2] | m[3]) {
-                c4[q] = make_uint4(c.x | m[0], c.y | m[1], c.z | m[2], c.w | m[3]);
-                seen = true;
-            }
+            for (int b = 0; b < 4; ++b)
+                nw[w] |= join_byte(16 * q + 4 * w + b, (cw[w] >> (8 * b)) & 0xffu, (lw[w] >> (8 * b)) & 0xffu, lv,
+                                   parent, flag) << (8 * b);
         }
-        m4[q] = make_uint4(m[0], m[1], m[2], m[3]);
+        if (nw[0] != cw[0] || nw[1] != cw[1] || nw[2] != cw[2] || nw[3] != cw[3]) {
+            c4[q] = make_uint4(nw[0], nw[1], nw[2], nw[3]);
+            seen = true;
+        }
     }
     for (int64_t i = 16 * nv + tid; i < n; i += stride) {
-        const uint32_t f = join_byte(i, in_c[i], level[i], lv, parent, flag);
-        if (f) in_c[i] = 1;
-        mask[i] = (uint8_t)f;
-        seen |= f != 0u;
+        const uint32_t c = in_c[i], f = join_byte(i, c, level[i], lv, parent, flag);
+        if (f != c) {
+            in_c[i] = (uint8_t)f;
+            seen = true;
+        }
     }
     if (__any(seen) && (threadIdx.x & 63) == 0) blk_seen = 1;
     __syncthreads();
@@ -317,33 +329,6 @@ __global__ void k_store_thresholds(Thresholds th, int n_levels, int64_t *te)
 
 // Without an inlet requirement the eroded sets themselves are the centres:
 // level lv's new centres are the voxels whose erosion level is lv.
-// 1 in each byte of w equal to the byte in pat, else 0 (exact: no carry
-// crosses a byte, (b & 0x7f) + 0x7f <= 0xfe)
-__device__ __forceinline__ uint32_t eq_bytes(uint32_t w, uint32_t pat)
-{
-    const uint32_t x = w ^ pat;
-    return ~(((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x | 0x7f7f7f7fu) >> 7;
-}
-
-// 16 voxels per lane step (level and mask are 256-byte aligned workspace
-// slices).  Whether the level has any voxel comes from the level histogram
-// (k_any_from_counts), not from here: a flag word shared by every wave is
-// read at device scope, past the per-XCD L2, and those reads serialise.
-__global__ __launch_bounds__(256) void k_mask_level(int64_t n, const uint8_t *__restrict__ level, int lv,
-                                                    uint8_t *__restrict__ mask)
-{
-    const uint32_t pat = 0x01010101u * (uint32_t)(lv & 0xff);
-    const int64_t nv = n / 16, stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint4 *lv4 = reinterpret_cast<const uint4 *>(level);
-    uint4 *m4 = reinterpret_cast<uint4 *>(mask);
-    for (int64_t i = tid; i < nv; i += stride) {
-        const uint4 q = lv4[i];
-        m4[i] = make_uint4(eq_bytes(q.x, pat), eq_bytes(q.y, pat), eq_bytes(q.z, pat), eq_bytes(q.w, pat));
-    }
-    for (int64_t i = 16 * nv + tid; i < n; i += stride) mask[i] = level[i] == lv;
-}
-
 // any[k] = 1 where level k has voxels (its new centres without an inlet requirement)
 __global__ void k_any_from_counts(const unsigned int *__restrict__ counts, int n_levels, unsigned int *__restrict__ any)
 {
@@ -367,7 +352,7 @@ __global__ __launch_bounds__(256) void k_zero_map(int32_t *__restrict__ map, Dim
 
 struct PsdPlan {
     int32_t *d2 = nullptr;
-    uint8_t *level = nullptr, *flag = nullptr, *mask = nullptr, *in_c = nullptr;
+    uint8_t *level = nullptr, *flag = nullptr, *in_c = nullptr;
     int *parent = nullptr, *order = nullptr;
     unsigned int *counts = nullptr, *offsets = nullptr, *cursor = nullptr, *any = nullptr;
     int64_t *te = nullptr;
@@ -382,7 +367,6 @@ size_t plan_psd(const pmx_geom *g, void *ws, PsdPlan *p)
     p->d2 = c.take<int32_t>(n);
     p->level = c.take<uint8_t>(n);
     p->flag = c.take<uint8_t>(n);
-    p->mask = c.take<uint8_t>(n);
     p->in_c = c.take<uint8_t>(n);
     p->parent = c.take<int>(n);
     p->order = c.take<int>(n);
@@ -495,10 +479,9 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
 
     const unsigned gu = std::min<unsigned>(gr, 2048u);  // grid-stride over each level's bucket
     for (int lv = 0; lv < n_levels; ++lv) {
-        if (!has_inlet) {
-            ProfScope ps("psd_mask", s);
-            hipLaunchKernelGGL(k_mask_level, dim3(grid_for(n / 16 + 1)), dim3(256), 0, s, n, p.level, lv, p.mask);
-        } else {
+        // without an inlet the level's new centres are the voxels of level lv:
+        // the dilation matches them in the level map
+        if (has_inlet) {
             {
                 ProfScope ps("psd_union", s);
                 hipLaunchKernelGGL(k_activate, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, p.parent);
@@ -510,7 +493,7 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
             hipLaunchKernelGGL(k_flag_inlet, dim3(grid_for(ni)), dim3(256), 0, s, d, inlet_lo[0], inlet_lo[1],
                                inlet_lo[2], inlet_ext[0], inlet_ext[1], inlet_ext[2], p.parent, p.flag);
             hipLaunchKernelGGL(k_mask, dim3(grid_for(n / 16 + 1)), dim3(256), 0, s, n, p.level, lv, p.parent, p.flag,
-                               p.in_c, p.mask, p.any + lv);
+                               p.in_c, p.any + lv);
         }
         if (int rc = check_launch("psd level")) return rc;
         // dilation of the level's new centres with the first-invasion update
@@ -524,8 +507,8 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
         o.psd_code = lv + 1;
         ProfScope ps("psd_dilate", s);
         // skipped on the GPU when the level has no new centre
-        if (int rc = run_edt(p.mask, &gd, true, o, (int64_t)o.t + 1, p.edt_ws, p.edt_bytes, s, p.any + lv,
-                             tile_rows)) return rc;
+        if (int rc = run_edt(has_inlet ? p.in_c : p.level, &gd, true, o, (int64_t)o.t + 1, p.edt_ws, p.edt_bytes, s,
+                             p.any + lv, tile_rows, lv)) return rc;
     }
     return PMX_OK;
 }

@@ -137,19 +137,31 @@ __device__ __forceinline__ void lunite(int *lp, int a, int b)
 }
 
 __global__ __launch_bounds__(256) void k_local(const uint8_t *__restrict__ img, Box b, TileGeom tg,
-                                               int *__restrict__ parent)
+                                               int vec16, int *__restrict__ parent)
 {
+    static_assert(kTVox == 256 * 16, "one 16-byte quarter row per thread");
     __shared__ int lp[kTVox];
-    __shared__ uint8_t lv[kTVox];
+    __shared__ __align__(16) uint8_t lv[kTVox];
     int x0, y0, z0, ex, ey, ez;
     tile_of(b, tg, blockIdx.x, x0, y0, z0, ex, ey, ez);
-    for (int l = threadIdx.x; l < kTVox; l += blockDim.x) {
-        const int lx = l / (kTY * kTZ), ly = (l / kTZ) % kTY, lz = l % kTZ;
-        uint8_t v = 0;
-        if (lx < ex && ly < ey && lz < ez) v = img[(int64_t)(x0 + lx) * b.is0 + (int64_t)(y0 + ly) * b.is1 + z0 + lz];
-        lv[l] = v;
-        lp[l] = l;
+    {
+        // thread t stages bytes 16 (t % 4) .. +15 of tile row t / 4 (64 rows of 64 bytes)
+        const int r = threadIdx.x >> 2, c0 = (threadIdx.x & 3) * 16;
+        const int lx = r / kTY, ly = r % kTY;
+        const bool row_in = lx < ex && ly < ey;
+        const uint8_t *src = img + (int64_t)(x0 + lx) * b.is0 + (int64_t)(y0 + ly) * b.is1 + z0 + c0;
+        uint4 q = make_uint4(0u, 0u, 0u, 0u);
+        if (row_in && vec16 && c0 + 16 <= ez) {
+            q = *reinterpret_cast<const uint4 *>(src);  // rows 16-byte aligned (checked on the host)
+        } else if (row_in) {
+            uint32_t w[4] = {0u, 0u, 0u, 0u};
+            for (int k = 0; k < 16; ++k)
+                if (c0 + k < ez) w[k >> 2] |= (uint32_t)src[k] << (8 * (k & 3));
+            q = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        *reinterpret_cast<uint4 *>(lv + r * kTZ + c0) = q;
     }
+    for (int l = threadIdx.x; l < kTVox; l += blockDim.x) lp[l] = l;
     __syncthreads();
     for (int l = threadIdx.x; l < kTVox; l += blockDim.x) {
         const uint8_t v = lv[l];
@@ -529,7 +541,8 @@ int pmx_ccl26(const uint8_t *img, const pmx_geom *g, int32_t *labels32, uint64_t
     if (ntile > UINT32_MAX / 2) return fail(PMX_ERR_INVALID, "too many tiles");
     {
         ProfScope ps("ccl_init", s);
-        hipLaunchKernelGGL(k_local, dim3((unsigned)ntile), dim3(256), 0, s, img, b, tg, p.parent);
+        const int vec16 = ((uintptr_t)img % 16 == 0 && g->in_s[0] % 16 == 0 && g->in_s[1] % 16 == 0) ? 1 : 0;
+        hipLaunchKernelGGL(k_local, dim3((unsigned)ntile), dim3(256), 0, s, img, b, tg, vec16, p.parent);
     }
     if (int e = check_launch("ccl local")) return e;
     {

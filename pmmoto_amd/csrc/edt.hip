@@ -1025,13 +1025,18 @@ int run_yx(const EdtPlan &p, const pmx_geom *g, const PassXOut &o, int ccap, boo
     const int nx = (int)g->n[0], ny = (int)g->n[1], nz = (int)g->n[2];
     const int64_t pitch = p.pitch;
     const int64_t bsx = pitch, bsy = (int64_t)nx * pitch;
+    // Every value is <= the cap, so a scan reaching row offset kReach + 1
+    // has k^2 > cap >= best: under a cap below (kReach + 1)^2 no line is ever
+    // deferred (the bitmaps start each EDT clear) and the fallbacks, which
+    // would find empty lists, are not launched.
+    const bool may_defer = ccap >= (kReach + 1) * (kReach + 1);
     {
     ProfScope ps("edt_pass_y", s);
     const WinGeom wg{ny, pitch, ny * pitch, nx, nz, g->bc[2], g->bc[3], ccap, pitch, bsx, bsy};
     dispatch_window(tr, wide, p.dz, b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, proceed, s);
     }
     if (int e = check_launch("pass y")) return e;
-    {
+    if (may_defer) {
     ProfScope ps("edt_fallback_y", s);
     hipLaunchKernelGGL(k_fallback_y<B>, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.dz, b, nx, ny, nz, pitch,
                        bsx, bsy, g->bc[2], g->bc[3], ccap, p.counters + 0, p.ovf_list, p.stk, p.cap, proceed);
@@ -1044,7 +1049,7 @@ int run_yx(const EdtPlan &p, const pmx_geom *g, const PassXOut &o, int ccap, boo
                     p.ovf_list, proceed, s);
     }
     if (int e = check_launch("pass x")) return e;
-    {
+    if (may_defer) {
     ProfScope ps("edt_fallback_x", s);
     hipLaunchKernelGGL(k_fallback_x<B>, dim3(p.fallback_lanes / 256), dim3(256), 0, s, (const B *)b, o, nx, ny, nz,
                        bsx, bsy, g->bc[0], g->bc[1], ccap, p.counters + 1, p.ovf_list, p.stk, p.cap, proceed);

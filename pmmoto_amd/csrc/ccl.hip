@@ -421,17 +421,35 @@ __global__ __launch_bounds__(256) void k_rank_roots(int *__restrict__ parent, in
     }
 }
 
+__device__ __forceinline__ int64_t label_of(const int *parent, int p)
+{
+    return p == kBg ? 0 : (p < 0 ? -(int64_t)p : -(int64_t)parent[p]);
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_labels(const int *__restrict__ parent, Box b, int64_t os0,
                                                 int64_t os1, T *__restrict__ out)
 {
     const int64_t n = (int64_t)b.n0 * b.n1 * b.n2;
     const bool dense = os1 == b.n2 && os0 == (int64_t)b.n1 * b.n2;  // C-contiguous labels: no index split
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int p = parent[i];
-        T lab = 0;
-        if (p != kBg) lab = (T)(p < 0 ? -(int64_t)p : -(int64_t)parent[p]);
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    if constexpr (std::is_same<T, int32_t>::value) {
+        if (dense && (uintptr_t)out % 16 == 0) {
+            // four voxels per lane: one 16-byte parent load and one 16-byte label store
+            // (the parent array is a 256-byte aligned workspace slice)
+            const int4 *p4 = reinterpret_cast<const int4 *>(parent);
+            int4 *o4 = reinterpret_cast<int4 *>(out);
+            for (int64_t q = tid; q < n / 4; q += stride) {
+                const int4 p = p4[q];
+                o4[q] = make_int4((int)label_of(parent, p.x), (int)label_of(parent, p.y), (int)label_of(parent, p.z),
+                                  (int)label_of(parent, p.w));
+            }
+            for (int64_t i = n / 4 * 4 + tid; i < n; i += stride) out[i] = (T)label_of(parent, parent[i]);
+            return;
+        }
+    }
+    for (int64_t i = tid; i < n; i += stride) {
+        const T lab = (T)label_of(parent, parent[i]);
         if (dense) {
             out[i] = lab;
         } else {

@@ -32,6 +32,22 @@ def test_erode_dilate_match_oracle(bts, radius):
     np.testing.assert_array_equal(M.dilate(sd, img2, radius), O.dilate(box, img2, radius))
 
 
+@pytest.mark.parametrize("bts", ["wall", "periodic", "mixed"])
+@pytest.mark.parametrize("radius", [12, 20])
+def test_erode_dilate_large_radius_match_oracle(bts, radius):
+    """Radii whose threshold needs the 32-row halo of the windowed passes
+    (20) or fills the 16-row one (12), against the oracle."""
+    from pmmoto_amd.filters import morphological_operators as M
+    density = {12: 3e-4, 20: 6e-5}[radius]  # about 10 % eroded-kept and 70-90 % dilated: mixed results
+    own = rand_binary((44, 38, 70), density, 17)
+    img, sd, box = padded_case(own, BT_CASES[bts])
+    print(f"[pmx_morph] {bts} r={radius} shape={img.shape}", flush=True)
+    np.testing.assert_array_equal(M.erode(sd, img, radius), O.erode(box, img, radius))
+    sparse = rand_binary((44, 38, 70), 1 - density, 18)
+    img2, _, _ = padded_case(sparse, BT_CASES[bts])
+    np.testing.assert_array_equal(M.dilate(sd, img2, radius), O.dilate(box, img2, radius))
+
+
 def test_opening_closing():
     from pmmoto_amd.filters import morphological_operators as M
     own = rand_binary((30, 30, 30), 0.1, 9)

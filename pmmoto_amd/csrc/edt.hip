@@ -572,7 +572,8 @@ __device__ __forceinline__ void emit_x(const PassXOut &o, int64_t x, int64_t y, 
 // (f = 0) stops the scan at its distance, so the work per voxel is about
 // its distance to the nearest solid (a few voxels in a porous medium).
 // A workgroup owns a strip of 64 z-columns of one line bundle and walks
-// kTilesPerBlock row tiles of T rows, each staged in LDS with W halo rows
+// up to kTilesPerBlock row tiles of T rows (fewer on small boxes, so that a
+// launch still has enough blocks to fill the chip), each staged in LDS with W halo rows
 // on each side; one lane per z-column.  The next tile's loads are issued
 // before the current tile is scanned.  Voxels not settled within W
 // continue the scan in global memory up to kReach rows; a voxel still
@@ -588,10 +589,8 @@ constexpr int kTileZ = 64;
 constexpr int kWinThreads = 256;
 constexpr int kReach = 128;     // farthest row offset scanned before a line is deferred
 constexpr int kRowWall = -1, kRowNone = -2;
-#ifndef PMX_WIN_TILES
-#define PMX_WIN_TILES 8
-#endif
-constexpr int kTilesPerBlock = PMX_WIN_TILES;
+constexpr int kTilesPerBlock = 8;
+constexpr int64_t kFillBlocks = 1024;  // fewer tiles per block below this many blocks per launch
 
 // row r of a line of n rows under the boundary rule: an in-range row,
 // kRowWall (the solid layer just beyond a WALL face) or kRowNone
@@ -616,7 +615,25 @@ struct WinGeom {
     int cap;        // values are clamped to [0, cap], cap <= kBig
     int64_t pitch;  // input row length in memory: a multiple of 8 >= nz, 16-byte aligned rows
     int64_t out_os, out_rs;  // output strides of the outer index and of a row (pass Y)
+    int tpb = kTilesPerBlock;  // row tiles per block (a power of two <= kTilesPerBlock)
 };
+
+// blocks of a windowed pass with T-row tiles and tpb tiles per block
+__host__ __device__ __forceinline__ int64_t window_blocks(const WinGeom &g, int T, int tpb)
+{
+    const int nzt = (g.nz + kTileZ - 1) / kTileZ, nrt = (g.n + T - 1) / T;
+    return (int64_t)g.nouter * nzt * ((nrt + tpb - 1) / tpb);
+}
+
+// tiles per block: halved from kTilesPerBlock while the launch has fewer
+// than kFillBlocks blocks (small boxes; each halving re-reads 2W halo rows
+// per extra block, which matters less there than idle compute units)
+inline int tiles_per_block(const WinGeom &g, int T)
+{
+    int tpb = kTilesPerBlock;
+    while (tpb > 1 && window_blocks(g, T, tpb) < kFillBlocks) tpb /= 2;
+    return tpb;
+}
 
 // A squared distance already clamped to a cap <= 65535, stored in 16 bits:
 // the pass-Y output of a clamped EDT (morphology, porosimetry), which
@@ -751,14 +768,14 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
     __shared__ __align__(16) int part[kWaves][kTileZ];
 
     const int nzt = (g.nz + kTileZ - 1) / kTileZ, nrt = (g.n + T - 1) / T;
-    const int nchunk = (nrt + kTilesPerBlock - 1) / kTilesPerBlock;
+    const int nchunk = (nrt + g.tpb - 1) / g.tpb;
     const int64_t nblocks = (int64_t)g.nouter * nzt * nchunk;
     const int64_t t = xcd_tile(blockIdx.x, nblocks);
     const int chunk = (int)(t % nchunk);
     const int64_t t2 = t / nchunk;
     const int zt = (int)(t2 % nzt);
     const int outer = (int)(t2 / nzt);
-    const int rt_end = min(nrt, (chunk + 1) * kTilesPerBlock);
+    const int rt_end = min(nrt, (chunk + 1) * g.tpb);
     const int z0 = zt * kTileZ;
 
     // the wave index is uniform: keep it (and every row index derived from
@@ -776,7 +793,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
     const E *base = in + (int64_t)outer * g.os + z0;
 
     int4 acc = make_int4(kBig, kBig, kBig, kBig);
-    int r0 = chunk * kTilesPerBlock * T;
+    int r0 = chunk * g.tpb * T;
     int xm[T / 16];
     V xr[T / 16];
     {
@@ -786,7 +803,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
         issue_rows<kRows / 16>(base, g, r0 - W, row0, c4, fm, fr);
         commit_rows<E, kRows / 16>(tile, g, 0, row0, q, zl, fm, fr, acc, r0 - W >= 0 && r0 - W + kRows <= g.n);
     }
-    for (int rt = chunk * kTilesPerBlock; rt < rt_end; ++rt, r0 += T) {
+    for (int rt = chunk * g.tpb; rt < rt_end; ++rt, r0 += T) {
 #pragma unroll
         for (int s = 16; s < kWave; s <<= 1) {
             acc.x = min(acc.x, __shfl_xor(acc.x, s, kWave));
@@ -952,11 +969,12 @@ constexpr size_t kFallbackStackBytes = size_t(256) << 20;
 
 // launches one windowed pass with tile rows T and halo W
 template <int T, int W, class E, class O>
-void launch_window(const E *in, O *out, const PassXOut &o, const WinGeom &wg, int nx, int ny,
+void launch_window(const E *in, O *out, const PassXOut &o, const WinGeom &wg0, int nx, int ny,
                    uint32_t *bits, uint32_t *count, uint32_t *list, const unsigned int *proceed, hipStream_t s)
 {
-    const int nzt = (wg.nz + kTileZ - 1) / kTileZ;
-    const int64_t blocks = (int64_t)wg.nouter * nzt * (((wg.n + T - 1) / T + kTilesPerBlock - 1) / kTilesPerBlock);
+    WinGeom wg = wg0;
+    wg.tpb = tiles_per_block(wg, T);
+    const int64_t blocks = window_blocks(wg, T, wg.tpb);
     const bool d2only = !std::is_same<E, uint16_t>::value && o.d2 && !o.f32sq && !o.f32 && !o.f64 && !o.u8 &&
                         !o.psd_map;
     if (d2only)

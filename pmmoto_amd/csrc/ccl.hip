@@ -315,22 +315,13 @@ __global__ __launch_bounds__(256) void k_border(const uint8_t *__restrict__ img,
     }
 }
 
-// Read-only walk to the root.  Compression must not halve other voxels'
+// The root walks here only read.  Compression must not halve other voxels'
 // paths: a halving write computed from a stale read could overwrite a
 // pointer another lane has just set to the root with a non-root ancestor,
 // and the label gather relies on every parent being a root.  No union runs
 // any more, so plain (L2) reads are enough: an entry only ever changes to
 // its root.
-__device__ __forceinline__ int root_of(const int *parent, int x)
-{
-    int p = parent[x];
-    while (p != x) {
-        x = p;
-        p = parent[x];
-    }
-    return x;
-}
-
+//
 // One block per rank tile of kTile voxels, in chunks of 256 consecutive
 // voxels: parent[i] = root(i) (each lane writes only its own entry, so roots
 // and pointers already set to a root stay as written), the tile's roots as
@@ -342,21 +333,42 @@ __global__ __launch_bounds__(256) void k_compress(int *__restrict__ parent, int6
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t base = (int64_t)blockIdx.x * kTile;
     int c = 0;
-    for (int k = 0; k < kTile / 256; ++k) {
-        const int64_t i = base + 256 * k + threadIdx.x;
-        bool root = false;
-        if (i < n) {
-            const int p = parent[i];
-            if (p == (int)i) {
-                root = true;
-            } else if (p != kBg) {
-                const int r = root_of(parent, p);
-                if (r != p) parent[i] = r;
+    // four chunks at a time: their four root walks advance in lockstep, so a
+    // lane keeps four independent parent loads in flight
+    constexpr int kWays = 4;
+    for (int k = 0; k < kTile / 256; k += kWays) {
+        int64_t i[kWays];
+        int p[kWays], r[kWays];
+        bool root[kWays];
+#pragma unroll
+        for (int s = 0; s < kWays; ++s) {
+            i[s] = base + 256 * (k + s) + threadIdx.x;
+            p[s] = i[s] < n ? parent[i[s]] : kBg;
+            root[s] = p[s] == (int)i[s];
+            r[s] = p[s];
+        }
+        // r[s] walks from p[s] to its root (the entry of a root is itself); background and
+        // roots need no walk
+        bool open[kWays];
+#pragma unroll
+        for (int s = 0; s < kWays; ++s) open[s] = p[s] != kBg && !root[s];
+        while (open[0] || open[1] || open[2] || open[3]) {
+            int q[kWays];
+#pragma unroll
+            for (int s = 0; s < kWays; ++s) q[s] = open[s] ? parent[r[s]] : r[s];
+#pragma unroll
+            for (int s = 0; s < kWays; ++s) {
+                if (q[s] == r[s]) open[s] = false;
+                r[s] = q[s];
             }
         }
-        const uint64_t m = __ballot(root);
-        if (lane == 0) root_bits[(base + 256 * k) / 64 + wv] = m;
-        c += root;
+#pragma unroll
+        for (int s = 0; s < kWays; ++s) {
+            if (p[s] != kBg && !root[s] && r[s] != p[s]) parent[i[s]] = r[s];
+            const uint64_t m = __ballot(root[s]);
+            if (lane == 0) root_bits[(base + 256 * (k + s)) / 64 + wv] = m;
+            c += root[s];
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);

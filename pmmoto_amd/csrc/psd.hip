@@ -288,14 +288,36 @@ __global__ __launch_bounds__(256) void k_mask(int64_t n, const uint8_t *__restri
         if (!open) continue;  // all 16 joined at earlier levels
         const uint4 l = l4[q];
         const uint32_t cw[4] = {c.x, c.y, c.z, c.w}, lw[4] = {l.x, l.y, l.z, l.w};
-        uint32_t nw[4];
+        // the group's candidates (not joined, level <= lv) walk to their roots in lockstep,
+        // up to 16 independent parent loads in flight; each candidate's entry is then set to
+        // its root (no union runs here, so the root is a valid ancestor), which keeps the
+        // walks of the voxels that stay unjoined short at later levels
+        int r[16];
+        uint32_t cand = 0;
 #pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            nw[w] = 0;
+        for (int e = 0; e < 16; ++e) {
+            const uint32_t cb = (cw[e >> 2] >> (8 * (e & 3))) & 0xffu, lb = (lw[e >> 2] >> (8 * (e & 3))) & 0xffu;
+            r[e] = (int)(16 * q + e);
+            if (cb == kNotJoined && (int)lb <= lv) cand |= 1u << e;
+        }
+        uint32_t walking = cand;
+        while (walking) {
+            int nx[16];
 #pragma unroll
-            for (int b = 0; b < 4; ++b)
-                nw[w] |= join_byte(16 * q + 4 * w + b, (cw[w] >> (8 * b)) & 0xffu, (lw[w] >> (8 * b)) & 0xffu, lv,
-                                   parent, flag) << (8 * b);
+            for (int e = 0; e < 16; ++e) nx[e] = (walking >> e) & 1u ? parent[r[e]] : r[e];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                if (nx[e] == r[e]) walking &= ~(1u << e);
+                r[e] = nx[e];
+            }
+        }
+        uint32_t nw[4] = {cw[0], cw[1], cw[2], cw[3]};
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            if (!((cand >> e) & 1u)) continue;
+            const int64_t i = 16 * q + e;
+            if (parent[i] != r[e]) parent[i] = r[e];
+            if (flag[r[e]]) nw[e >> 2] = (nw[e >> 2] & ~(0xffu << (8 * (e & 3)))) | ((uint32_t)lv << (8 * (e & 3)));
         }
         if (nw[0] != cw[0] || nw[1] != cw[1] || nw[2] != cw[2] || nw[3] != cw[3]) {
             c4[q] = make_uint4(nw[0], nw[1], nw[2], nw[3]);

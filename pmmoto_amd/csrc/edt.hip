@@ -546,9 +546,14 @@ __device__ __forceinline__ void emit_x(const PassXOut &o, int64_t x, int64_t y, 
     if (o.f32sq) o.f32sq[idx] = v == kInf ? 3.40282347e+38f : (float)v;
     if (o.f32) o.f32[idx] = v == kInf ? sqrtf(3.40282347e+38f) : sqrtf((float)v);
     if (o.f64) o.f64[idx] = v == kInf ? sqrt((double)3.40282347e+38f) : sqrt((double)v);
-    if (o.psd_map) {
-        // first invasion (porosimetry.py:220-225): covered pore voxels not yet assigned
-        if (v <= o.t && o.psd_pm[idx] == 1 && o.psd_map[idx] == 0) o.psd_map[idx] = o.psd_code;
+    if (o.psd_map && v <= o.t) {
+        // first invasion (porosimetry.py:220-225): covered pore voxels not yet assigned; one
+        // state byte says both (1: pore, not invaded), so a covered voxel costs one byte read
+        const int64_t si = x * o.ssx + y * o.ssy + z;
+        if (o.psd_state[si] == 1) {
+            o.psd_state[si] = 2;
+            o.psd_map[idx] = o.psd_code;
+        }
     }
     if (o.u8) {
         uint8_t r = v <= o.t ? o.v_le : o.v_gt;

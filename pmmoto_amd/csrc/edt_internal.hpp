@@ -17,8 +17,11 @@ struct PassXOut {
     int32_t t = 0;
     uint8_t v_le = 0, v_gt = 0;
     uint32_t wall_mask = 0;
-    // porosimetry: psd_map = psd_code where d2 <= t, psd_pm == 1, psd_map == 0
-    const uint8_t *psd_pm = nullptr;
+    // porosimetry (first invasion): where d2 <= t and psd_state == 1 (a pore
+    // voxel not yet invaded), psd_state = 2 and psd_map = psd_code.  psd_map
+    // has the strides sx, sy; psd_state its own, ssx and ssy.
+    uint8_t *psd_state = nullptr;
+    int64_t ssx = 0, ssy = 0;
     int32_t *psd_map = nullptr;
     int32_t psd_code = 0;
 };

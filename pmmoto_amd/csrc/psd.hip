@@ -362,19 +362,24 @@ __global__ void k_clear_u32(unsigned int *a, int n)
     for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_zero_map(int32_t *__restrict__ map, Dims d, int64_t s0, int64_t s1)
+// code map = 0 over the box; state (contiguous box) = 1 on pore voxels
+// (img == 1), the ones the first-invasion update may still assign
+__global__ __launch_bounds__(256) void k_zero_map(int32_t *__restrict__ map, const uint8_t *__restrict__ img,
+                                                  uint8_t *__restrict__ state, Dims d, int64_t s0, int64_t s1)
 {
     const int64_t n = n_of(d);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t z = i % d.n2, t = i / d.n2, y = t % d.n1, x = t / d.n1;
-        map[x * s0 + y * s1 + z] = 0;
+        const int64_t j = x * s0 + y * s1 + z;
+        map[j] = 0;
+        state[i] = img[j] == 1 ? 1 : 0;
     }
 }
 
 struct PsdPlan {
     int32_t *d2 = nullptr;
-    uint8_t *level = nullptr, *flag = nullptr, *in_c = nullptr;
+    uint8_t *level = nullptr, *flag = nullptr, *in_c = nullptr, *state = nullptr;
     int *parent = nullptr, *order = nullptr;
     unsigned int *counts = nullptr, *offsets = nullptr, *cursor = nullptr, *any = nullptr;
     int64_t *te = nullptr;
@@ -390,6 +395,7 @@ size_t plan_psd(const pmx_geom *g, void *ws, PsdPlan *p)
     p->level = c.take<uint8_t>(n);
     p->flag = c.take<uint8_t>(n);
     p->in_c = c.take<uint8_t>(n);
+    p->state = c.take<uint8_t>(n);
     p->parent = c.take<int>(n);
     p->order = c.take<int>(n);
     p->counts = c.take<unsigned int>(kMaxLevels + 1);
@@ -495,7 +501,8 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
             hipLaunchKernelGGL(k_bucket, dim3(grid_for((n + kBucketItems - 1) / kBucketItems)), dim3(256), 0, s,
                                p.level, n, n_levels, p.cursor, p.order);
         }
-        hipLaunchKernelGGL(k_zero_map, dim3(gr), dim3(256), 0, s, code_map, d, g->out_s[0], g->out_s[1]);
+        hipLaunchKernelGGL(k_zero_map, dim3(gr), dim3(256), 0, s, code_map, img, p.state, d, g->out_s[0],
+                           g->out_s[1]);
     }
     if (int e = check_launch("psd setup")) return e;
 
@@ -524,7 +531,9 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
         o.sy = g->out_s[1];
         o.t = (int32_t)std::min<int64_t>(std::max<int64_t>(t_dilate[lv], -1), INT32_MAX - 1);
         o.wall_mask = wall_mask;
-        o.psd_pm = img;
+        o.psd_state = p.state;
+        o.ssx = g->n[1] * g->n[2];
+        o.ssy = g->n[2];
         o.psd_map = code_map;
         o.psd_code = lv + 1;
         ProfScope ps("psd_dilate", s);

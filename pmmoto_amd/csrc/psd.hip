@@ -22,6 +22,7 @@ namespace {
 
 constexpr int kInactive = INT32_MIN;
 constexpr uint32_t kNotJoined = 0xff;  // in_c: not (yet) in the inlet-connected set; levels are < 250
+constexpr int kNoLevel = 256;          // above every level: no neighbour there
 
 __device__ __forceinline__ int ld(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
@@ -202,16 +203,38 @@ __global__ __launch_bounds__(256) void k_merge_new(const int *__restrict__ order
                     if (!d.per1) continue;
                     ny = ny < 0 ? ny + d.n1 : ny - d.n1;
                 }
-                for (int dz = -1; dz <= 1; ++dz) {
-                    if (!dx && !dy && !dz) continue;
-                    int nz = z + dz;
-                    if (nz < 0 || nz >= d.n2) {
-                        if (!d.per2) continue;
-                        nz = nz < 0 ? nz + d.n2 : nz - d.n2;
+                const int row = (nx * d.n1 + ny) * d.n2;
+                // the levels at z-1, z, z+1: away from the z faces one aligned 8-byte load
+                // holds all three (two when they straddle an 8-byte boundary; the level map is
+                // a 256-byte aligned workspace slice with slack after it)
+                int lz[3] = {kNoLevel, kNoLevel, kNoLevel}, uz[3] = {-1, -1, -1};
+                if (z >= 1 && z + 1 < d.n2) {
+                    const int a = row + z - 1, off = a & 7;
+                    const uint64_t *q = reinterpret_cast<const uint64_t *>(level + (a - off));
+                    uint64_t w = q[0] >> (8 * off);
+                    if (off > 5) w |= q[1] << (64 - 8 * off);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        lz[k] = (int)((w >> (8 * k)) & 0xffu);
+                        uz[k] = a + k;
                     }
-                    const int u = (nx * d.n1 + ny) * d.n2 + nz;
-                    const int lu = level[u];
-                    if (lu > lv || (lu == lv && u > v)) continue;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        int nz = z + k - 1;
+                        if (nz < 0 || nz >= d.n2) {
+                            if (!d.per2) continue;
+                            nz = nz < 0 ? nz + d.n2 : nz - d.n2;
+                        }
+                        uz[k] = row + nz;
+                        lz[k] = level[uz[k]];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    if (!dx && !dy && k == 1) continue;
+                    const int u = uz[k], lu = lz[k];
+                    if (u < 0 || lu > lv || (lu == lv && u > v)) continue;
                     const int pu = parent[u];
                     if (pu == rv || pu == last) continue;
                     last = pu;

@@ -50,3 +50,30 @@ def test_radius_threshold_vs_brute_force(radius):
 
 def test_radius_threshold_negative():
     assert M.radius_threshold(-1.0) == -1
+
+
+def test_sweep_box_cuts_only_all_solid_wall_pads():
+    """The fused sweep's box: a WALL face's pad layer is cut off (and the face
+    marked WALL) only when it holds no pore voxel; END and periodic faces and
+    pads with pore voxels stay in the box as END faces."""
+    import torch
+
+    import pmmoto_amd
+    from pmmoto_amd import _lib
+    BT = pmmoto_amd.BoundaryType
+    sd = pmmoto_amd.initialize((10, 12, 14), box=((0, 10), (0, 12), (0, 14)),
+                               boundary_types=((BT.END, BT.END), (BT.WALL, BT.WALL), (BT.WALL, BT.WALL)))
+    pm = torch.zeros(sd.voxels, dtype=torch.uint8)
+    pm[:, 1:-1, 1:-1] = 1
+    origin, n, bc = PO._sweep_box(sd, pm)
+    shape = tuple(pm.shape)
+    assert origin == [0, 1, 1] and n == [shape[0], shape[1] - 2, shape[2] - 2]
+    assert bc == [_lib.BC_END, _lib.BC_END] + [_lib.BC_WALL] * 4
+    pm[3, 0, 5] = 1  # a pore voxel in the y = 0 pad keeps that layer
+    origin, n, bc = PO._sweep_box(sd, pm)
+    assert origin == [0, 0, 1] and n == [shape[0], shape[1] - 1, shape[2] - 2]
+    assert bc == [_lib.BC_END, _lib.BC_END, _lib.BC_END, _lib.BC_WALL, _lib.BC_WALL, _lib.BC_WALL]
+    per = pmmoto_amd.initialize((10, 12, 14), box=((0, 10), (0, 12), (0, 14)),
+                                boundary_types=((BT.PERIODIC, BT.PERIODIC),) * 3)
+    origin, n, bc = PO._sweep_box(per, torch.zeros(per.voxels, dtype=torch.uint8))
+    assert origin == [0, 0, 0] and n == list(per.voxels) and bc == [_lib.BC_END] * 6

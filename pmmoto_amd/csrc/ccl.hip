@@ -377,32 +377,61 @@ __global__ __launch_bounds__(256) void k_compress(int *__restrict__ parent, int6
     if (threadIdx.x == 0) tile_counts[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-// single block: exclusive scan of tile counts in place, total to *count
+// single block: exclusive scan of tile counts in place, total to *count.
+// Rounds of 8192 counts: a coalesced load into LDS, thread t scans its 8
+// consecutive counts, a wave-shuffle scan plus 16 wave totals give the
+// thread offsets, and a coalesced store writes the round back.
 __global__ __launch_bounds__(1024) void k_scan_tiles(int *__restrict__ tile_counts, int64_t ntiles,
                                                      int64_t *__restrict__ count)
 {
-    __shared__ int64_t sh[1024];
-    const int64_t per = (ntiles + 1023) / 1024;
-    const int64_t b0 = threadIdx.x * per;
-    int64_t s = 0;
-    for (int64_t k = 0; k < per; ++k)
-        if (b0 + k < ntiles) s += tile_counts[b0 + k];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        int64_t t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+    constexpr int kPer = 8, kRound = 1024 * kPer;
+    __shared__ int buf[kRound];
+    __shared__ int64_t wsum[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int64_t carry = 0;
+    for (int64_t r0 = 0; r0 < ntiles; r0 += kRound) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int64_t i = r0 + k * 1024 + threadIdx.x;
+            buf[k * 1024 + threadIdx.x] = i < ntiles ? tile_counts[i] : 0;
+        }
         __syncthreads();
-        sh[threadIdx.x] += t;
+        int v[kPer];
+        int s = 0;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            v[k] = buf[kPer * threadIdx.x + k];
+            s += v[k];
+        }
+        int64_t inc = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int64_t u = __shfl_up(inc, o);
+            if (lane >= o) inc += u;
+        }
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        int64_t woff = 0, tot = 0;
+        for (int k = 0; k < 16; ++k) {
+            if (k < w) woff += wsum[k];
+            tot += wsum[k];
+        }
+        int64_t run = carry + woff + inc - s;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            buf[kPer * threadIdx.x + k] = (int)run;
+            run += v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int64_t i = r0 + k * 1024 + threadIdx.x;
+            if (i < ntiles) tile_counts[i] = buf[k * 1024 + threadIdx.x];
+        }
+        carry += tot;
         __syncthreads();
     }
-    int64_t run = sh[threadIdx.x] - s;
-    for (int64_t k = 0; k < per; ++k)
-        if (b0 + k < ntiles) {
-            const int c = tile_counts[b0 + k];
-            tile_counts[b0 + k] = (int)run;
-            run += c;
-        }
-    if (threadIdx.x == 1023) *count = sh[1023];
+    if (threadIdx.x == 0) *count = carry;
 }
 
 // One wave per rank tile (its kTile / 64 root words, one per lane): each

@@ -223,6 +223,41 @@ def test_porosimetry_modes_and_inlet_kat():
         PO.porosimetry(sd, pm, [2, 2, 2])
 
 
+def test_ccl26_many_rank_tiles_match_oracle():
+    """More than 8192 rank tiles of 4096 voxels: the tile-count scan runs
+    several rounds, and every later round's offsets reach the labels."""
+    rng = np.random.default_rng(5)
+    img = rng.choice([0, 1], size=(344, 336, 300), p=[0.9, 0.1]).astype(np.uint8)
+    labels, count = _ccl26_gpu(img, (True, False, True))
+    want, n = O.ccl26(img, (True, False, True))
+    assert count == n
+    np.testing.assert_array_equal(labels, want)
+
+
+def _ccl26_gpu(img, per):
+    import ctypes as C
+
+    import torch
+
+    from pmmoto_amd import _device, _lib
+    from pmmoto_amd.filters.distance import euclidean_distance_transform as EDT
+    shape = img.shape
+    bc = [_lib.BC_PERIODIC if per[a // 2] else _lib.BC_END for a in range(6)]
+    g = EDT._geom(shape, (0, 0, 0), shape, bc)
+    L = _lib.lib()
+    nb = C.c_size_t()
+    _lib.check(L.pmx_ccl_workspace_size(C.byref(g), C.byref(nb)))
+    ws = _device.workspace(nb.value)
+    dimg = torch.from_numpy(img).cuda()
+    labels = torch.empty(shape, dtype=torch.int32, device="cuda")
+    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    print(f"[pmx_ccl26] shape={shape} periodic={per}", flush=True)
+    _lib.check(L.pmx_ccl26(_device.ptr(dimg), C.byref(g), _device.ptr(labels), None, _device.ptr(count),
+                           _device.ptr(ws), nb.value, _device.stream()))
+    torch.cuda.synchronize()
+    return labels.cpu().numpy().astype(np.int64), int(count.item())
+
+
 @pytest.mark.parametrize("shape,per", [
     ((20, 19, 150), (True, True, True)),     # three z tiles, the last partial; wrap on every axis
     ((17, 9, 130), (False, False, False)),

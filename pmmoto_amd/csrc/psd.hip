@@ -20,7 +20,6 @@
 namespace pmx {
 namespace {
 
-constexpr int kInactive = INT32_MIN;
 constexpr uint32_t kNotJoined = 0xff;  // in_c: not (yet) in the inlet-connected set; levels are < 250
 constexpr int kNoLevel = 256;          // above every level: no neighbour there
 
@@ -81,11 +80,13 @@ constexpr int kBucketItems = 16;  // voxels per thread between bucket reservatio
 
 // level[v] = first i with d2 > te[i] (the erosion at radius i keeps v),
 // or n_levels if none (te does not increase, so a binary search finds
-// it); counts[level] += per-block totals from an LDS histogram
+// it); counts[level] += per-block totals from an LDS histogram.  The parent
+// array needs no initial value: a voxel's entry is read only once the level
+// map says it is active (level <= the current level), after k_activate.
 __global__ __launch_bounds__(256) void k_levels(const int32_t *__restrict__ d2, int64_t n, const int64_t *__restrict__ te,
                                                 int n_levels, uint8_t *__restrict__ level,
-                                                unsigned int *__restrict__ counts, int *__restrict__ parent,
-                                                uint8_t *__restrict__ flag, uint8_t *__restrict__ in_c)
+                                                unsigned int *__restrict__ counts, uint8_t *__restrict__ flag,
+                                                uint8_t *__restrict__ in_c)
 {
     __shared__ int64_t ste[kMaxLevels];
     __shared__ unsigned int hist[kMaxLevels];
@@ -106,7 +107,6 @@ __global__ __launch_bounds__(256) void k_levels(const int32_t *__restrict__ d2, 
                 lo = mid + 1;
         }
         level[i] = (uint8_t)lo;
-        parent[i] = kInactive;
         flag[i] = 0;
         in_c[i] = kNotJoined;
         if (lo < n_levels) atomicAdd(hist + lo, 1u);
@@ -249,15 +249,15 @@ __global__ __launch_bounds__(256) void k_merge_new(const int *__restrict__ order
 // k_mask no union runs, so the forest only changes by path halving (a
 // pointer moves to an ancestor) and its reads go through L2.
 __global__ __launch_bounds__(256) void k_flag_inlet(Dims d, int64_t lo0, int64_t lo1, int64_t lo2, int64_t e0,
-                                                    int64_t e1, int64_t e2, int *__restrict__ parent,
-                                                    uint8_t *__restrict__ flag)
+                                                    int64_t e1, int64_t e2, const uint8_t *__restrict__ level, int lv,
+                                                    int *__restrict__ parent, uint8_t *__restrict__ flag)
 {
     const int64_t n = e0 * e1 * e2;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t z = i % e2, t = i / e2, y = t % e1, x = t / e1;
         const int v = (int)(((lo0 + x) * d.n1 + lo1 + y) * d.n2 + lo2 + z);
-        if (parent[v] != kInactive) {
+        if (level[v] <= lv) {
             const int r = find_root<true>(parent, v);
             if (!flag[r]) flag[r] = 1;  // read first: a large component's voxels share one root
         }
@@ -516,8 +516,7 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
         hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(256), 0, s, p.any, kMaxLevels);
         // few blocks: each adds its per-level totals with one global atomic per level
         hipLaunchKernelGGL(k_levels, dim3(std::min<unsigned>(gr, 2048u)), dim3(256), 0, s, p.d2, n, p.te, n_levels,
-                           p.level, p.counts,
-                           p.parent, p.flag, p.in_c);
+                           p.level, p.counts, p.flag, p.in_c);
         if (!has_inlet) hipLaunchKernelGGL(k_any_from_counts, dim3(1), dim3(256), 0, s, p.counts, n_levels, p.any);
         if (has_inlet) {  // the union-find absorbs each level's voxels from these buckets
             hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, s, p.counts, n_levels, p.offsets, p.cursor);
@@ -543,7 +542,7 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
             ProfScope ps("psd_mask", s);
             const int64_t ni = inlet_ext[0] * inlet_ext[1] * inlet_ext[2];
             hipLaunchKernelGGL(k_flag_inlet, dim3(grid_for(ni)), dim3(256), 0, s, d, inlet_lo[0], inlet_lo[1],
-                               inlet_lo[2], inlet_ext[0], inlet_ext[1], inlet_ext[2], p.parent, p.flag);
+                               inlet_lo[2], inlet_ext[0], inlet_ext[1], inlet_ext[2], p.level, lv, p.parent, p.flag);
             hipLaunchKernelGGL(k_mask, dim3(grid_for(n / 16 + 1)), dim3(256), 0, s, n, p.level, lv, p.parent, p.flag,
                                p.in_c, p.any + lv);
         }

@@ -23,8 +23,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per voxel of each EDT kernel as built: pass Z reads the
 # u8 image and writes uint16 z distances, pass Y reads those and writes
-# int32 squared distances, pass X reads those and writes the int32 output
-BYTES_PER_VOXEL = {"edt_pass_z": 1 + 2, "edt_pass_y": 2 + 4, "edt_pass_x": 4 + 4}
+# 16-bit squared distances (saturating at 65535), pass X reads those and
+# writes the int32 output.  When a squared distance exceeds 65535, pass Y
+# runs again to int32 and pass X reads that (the *32 kernels; otherwise they
+# exit at once).
+BYTES_PER_VOXEL = {"edt_pass_z": 1 + 2, "edt_pass_y": 2 + 2, "edt_pass_x": 2 + 4,
+                   "edt_pass_y32": 2 + 4, "edt_pass_x32": 4 + 4}
 # the squared-EDT figure of the three-pass algorithm with int32 passes
 # (1 R + 4 W, 4 R + 4 W, 4 R + 4 W): the %HBM definition of the metric
 EDT_BYTES_PER_VOXEL = 21
@@ -434,7 +438,8 @@ def main() -> int:
                         "(mean r 10, sigma 0.3, porosity 0.35, seed 3), x END / y,z WALL",
             "voxels": int(vox),
             "edt_hbm_frac_21B": round(EDT_BYTES_PER_VOXEL * vox / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "edt_bytes_per_voxel_moved": sum(BYTES_PER_VOXEL.values()),
+            "edt_bytes_per_voxel_moved": sum(b for k, b in BYTES_PER_VOXEL.items()
+                                             if k in prof and prof[k][0] / prof[k][1] > 0.05),
             "kernels": per_kernel,
             "porosimetry": psd,
             "periodic_512": per512,

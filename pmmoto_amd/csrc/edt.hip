@@ -176,7 +176,9 @@ __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img,
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
     const int64_t row = (int64_t)blockIdx.x * (blockDim.x / kWave) + w;
-    if (blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0;  // deferred-line counts
+    // counters: deferred-line counts of passes Y and X, then the 16-bit flags of pass Y
+    // ([2] = 1: a value above 65535 was saturated; [3] = 1: none was)
+    if (blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = threadIdx.x == 3 ? 1u : 0u;
     if (row >= rows) return;
     const int64_t x = row / ny, y = row - x * ny;
     const uint8_t *src = img + x * in_sx + y * in_sy;
@@ -332,7 +334,9 @@ __global__ __launch_bounds__(256) void k_pass_z_short(const uint8_t *__restrict_
     const int lane = threadIdx.x & (kWave - 1);
     const int seg = lane / L, sl = lane % L;
     const int64_t row0 = ((int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * R;
-    if (blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0;  // deferred-line counts
+    // counters: deferred-line counts of passes Y and X, then the 16-bit flags of pass Y
+    // ([2] = 1: a value above 65535 was saturated; [3] = 1: none was)
+    if (blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = threadIdx.x == 3 ? 1u : 0u;
     if (row0 >= rows) return;  // wave-uniform: the scans below need every lane of the wave
     const int64_t row = row0 + seg;
     const bool valid = row < rows;
@@ -865,10 +869,17 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
             if (!ok || z >= g.nz) continue;
             if constexpr (kY) {
                 const int32_t v = best >= kBig ? kInf : best;
-                if constexpr (std::is_same<O, Sq16>::value)
-                    out[(int64_t)outer * g.out_os + (int64_t)y * g.out_rs + z] = Sq16{(uint16_t)v};
-                else
+                if constexpr (std::is_same<O, Sq16>::value) {
+                    // 16-bit output saturates; a saturated value sends the EDT to its int32 pass
+                    // Y and pass X (the flags after the two deferred-line counts)
+                    if (v > 65535) {
+                        ovf_count[2] = 1u;
+                        ovf_count[3] = 0u;
+                    }
+                    out[(int64_t)outer * g.out_os + (int64_t)y * g.out_rs + z] = Sq16{(uint16_t)min(v, 65535)};
+                } else {
                     out[(int64_t)outer * g.out_os + (int64_t)y * g.out_rs + z] = v;
+                }
             } else if constexpr (D2ONLY) {
                 o.d2[(int64_t)y * o.sx + (int64_t)outer * o.sy + z] = best >= kBig ? kInf : best;
             } else {
@@ -906,7 +917,7 @@ template <class O>
 __global__ __launch_bounds__(256) void k_fallback_y(const uint16_t *__restrict__ in, O *__restrict__ out,
                                                     int nx, int ny, int nz, int64_t pitch, int64_t out_sx,
                                                     int64_t out_sy, int bc_lo, int bc_hi, int cap,
-                                                    const uint32_t *__restrict__ ovf_count,
+                                                    uint32_t *__restrict__ ovf_count,
                                                     const uint32_t *__restrict__ ovf_list,
                                                     int *__restrict__ stk, int stk_cap,
                                                     const unsigned int *__restrict__ proceed)
@@ -929,10 +940,16 @@ __global__ __launch_bounds__(256) void k_fallback_y(const uint16_t *__restrict__
                 return d == kNoSolid ? kInf : (int)d * (int)d;
             },
             [&](int j, int v) {
-                if constexpr (std::is_same<O, Sq16>::value)
-                    dst[(int64_t)j * out_sy] = Sq16{(uint16_t)finish(v, cap)};
-                else
-                    dst[(int64_t)j * out_sy] = finish(v, cap);
+                const int f = finish(v, cap);
+                if constexpr (std::is_same<O, Sq16>::value) {
+                    if (f > 65535) {  // saturated: the EDT takes its int32 passes
+                        ovf_count[2] = 1u;
+                        ovf_count[3] = 0u;
+                    }
+                    dst[(int64_t)j * out_sy] = Sq16{(uint16_t)min(f, 65535)};
+                } else {
+                    dst[(int64_t)j * out_sy] = f;
+                }
             });
     }
 }
@@ -1038,46 +1055,96 @@ size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p)
 
 namespace {
 
-// passes Y and X (and their fallbacks) with the intermediate array `b` of
-// type B, stored [y][x][z] with the row pitch of pass Z's array: pass X then
-// reads rows pitch * sizeof(B) bytes apart
+// The launches of passes Y and X and their fallbacks, with the intermediate
+// array `b` of type B (stored [y][x][z] with the row pitch of pass Z's
+// array: pass X reads rows pitch * sizeof(B) bytes apart), each gated by
+// `gate` (device memory; nothing runs while it reads 0).  name: profiling
+// scope.
+struct YX {
+    const EdtPlan &p;
+    const pmx_geom *g;
+    const PassXOut &o;
+    int ccap, tr;
+    bool wide;
+    hipStream_t s;
+    int nx() const { return (int)g->n[0]; }
+    int ny() const { return (int)g->n[1]; }
+    int nz() const { return (int)g->n[2]; }
+    int64_t bsx() const { return p.pitch; }
+    int64_t bsy() const { return (int64_t)nx() * p.pitch; }
+
+    template <class B> int pass_y(B *b, const unsigned int *gate, const char *name) const
+    {
+        ProfScope ps(name, s);
+        const WinGeom wg{ny(), p.pitch, ny() * p.pitch, nx(), nz(), g->bc[2], g->bc[3], ccap, p.pitch, bsx(), bsy()};
+        dispatch_window(tr, wide, p.dz, b, o, wg, nx(), ny(), p.bits_y, p.counters + 0, p.ovf_list, gate, s);
+        return check_launch("pass y");
+    }
+    template <class B> int fallback_y(B *b, const unsigned int *gate, const char *name) const
+    {
+        ProfScope ps(name, s);
+        hipLaunchKernelGGL(k_fallback_y<B>, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.dz, b, nx(), ny(), nz(),
+                           p.pitch, bsx(), bsy(), g->bc[2], g->bc[3], ccap, p.counters + 0, p.ovf_list, p.stk, p.cap,
+                           gate);
+        return check_launch("pass y fallback");
+    }
+    template <class B> int pass_x(const B *b, const unsigned int *gate, const char *name) const
+    {
+        ProfScope ps(name, s);
+        const WinGeom wg{nx(), bsx(), bsy(), ny(), nz(), g->bc[0], g->bc[1], ccap, p.pitch, 0, 0};
+        dispatch_window(tr, wide, b, (int32_t *)nullptr, o, wg, nx(), ny(), p.bits_x, p.counters + 1, p.ovf_list, gate,
+                        s);
+        return check_launch("pass x");
+    }
+    template <class B> int fallback_x(const B *b, const unsigned int *gate, const char *name) const
+    {
+        ProfScope ps(name, s);
+        hipLaunchKernelGGL(k_fallback_x<B>, dim3(p.fallback_lanes / 256), dim3(256), 0, s, b, o, nx(), ny(), nz(),
+                           bsx(), bsy(), g->bc[0], g->bc[1], ccap, p.counters + 1, p.ovf_list, p.stk, p.cap, gate);
+        return check_launch("pass x fallback");
+    }
+};
+
+// Every value is <= the cap, so a scan reaching row offset kReach + 1 has
+// k^2 > cap >= best: under a cap below (kReach + 1)^2 no line is ever
+// deferred (the bitmaps start each EDT clear) and the fallbacks, which would
+// find empty lists, are not launched.
+inline bool may_defer(int ccap) { return ccap >= (kReach + 1) * (kReach + 1); }
+
 template <class B>
-int run_yx(const EdtPlan &p, const pmx_geom *g, const PassXOut &o, int ccap, bool wide, int tr, B *b,
-           const unsigned int *proceed, hipStream_t s)
+int run_yx(const YX &yx, B *b, const unsigned int *proceed)
 {
-    const int nx = (int)g->n[0], ny = (int)g->n[1], nz = (int)g->n[2];
-    const int64_t pitch = p.pitch;
-    const int64_t bsx = pitch, bsy = (int64_t)nx * pitch;
-    // Every value is <= the cap, so a scan reaching row offset kReach + 1
-    // has k^2 > cap >= best: under a cap below (kReach + 1)^2 no line is ever
-    // deferred (the bitmaps start each EDT clear) and the fallbacks, which
-    // would find empty lists, are not launched.
-    const bool may_defer = ccap >= (kReach + 1) * (kReach + 1);
-    {
-    ProfScope ps("edt_pass_y", s);
-    const WinGeom wg{ny, pitch, ny * pitch, nx, nz, g->bc[2], g->bc[3], ccap, pitch, bsx, bsy};
-    dispatch_window(tr, wide, p.dz, b, o, wg, nx, ny, p.bits_y, p.counters + 0, p.ovf_list, proceed, s);
-    }
-    if (int e = check_launch("pass y")) return e;
-    if (may_defer) {
-    ProfScope ps("edt_fallback_y", s);
-    hipLaunchKernelGGL(k_fallback_y<B>, dim3(p.fallback_lanes / 256), dim3(256), 0, s, p.dz, b, nx, ny, nz, pitch,
-                       bsx, bsy, g->bc[2], g->bc[3], ccap, p.counters + 0, p.ovf_list, p.stk, p.cap, proceed);
-    }
-    if (int e = check_launch("pass y fallback")) return e;
-    {
-    ProfScope ps("edt_pass_x", s);
-    const WinGeom wg{nx, bsx, bsy, ny, nz, g->bc[0], g->bc[1], ccap, pitch, 0, 0};
-    dispatch_window(tr, wide, (const B *)b, (int32_t *)nullptr, o, wg, nx, ny, p.bits_x, p.counters + 1,
-                    p.ovf_list, proceed, s);
-    }
-    if (int e = check_launch("pass x")) return e;
-    if (may_defer) {
-    ProfScope ps("edt_fallback_x", s);
-    hipLaunchKernelGGL(k_fallback_x<B>, dim3(p.fallback_lanes / 256), dim3(256), 0, s, (const B *)b, o, nx, ny, nz,
-                       bsx, bsy, g->bc[0], g->bc[1], ccap, p.counters + 1, p.ovf_list, p.stk, p.cap, proceed);
-    }
-    return check_launch("pass x fallback");
+    const bool fb = may_defer(yx.ccap);
+    if (int e = yx.pass_y(b, proceed, "edt_pass_y")) return e;
+    if (fb)
+        if (int e = yx.fallback_y(b, proceed, "edt_fallback_y")) return e;
+    if (int e = yx.pass_x((const B *)b, proceed, "edt_pass_x")) return e;
+    if (fb)
+        if (int e = yx.fallback_x((const B *)b, proceed, "edt_fallback_x")) return e;
+    return PMX_OK;
+}
+
+// A cap above 65535 with no caller gate (the plain EDT).  Clamping commutes
+// with the passes, so a pass Y that saturates at 65535 gives, through pass
+// X, min(d2, 65535) exactly: every output below 65535 is the true d2.  Pass
+// Y writes 16 bits and raises a flag when it saturates anything (squared
+// distances above 65535, i.e. distances above 255 voxels: rare in a porous
+// medium).  Then pass Y runs again to int32 and pass X reads that; without
+// the flag pass X reads the 16-bit array.  The choice is made on the device
+// (the gates), so the host never waits.  The int32 array aliases the 16-bit
+// one, which is dead once the flag is raised.
+int run_yx_narrow_first(const YX &yx)
+{
+    const unsigned int *over = yx.p.counters + 2, *none = yx.p.counters + 3;
+    Sq16 *b16 = reinterpret_cast<Sq16 *>(yx.p.b);
+    if (int e = yx.pass_y(b16, nullptr, "edt_pass_y")) return e;
+    if (int e = yx.fallback_y(b16, nullptr, "edt_fallback_y")) return e;
+    if (int e = yx.pass_y(yx.p.b, over, "edt_pass_y32")) return e;
+    if (int e = yx.fallback_y(yx.p.b, over, "edt_fallback_y32")) return e;
+    if (int e = yx.pass_x((const Sq16 *)b16, none, "edt_pass_x")) return e;
+    if (int e = yx.fallback_x((const Sq16 *)b16, none, "edt_fallback_x")) return e;
+    if (int e = yx.pass_x((const int32_t *)yx.p.b, over, "edt_pass_x32")) return e;
+    return yx.fallback_x((const int32_t *)yx.p.b, over, "edt_fallback_x32");
 }
 
 }  // namespace
@@ -1140,10 +1207,12 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     // halo: 16 rows settle nearly every voxel of a porous medium; a clamped
     // EDT whose threshold reaches past 17^2 scans up to 32 rows in LDS
     const bool wide = ccap < kBig && ccap > 17 * 17;
+    const YX yx{p, g, o, ccap, tr, wide, s};
     // a cap that fits in 16 bits: pass Y hands pass X 16-bit squared distances
-    if (ccap <= 65535)
-        return run_yx(p, g, o, ccap, wide, tr, reinterpret_cast<Sq16 *>(p.b), proceed, s);
-    return run_yx(p, g, o, ccap, wide, tr, p.b, proceed, s);
+    if (ccap <= 65535) return run_yx(yx, reinterpret_cast<Sq16 *>(p.b), proceed);
+    // otherwise 16 bits first, int32 only where a value saturated (decided on the device)
+    if (!proceed) return run_yx_narrow_first(yx);
+    return run_yx(yx, p.b, proceed);
 }
 
 

@@ -28,11 +28,16 @@ GIB = 1 << 30
 # kernel symbol -> the name bench.py reports it under
 BENCH_NAMES = (
     (re.compile(r"k_pass_z<"), "edt_pass_z"),
-    # k_window<tile rows, halo, input element, pass-Y output element, d2 only>
-    (re.compile(r"k_window<\d+, \d+, unsigned short,"), "edt_pass_y"),
-    (re.compile(r"k_window<\d+, \d+, (int|[\w:() ]*Sq16),"), "edt_pass_x"),
-    (re.compile(r"k_fallback_y"), "edt_fallback_y"),
-    (re.compile(r"k_fallback_x"), "edt_fallback_x"),
+    # k_window<tile rows, halo, input element, pass-Y output element, d2 only>: the plain
+    # EDT's 16-bit passes, and the int32 ones that run when a value exceeds 65535
+    (re.compile(r"k_window<\d+, \d+, unsigned short, [\w:() ]*Sq16,"), "edt_pass_y"),
+    (re.compile(r"k_window<\d+, \d+, unsigned short, int,"), "edt_pass_y32"),
+    (re.compile(r"k_window<\d+, \d+, [\w:() ]*Sq16, int,"), "edt_pass_x"),
+    (re.compile(r"k_window<\d+, \d+, int, int,"), "edt_pass_x32"),
+    (re.compile(r"k_fallback_y<[\w:() ]*Sq16>"), "edt_fallback_y"),
+    (re.compile(r"k_fallback_y<int>"), "edt_fallback_y32"),
+    (re.compile(r"k_fallback_x<[\w:() ]*Sq16>"), "edt_fallback_x"),
+    (re.compile(r"k_fallback_x<int>"), "edt_fallback_x32"),
     (re.compile(r"k_conv_first<"), None),  # surrogate inference kernels: reported under their symbol
     (re.compile(r"k_conv_igemm<"), None),
 )

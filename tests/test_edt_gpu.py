@@ -63,6 +63,23 @@ def test_single_solid_voxel_uses_fallback_and_is_exact():
         np.testing.assert_array_equal(gpu_edt(img, bc)["d2"], O.exact_edt_sq(img, bc))
 
 
+def test_distances_above_255_take_the_int32_passes():
+    """Squared distances above 65535 saturate the 16-bit pass Y; the EDT then
+    runs pass Y again to int32 and pass X on that array (decided on the
+    device).  Images where only part of the box is that far from a solid."""
+    img = np.ones((300, 12, 10), np.uint8)
+    img[0, 5, 4] = 0
+    for bc in (BCS["end"], BCS["wall"], BCS["mixed"]):
+        np.testing.assert_array_equal(gpu_edt(img, bc, ("d2", "f64"))["d2"], O.exact_edt_sq(img, bc))
+    img = rand_binary((280, 30, 40), 0.0, 3)
+    img[:20] = rand_binary((20, 30, 40), 0.05, 4)
+    got = gpu_edt(img, BCS["end"], ("d2", "f64"))
+    want = O.exact_edt_sq(img, BCS["end"])
+    assert want.max() > 65535
+    np.testing.assert_array_equal(got["d2"], want)
+    np.testing.assert_array_equal(got["f64"], np.sqrt(want.astype(np.float64)))
+
+
 def test_periodic_lines_without_solid():
     img = np.ones((50, 60, 40), np.uint8)
     img[10:12, :, 5] = 0  # solids only in two x-planes of one z column

@@ -45,9 +45,12 @@ COPY = re.compile(r"copyBuffer")
 
 
 def _short(name: str) -> str:
-    name = name.replace("(anonymous namespace)", "")
+    """Kernel symbol without its return type, arguments and namespace
+    qualifiers (also those inside template arguments)."""
+    name = name.replace("(anonymous namespace)::", "")
+    name = re.sub(r"\b[A-Za-z_]\w*::", "", name)
     name = name.split("(")[0]
-    return name.split("::")[-1]
+    return name.replace("void ", "", 1).strip()
 
 
 def _db(d: str) -> str:

@@ -709,11 +709,37 @@ template <> __device__ __forceinline__ int4 unpack<int32_t>(int4 r, int cap)
     return make_int4(min(r.x, cap), min(r.y, cap), min(r.z, cap), min(r.w, cap));
 }
 
+// The LDS tile holds int32 values, or uint16 ones (half the LDS, so more
+// blocks per CU) when every value the pass needs is at most 65535: a pass X
+// over 16-bit input, and a pass Y with 16-bit output.  A uint16 tile
+// saturates at 65535.  For pass Y that leaves its 16-bit output as it was:
+// every candidate s(f) + k^2 lies between min(f + k^2, 65535) and f + k^2, so
+// the windowed minimum equals the true one below 65535 and is >= 65535
+// (saturated and flagged) otherwise.
+template <class L> struct TileIO;
+template <> struct TileIO<int> {
+    static __device__ __forceinline__ void put4(int *p, int4 v) { *reinterpret_cast<int4 *>(p) = v; }
+    static __device__ __forceinline__ int4 get4(const int *p) { return *reinterpret_cast<const int4 *>(p); }
+};
+template <> struct TileIO<uint16_t> {
+    static __device__ __forceinline__ void put4(uint16_t *p, int4 v)
+    {
+        const uint32_t a = min((uint32_t)v.x, 65535u) | (min((uint32_t)v.y, 65535u) << 16);
+        const uint32_t b = min((uint32_t)v.z, 65535u) | (min((uint32_t)v.w, 65535u) << 16);
+        *reinterpret_cast<uint2 *>(p) = make_uint2(a, b);
+    }
+    static __device__ __forceinline__ int4 get4(const uint16_t *p)
+    {
+        const uint2 r = *reinterpret_cast<const uint2 *>(p);
+        return make_int4((int)(r.x & 0xffffu), (int)(r.x >> 16), (int)(r.y & 0xffffu), (int)(r.y >> 16));
+    }
+};
+
 // clamp (columns past nz hold 0, settled at once; rows beyond the line the
 // boundary value) and store at tile rows dst + row0 + 16 i; acc keeps the
 // per-column minimum
-template <class E, int N, class V>
-__device__ __forceinline__ void commit_rows(int *tile, const WinGeom &g, int dst, int row0, int q, int zl,
+template <class E, int N, class V, class L>
+__device__ __forceinline__ void commit_rows(L *tile, const WinGeom &g, int dst, int row0, int q, int zl,
                                             const int (&mrow)[N], const V (&raw)[N], int4 &acc, bool inside)
 {
     // inside (wave-uniform): every staged row lies in the line; with a full quad of columns
@@ -726,7 +752,7 @@ __device__ __forceinline__ void commit_rows(int *tile, const WinGeom &g, int dst
             acc.y = min(acc.y, v.y);
             acc.z = min(acc.z, v.z);
             acc.w = min(acc.w, v.w);
-            *reinterpret_cast<int4 *>(tile + (dst + row0 + 16 * i) * kTileZ + 4 * q) = v;
+            TileIO<L>::put4(tile + (dst + row0 + 16 * i) * kTileZ + 4 * q, v);
         }
         return;
     }
@@ -744,7 +770,7 @@ __device__ __forceinline__ void commit_rows(int *tile, const WinGeom &g, int dst
         acc.y = min(acc.y, v.y);
         acc.z = min(acc.z, v.z);
         acc.w = min(acc.w, v.w);
-        *reinterpret_cast<int4 *>(tile + (dst + row0 + 16 * i) * kTileZ + 4 * q) = v;
+        TileIO<L>::put4(tile + (dst + row0 + 16 * i) * kTileZ + 4 * q, v);
     }
 }
 
@@ -773,7 +799,9 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
     constexpr int kWaves = kWinThreads / kWave;
     static_assert(kWaves == 4 && kWinThreads == 256, "four waves, 16 threads per row of four columns");
     constexpr int kHalo = 2 * W / 16;  // rows of four columns per thread in the halo move
-    __shared__ __align__(16) int tile[kRows * kTileZ];
+    constexpr bool kNarrow = kY ? std::is_same<O, Sq16>::value : std::is_same<E, Sq16>::value;
+    using L = typename std::conditional<kNarrow, uint16_t, int>::type;
+    __shared__ __align__(16) L tile[kRows * kTileZ];
     __shared__ __align__(16) int part[kWaves][kTileZ];
 
     const int nzt = (g.nz + kTileZ - 1) / kTileZ, nrt = (g.n + T - 1) / T;
@@ -792,7 +820,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
     // scalar unit
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
     const int z = z0 + lane;
-    const int *col = tile + lane;
+    const L *col = tile + lane;
     const int64_t line = (int64_t)outer * g.nz + z;
     const int q = threadIdx.x % (kTileZ / 4), row0 = threadIdx.x / (kTileZ / 4);
     const int zl = g.nz - 1 - z0 - 4 * q;  // valid columns of this thread's quad: 0 .. zl
@@ -832,7 +860,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
         const int jend = min(T, g.n - r0);  // wave-uniform: the row loop needs no exec mask
         for (int j = wv; j < jend; j += kWaves) {
             const int y = r0 + j;
-            const int *c = col + (W + j) * kTileZ;
+            const L *c = col + (W + j) * kTileZ;
             int best = c[0];
             // blocks of four offsets while any lane of the wave can still improve; a lane that
             // is done only folds in candidates >= k^2 >= best, which leave it unchanged, so the
@@ -872,7 +900,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
                 if constexpr (std::is_same<O, Sq16>::value) {
                     // 16-bit output saturates; a saturated value sends the EDT to its int32 pass
                     // Y and pass X (the flags after the two deferred-line counts)
-                    if (v > 65535) {
+                    if (v >= 65535) {  // >=: with a uint16 tile a larger true value reads 65535
                         ovf_count[2] = 1u;
                         ovf_count[3] = 0u;
                     }
@@ -890,12 +918,12 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
         // slide: rows T .. T+2W-1 become rows 0 .. 2W-1, the new rows follow
         int4 h[kHalo];
 #pragma unroll
-        for (int i = 0; i < kHalo; ++i) h[i] = *reinterpret_cast<const int4 *>(tile + (T + row0 + 16 * i) * kTileZ + 4 * q);
+        for (int i = 0; i < kHalo; ++i) h[i] = TileIO<L>::get4(tile + (T + row0 + 16 * i) * kTileZ + 4 * q);
         __syncthreads();
         acc = make_int4(kBig, kBig, kBig, kBig);
 #pragma unroll
         for (int i = 0; i < kHalo; ++i) {
-            *reinterpret_cast<int4 *>(tile + (row0 + 16 * i) * kTileZ + 4 * q) = h[i];
+            TileIO<L>::put4(tile + (row0 + 16 * i) * kTileZ + 4 * q, h[i]);
             acc.x = min(acc.x, h[i].x);
             acc.y = min(acc.y, h[i].y);
             acc.z = min(acc.z, h[i].z);
@@ -942,7 +970,7 @@ __global__ __launch_bounds__(256) void k_fallback_y(const uint16_t *__restrict__
             [&](int j, int v) {
                 const int f = finish(v, cap);
                 if constexpr (std::is_same<O, Sq16>::value) {
-                    if (f > 65535) {  // saturated: the EDT takes its int32 passes
+                    if (f >= 65535) {  // saturated (or exactly 65535): the EDT takes its int32 passes
                         ovf_count[2] = 1u;
                         ovf_count[3] = 0u;
                     }

@@ -331,8 +331,8 @@ def main() -> int:
                          "reference kernel for HBM counter scaling under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
-              "the multi-GPU sharding design is in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the library runs one "
+              "domain on one GPU; sharding across GPUs is a design only)", file=sys.stderr)
         return 2
 
     import torch

@@ -858,22 +858,9 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
         if (more) issue_rows<T / 16>(base, g, r0 + T + W, row0, c4, xm, xr);
 
         const int jend = min(T, g.n - r0);  // wave-uniform: the row loop needs no exec mask
-        for (int j = wv; j < jend; j += kWaves) {
+        // the rest of a row once its tile scan is done (wave-uniform calls: it holds a ballot)
+        auto finish_row = [&](int j, int best) {
             const int y = r0 + j;
-            const L *c = col + (W + j) * kTileZ;
-            int best = c[0];
-            // blocks of four offsets while any lane of the wave can still improve; a lane that
-            // is done only folds in candidates >= k^2 >= best, which leave it unchanged, so the
-            // lanes need no exec mask and the loop exit is one wave-wide ballot
-#pragma unroll
-            for (int k = 1; k <= W; k += 4) {
-                if (__builtin_amdgcn_ballot_w64(cm + k * k < best) == 0) break;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int kk = k + u;
-                    best = min(best, min(c[-kk * kTileZ], c[kk * kTileZ]) + kk * kk);
-                }
-            }
             // the tile is done; rows beyond it are at least W + 1 away
             int k = W + 1;
             bool ok = k * k >= best;
@@ -894,7 +881,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
                 if (!ok && !(atomicOr(line_bits + (line >> 5), bit) & bit))
                     ovf_list[atomicAdd(ovf_count, 1u)] = (uint32_t)line;
             }
-            if (!ok || z >= g.nz) continue;
+            if (!ok || z >= g.nz) return;
             if constexpr (kY) {
                 const int32_t v = best >= kBig ? kInf : best;
                 if constexpr (std::is_same<O, Sq16>::value) {
@@ -913,6 +900,30 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
             } else {
                 emit_x(o, y, outer, z, nx, ny, g.nz, best >= kBig ? kInf : best);
             }
+        };
+        // two rows per step (j and j + kWaves): their scans run in one loop, so a wave keeps
+        // the LDS reads of both in flight; the loop ends when neither can improve
+        for (int j = wv; j < jend; j += 2 * kWaves) {
+            const int j2 = j + kWaves;
+            const bool two = j2 < jend;  // wave-uniform
+            const L *c = col + (W + j) * kTileZ;
+            const L *c2 = col + (W + (two ? j2 : j)) * kTileZ;
+            int best = c[0], best2 = c2[0];
+            // blocks of four offsets while any lane of the wave can still improve; a lane that
+            // is done only folds in candidates >= k^2 >= best, which leave it unchanged, so the
+            // lanes need no exec mask and the loop exit is one wave-wide ballot
+#pragma unroll
+            for (int k = 1; k <= W; k += 4) {
+                if (__builtin_amdgcn_ballot_w64(cm + k * k < max(best, best2)) == 0) break;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int kk = k + u;
+                    best = min(best, min(c[-kk * kTileZ], c[kk * kTileZ]) + kk * kk);
+                    best2 = min(best2, min(c2[-kk * kTileZ], c2[kk * kTileZ]) + kk * kk);
+                }
+            }
+            finish_row(j, best);
+            if (two) finish_row(j2, best2);
         }
         if (!more) break;
         // slide: rows T .. T+2W-1 become rows 0 .. 2W-1, the new rows follow

@@ -68,8 +68,10 @@ def lib() -> C.CDLL:
         L.orc_ref_nearest_index_1d.restype = C.c_long
         L.orc_ref_edt3d_sq.argtypes = [P, C.c_long, C.c_long, C.c_long, P, P, P, P]
         L.orc_ref_edt2d_sq.argtypes = [P, C.c_long, C.c_long, P, P, P, P]
+        L.orc_ref_corrected_edt_sq.argtypes = [P, C.c_long, C.c_long, C.c_long, P, P, P, P, P]
         L.orc_ref_edt3d_sq_mt.argtypes = [P, C.c_long, C.c_long, C.c_long, P, P, P, C.c_int]
         L.orc_exact_edt_sq.argtypes = [P, C.c_int, P, P, P]
+        L.orc_exact_edt_sq_mt.argtypes = [P, C.c_int, P, P, P, C.c_int]
         L.orc_exact_ccl26.argtypes = [P, C.c_long, C.c_long, C.c_long, P, P]
         L.orc_exact_ccl26.restype = C.c_int64
         L.orc_rasterize_spheres.argtypes = [P, C.c_long, C.c_long, C.c_long, P, P, C.c_long, C.c_double, P]
@@ -187,6 +189,24 @@ def ref_edt_sq(img, periodic=None, resolution=None, use_ref=False):
     return out
 
 
+def ref_corrected_edt_sq(img, periodic, pads, resolution, use_ref=False):
+    """Squared float32 output of corrected_edt(img, subdomain) for one rank
+    of a padded subdomain (euclidean_distance_transform.py:56-101, before
+    its np.sqrt): periodic[a] per axis, pads = six pad widths (lo, hi per
+    axis), over the whole padded array."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    per = np.array([1 if p else 0 for p in periodic], dtype=np.int32)
+    pd = np.array([int(v) for v in pads], dtype=np.int64)
+    res = np.array(resolution, dtype=np.float32)
+    out = np.empty(img.shape, dtype=np.float32)
+    ops = ref_ops() if use_ref else None
+    if use_ref and ops is None:
+        raise RuntimeError("reference kernels not built (make -C oracle ref)")
+    lib().orc_ref_corrected_edt_sq(_p(img), *img.shape, _p(per), _p(pd), _p(res), _p(out),
+                                   C.byref(ops) if ops is not None else None)
+    return out
+
+
 def ref_edt_sq_threads(img, threads, resolution=None, use_ref=False):
     """ref_edt_sq of a non-periodic 3-D image with each pass's lines split
     over `threads` host threads (orc_ref_edt3d_sq_mt; identical output)."""
@@ -212,15 +232,16 @@ def ref_edt(img, periodic=None, resolution=None, use_ref=False):
 # Exact integer EDT and labelling
 # --------------------------------------------------------------------------
 
-def exact_edt_sq(img, bc=None):
+def exact_edt_sq(img, bc=None, threads=1):
     """Exact squared EDT (int32; D2_INF where no solid is reachable).
-    bc: per-face boundary types, sequence of 2*ndim ints (END/WALL/PERIODIC)."""
+    bc: per-face boundary types, sequence of 2*ndim ints (END/WALL/PERIODIC).
+    threads > 1 splits each pass's lines over host threads (same output)."""
     img = np.ascontiguousarray(img, dtype=np.uint8)
     nd = img.ndim
     bc = np.array(bc if bc is not None else [BC_END] * (2 * nd), dtype=np.int32)
     shape = np.array(img.shape, dtype=np.int64)
     out = np.empty(img.shape, dtype=np.int32)
-    lib().orc_exact_edt_sq(_p(img), nd, _p(shape), _p(bc), _p(out))
+    lib().orc_exact_edt_sq_mt(_p(img), nd, _p(shape), _p(bc), _p(out), int(threads))
     return out
 
 
@@ -525,4 +546,97 @@ def pore_size_distribution(box: Box, pm_img, radii, inlet=False, mode="distance"
         t = porosimetry(box, pm_img, r, inlet, mode=mode)
         if np.any(t):
             out = np.where((pm_img == 1) & (out == 0) & (t == 1), r, out)
+    return out
+
+
+def psd_codes_exact(box: Box, pm_img, radii, inlet=False, threads=1):
+    """pore_size_distribution (porosimetry.py:163-227) in mode "distance"
+    at unit resolution on a box with no periodic face, as radius codes:
+    code i+1 where radii[i] (descending) first covers a pore voxel, 0 where
+    none does.  The same steps as ``pore_size_distribution``, with the
+    float32 EDT replaced by the exact integer EDT it equals at unit
+    resolution, the erosion EDT computed once (it does not depend on the
+    radius), the threshold sqrt_f32(d2) <= r written d2 <= T(r) and the
+    labelling done by scipy.ndimage.label with a 3x3x3 structure; for sizes
+    where that chain is too slow.  tests/test_oracle_morph_ccl.py checks it
+    against ``pore_size_distribution``."""
+    from scipy import ndimage
+
+    if any(b == BC_PERIODIC for pair in box.bc for b in pair):
+        raise ValueError("psd_codes_exact covers boxes without periodic faces")
+    if any(float(r) != 1.0 for r in box.resolution):
+        raise ValueError("psd_codes_exact covers unit resolution")
+    pm_img = np.ascontiguousarray(pm_img, dtype=np.uint8)
+    own = _own_slices(box, pm_img.shape)
+    d2_pm = exact_edt_sq(pm_img, [BC_END] * 6, threads=threads)
+    codes = np.zeros(pm_img.shape, dtype=np.uint8)
+    pore = pm_img == 1
+    for i, r in enumerate(radii):
+        t = radius_threshold(r)
+        eroded = d2_pm > t
+        if inlet:
+            lab, _ = ndimage.label(eroded, structure=np.ones((3, 3, 3), dtype=bool))
+            keep = np.zeros(int(lab.max()) + 1, dtype=bool)
+            for ax in range(3):
+                for side in (0, 1):
+                    if box.inlet[ax][side] and box.bc[ax][side] == BC_END:
+                        idx = list(own)
+                        p = box.pad[ax][side]
+                        n = pm_img.shape[ax]
+                        idx[ax] = (slice(p, 2 * p) if p else slice(0, 1)) if side == 0 else \
+                            (slice(n - 2 * p, n - p) if p else slice(n - 1, n))
+                        keep[np.unique(lab[tuple(idx)])] = True
+            keep[0] = False
+            eroded = keep[lab]
+            del lab
+        if not eroded.any():
+            continue
+        d2 = exact_edt_sq(np.logical_not(eroded).astype(np.uint8), [BC_END] * 6, threads=threads)
+        cover = d2 <= t
+        del d2
+        cover = set_wall_bcs(box, cover.astype(np.uint8)).astype(bool)
+        codes[pore & (codes == 0) & cover] = i + 1
+    return codes
+
+
+def radius_threshold(r):
+    """Largest integer d2 with np.sqrt(np.float32(d2)) <= r under NumPy 2
+    promotion (a Python float compares in float32, a numpy float64 in
+    float64): the threshold the EDT-mode erosion and dilation apply
+    (morphological_operators.py:147,226)."""
+    lo, hi = 0, 1
+    dt = np.float64 if isinstance(r, np.floating) and np.asarray(r).dtype == np.float64 else np.float32
+    rr = dt(r)
+
+    def ok(d):
+        return bool(np.sqrt(np.float32(d)) <= rr) if dt is np.float32 else bool(np.float64(np.sqrt(np.float32(d))) <= rr)
+
+    if not ok(0):
+        return -1
+    while ok(hi):
+        hi *= 2
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        if ok(mid):
+            lo = mid
+        else:
+            hi = mid
+    return lo
+
+
+def saturation_from_codes(box: Box, pm_img, codes, n_radii, domain_voxels):
+    """Wetting saturation after each radius of a sweep, in float64 as the
+    reference forms it: S_w = (count(own pore not yet invaded) / V) /
+    (count(own pore) / V) (multiphase.py:56-95, porousmedia.py:56-66)."""
+    own = _own_slices(box, pm_img.shape)
+    pm_own = pm_img[own] == 1
+    c = np.bincount(codes[own][pm_own].ravel(), minlength=n_radii + 1)
+    vol = float(np.prod(domain_voxels))
+    pore = int(pm_own.sum())
+    porosity = pore / vol
+    out = np.zeros(n_radii, dtype=np.float64)
+    invaded = 0
+    for i in range(n_radii):
+        invaded += int(c[i + 1])
+        out[i] = ((pore - invaded) / vol) / porosity
     return out

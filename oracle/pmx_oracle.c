@@ -491,6 +491,123 @@ ORC_EXPORT void orc_ref_edt3d_sq_mt(const uint8_t *img, long nx, long ny, long n
     }
 }
 
+/* corrected_edt(img, subdomain) for one rank of a padded subdomain
+ * (euclidean_distance_transform.py:56-101): the passes run over the whole
+ * padded array; on a periodic axis the rank exchanges features with itself
+ * (communication.py:278-285), so the correctors and hulls it receives are
+ * the ones it computed for the opposite face.
+ *
+ * Pass 0 correctors (get_initial_correctors, :395-447, through
+ * get_nearest_boundary_distance, :304-392, with which_voxels="pad" and
+ * distance_to="own").  The faces are visited lower face first
+ * (orientation.py:79-114) and that function's skip variables persist
+ * across faces, so the upper face's search also skips the lower pad:
+ *   lower face: first zero in [2 p_lo, S)          -> d_lo = idx - p_lo
+ *   upper face: last zero in [2 p_lo, S - 2 p_hi)  -> d_hi = S - idx - p_hi - 1
+ * swapped: the line start is seeded with d_hi, the line end capped at d_lo.
+ *
+ * Passes 1, 2 hulls (get_boundary_hull, :450-528): the lower face's
+ * forward hull of the segment from 2 p_lo to the first zero at or after it
+ * (vertices relative to 2 p_lo), the upper face's backward hull of the
+ * segment from the last zero before S - 2 p_hi to S - 2 p_hi (vertices
+ * relative to that end, negative); swapped into the line's lower and
+ * upper hulls.  Output: squared float32 over the whole padded array. */
+ORC_EXPORT void orc_ref_corrected_edt_sq(const uint8_t *img, long nx, long ny, long nz,
+                                         const int *periodic, const long *pads, const float *res,
+                                         float *out, const orc_line_ops *ops)
+{
+    if (!ops) ops = &restated_ops;
+    long shape[3] = {nx, ny, nz};
+    geom g = make_geom(3, shape);
+    long total = nx * ny * nz;
+    const int num_hull = 4;
+    /* pass 0 */
+    {
+        int d1, d2;
+        other_dims(3, 0, &d1, &d2);
+        int S = (int)g.shape[0];
+        long st = g.stride[0];
+        int plo = (int)pads[0], phi = (int)pads[1];
+        for (long a = 0; a < g.shape[d1]; a++)
+            for (long b = 0; b < g.shape[d2]; b++) {
+                long base = a * g.stride[d1] + b * g.stride[d2];
+                float lo = INFINITY, hi = INFINITY;
+                if (periodic[0]) {
+                    long f = ops->nearest_index(img + base, S, st, 0, 2 * plo, 0, 1);
+                    long l = ops->nearest_index(img + base, S, st, 0, 2 * plo, 2 * phi, 0);
+                    float d_lo = f != -1 ? (float)(f - plo) : INFINITY;
+                    float d_hi = l != -1 ? (float)(S - l - phi - 1) : INFINITY;
+                    lo = d_hi;
+                    hi = d_lo;
+                }
+                ops->sq_edt_1d(img + base, out + base, S, st, res[0], lo, hi);
+            }
+        to_finite(out, (size_t)total);
+    }
+    for (int dim = 1; dim <= 2; dim++) {
+        int d1, d2;
+        other_dims(3, dim, &d1, &d2);
+        long s1 = g.shape[d1], s2 = g.shape[d2];
+        int S = (int)g.shape[dim];
+        long st = g.stride[dim];
+        if (!periodic[dim]) {
+            for (long a = 0; a < s1; a++)
+                for (long b = 0; b < s2; b++)
+                    ops->parabolic(out + a * g.stride[d1] + b * g.stride[d2], S, res[dim], st, NULL, 0, NULL, 0);
+            continue;
+        }
+        int plo = (int)pads[2 * dim], phi = (int)pads[2 * dim + 1];
+        long lines = s1 * s2;
+        orc_hull *hf = (orc_hull *)malloc(sizeof(orc_hull) * (size_t)lines * (num_hull + 1));
+        orc_hull *hb = (orc_hull *)malloc(sizeof(orc_hull) * (size_t)lines * (num_hull + 1));
+        int *nf = (int *)malloc(sizeof(int) * (size_t)lines);
+        int *nb = (int *)malloc(sizeof(int) * (size_t)lines);
+        for (long a = 0; a < s1; a++)
+            for (long b = 0; b < s2; b++) {
+                long base = a * g.stride[d1] + b * g.stride[d2];
+                long li = a * s2 + b;
+                for (int fw = 1; fw >= 0; fw--) {
+                    int lsk = fw ? 2 * plo : 0, usk = fw ? 0 : 2 * phi;
+                    long bound = ops->nearest_index(img + base, S, st, 0, lsk, usk, fw);
+                    long start;
+                    int nv, ic;
+                    if (bound == -1) {
+                        start = lsk;
+                        nv = S - usk - lsk;
+                        ic = fw ? 0 : -nv;
+                    } else if (fw) {
+                        start = lsk;
+                        nv = (int)bound - lsk + 1;
+                        ic = 0;
+                    } else {
+                        start = bound;
+                        nv = S - (int)bound - usk;
+                        ic = -nv;
+                    }
+                    orc_hull *dst = fw ? &hf[li * (num_hull + 1)] : &hb[li * (num_hull + 1)];
+                    int cnt = nv > 0 ? ops->boundary_hull(out + base + start * st, nv, res[dim], st, num_hull, ic, fw,
+                                                          dst)
+                                     : 0;
+                    if (fw)
+                        nf[li] = cnt;
+                    else
+                        nb[li] = cnt;
+                }
+            }
+        for (long a = 0; a < s1; a++)
+            for (long b = 0; b < s2; b++) {
+                long li = a * s2 + b;
+                /* lower <- the upper face's backward hull, upper <- the lower face's forward hull */
+                ops->parabolic(out + a * g.stride[d1] + b * g.stride[d2], S, res[dim], st, &hb[li * (num_hull + 1)],
+                               nb[li], &hf[li * (num_hull + 1)], nf[li]);
+            }
+        free(hf);
+        free(hb);
+        free(nf);
+        free(nb);
+    }
+}
+
 /* edt2d(img, periodic, resolution): euclidean_distance_transform.py:104-205
  * (pass 0 along axis 1, parabolic along axis 0).  Output: squared float32. */
 ORC_EXPORT void orc_ref_edt2d_sq(const uint8_t *img, long nx, long ny, const int *periodic,
@@ -572,45 +689,84 @@ static void exact_line(const int64_t *f, int64_t *outv, int n, int bc_lo, int bc
     }
 }
 
-/* Exact squared EDT of a C-order 3-D (nd=3) or 2-D (nd=2) binary image,
- * solid = (img == 0), with per-face boundary types bc[2*axis + side].
- * Output int32; ORC_D2_INF where no solid is reachable. */
-ORC_EXPORT void orc_exact_edt_sq(const uint8_t *img, int nd, const long *shape, const int *bc,
-                                 int32_t *out)
+/* One pass of the exact squared EDT over the lines along `dim` whose first
+ * other-axis index lies in [a0, a1): each line is read from `out`, run
+ * through exact_line and written back. */
+typedef struct {
+    int32_t *out;
+    geom g;
+    int nd, dim, bc_lo, bc_hi;
+    long a0, a1;
+} exact_job;
+
+static void *exact_worker(void *arg)
 {
-    geom g = make_geom(nd, shape);
-    long total = g.shape[0] * g.shape[1] * g.shape[2];
-    long nmax = 0;
-    for (int i = 0; i < nd; i++)
-        if (g.shape[i] > nmax) nmax = g.shape[i];
-    int64_t *f = (int64_t *)malloc(sizeof(int64_t) * (size_t)nmax);
-    int64_t *o = (int64_t *)malloc(sizeof(int64_t) * (size_t)nmax);
-    size_t cap = (size_t)(3 * nmax + 4);
+    exact_job *j = (exact_job *)arg;
+    int d1, d2;
+    other_dims(j->nd, j->dim, &d1, &d2);
+    long s2 = d2 >= 0 ? j->g.shape[d2] : 1;
+    long st1 = j->g.stride[d1], st2 = d2 >= 0 ? j->g.stride[d2] : 0;
+    int n = (int)j->g.shape[j->dim];
+    long st = j->g.stride[j->dim];
+    size_t cap = (size_t)(3 * n + 4);
+    int64_t *f = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t *o = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
     int64_t *v = (int64_t *)malloc(sizeof(int64_t) * cap), *h = (int64_t *)malloc(sizeof(int64_t) * cap);
     int64_t *zn = (int64_t *)malloc(sizeof(int64_t) * cap), *zd = (int64_t *)malloc(sizeof(int64_t) * cap);
-    for (long i = 0; i < total; i++) out[i] = img[i] == 0 ? 0 : ORC_D2_INF;
-    for (int dim = 0; dim < nd; dim++) {
-        int d1, d2;
-        other_dims(nd, dim, &d1, &d2);
-        long s1 = g.shape[d1], s2 = d2 >= 0 ? g.shape[d2] : 1;
-        long st1 = g.stride[d1], st2 = d2 >= 0 ? g.stride[d2] : 0;
-        int n = (int)g.shape[dim];
-        long st = g.stride[dim];
-        for (long a = 0; a < s1; a++)
-            for (long b = 0; b < s2; b++) {
-                long base = a * st1 + b * st2;
-                for (int i = 0; i < n; i++) f[i] = out[base + i * st];
-                exact_line(f, o, n, bc[2 * dim], bc[2 * dim + 1], v, h, zn, zd);
-                for (int i = 0; i < n; i++)
-                    out[base + i * st] = o[i] >= ORC_D2_INF ? ORC_D2_INF : (int32_t)o[i];
-            }
-    }
+    for (long a = j->a0; a < j->a1; a++)
+        for (long b = 0; b < s2; b++) {
+            long base = a * st1 + b * st2;
+            for (int i = 0; i < n; i++) f[i] = j->out[base + i * st];
+            exact_line(f, o, n, j->bc_lo, j->bc_hi, v, h, zn, zd);
+            for (int i = 0; i < n; i++)
+                j->out[base + i * st] = o[i] >= ORC_D2_INF ? ORC_D2_INF : (int32_t)o[i];
+        }
     free(f);
     free(o);
     free(v);
     free(h);
     free(zn);
     free(zd);
+    return NULL;
+}
+
+/* Exact squared EDT of a C-order 3-D (nd=3) or 2-D (nd=2) binary image,
+ * solid = (img == 0), with per-face boundary types bc[2*axis + side].
+ * Output int32; ORC_D2_INF where no solid is reachable.  The lines of each
+ * pass are split over `nthreads` POSIX threads by their first other-axis
+ * index (a join between passes); every line is computed the same way
+ * whatever the split, so the output does not depend on nthreads. */
+ORC_EXPORT void orc_exact_edt_sq_mt(const uint8_t *img, int nd, const long *shape, const int *bc,
+                                    int32_t *out, int nthreads)
+{
+    geom g = make_geom(nd, shape);
+    long total = g.shape[0] * g.shape[1] * g.shape[2];
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    for (long i = 0; i < total; i++) out[i] = img[i] == 0 ? 0 : ORC_D2_INF;
+    pthread_t th[256];
+    exact_job jobs[256];
+    int created[256];
+    for (int dim = 0; dim < nd; dim++) {
+        int d1, d2;
+        other_dims(nd, dim, &d1, &d2);
+        long s1 = g.shape[d1];
+        for (int t = 0; t < nthreads; t++) {
+            exact_job j = {out, g, nd, dim, bc[2 * dim], bc[2 * dim + 1], s1 * t / nthreads,
+                           s1 * (t + 1) / nthreads};
+            jobs[t] = j;
+            created[t] = nthreads > 1 && pthread_create(&th[t], NULL, exact_worker, &jobs[t]) == 0;
+            if (!created[t]) exact_worker(&jobs[t]); /* no thread: run the share here */
+        }
+        for (int t = 0; t < nthreads; t++)
+            if (created[t]) pthread_join(th[t], NULL);
+    }
+}
+
+ORC_EXPORT void orc_exact_edt_sq(const uint8_t *img, int nd, const long *shape, const int *bc,
+                                 int32_t *out)
+{
+    orc_exact_edt_sq_mt(img, nd, shape, bc, out, 1);
 }
 
 /* ------------------------------------------------------------------------ */

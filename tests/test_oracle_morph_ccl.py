@@ -98,3 +98,45 @@ def test_ccl_reference_kats():
     img = (np.arange(120) % 256).astype(np.uint8).reshape(6, 5, 4)
     assert O.ccl26(img)[1] == 119
     assert O.ccl26(img + 1)[1] == 120
+
+
+@pytest.mark.parametrize("seed,inlet,radii", [
+    (1, True, [np.float64(r) for r in range(6, 0, -1)]),
+    (2, False, [4, 3.5, 2, 1]),
+    (3, True, [5.5, 3, 1.5]),
+])
+def test_psd_codes_exact_equals_reference_chain(seed, inlet, radii):
+    """The fast pore-size-distribution restatement (exact EDT computed once,
+    scipy labelling) gives the radius codes of the reference chain
+    (pore_size_distribution: float32 EDT per radius, the oracle's
+    labelling), on a padded x END / y,z WALL box with inlet x = 0."""
+    from tests.helpers import padded_case
+    own = rand_binary((26, 22, 20), 0.25, seed)
+    img, _, box = padded_case(own, (("end", "end"), ("wall", "wall"), ("wall", "wall")),
+                              inlet=((True, False), (False, False), (False, False)))
+    got = O.psd_codes_exact(box, img, radii, inlet=inlet, threads=2)
+    ref = O.pore_size_distribution(box, img, radii, inlet=inlet, mode="distance")
+    want = np.zeros(img.shape, np.uint8)
+    for i, r in enumerate(radii):
+        want[ref == r] = i + 1
+    assert got.max() > 0
+    np.testing.assert_array_equal(got, want)
+    # the saturation arithmetic against a direct float64 evaluation
+    sw = O.saturation_from_codes(box, img, got, len(radii), own.shape)
+    own_sl = (slice(None), slice(1, -1), slice(1, -1))
+    pore = int((img[own_sl] == 1).sum())
+    for i in range(len(radii)):
+        wet = int(((img[own_sl] == 1) & ((got[own_sl] == 0) | (got[own_sl] > i + 1))).sum())
+        assert sw[i] == (wet / own.size) / (pore / own.size)
+
+
+def test_radius_threshold_closed_form():
+    """T(r) = r^2 for integer radii (Python int compares in float32, numpy
+    float64 in float64); for other radii the largest d2 whose float32 root is
+    within r, checked against a direct scan."""
+    for r in range(1, 40):
+        assert O.radius_threshold(r) == r * r
+        assert O.radius_threshold(np.float64(r)) == r * r
+    for r in (0.5, 1.5, 2.25, 7.1, np.float64(3.3)):
+        want = max(d for d in range(200) if np.sqrt(np.float32(d)) <= (np.float64(r) if isinstance(r, np.float64) else np.float32(r)))
+        assert O.radius_threshold(r) == want

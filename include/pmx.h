@@ -103,7 +103,7 @@ int pmx_edt(const uint8_t *img, const pmx_geom *g, int32_t *d2, float *f32sq, fl
             double *f64, void *workspace, size_t workspace_bytes, hipStream_t stream);
 
 /* The reference's float32 EDT arithmetic, bit for bit, at any resolution
- * (edt3d / edt2d / corrected_edt, euclidean_distance_transform.py:104-303,
+ * (edt3d / edt2d / corrected_edt, euclidean_distance_transform.py:56-303,
  * per-line kernels _distance.hpp:135-356).  img: uint8 view (x, y strides
  * in_s[2]) of the box n[3]; d: float32 view (strides out_s[2]) receiving the
  * squared distance, or its float32 sqrt with take_sqrt.  first_axis gets the
@@ -111,10 +111,16 @@ int pmx_edt(const uint8_t *img, const pmx_geom *g, int32_t *d2, float *f32sq, fl
  * parabolic envelope (3-D: first_axis 0; 2-D image as a (1, X, Y) box:
  * first_axis 2).  periodic[3] selects the reference's corrector and
  * five-member boundary-hull handling on that axis; resolution[3] are the
- * float32 voxel sizes.  Voxels with no reachable solid hold FLT_MAX. */
+ * float32 voxel sizes.  pads: NULL for edt3d / edt2d (correctors and hulls
+ * from whole lines); or the six pad widths (x-, x+, y-, y+, z-, z+) of a
+ * PMMoTo padded subdomain, whose whole padded array is the box: then the
+ * periodic axes follow corrected_edt for one rank (pad-skipping face
+ * searches, :304-528), first_axis must be 0.  Voxels with no reachable
+ * solid hold FLT_MAX.  Replaces the per-line loops of
+ * _distance.pyx:86-127,308-353,544-618. */
 int pmx_edt_ref_workspace_size(const int64_t *n, size_t *bytes);
 int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float *d,
-                const int64_t *out_s, int first_axis, const int32_t *periodic,
+                const int64_t *out_s, int first_axis, const int32_t *periodic, const int64_t *pads,
                 const float *resolution, int take_sqrt, void *workspace, size_t workspace_bytes,
                 hipStream_t stream);
 

@@ -143,7 +143,7 @@ int pmx_halo_morph(const uint8_t *img, const int64_t *shape, const int64_t *pad,
     }
     const int64_t st[2] = {e.dst[1] * e.dst[2], e.dst[2]};
     const int32_t none[3] = {0, 0, 0};
-    if (int rc = pmx_edt_ref(halo, e.dst, st, d, st, 0, none, resolution, 1, rws, ref, s)) return rc;
+    if (int rc = pmx_edt_ref(halo, e.dst, st, d, st, 0, none, nullptr, resolution, 1, rws, ref, s)) return rc;
     hipLaunchKernelGGL(k_crop_threshold, dim3(4096), dim3(256), 0, s, d, e, radius, (float)radius,
                        compare_f64, v_le, v_gt, wall_mask, out);
     return check_launch("halo threshold");

@@ -34,10 +34,12 @@ def _unit(resolution) -> bool:
     return all(float(r) == 1.0 for r in resolution)
 
 
-def run_edt_ref(img_t: torch.Tensor, origin, n, first_axis, periodic, resolution, take_sqrt, out=None):
+def run_edt_ref(img_t: torch.Tensor, origin, n, first_axis, periodic, resolution, take_sqrt, out=None, pads=None):
     """pmx_edt_ref: the reference's float32 arithmetic (any resolution) on
     the box `origin`+`n` of a contiguous 3-D uint8 tensor; returns a
-    float32 tensor of the full shape (box region written)."""
+    float32 tensor of the full shape (box region written).  ``pads`` (six
+    widths) selects corrected_edt's padded-subdomain handling of periodic
+    axes, with the box the whole padded array."""
     shape = tuple(img_t.shape)
     off = (origin[0] * shape[1] + origin[1]) * shape[2] + origin[2]
     if out is None:
@@ -46,12 +48,13 @@ def run_edt_ref(img_t: torch.Tensor, origin, n, first_axis, periodic, resolution
     st = (C.c_int64 * 2)(shape[1] * shape[2], shape[2])
     per = (C.c_int32 * 3)(*[1 if p else 0 for p in periodic])
     res = (C.c_float * 3)(*[float(r) for r in resolution])
+    pd = None if pads is None else (C.c_int64 * 6)(*[int(v) for v in pads])
     L = _lib.lib()
     nbytes = C.c_size_t()
     _lib.check(L.pmx_edt_ref_workspace_size(nn, C.byref(nbytes)))
     ws = _device.workspace(nbytes.value)
-    _lib.check(L.pmx_edt_ref(_device.ptr(img_t, off), nn, st, _device.ptr(out, off), st, int(first_axis), per, res,
-                             1 if take_sqrt else 0, _device.ptr(ws), nbytes.value, _device.stream()))
+    _lib.check(L.pmx_edt_ref(_device.ptr(img_t, off), nn, st, _device.ptr(out, off), st, int(first_axis), per, pd,
+                             res, 1 if take_sqrt else 0, _device.ptr(ws), nbytes.value, _device.stream()))
     return out
 
 
@@ -215,10 +218,15 @@ def edt(img, subdomain=None, mode="auto"):
     res = subdomain.domain.resolution
     per = [bc[2 * a] == _lib.BC_PERIODIC for a in range(3)]
     if _use_reference(mode, res, per, t):
-        # non-periodic: edt3d of the padded image, as the reference; periodic:
-        # the reference arithmetic on the own box with wrap, pads copied
-        out = run_edt_ref(t, origin, n, 0, per, res, True)
-        fill_periodic_pads([out], tuple(t.shape), pads)
+        # non-periodic: edt3d of the padded image; periodic: corrected_edt,
+        # whose passes run over the whole padded array with correctors and
+        # boundary hulls measured past the pads (the pad voxels are computed,
+        # not copied), as the reference does for one rank
+        if any(per):
+            sd_pads = [int(p) for a in range(3) for p in subdomain.pad[a]]
+            out = run_edt_ref(t, (0, 0, 0), t.shape, 0, per, res, True, pads=sd_pads)
+        else:
+            out = run_edt_ref(t, (0, 0, 0), t.shape, 0, per, res, True)
         return _device.to_host(out, as_np)
     return _device.to_host(run_edt(t, origin, n, bc, pads, ("f32",))["f32"], as_np)
 

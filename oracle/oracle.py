@@ -392,18 +392,26 @@ def struct_ratio(resolution, radius):
     return tuple(math.ceil(radius / r) for r in resolution)
 
 
-def erode(box: Box, img, radius):
+def _halo_edt(halo, resolution, threads):
+    """The non-periodic float32 EDT of a halo image (np.sqrt of edt3d's
+    squared output); threads > 1 splits each pass's lines (same output)."""
+    if threads > 1:
+        return np.sqrt(ref_edt_sq_threads(halo, threads, resolution=resolution))
+    return ref_edt(halo, resolution=resolution)
+
+
+def erode(box: Box, img, radius, threads=1):
     """subtraction, EDT mode (morphological_operators.py:175-228)."""
     halo, pads = halo_image(box, img, struct_ratio(box.resolution, radius))
-    d = ref_edt(halo.astype(np.uint8), resolution=box.resolution)
+    d = _halo_edt(halo.astype(np.uint8), box.resolution, threads)
     out = np.where(d <= radius, 0, 1).astype(np.uint8)
     return set_wall_bcs(box, unpad(out, pads))
 
 
-def dilate(box: Box, img, radius):
+def dilate(box: Box, img, radius, threads=1):
     """addition, EDT mode (morphological_operators.py:103-149)."""
     halo, pads = halo_image(box, img, struct_ratio(box.resolution, radius))
-    d = ref_edt(np.logical_not(halo).astype(np.uint8), resolution=box.resolution)
+    d = _halo_edt(np.logical_not(halo).astype(np.uint8), box.resolution, threads)
     out = np.where(d <= radius, 1, 0).astype(np.uint8)
     return set_wall_bcs(box, unpad(out, pads))
 

@@ -1,0 +1,214 @@
+"""Parity at the benchmark workloads' real sizes.
+
+* 1024^3 headline: the exact squared EDT (pmx_edt, int32) of the 1024^3
+  lognormal sphere pack (x END, y/z WALL) bit for bit against the oracle's
+  exact EDT (threaded), and its float32 sqrt output;
+* config 3: the 1024^3 drainage-radius sweep (32 radii, inlet x = 0) against
+  the committed oracle fixture tests/golden/config3_psd_1024.json (SHA-256 of
+  the radius-code map, code histogram, float64 saturation curve bit for bit),
+  the fused sweep against the per-radius path at full size, and a 160^3
+  crop against the oracle's reference-driver chain;
+* config 2: the 512^3 fully periodic pack through the drop-in API on its
+  padded subdomain: edt3d(periodic) and edt(img, subdomain) in the
+  reference's float32 arithmetic, erosion and dilation at r = 5, and
+  26-connected labelling with the periodic N + 1 count;
+* config 4: all 256 batched domains equal their single-domain sweeps;
+* config 5: the shipped surrogate weights on the 512 held-out packs.
+
+Every test prints the case before its GPU work.
+"""
+
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+HOST_THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+@pytest.fixture(scope="module")
+def pack1024():
+    from pmmoto_amd.domain_generation import pack_radii, sphere_pack
+    print("[config 3] 1024^3 lognormal pack, seed 3", flush=True)
+    img_t, _ = sphere_pack((1024,) * 3, 0.35, pack_radii("lognormal", mean=10.0, sigma=0.3), seed=3)
+    with open(os.path.join(GOLD, "config3_psd_1024.json")) as f:
+        fix = json.load(f)
+    img = img_t.cpu().numpy()
+    assert hashlib.sha256(img.tobytes()).hexdigest() == fix["image_sha256"]
+    return img_t, img, fix
+
+
+def test_headline_edt_1024_bit_exact(pack1024):
+    import torch
+
+    from pmmoto_amd.filters import distance
+    img_t, img, _ = pack1024
+    bc = [O.BC_END, O.BC_END] + [O.BC_WALL] * 4
+    print("[pmx_edt] 1024^3, x END / y,z WALL, outputs d2 + f32", flush=True)
+    out = distance.edt_squared_int(img_t, bc=bc, outputs=("d2", "f32"))
+    want = O.exact_edt_sq(img, bc, threads=HOST_THREADS)
+    want_t = torch.from_numpy(want).cuda()
+    assert torch.equal(out["d2"], want_t), "1024^3 d2 differs from the exact oracle"
+    assert int(want.max()) < O.D2_INF
+    want_f32 = torch.sqrt(want_t.to(torch.float32))
+    assert torch.equal(out["f32"].view(torch.int32), want_f32.view(torch.int32))
+
+
+def _config3_subdomain(n):
+    import pmmoto_amd
+    BT = pmmoto_amd.BoundaryType
+    return pmmoto_amd.initialize((n, n, n), box=((0, n),) * 3,
+                                 boundary_types=((BT.END, BT.END), (BT.WALL, BT.WALL), (BT.WALL, BT.WALL)),
+                                 inlet=((True, False), (False, False), (False, False)),
+                                 outlet=((False, True), (False, False), (False, False)))
+
+
+def test_config3_sweep_1024_matches_oracle_fixture(pack1024):
+    import torch
+
+    from pmmoto_amd.filters import porosimetry as PO
+    img_t, _, fix = pack1024
+    n = 1024
+    sd = _config3_subdomain(n)
+    pm = torch.zeros(sd.voxels, dtype=torch.uint8, device="cuda")
+    pm[:, 1:-1, 1:-1] = img_t
+    radii = [np.float64(r) for r in range(32, 0, -1)]
+    print("[pmx_psd_sweep] 1024^3 + y/z wall pads, radii 32..1, inlet x=0", flush=True)
+    code = PO.psd_sweep_fused(sd, pm, radii, inlet=True, mode="distance")
+    codes8 = code.to(torch.uint8).cpu().numpy()
+    assert list(codes8.shape) == fix["shape"]
+    np.testing.assert_array_equal(np.bincount(codes8.ravel(), minlength=33), fix["code_counts"])
+    assert hashlib.sha256(codes8.tobytes()).hexdigest() == fix["code_sha256"]
+    sw = PO.saturation_curve(sd, pm, code, len(radii))
+    assert [float(v).hex() for v in sw] == fix["saturation_hex"]
+    del codes8
+    print("[psd_sweep_levels] 1024^3 per-radius path", flush=True)
+    levels = PO.psd_sweep_levels(sd, pm, radii, inlet=True, mode="distance")
+    assert torch.equal(levels, code), "fused sweep differs from the per-radius path"
+
+
+def test_config3_sweep_crop_matches_oracle_chain(pack1024):
+    """A 160^3 corner crop of the same pack, padded the same way, against the
+    oracle's restatement of the reference drivers (float32 EDT per radius)."""
+    import torch
+
+    from pmmoto_amd.filters import porosimetry as PO
+    _, img, _ = pack1024
+    n = 160
+    own = np.ascontiguousarray(img[:n, :n, :n])
+    sd = _config3_subdomain(n)
+    pm = np.zeros(sd.voxels, np.uint8)
+    pm[:, 1:-1, 1:-1] = own
+    radii = np.arange(32, 0, -1).astype(np.float64)
+    print("[pore_size_distribution] 160^3 crop vs oracle chain", flush=True)
+    pm_t = torch.from_numpy(pm).cuda()
+    got = PO.psd_sweep(sd, pm_t, list(radii), inlet=True, mode="distance").cpu().numpy()
+    box = O.Box(bc=((O.BC_END, O.BC_END), (O.BC_WALL, O.BC_WALL), (O.BC_WALL, O.BC_WALL)),
+                pad=((0, 0), (1, 1), (1, 1)), inlet=((True, False), (False, False), (False, False)))
+    want = O.pore_size_distribution(box, pm, list(radii), inlet=True, mode="distance")
+    got_r = np.where(got > 0, np.concatenate([[0.0], radii])[got], 0.0)
+    assert (got > 0).sum() > 0
+    np.testing.assert_array_equal(got_r, want)
+
+
+@pytest.fixture(scope="module")
+def pack512():
+    import pmmoto_amd
+    from pmmoto_amd.domain_generation import pack_radii, sphere_pack
+    print("[config 2] 512^3 periodic pack, seed 2", flush=True)
+    img_t, _ = sphere_pack((512,) * 3, 0.50, pack_radii("uniform", lo=4, hi=12), seed=2,
+                           periodic=(True, True, True))
+    own = img_t.cpu().numpy()
+    BT = pmmoto_amd.BoundaryType
+    sd = pmmoto_amd.initialize((512,) * 3, box=((0, 512),) * 3, boundary_types=((BT.PERIODIC, BT.PERIODIC),) * 3)
+    padded = np.ascontiguousarray(np.pad(own, 1, mode="wrap"))
+    box = O.Box(bc=((O.BC_PERIODIC, O.BC_PERIODIC),) * 3, pad=((1, 1),) * 3)
+    return img_t, own, sd, padded, box
+
+
+def test_config2_edt_reference_mode_512(pack512):
+    from pmmoto_amd.filters import distance
+    img_t, own, sd, padded, _ = pack512
+    print("[edt3d periodic] 512^3 reference arithmetic", flush=True)
+    got = distance.edt3d(img_t, periodic=(True, True, True), squared=True).cpu().numpy()
+    want = O.ref_edt_sq(own, (True, True, True))
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    del got, want
+    print("[edt subdomain] 514^3 padded periodic (corrected_edt)", flush=True)
+    got = distance.edt(padded, sd)
+    want = np.sqrt(O.ref_corrected_edt_sq(padded, [True] * 3, [1] * 6, sd.domain.resolution))
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_config2_erode_dilate_r5_512(pack512):
+    from pmmoto_amd.filters import morphological_operators as M
+    _, _, sd, padded, box = pack512
+    print("[erode/dilate r=5] 514^3 padded periodic", flush=True)
+    np.testing.assert_array_equal(M.erode(sd, padded, 5), O.erode(box, padded, 5, threads=HOST_THREADS))
+    np.testing.assert_array_equal(M.dilate(sd, padded, 5), O.dilate(box, padded, 5, threads=HOST_THREADS))
+
+
+def test_config2_ccl26_512(pack512):
+    from pmmoto_amd.filters import connected_components as CC
+    _, _, sd, padded, box = pack512
+    print("[connect_components] 514^3 padded periodic", flush=True)
+    labels, count = CC.connect_components(padded, sd)
+    want, n = O.label_padded(box, padded)
+    np.testing.assert_array_equal(labels.astype(np.int64), want)
+    assert n > 1
+    assert count == n + 1  # the reference's count on periodic domains (test_connect_components.py)
+
+
+def test_config4_batch_256_equals_single_domain_sweeps():
+    import torch
+
+    from pmmoto_amd.domain_generation import pack_radii, sphere_pack
+    from pmmoto_amd.filters.batch import BatchDomain, BatchSweep
+    from tests.test_batch_gpu import single_sweep
+    doms = []
+    for d in range(256):
+        rng = np.random.default_rng(4000 + d)
+        shape = tuple(int(v) for v in rng.integers(64, 385, 3))
+        por = float(rng.uniform(0.25, 0.6))
+        bc = tuple("periodic" if rng.random() < 0.5 else "wall" for _ in range(3))
+        img, _ = sphere_pack(shape, por, pack_radii("uniform", lo=4, hi=12), seed=4000 + d,
+                             periodic=tuple(b == "periodic" for b in bc))
+        doms.append(BatchDomain(img, bc, None))
+    radii = list(range(24, 0, -1))
+    print("[batch] 256 domains, 24 radii, captured graph", flush=True)
+    sweep = BatchSweep(doms, radii, lanes=4, autotune=True, graph=True)
+    maps = sweep.run()
+    maps = sweep.run()  # a replay
+    torch.cuda.synchronize()
+    for i, d in enumerate(doms):
+        want = single_sweep(d.img, d.bc, radii, None)
+        assert torch.equal(maps[i], want), f"domain {i} {tuple(d.img.shape)} {d.bc}"
+
+
+def test_config5_surrogate_512_heldout():
+    """The shipped weights on the 512 held-out packs (seeds 6000000 + i):
+    mean absolute saturation error <= 0.02 over the returned curves, every
+    fallback curve is the exact sweep's, and with tau = 0 every domain falls
+    back and the curves are the exact ones."""
+    from pmmoto_amd.surrogate.data import RADII, make_dataset
+    from pmmoto_amd.surrogate.train import SurrogatePorosimetry, load, mae
+    path = os.path.join(os.path.dirname(os.path.dirname(__file__)), "pmmoto_amd", "surrogate", "weights_128.npz")
+    net = load(path)
+    print("[surrogate] 512 held-out 128^3 packs", flush=True)
+    x, exact, bcs = make_dataset(512, 6_000_000, size=net.size)
+    sp = SurrogatePorosimetry(net, 0.02, RADII)
+    curve, sigma, fall = sp.curves(x, bcs, batch=512)
+    assert mae(curve, exact) <= 0.02
+    if fall.size:
+        np.testing.assert_array_equal(curve[fall], exact[fall])
+    sub = np.arange(16)
+    curve0, _, fall0 = SurrogatePorosimetry(net, 0.0, RADII).curves(x[:16], bcs[:16], batch=16)
+    assert fall0.tolist() == sub.tolist()
+    np.testing.assert_array_equal(curve0, exact[:16])

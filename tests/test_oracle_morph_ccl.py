@@ -140,3 +140,14 @@ def test_radius_threshold_closed_form():
     for r in (0.5, 1.5, 2.25, 7.1, np.float64(3.3)):
         want = max(d for d in range(200) if np.sqrt(np.float32(d)) <= (np.float64(r) if isinstance(r, np.float64) else np.float32(r)))
         assert O.radius_threshold(r) == want
+
+
+def test_threaded_halo_edt_gives_the_same_morphology():
+    """erode/dilate with the halo EDT's lines split over threads equal the
+    single-threaded chain (the threaded driver runs the same line code)."""
+    img = rand_binary((40, 36, 30), 0.1, 9)
+    box = O.Box(bc=((O.BC_PERIODIC, O.BC_PERIODIC), (O.BC_END, O.BC_END), (O.BC_WALL, O.BC_WALL)),
+                pad=((1, 1), (0, 0), (1, 1)))
+    for r in (2, 3.5):
+        np.testing.assert_array_equal(O.erode(box, img, r, threads=4), O.erode(box, img, r))
+        np.testing.assert_array_equal(O.dilate(box, img, r, threads=4), O.dilate(box, img, r))

@@ -102,6 +102,19 @@ int pmx_edt_workspace_size(const pmx_geom *g, size_t *bytes);
 int pmx_edt(const uint8_t *img, const pmx_geom *g, int32_t *d2, float *f32sq, float *f32,
             double *f64, void *workspace, size_t workspace_bytes, hipStream_t stream);
 
+/* No counterpart in the reference.  pmx_edt for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): the same kernels and outputs, with the workspace
+ * in a static device array of PMX_EDT_NOALLOC_SCRATCH_BYTES that every GPU
+ * the library runs on holds until the process ends.  A box whose workspace
+ * exceeds it (beyond about 340^3 voxels) returns PMX_ERR_INVALID; use pmx_edt.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (other streams or host threads) race, so serialise them.  Allocates
+ * nothing. */
+#define PMX_EDT_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int pmx_edt_noalloc(const uint8_t *img, const pmx_geom *g, int32_t *d2, float *f32sq, float *f32,
+                    double *f64, hipStream_t stream);
+
 /* The reference's float32 EDT arithmetic, bit for bit, at any resolution
  * (edt3d / edt2d / corrected_edt, euclidean_distance_transform.py:56-303,
  * per-line kernels _distance.hpp:135-356).  img: uint8 view (x, y strides

@@ -56,6 +56,7 @@ SIGNATURES = {
     "pmx_profile_end": (C.c_int, [P, C.c_int, P, P, C.c_int, P]),
     "pmx_edt_workspace_size": (C.c_int, [P, P]),
     "pmx_edt": (C.c_int, [P, P, P, P, P, P, P, SZ, P]),
+    "pmx_edt_noalloc": (C.c_int, [P, P, P, P, P, P, P]),
     "pmx_edt_ref_workspace_size": (C.c_int, [P, P]),
     "pmx_edt_ref": (C.c_int, [P, P, P, P, P, C.c_int, P, P, P, C.c_int, P, SZ, P]),
     "pmx_halo_morph_workspace_size": (C.c_int, [P, P, P]),

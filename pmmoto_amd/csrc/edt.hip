@@ -1066,7 +1066,7 @@ void dispatch_window(int tile_rows, bool wide, const E *in, O *out, const PassXO
 
 }  // namespace
 
-size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p)
+size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p, int max_lanes)
 {
     Carver c(ws, SIZE_MAX);
     // working rows padded to a multiple of 8 voxels: 16-byte aligned rows
@@ -1085,7 +1085,7 @@ size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p)
     // persistent fallback lanes: one per deferred line up to 32768, with
     // the stacks bounded to kFallbackStackBytes
     const size_t per_lane = (size_t)(2 * p->cap + kChunk) * sizeof(int);
-    size_t lanes = std::min<size_t>(32768, (std::max(ly, lx) + 255) / 256 * 256);
+    size_t lanes = std::min<size_t>((size_t)std::max(max_lanes, 256) / 256 * 256, (std::max(ly, lx) + 255) / 256 * 256);
     lanes = std::min(lanes, std::max<size_t>(256, kFallbackStackBytes / per_lane / 256 * 256));
     p->fallback_lanes = (int)lanes;
     p->stk = c.take<int>((size_t)(2 * p->cap + kChunk) * lanes);
@@ -1189,7 +1189,8 @@ int run_yx_narrow_first(const YX &yx)
 }  // namespace
 
 int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &o, int64_t cap,
-            void *ws, size_t ws_bytes, hipStream_t s, const unsigned int *proceed, int tile_rows, int match)
+            void *ws, size_t ws_bytes, hipStream_t s, const unsigned int *proceed, int tile_rows, int match,
+            int max_lanes)
 {
     if (tile_rows != 0 && tile_rows != 32 && tile_rows != 64)
         return fail(PMX_ERR_INVALID, "tile_rows %d: use 0 (default), 32 or 64", tile_rows);
@@ -1197,10 +1198,10 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     if (int e = validate_geom(g)) return e;
     if (!img) return fail(PMX_ERR_INVALID, "img is NULL");
     EdtPlan p;
-    const size_t need = plan_edt(g, nullptr, &p);
+    const size_t need = plan_edt(g, nullptr, &p, max_lanes);
     if (!ws || ws_bytes < need)
         return fail(PMX_ERR_WORKSPACE, "workspace %zu bytes < required %zu", ws_bytes, need);
-    plan_edt(g, ws, &p);
+    plan_edt(g, ws, &p, max_lanes);
     const int ccap = (int)std::min<int64_t>(std::max<int64_t>(cap, 0), kBig);
     const int nx = (int)g->n[0], ny = (int)g->n[1], nz = (int)g->n[2];
     const int64_t rows = (int64_t)nx * ny;

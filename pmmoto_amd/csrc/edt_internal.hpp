@@ -38,8 +38,12 @@ struct EdtPlan {
     int fallback_lanes = 0;
 };
 
-// Workspace layout of run_edt for geometry g (sizes only when ws is null).
-size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p);
+// Fallback lanes of the exact EDT at most (one per deferred line).
+constexpr int kMaxFallbackLanes = 32768;
+
+// Workspace layout of run_edt for geometry g (sizes only when ws is null);
+// max_lanes bounds the persistent fallback lanes (and their stacks).
+size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p, int max_lanes = kMaxFallbackLanes);
 
 // Exact squared EDT of img over the box of g (distance to voxels == 0, or
 // != 0 with invert), clamped to min(d2, cap) (cap >= 2^30: unclamped),
@@ -48,8 +52,9 @@ size_t plan_edt(const pmx_geom *g, void *ws, EdtPlan *p);
 // kernel does nothing when *proceed == 0 at run time.  tile_rows: rows per
 // tile of the windowed passes, 0 (default 64), 32 or 64.  match >= 0: the
 // distance is to the voxels whose byte equals match (invert is ignored).
+// max_lanes: as for plan_edt (the workspace must follow the same plan).
 int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &o, int64_t cap,
             void *ws, size_t ws_bytes, hipStream_t s, const unsigned int *proceed = nullptr,
-            int tile_rows = 0, int match = -1);
+            int tile_rows = 0, int match = -1, int max_lanes = kMaxFallbackLanes);
 
 }  // namespace pmx

@@ -14,30 +14,40 @@
  *
  * Entry points and the reference interface each one replaces:
  *
- *   pmx_edt           the three per-line passes of edt3d/corrected_edt
- *                     (src/pmmoto/filters/distance/euclidean_distance_transform.py:56-101,
- *                     :208-303) driven line by line through
+ *   pmx_edt           the three per-line passes of edt3d (src/pmmoto/filters/distance/
+ *                     euclidean_distance_transform.py:208-303) driven line by line through
  *                     _distance.get_initial_envelope (_distance.pyx:308-353) and
- *                     _distance.get_parabolic_envelope (_distance.pyx:86-127),
- *                     whose C++ kernels are squared_edt_1d / squared_edt_1d_parabolic
- *                     (_distance.hpp:135-272); the periodic correctors and hull
- *                     exchange (euclidean_distance_transform.py:395-528) become
- *                     in-kernel wrap handling.
+ *                     _distance.get_parabolic_envelope (_distance.pyx:86-127), whose C++
+ *                     kernels are squared_edt_1d / squared_edt_1d_parabolic
+ *                     (_distance.hpp:135-272): the exact integer EDT, equal to the
+ *                     reference's float32 output at unit resolution on 0/1 images without
+ *                     periodic axes.
+ *   pmx_edt_ref       the same passes in the reference's float32 arithmetic at any
+ *                     resolution, with its label-aware first pass, its five-member
+ *                     periodic boundary hulls (edt3d) and corrected_edt's padded-subdomain
+ *                     correctors and hulls (euclidean_distance_transform.py:56-101,
+ *                     :304-528): the drop-in for periodic axes, multi-label images and
+ *                     non-unit resolution.
  *   pmx_morph         the EDT-mode dilation / erosion bodies of
  *                     addition / subtraction (src/pmmoto/filters/morphological_operators.py:103-228):
  *                     EDT of the (inverted) halo image, threshold, set_wall_bcs.
  *   pmx_ccl26         cc3d.connected_components(img, return_N=True) at
  *                     src/pmmoto/filters/connected_components.py:52-54 plus the periodic
  *                     merge of connect_subdomain_boundaries (:65-101).
- *   pmx_face_labels   gen_inlet_label_map / gen_outlet_label_map
+ *   pmx_mark_labels   gen_inlet_label_map / gen_outlet_label_map
  *                     (src/pmmoto/core/voxels.py:400-454).
  *   pmx_keep_labels   the label->phase map + renumber_img of inlet_connected_img and
  *                     its siblings (connected_components.py:254-392, _voxels.pyx:453-494).
  *   pmx_fill_periodic_pads  update_buffer's self-neighbour copy for periodic faces
  *                     (src/pmmoto/core/communication.py:101-118, :278-285).
- *   pmx_rasterize_spheres   gen_pm_spheres_domain's centroid-in-sphere rasterisation
+ *   pmx_spheres_on_grid, pmx_sphere_first_cover, pmx_sphere_image
+ *                     gen_pm_spheres_domain's centroid-in-sphere rasterisation
  *                     (src/pmmoto/domain_generation/domain_generation.py:125-155,
- *                     particles/spheres.hpp:42-48), with a porosity stopping rule.
+ *                     particles/spheres.hpp:42-48), and seeded packs grown to a porosity.
+ *   pmx_psd_sweep     the radius loop of pore_size_distribution (porosimetry.py:163-227)
+ *                     as one call.
+ * Each declaration below cites the reference code it replaces, or says that it
+ * has none.
  */
 #ifndef PMX_H
 #define PMX_H
@@ -179,17 +189,25 @@ int pmx_ccl26(const uint8_t *img, const pmx_geom *g, int32_t *labels32, uint64_t
 
 /* flags[l] = 1 for every label l in [1, n_flags) in the sub-box lo[3] +
  * [0, extent[3]) of a C-order label array (label_bytes 4 or 8, element
- * strides[2] for x and y); flags holds n_flags bytes (count + 1). */
+ * strides[2] for x and y); flags holds n_flags bytes (count + 1).  A label
+ * >= n_flags (a stale count) is not written but counted into *bad (device
+ * uint32, may be NULL), which the caller checks.  Replaces the label
+ * collection of gen_inlet_label_map / gen_outlet_label_map
+ * (src/pmmoto/core/voxels.py:400-454). */
 int pmx_mark_labels(const void *labels, int label_bytes, const int64_t *strides, const int64_t *lo,
-                    const int64_t *extent, uint8_t *flags, uint64_t n_flags, hipStream_t stream);
+                    const int64_t *extent, uint8_t *flags, uint64_t n_flags, unsigned int *bad,
+                    hipStream_t stream);
 
 /* out[i] = keep ? img[i] : 0 over n contiguous voxels, keep decided from
  * the voxel's label l in [1, n_flags): mode 0 flags_a[l]; mode 1
  * flags_a[l] && flags_b[l]; mode 2 !flags_a[l] && !flags_b[l]; and, if
- * phase >= 0, img[i] == phase. */
+ * phase >= 0, img[i] == phase.  A nonzero label >= n_flags gives 0 and is
+ * counted into *bad (may be NULL).  Replaces the label->phase map and
+ * renumber_img of inlet_connected_img and its siblings
+ * (connected_components.py:254-392, _voxels.pyx:453-494). */
 int pmx_keep_labels(const uint8_t *img, const void *labels, int label_bytes, int64_t n,
                     const uint8_t *flags_a, const uint8_t *flags_b, uint64_t n_flags, int mode,
-                    int phase, uint8_t *out, hipStream_t stream);
+                    int phase, uint8_t *out, unsigned int *bad, hipStream_t stream);
 
 /* Porosimetry steps over n contiguous voxels.  pmx_psd_update: map[i] =
  * code where pm[i] == 1 && temp[i] == 1 && map[i] == 0 (the first-invasion

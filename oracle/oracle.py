@@ -429,24 +429,36 @@ def label_padded(box: Box, img):
     return full, n
 
 
+# faces in the reference's feature order (core/orientation.py:79-114): the
+# unit directions of itertools.product([-1, 0, 1], repeat=3)
+FACE_ORDER = ((0, 0), (1, 0), (2, 0), (2, 1), (1, 1), (0, 1))
+
+
+def face_slab(box: Box, shape, ax, side):
+    """The voxels a face feature owns (core/features.py:76-139): on the
+    face's axis 1 voxel thick on END faces (the outermost layer of a
+    reservoir); on the other axes the array without its pads, reservoir
+    layers included (subdomain_features.py:46-50 passes the padded voxel
+    counts and the pads)."""
+    idx = [slice(box.pad[a][0], shape[a] - box.pad[a][1]) for a in range(3)]
+    p = box.pad[ax][side]
+    n = shape[ax]
+    if side == 0:
+        idx[ax] = slice(p, 2 * p) if p else slice(0, 1)
+    else:
+        idx[ax] = slice(n - 2 * p, n - p) if p else slice(n - 1, n)
+    return tuple(idx)
+
+
 def inlet_labels(box: Box, labels, which="inlet"):
-    """gen_inlet_label_map / gen_outlet_label_map (core/voxels.py:400-454)
-    with the face own slabs of core/features.py:76-139 (1 voxel thick on
-    END faces, the outermost layer of a reservoir)."""
-    own = _own_slices(box, labels.shape)
+    """gen_inlet_label_map / gen_outlet_label_map (core/voxels.py:400-454):
+    each flagged END face overwrites the labels of the faces before it in
+    the reference's face order, so the last one's labels are returned."""
     faces = box.inlet if which == "inlet" else box.outlet
     found = set()
-    for ax in range(3):
-        for side in (0, 1):
-            if faces[ax][side] and box.bc[ax][side] == BC_END:
-                idx = list(own)
-                p = box.pad[ax][side]
-                if side == 0:
-                    idx[ax] = slice(p, 2 * p) if p else slice(0, 1)
-                else:
-                    n = labels.shape[ax]
-                    idx[ax] = slice(n - 2 * p, n - p) if p else slice(n - 1, n)
-                found |= set(np.unique(labels[tuple(idx)]).tolist())
+    for ax, side in FACE_ORDER:
+        if faces[ax][side] and box.bc[ax][side] == BC_END:
+            found = set(np.unique(labels[face_slab(box, labels.shape, ax, side)]).tolist())
     found.discard(0)
     return found
 
@@ -585,15 +597,7 @@ def psd_codes_exact(box: Box, pm_img, radii, inlet=False, threads=1):
         if inlet:
             lab, _ = ndimage.label(eroded, structure=np.ones((3, 3, 3), dtype=bool))
             keep = np.zeros(int(lab.max()) + 1, dtype=bool)
-            for ax in range(3):
-                for side in (0, 1):
-                    if box.inlet[ax][side] and box.bc[ax][side] == BC_END:
-                        idx = list(own)
-                        p = box.pad[ax][side]
-                        n = pm_img.shape[ax]
-                        idx[ax] = (slice(p, 2 * p) if p else slice(0, 1)) if side == 0 else \
-                            (slice(n - 2 * p, n - p) if p else slice(n - 1, n))
-                        keep[np.unique(lab[tuple(idx)])] = True
+            keep[list(inlet_labels(box, lab))] = True
             keep[0] = False
             eroded = keep[lab]
             del lab

@@ -65,8 +65,8 @@ SIGNATURES = {
     "pmx_fill_periodic_pads": (C.c_int, [P, C.c_int, P, P, P, P]),
     "pmx_ccl_workspace_size": (C.c_int, [P, P]),
     "pmx_ccl26": (C.c_int, [P, P, P, P, P, P, SZ, P]),
-    "pmx_mark_labels": (C.c_int, [P, C.c_int, P, P, P, P, C.c_uint64, P]),
-    "pmx_keep_labels": (C.c_int, [P, P, C.c_int, I64, P, P, C.c_uint64, C.c_int, C.c_int, P, P]),
+    "pmx_mark_labels": (C.c_int, [P, C.c_int, P, P, P, P, C.c_uint64, P, P]),
+    "pmx_keep_labels": (C.c_int, [P, P, C.c_int, I64, P, P, C.c_uint64, C.c_int, C.c_int, P, P, P]),
     "pmx_psd_workspace_size": (C.c_int, [P, P]),
     "pmx_psd_sweep": (C.c_int, [P, P, P, P, C.c_int, P, P, U32, P, P, SZ, P]),
     "pmx_psd_batch_workspace_size": (C.c_int, [C.c_int, P, C.c_int, P]),

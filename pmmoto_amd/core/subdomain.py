@@ -60,6 +60,9 @@ class PaddedSubdomain:
     reservoir_pad: tuple
     rank: int = 0
     index: tuple = (0, 0, 0)
+    # False for initialize(return_subdomain=True): the reference's plain
+    # Subdomain, which has no pads, no reservoir and no update_reservoir
+    padded: bool = True
     boundary_types: tuple = field(init=False)
     inlet: tuple = field(init=False)
     outlet: tuple = field(init=False)
@@ -141,9 +144,12 @@ class PaddedSubdomain:
         return img
 
     def face_own_slab(self, axis: int, side: int) -> tuple:
-        """Index of the own voxels adjacent to a face (features.py:76-139)."""
-        own = self.get_own_voxels()
-        idx = [slice(int(own[2 * a]), int(own[2 * a + 1])) for a in range(3)]
+        """Index of the voxels a face feature owns (features.py:76-139): on the
+        face's axis the layer next to the face; on the other axes the padded
+        extent without the pads (reservoir layers included), as
+        SubdomainFeatures builds them from the padded voxel counts and the
+        pads (subdomain_features.py:46-50)."""
+        idx = [slice(int(self.pad[a][0]), int(self.voxels[a] - self.pad[a][1])) for a in range(3)]
         p = self.pad[axis][side]
         n = self.voxels[axis]
         wall = self.boundary_types[axis][side] == BoundaryType.WALL
@@ -187,8 +193,8 @@ def initialize(
         pads = tuple(tuple(0 if bts[a][s] == BoundaryType.END else (1 if bts[a][s] == BoundaryType.WALL
                                                                     else int(pad[a])) for s in (0, 1))
                      for a in range(3))
-    sd = PaddedSubdomain(domain=dom, pad=pads, reservoir_pad=((0, 0),) * 3)
-    if reservoir_voxels:
+    sd = PaddedSubdomain(domain=dom, pad=pads, reservoir_pad=((0, 0),) * 3, padded=not return_subdomain)
+    if reservoir_voxels and not return_subdomain:
         # subdomain_padded.py:218-240: reservoir pads on inlet faces
         sd.reservoir_pad = tuple(tuple(reservoir_voxels if sd.inlet[a][s] else 0 for s in (0, 1))
                                  for a in range(3))

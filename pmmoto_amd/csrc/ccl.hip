@@ -500,31 +500,36 @@ __global__ __launch_bounds__(256) void k_labels(const int *__restrict__ parent, 
     }
 }
 
-// flags[label] = 1 for every label inside the sub-box [lo, hi)
-// labels outside [1, n_flags) are skipped (they cannot come from pmx_ccl26
-// with a matching count; the check keeps a stale count from writing past
-// the flag array)
+// flags[label] = 1 for every label inside the sub-box [lo, hi).  Labels
+// outside [0, n_flags) cannot come from pmx_ccl26 with a matching count: they
+// are not written (nothing lands past the flag array) but counted in *bad,
+// which the caller turns into an error.
 template <class T>
 __global__ __launch_bounds__(256) void k_mark(const T *__restrict__ labels, int64_t s0, int64_t s1,
                                               int64_t lo0, int64_t lo1, int64_t lo2, int64_t e0,
                                               int64_t e1, int64_t e2, uint8_t *__restrict__ flags,
-                                              uint64_t n_flags)
+                                              uint64_t n_flags, unsigned int *__restrict__ bad)
 {
     const int64_t n = e0 * e1 * e2;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t z = i % e2, t = i / e2, y = t % e1, x = t / e1;
         const uint64_t l = (uint64_t)labels[(lo0 + x) * s0 + (lo1 + y) * s1 + lo2 + z];
-        if (l && l < n_flags) flags[l] = 1;
+        if (l < n_flags) {
+            if (l) flags[l] = 1;
+        } else if (bad) {
+            atomicAdd(bad, 1u);
+        }
     }
 }
 
-// out = keep(label) ? img : 0 over the whole array
+// out = keep(label) ? img : 0 over the whole array; out-of-range labels are
+// dropped and counted in *bad, as in k_mark
 template <class T>
 __global__ __launch_bounds__(256) void k_keep(const uint8_t *__restrict__ img, const T *__restrict__ labels,
                                               int64_t n, const uint8_t *__restrict__ fa,
                                               const uint8_t *__restrict__ fb, uint64_t n_flags, int mode,
-                                              int phase, uint8_t *__restrict__ out)
+                                              int phase, uint8_t *__restrict__ out, unsigned int *__restrict__ bad)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -535,6 +540,8 @@ __global__ __launch_bounds__(256) void k_keep(const uint8_t *__restrict__ img, c
             const bool a = fa[l] != 0, bb = fb ? fb[l] != 0 : false;
             keep = mode == 0 ? a : (mode == 1 ? (a && bb) : (!a && !bb));
             if (phase >= 0 && v != phase) keep = false;
+        } else if (l && bad) {
+            atomicAdd(bad, 1u);
         }
         out[i] = keep ? v : 0;
     }
@@ -634,7 +641,7 @@ int pmx_ccl26(const uint8_t *img, const pmx_geom *g, int32_t *labels32, uint64_t
 }
 
 int pmx_mark_labels(const void *labels, int label_bytes, const int64_t *strides, const int64_t *lo,
-                    const int64_t *extent, uint8_t *flags, uint64_t n_flags, hipStream_t s)
+                    const int64_t *extent, uint8_t *flags, uint64_t n_flags, unsigned int *bad, hipStream_t s)
 {
     using namespace pmx;
     clear_error();
@@ -644,11 +651,11 @@ int pmx_mark_labels(const void *labels, int label_bytes, const int64_t *strides,
     if (label_bytes == 4)
         hipLaunchKernelGGL(k_mark<int32_t>, dim3(grid_for(n)), dim3(256), 0, s,
                            static_cast<const int32_t *>(labels), strides[0], strides[1], lo[0], lo[1],
-                           lo[2], extent[0], extent[1], extent[2], flags, n_flags);
+                           lo[2], extent[0], extent[1], extent[2], flags, n_flags, bad);
     else if (label_bytes == 8)
         hipLaunchKernelGGL(k_mark<uint64_t>, dim3(grid_for(n)), dim3(256), 0, s,
                            static_cast<const uint64_t *>(labels), strides[0], strides[1], lo[0], lo[1],
-                           lo[2], extent[0], extent[1], extent[2], flags, n_flags);
+                           lo[2], extent[0], extent[1], extent[2], flags, n_flags, bad);
     else
         return fail(PMX_ERR_INVALID, "label_bytes must be 4 or 8");
     return check_launch("mark labels");
@@ -656,7 +663,7 @@ int pmx_mark_labels(const void *labels, int label_bytes, const int64_t *strides,
 
 int pmx_keep_labels(const uint8_t *img, const void *labels, int label_bytes, int64_t n,
                     const uint8_t *flags_a, const uint8_t *flags_b, uint64_t n_flags, int mode, int phase,
-                    uint8_t *out, hipStream_t s)
+                    uint8_t *out, unsigned int *bad, hipStream_t s)
 {
     using namespace pmx;
     clear_error();
@@ -666,10 +673,10 @@ int pmx_keep_labels(const uint8_t *img, const void *labels, int label_bytes, int
     if (n == 0) return PMX_OK;
     if (label_bytes == 4)
         hipLaunchKernelGGL(k_keep<int32_t>, dim3(grid_for(n)), dim3(256), 0, s, img,
-                           static_cast<const int32_t *>(labels), n, flags_a, flags_b, n_flags, mode, phase, out);
+                           static_cast<const int32_t *>(labels), n, flags_a, flags_b, n_flags, mode, phase, out, bad);
     else if (label_bytes == 8)
         hipLaunchKernelGGL(k_keep<uint64_t>, dim3(grid_for(n)), dim3(256), 0, s, img,
-                           static_cast<const uint64_t *>(labels), n, flags_a, flags_b, n_flags, mode, phase, out);
+                           static_cast<const uint64_t *>(labels), n, flags_a, flags_b, n_flags, mode, phase, out, bad);
     else
         return fail(PMX_ERR_INVALID, "label_bytes must be 4 or 8");
     return check_launch("keep labels");

@@ -156,5 +156,9 @@ def gen_mp_constant(porous_media, fluid_phase: int = 1):
     """domain_generation.py:290-306: every pore voxel holds fluid_phase."""
     from .multiphase import Multiphase
 
-    img = np.where(np.asarray(porous_media.img) == 1, fluid_phase, 0).astype(np.uint8)
+    src = porous_media.img
+    if isinstance(src, torch.Tensor):
+        img = torch.where(src == 1, int(fluid_phase), 0).to(torch.uint8)
+    else:
+        img = np.where(np.asarray(src) == 1, fluid_phase, 0).astype(np.uint8)
     return Multiphase(porous_media, img, 2)

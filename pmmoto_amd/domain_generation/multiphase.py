@@ -53,7 +53,11 @@ class Multiphase:
         self.fluids = list(range(1, num_phases + 1))
 
     def update_img(self, img) -> None:
-        """multiphase.py:43-54: reservoir voxels are forced to phase 1."""
+        """multiphase.py:43-54: reservoir voxels are forced to phase 1; a
+        subdomain without pads (initialize(return_subdomain=True)) raises
+        TypeError, as the reference's plain Subdomain does."""
+        if not getattr(self.subdomain, "padded", True):
+            raise TypeError("subdomain does not support update_reservoir")
         if isinstance(img, torch.Tensor):
             update_reservoir_device(img, self.subdomain, 1)
         else:

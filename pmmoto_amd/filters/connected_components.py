@@ -56,21 +56,51 @@ def label_device(img_t: torch.Tensor, subdomain, dtype=torch.int64):
     return out, count
 
 
-def _face_flags(subdomain, labels: torch.Tensor, n_labels: int, which: str) -> torch.Tensor:
-    """flags[l] = 1 for labels on the own slab of inlet (or outlet) faces
-    (gen_inlet_label_map / gen_outlet_label_map, core/voxels.py:400-454)."""
-    flags = torch.zeros(n_labels + 1, dtype=torch.uint8, device=labels.device)
+def _check_bad(bad: torch.Tensor, what: str) -> None:
+    """Raise if a label kernel met labels beyond the flag array (a label
+    image that does not match its count)."""
+    n = int(bad.item())
+    if n:
+        raise _lib.PmxError(f"{what}: {n} voxels hold labels beyond the component count (stale labels?)")
+
+
+# (axis, side) of the six faces in the reference's feature order: the unit
+# directions of itertools.product([-1, 0, 1], repeat=3), i.e. x-, y-, z-, z+,
+# y+, x+ (core/orientation.py:79-114)
+FACE_ORDER = ((0, 0), (1, 0), (2, 0), (2, 1), (1, 1), (0, 1))
+
+
+def selected_face(subdomain, which: str):
+    """The face whose labels gen_inlet_label_map / gen_outlet_label_map
+    return: each flagged face overwrites the previous one's labels
+    (core/voxels.py:400-454, "currently restricted to a single face"), so the
+    last flagged face in the reference's face order; None if no face is
+    flagged."""
     faces = subdomain.inlet if which == "inlet" else subdomain.outlet
+    last = None
+    for a, s in FACE_ORDER:
+        if faces[a][s]:
+            last = (a, s)
+    return last
+
+
+def _face_flags(subdomain, labels: torch.Tensor, n_labels: int, which: str) -> torch.Tensor:
+    """flags[l] = 1 for labels on the own slab of the selected inlet (or
+    outlet) face (gen_inlet_label_map / gen_outlet_label_map,
+    core/voxels.py:400-454)."""
+    flags = torch.zeros(n_labels + 1, dtype=torch.uint8, device=labels.device)
+    face = selected_face(subdomain, which)
+    if face is None:
+        return flags
+    bad = torch.zeros(1, dtype=torch.int32, device=labels.device)
     shape = tuple(labels.shape)
     strides = (C.c_int64 * 2)(shape[1] * shape[2], shape[2])
-    for a in range(3):
-        for s in (0, 1):
-            if faces[a][s]:
-                sl = subdomain.face_own_slab(a, s)
-                lo = (C.c_int64 * 3)(*[x.start for x in sl])
-                ext = (C.c_int64 * 3)(*[x.stop - x.start for x in sl])
-                _lib.check(_lib.lib().pmx_mark_labels(_device.ptr(labels), labels.element_size(), strides, lo,
-                                                      ext, _device.ptr(flags), flags.numel(), _device.stream()))
+    sl = subdomain.face_own_slab(*face)
+    lo = (C.c_int64 * 3)(*[x.start for x in sl])
+    ext = (C.c_int64 * 3)(*[x.stop - x.start for x in sl])
+    _lib.check(_lib.lib().pmx_mark_labels(_device.ptr(labels), labels.element_size(), strides, lo, ext,
+                                          _device.ptr(flags), flags.numel(), _device.ptr(bad), _device.stream()))
+    _check_bad(bad, "face labels")
     return flags
 
 
@@ -78,10 +108,12 @@ def keep_device(img_t, labels, fa, fb, mode, phase):
     out = torch.empty_like(img_t)
     if fb is not None and fb.numel() != fa.numel():
         raise ValueError("flag arrays differ in size")
+    bad = torch.zeros(1, dtype=torch.int32, device=img_t.device)
     _lib.check(_lib.lib().pmx_keep_labels(_device.ptr(img_t), _device.ptr(labels), labels.element_size(),
                                           img_t.numel(), _device.ptr(fa), _device.ptr(fb) if fb is not None else None,
                                           fa.numel(), mode, -1 if phase is None else int(phase), _device.ptr(out),
-                                          _device.stream()))
+                                          _device.ptr(bad), _device.stream()))
+    _check_bad(bad, "keep labels")
     return out
 
 
@@ -119,13 +151,12 @@ def inlet_outlet_labels(subdomain, labeled_img):
     """{label: {"inlet": bool, "outlet": bool}} (connected_components.py:144-177)."""
     lab = np.asarray(labeled_img.cpu() if isinstance(labeled_img, torch.Tensor) else labeled_img)
     connected = defaultdict(lambda: {"inlet": False, "outlet": False})
-    for which, faces in (("inlet", subdomain.inlet), ("outlet", subdomain.outlet)):
-        for a in range(3):
-            for s in (0, 1):
-                if faces[a][s]:
-                    for l in np.unique(lab[subdomain.face_own_slab(a, s)]):
-                        if l > 0:
-                            connected[np.uint64(l)][which] = True
+    for which in ("inlet", "outlet"):
+        face = selected_face(subdomain, which)
+        if face is not None:
+            for l in np.unique(lab[subdomain.face_own_slab(*face)]):
+                if l > 0:
+                    connected[np.uint64(l)][which] = True
     return connected
 
 

@@ -74,8 +74,6 @@ def drainage(multiphase, capillary_pressures, gamma=1, contact_angle=0, method="
             logging.warning("The contact angle is zero. This will yield same results as the standard approach.")
     else:
         raise ValueError(f"{method} is not implemented. ")
-    if save:
-        raise NotImplementedError("save=True writes VTK files; image output is outside this package")
 
     sd = multiphase.subdomain
     pm_t, _ = _device.as_device_u8(multiphase.porous_media.img)
@@ -93,6 +91,12 @@ def drainage(multiphase, capillary_pressures, gamma=1, contact_angle=0, method="
                                                _device.ptr(mp_t), _device.stream()))
         update_reservoir_device(mp_t, sd, 1)
         multiphase.img = mp_t
+        if save:
+            # equilibrium_distribution.py:109-118: one image per pressure, the
+            # porosimetry image alongside
+            from ..io.output import save_img
+            file_out = str(out_folder) + f"/capillary_pressure_{pc:.3f}".replace(".", "_")
+            save_img(file_out, sd, mp_t, additional_img={"morph": morph})
         w_sat[n] = multiphase.get_saturation(2, mp_t)
         logging.info("Wetting phase saturation at capillary pressure of %f: %f", pc, w_sat[n])
     multiphase.img = mp_t.cpu().numpy() if as_np else mp_t

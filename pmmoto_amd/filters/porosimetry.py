@@ -103,9 +103,10 @@ def _sorted_radii(subdomain, porous_media, radii, num_radii):
         if isinstance(radii, list):
             return sorted(radii, reverse=True)
         return np.sort(np.asarray(radii))[::-1]
-    # porosimetry.py:193-201
-    edt = np.asarray(porous_media.distance)
-    global_max = float(np.max(edt)) - 1e-6 * porous_media.max_distance
+    # porosimetry.py:193-201: the global maximum of the distance, less 1e-6
+    # of the same maximum (porous_media.max_distance), in float64
+    max_d = float(porous_media.max_distance)
+    global_max = max_d - 1e-6 * max_d
     r_min = np.nextafter(np.min(subdomain.domain.resolution), np.inf)
     return get_sizes(r_min, global_max, num_radii, "linear")
 
@@ -166,11 +167,10 @@ def psd_sweep_fused(subdomain, pm_t: torch.Tensor, radii, inlet=False, mode="hyb
     ws = _device.workspace(nbytes.value)
     lo = ext = None
     if inlet:
-        slabs = [subdomain.face_own_slab(a, s) for a in range(3) for s in (0, 1) if subdomain.inlet[a][s]]
-        if len(slabs) > 1:
-            raise NotImplementedError("the fused sweep takes one inlet face")
-        if slabs:
-            sl = slabs[0]
+        # the one face whose labels the reference's inlet map keeps
+        face = cc.selected_face(subdomain, "inlet")
+        if face is not None:
+            sl = subdomain.face_own_slab(*face)
             # in box coordinates, clipped to the box (a cut pad layer holds no pore voxel)
             b0 = [min(max(sl[a].start - origin[a], 0), n[a]) for a in range(3)]
             b1 = [min(max(sl[a].stop - origin[a], 0), n[a]) for a in range(3)]

@@ -30,6 +30,9 @@ Outputs (committed, data only):
   mixed-face edt3d at non-unit resolution, and ``edt(img, subdomain)`` on
   padded periodic / mixed subdomains of the unit box (resolution 1/N) and
   on a 64^3 periodic pack at unit resolution.
+* tests/golden/ref_inlet_labels.json -- the labels gen_inlet_label_map and
+  gen_outlet_label_map (core/voxels.py:400-454) return on subdomains with
+  several inlet / outlet faces, for seeded random label images.
 
 Run:  python tests/golden/make_ref_edt_vectors.py   (REF_ROOT overrides
 /root/reference)
@@ -263,6 +266,43 @@ def main():
         print(f"{key}: padded {padded.shape}, resolution {tuple(sd.domain.resolution)}", flush=True)
     np.savez_compressed(os.path.join(HERE, "ref_edt_vectors.npz"), **arrays)
     print("wrote ref_edt_periodic_128.json and ref_edt_vectors.npz", flush=True)
+    inlet_label_vectors(core)
+
+
+def inlet_label_vectors(core):
+    """gen_inlet_label_map / gen_outlet_label_map (core/voxels.py:400-454)
+    on subdomains with several inlet or outlet faces: the labels each
+    returns for a seeded random label image of the padded shape."""
+    import numpy as np
+
+    BT = core.boundary_types.BoundaryType
+    E_, W_, P_ = BT.END, BT.WALL, BT.PERIODIC
+    F, T = False, True
+    cases = [
+        ("end_x_y", (12, 10, 14), ((E_, E_),) * 3, ((T, F), (T, F), (F, F)), ((F, T), (F, F), (F, T)), 0),
+        ("end_xlo_xhi_zhi", (10, 12, 9), ((E_, E_),) * 3, ((T, T), (F, F), (F, T)), ((F, F), (T, T), (F, F)), 0),
+        ("wall_y", (11, 9, 13), ((E_, E_), (W_, W_), (E_, E_)), ((T, F), (F, F), (F, T)), ((F, T), (F, F), (T, F)), 0),
+        ("periodic_z_reservoir", (10, 11, 12), ((E_, E_), (E_, E_), (P_, P_)), ((T, F), (F, T), (F, F)),
+         ((F, T), (T, F), (F, F)), 3),
+    ]
+    out = []
+    for k, (name, n, bts, inlet, outlet, res_vox) in enumerate(cases):
+        sd = core.pmmoto.initialize(n, boundary_types=bts, inlet=inlet, outlet=outlet, reservoir_voxels=res_vox)
+        rng = np.random.default_rng(500 + k)
+        lab = rng.integers(0, 2000, size=tuple(int(v) for v in sd.voxels)).astype(np.uint64)
+        got_in = core.voxels.gen_inlet_label_map(sd, lab)
+        got_out = core.voxels.gen_outlet_label_map(sd, lab)
+        out.append({
+            "name": name, "voxels": list(n), "boundary_types": [[b.value for b in pair] for pair in bts],
+            "inlet": [list(p) for p in inlet], "outlet": [list(p) for p in outlet], "reservoir_voxels": res_vox,
+            "padded_shape": [int(v) for v in sd.voxels], "pad": [[int(p) for p in pair] for pair in sd.pad],
+            "label_seed": 500 + k, "label_high": 2000,
+            "inlet_labels": sorted(int(v) for v in got_in), "outlet_labels": sorted(int(v) for v in got_out),
+        })
+        print(f"{name}: padded {tuple(sd.voxels)}, {len(got_in)} inlet / {len(got_out)} outlet labels", flush=True)
+    with open(os.path.join(HERE, "ref_inlet_labels.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote ref_inlet_labels.json", flush=True)
 
 
 if __name__ == "__main__":

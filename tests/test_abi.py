@@ -91,3 +91,15 @@ def test_struct_layouts_match_the_header(tmp_path):
     sizes = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
     assert sizes == [C.sizeof(_lib.Geom), C.sizeof(_lib.PsdDomain), _lib.PsdDomain.code_map.offset,
                      _lib.PsdDomain.tile_rows.offset]
+
+
+def test_documented_binding_signatures_match_the_library():
+    """The ctypes binding documented for PMMoTo's maintainers declares the
+    same argument counts as the package's own table (executed on the host:
+    it only loads the library and defines functions)."""
+    blocks = re.findall(r"```python\n(.*?)```", open(os.path.join(ROOT, "INTEGRATION.md")).read(), flags=re.S)
+    os.environ["LIBPMX"] = _lib.LIB_PATH
+    ns = {}
+    exec(compile(blocks[0], "binding", "exec"), ns)
+    for name in ("pmx_edt", "pmx_edt_workspace_size", "pmx_edt_ref", "pmx_edt_ref_workspace_size"):
+        assert len(getattr(ns["_L"], name).argtypes) == len(_lib.SIGNATURES[name][1]), name

@@ -77,3 +77,24 @@ def test_sweep_box_cuts_only_all_solid_wall_pads():
                                 boundary_types=((BT.PERIODIC, BT.PERIODIC),) * 3)
     origin, n, bc = PO._sweep_box(per, torch.zeros(per.voxels, dtype=torch.uint8))
     assert origin == [0, 0, 0] and n == list(per.voxels) and bc == [_lib.BC_END] * 6
+
+
+def test_probe_radius_and_pc_known_answers():
+    """tests/domain_generation/test_multiphase.py:60-120."""
+    from pmmoto_amd.domain_generation.multiphase import Multiphase as M
+    assert M.get_probe_radius(1, 1) == 2.0
+    assert M.get_probe_radius(2, 1) == 1.0
+    assert M.get_probe_radius(3, 0.5) == 1 / 3
+    assert M.get_probe_radius(0, 0.5) == 0
+    np.testing.assert_approx_equal(M.get_probe_radius(1, 1, 20), 1.87938524)
+    np.testing.assert_approx_equal(M.get_probe_radius(2, 1, 20), 0.93969262)
+    np.testing.assert_approx_equal(M.get_probe_radius(3, 0.5, 85), 0.029051914)
+    assert M.get_pc(M.get_probe_radius(1, 1), 1) == 1
+
+
+def test_plain_subdomain_has_no_pads_or_reservoir():
+    import pmmoto_amd
+    sd = pmmoto_amd.initialize((5, 6, 7), reservoir_voxels=5, return_subdomain=True,
+                               inlet=((True, False), (False, False), (False, False)))
+    assert not sd.padded and sd.voxels == (5, 6, 7)
+    assert sd.reservoir_pad == ((0, 0),) * 3

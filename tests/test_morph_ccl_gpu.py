@@ -293,3 +293,75 @@ def test_ccl26_tiles_match_oracle(shape, per):
     want, n = O.ccl26(img, per)
     assert int(count.item()) == n
     np.testing.assert_array_equal(labels.cpu().numpy().astype(np.int64), want)
+
+
+@pytest.mark.parametrize("inlet,outlet", [
+    (((True, False), (True, False), (False, False)), ((False, True), (False, False), (False, True))),
+    (((True, True), (False, False), (False, True)), ((False, False), (True, True), (False, False))),
+])
+def test_several_inlet_faces_follow_the_reference(inlet, outlet):
+    """With several inlet (outlet) faces the reference keeps the labels of the
+    last flagged face in its face order (gen_inlet_label_map overwrites per
+    face); the oracle restates that and is pinned to the reference's own
+    output in tests/test_oracle_golden.py."""
+    from pmmoto_amd.domain_generation import gen_pm, pack_radii, sphere_pack
+    from pmmoto_amd.filters import connected_components as CC
+    from pmmoto_amd.filters import porosimetry as PO
+    img_t, _ = sphere_pack((34, 30, 28), 0.45, pack_radii("uniform", lo=3, hi=6), seed=77)
+    img, sd, box = padded_case(img_t.cpu().numpy(), BT_CASES["end"], inlet=inlet, outlet=outlet)
+    print(f"[inlet faces] inlet={inlet} outlet={outlet}", flush=True)
+    np.testing.assert_array_equal(CC.inlet_connected_img(sd, img), O.inlet_connected(box, img))
+    np.testing.assert_array_equal(CC.outlet_connected_img(sd, img),
+                                  O.inlet_connected(box, img, which="outlet"))
+    pm = gen_pm(sd, img)
+    radii = np.arange(6, 0, -1).astype(np.float64)
+    got = PO.pore_size_distribution(sd, pm, radii, inlet=True, mode="distance")
+    np.testing.assert_array_equal(got, O.pore_size_distribution(box, img, radii, inlet=True))
+
+
+def test_stale_label_count_raises():
+    """Label kernels meeting labels beyond the component count (a stale
+    count) write nothing out of range and make the host raise."""
+    import torch
+
+    from pmmoto_amd import _lib
+    from pmmoto_amd.filters import connected_components as CC
+    own = rand_binary((20, 18, 16), 0.85, 5)  # sparse pores: many components
+    img, sd, _ = padded_case(own, BT_CASES["end"], inlet=((True, False), (False, False), (False, False)))
+    t = torch.from_numpy(img).cuda()
+    labels, count = CC.label_device(t, sd)
+    n = int(count.item())
+    assert n > 20
+    with pytest.raises(_lib.PmxError):
+        CC._face_flags(sd, labels, 2, "inlet")
+    fa = torch.ones(3, dtype=torch.uint8, device="cuda")
+    with pytest.raises(_lib.PmxError):
+        CC.keep_device(t, labels, fa, None, 0, None)
+    # the matching count passes
+    flags = CC._face_flags(sd, labels, n, "inlet")
+    CC.keep_device(t, labels, flags, None, 0, None)
+
+
+def test_images_beyond_uint8_are_rejected():
+    """The kernels take uint8 phase / label values: wider or fractional
+    images raise ValueError instead of wrapping (256 -> 0 would become
+    solid); integral values in 0..255 of any dtype are accepted."""
+    import torch
+
+    from pmmoto_amd.filters import connected_components as CC
+    from pmmoto_amd.filters import distance
+    img = rand_binary((12, 10, 8), 0.2, 1).astype(np.int64)
+    img2 = img.copy()
+    img2[3, 3, 3] = 256
+    with pytest.raises(ValueError, match="0..255"):
+        distance.edt(img2)
+    with pytest.raises(ValueError, match="0..255"):
+        CC.connect_components(img2, padded_case(img2.astype(np.uint8), BT_CASES["end"])[1])
+    with pytest.raises(ValueError, match="non-integer"):
+        distance.edt(img.astype(np.float64) * 0.5)
+    with pytest.raises(ValueError):
+        distance.edt(torch.from_numpy(img2).cuda())
+    with pytest.raises(ValueError):
+        distance.edt(-img)
+    np.testing.assert_array_equal(distance.edt(img), distance.edt(img.astype(np.uint8)))
+    np.testing.assert_array_equal(distance.edt(img.astype(np.float32)), distance.edt(img.astype(np.uint8)))

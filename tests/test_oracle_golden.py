@@ -86,3 +86,36 @@ def test_own_box_restatement_is_not_the_reference_on_packs():
         own = np.sqrt(O.ref_edt_sq(np.ascontiguousarray(img[sl]), tuple(periodic), res))
         full = np.pad(own, [(pads[2 * a], pads[2 * a + 1]) for a in range(3)], mode="wrap")
         assert np.any(full.view(np.uint32) != VEC[key + "_out"].view(np.uint32))
+
+
+def _label_cases():
+    with open(os.path.join(GOLD, "ref_inlet_labels.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", _label_cases(), ids=lambda c: c["name"])
+def test_inlet_outlet_label_maps_match_reference(case):
+    """gen_inlet_label_map / gen_outlet_label_map with several flagged faces
+    keep only the last flagged face's labels: the oracle's restatement and
+    the product's face selection (on the host) against the reference's own
+    output."""
+    import pmmoto_amd
+    from pmmoto_amd.filters.connected_components import selected_face
+    BT = pmmoto_amd.BoundaryType
+    bts = tuple((BT(a), BT(b)) for a, b in case["boundary_types"])
+    inlet = tuple(tuple(p) for p in case["inlet"])
+    outlet = tuple(tuple(p) for p in case["outlet"])
+    sd = pmmoto_amd.initialize(tuple(case["voxels"]), boundary_types=bts, inlet=inlet, outlet=outlet,
+                               reservoir_voxels=case["reservoir_voxels"])
+    assert list(sd.voxels) == case["padded_shape"]
+    assert [list(p) for p in sd.pad] == case["pad"]
+    lab = np.random.default_rng(case["label_seed"]).integers(0, case["label_high"], size=tuple(sd.voxels))
+    code = {"end": O.BC_END, "wall": O.BC_WALL, "periodic": O.BC_PERIODIC}
+    box = O.Box(bc=tuple((code[a], code[b]) for a, b in case["boundary_types"]), pad=tuple(sd.pad),
+                inlet=sd.inlet, outlet=sd.outlet, reservoir=tuple(sd.reservoir_pad))
+    for which, key in (("inlet", "inlet_labels"), ("outlet", "outlet_labels")):
+        want = set(case[key]) - {0}
+        assert O.inlet_labels(box, lab, which) == want
+        face = selected_face(sd, which)
+        got = set(np.unique(lab[sd.face_own_slab(*face)]).tolist()) - {0}
+        assert got == want

@@ -26,7 +26,7 @@ import torch
 from .. import _device, _lib
 from . import connected_components as cc
 from . import morphological_operators as morph
-from .distance.euclidean_distance_transform import _geom
+from .distance.euclidean_distance_transform import _geom, fill_periodic_pads
 
 __all__ = ["get_sizes", "porosimetry", "pore_size_distribution", "psd_sweep", "saturation_curve"]
 
@@ -118,11 +118,12 @@ def _thresholds(subdomain, radius, fft):
 
 
 def fused_sweep_ok(subdomain, mode) -> bool:
-    """pmx_psd_sweep covers non-periodic domains whose erosion and dilation
-    are integer thresholds (unit resolution, or FFT semantics throughout)."""
+    """pmx_psd_sweep covers domains whose erosion and dilation are integer
+    thresholds (unit resolution, or FFT semantics throughout).  Periodic
+    axes run on the own voxels with wrap: the reference's ceil(r) halo of
+    periodic copies (morphological_operators.py:125-129) gives the same
+    erosion and dilation, and its labelling wraps those axes."""
     from .distance.euclidean_distance_transform import _unit
-    if subdomain.domain.periodic:
-        return False
     return _unit(subdomain.domain.resolution) or mode == "morph"
 
 
@@ -139,6 +140,12 @@ def _sweep_box(subdomain, pm_t: torch.Tensor):
     shape = list(pm_t.shape)
     origin, n, bc = [0, 0, 0], list(shape), [_lib.BC_END] * 6
     for a in range(3):
+        if subdomain.boundary_types[a][0] == BoundaryType.PERIODIC:
+            # the own voxels with wrap; the pads are periodic copies, filled afterwards
+            lo, hi = subdomain.pad[a]
+            origin[a], n[a] = lo, shape[a] - lo - hi
+            bc[2 * a] = bc[2 * a + 1] = _lib.BC_PERIODIC
+            continue
         for s in (0, 1):
             if subdomain.boundary_types[a][s] != BoundaryType.WALL or not subdomain.pad[a][s] or n[a] < 3:
                 continue
@@ -182,6 +189,11 @@ def psd_sweep_fused(subdomain, pm_t: torch.Tensor, radii, inlet=False, mode="hyb
     _lib.check(L.pmx_psd_sweep(_device.ptr(pm_t, off), C.byref(g), lo, ext, len(radii), (C.c_int64 * len(te))(*te),
                                (C.c_int64 * len(td))(*td), morph._wall_mask(subdomain), _device.ptr(code_map, off),
                                _device.ptr(ws), nbytes.value, _device.stream()))
+    pads = [0] * 6
+    for a in range(3):
+        if bc[2 * a] == _lib.BC_PERIODIC:
+            pads[2 * a], pads[2 * a + 1] = subdomain.pad[a]
+    fill_periodic_pads([code_map], shape, pads)
     return code_map
 
 

@@ -365,3 +365,28 @@ def test_images_beyond_uint8_are_rejected():
         distance.edt(-img)
     np.testing.assert_array_equal(distance.edt(img), distance.edt(img.astype(np.uint8)))
     np.testing.assert_array_equal(distance.edt(img.astype(np.float32)), distance.edt(img.astype(np.uint8)))
+
+
+@pytest.mark.parametrize("bts,inlet", [("periodic", False), ("mixed", True), ("mixed", False)])
+def test_fused_sweep_periodic_axes(bts, inlet):
+    """Periodic axes in the fused sweep (own voxels with wrap, code-map pads
+    copied): equal to the per-radius path and to the oracle."""
+    import torch
+
+    from pmmoto_amd.domain_generation import pack_radii, sphere_pack
+    from pmmoto_amd.filters import porosimetry as PO
+    per = tuple(BT_CASES[bts][a][0] == P for a in range(3))
+    img_t, _ = sphere_pack((40, 36, 44), 0.42, pack_radii("uniform", lo=2, hi=6), seed=61, periodic=per)
+    inl = ((True, False), (False, False), (False, False)) if inlet else ((False, False),) * 3
+    img, sd, box = padded_case(img_t.cpu().numpy(), BT_CASES[bts], inlet=inl)
+    assert PO.fused_sweep_ok(sd, "distance")
+    pm_t = torch.from_numpy(img).cuda()
+    radii = [np.float64(r) for r in (6.0, 4.5, 3.0, 2.0, 1.0)]
+    print(f"[fused sweep] {bts} inlet={inlet}", flush=True)
+    fused = PO.psd_sweep_fused(sd, pm_t, radii, inlet=inlet, mode="distance").cpu().numpy()
+    levels = PO.psd_sweep_levels(sd, pm_t, radii, inlet=inlet, mode="distance").cpu().numpy()
+    np.testing.assert_array_equal(fused, levels)
+    want = O.pore_size_distribution(box, img, np.array(radii), inlet=inlet, mode="distance")
+    got = np.where(fused > 0, np.array([0.0] + [float(r) for r in radii])[fused], 0.0)
+    assert np.count_nonzero(got) > 0
+    np.testing.assert_array_equal(got, want)

@@ -225,6 +225,17 @@ int pmx_code_histogram(const uint8_t *pm, const int32_t *map, const int64_t *sha
                        const int64_t *extent, int32_t n_codes, uint64_t *counts, hipStream_t stream);
 int pmx_both_one(const uint8_t *a, const uint8_t *b, int64_t n, uint8_t *out, hipStream_t stream);
 
+/* Wetting-saturation curve of a radius-code map: the code histogram of
+ * pmx_code_histogram (counts: n_codes + 1 device uint64, also returned),
+ * then curve[i] (device float64, n_codes values) = (wet_i / volume) /
+ * (pore / volume), wet_i = pore voxels of the sub-box with code 0 or code >
+ * i + 1, in the reference's order of float64 operations (Multiphase.
+ * get_saturation = get_volume_fraction / porosity, multiphase.py:56-95,
+ * porousmedia.py:56-66); volume = the domain's voxel count. */
+int pmx_saturation_curve(const uint8_t *pm, const int32_t *map, const int64_t *shape, const int64_t *lo,
+                         const int64_t *extent, int32_t n_codes, double volume, uint64_t *counts,
+                         double *curve, hipStream_t stream);
+
 /* Drainage (equilibrium_distribution.py:92-123, multiphase.py:43-95).
  * pmx_drain_update: img[i] = 1 where morph[i] == 1 && wet[i] == 2 (n voxels).
  * pmx_box_op on the sub-box lo + [0, extent) of a uint8 array of `shape`:

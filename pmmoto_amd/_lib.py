@@ -77,6 +77,7 @@ SIGNATURES = {
     "pmx_codes_to_values": (C.c_int, [P, I64, P, I32, P, P]),
     "pmx_code_histogram": (C.c_int, [P, P, P, P, P, I32, P, P]),
     "pmx_both_one": (C.c_int, [P, P, I64, P, P]),
+    "pmx_saturation_curve": (C.c_int, [P, P, P, P, P, C.c_int32, C.c_double, P, P, P]),
     "pmx_drain_update": (C.c_int, [P, P, I64, P, P]),
     "pmx_box_op": (C.c_int, [P, P, P, P, C.c_int, U8, P, P]),
     "pmx_max_f32": (C.c_int, [P, I64, P, P]),

@@ -34,6 +34,26 @@ __global__ __launch_bounds__(256) void k_codes_to_values(const int32_t *__restri
     }
 }
 
+// Wetting saturation after each code from the code histogram, in float64
+// in the reference's order: S_i = (wet_i / V) / (pore / V) with wet_i the
+// pore voxels whose code is 0 or above i + 1 (Multiphase.get_saturation =
+// get_volume_fraction / porosity, multiphase.py:56-95, porousmedia.py:56-66).
+// One lane walks the codes (at most 4096); a domain with no pore voxel gives
+// 0 / 0 = NaN, as the reference's numpy division does.
+__global__ void k_saturation(const unsigned long long *__restrict__ counts, int32_t n_codes, double volume,
+                             double *__restrict__ curve)
+{
+    if (threadIdx.x != 0) return;
+    unsigned long long pore = 0;
+    for (int c = 0; c <= n_codes; ++c) pore += counts[c];
+    const double porosity = (double)pore / volume;
+    unsigned long long wet = pore;
+    for (int i = 0; i < n_codes; ++i) {
+        wet -= counts[i + 1];
+        curve[i] = ((double)wet / volume) / porosity;
+    }
+}
+
 // counts[c] += number of own voxels with pm == 1 and map == c
 __global__ __launch_bounds__(256) void k_code_hist(const uint8_t *__restrict__ pm,
                                                    const int32_t *__restrict__ map, int64_t n1,
@@ -188,6 +208,18 @@ int pmx_code_histogram(const uint8_t *pm, const int32_t *map, const int64_t *sha
                        pm, map, shape[1], shape[2], lo[0], lo[1], lo[2], extent[0], extent[1], extent[2],
                        n_codes, c);
     return pmx::check_launch("code histogram");
+}
+
+int pmx_saturation_curve(const uint8_t *pm, const int32_t *map, const int64_t *shape, const int64_t *lo,
+                         const int64_t *extent, int32_t n_codes, double volume, uint64_t *counts, double *curve,
+                         hipStream_t s)
+{
+    pmx::clear_error();
+    if (!curve || !(volume > 0.0)) return pmx::fail(PMX_ERR_INVALID, "curve is NULL or volume is not positive");
+    if (int e = pmx_code_histogram(pm, map, shape, lo, extent, n_codes, counts, s)) return e;
+    hipLaunchKernelGGL(pmx::k_saturation, dim3(1), dim3(64), 0, s,
+                       reinterpret_cast<const unsigned long long *>(counts), n_codes, volume, curve);
+    return pmx::check_launch("saturation curve");
 }
 
 int pmx_drain_update(const uint8_t *morph, const uint8_t *wet, int64_t n, uint8_t *img, hipStream_t s)

@@ -156,13 +156,9 @@ class BatchSweep:
 
 
 def batch_saturation(code_map: torch.Tensor, img: torch.Tensor, n_levels: int) -> np.ndarray:
-    """Wetting saturation after each level: pore voxels not invaded by
-    radii[0..i], over all pore voxels (float64)."""
-    pore = img == 1
-    n_pore = int(pore.sum().item())
-    if n_pore == 0:
-        return np.ones(n_levels)
-    codes = code_map[pore]
-    hist = torch.bincount(codes.to(torch.int64), minlength=n_levels + 1).cpu().numpy().astype(np.float64)
-    invaded = np.cumsum(hist[1:n_levels + 1])
-    return (n_pore - invaded) / n_pore
+    """Wetting saturation after each level of one batched domain (no pads:
+    V = its voxel count): (wet / V) / (pore / V) in float64, on the device
+    (pmx_saturation_curve, the arithmetic of Multiphase.get_saturation)."""
+    from .porosimetry import saturation_curve_box
+    shape = tuple(img.shape)
+    return saturation_curve_box(img, code_map, (0, 0, 0), shape, n_levels, float(img.numel()))

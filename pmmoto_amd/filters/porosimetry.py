@@ -231,21 +231,25 @@ def pore_size_distribution(subdomain, porous_media, radii=None, num_radii=25, in
     return _device.to_host(out, as_np)
 
 
+def saturation_curve_box(pm_t: torch.Tensor, code_map: torch.Tensor, lo, ext, n_radii: int, volume: float):
+    """pmx_saturation_curve on the sub-box lo + [0, ext): the float64 wetting
+    saturation after each radius, (wet / V) / (pore / V) on the device."""
+    shape = tuple(pm_t.shape)
+    counts = torch.empty(n_radii + 1, dtype=torch.int64, device=pm_t.device)
+    curve = torch.empty(n_radii, dtype=torch.float64, device=pm_t.device)
+    _lib.check(_lib.lib().pmx_saturation_curve(_device.ptr(pm_t), _device.ptr(code_map), (C.c_int64 * 3)(*shape),
+                                               (C.c_int64 * 3)(*lo), (C.c_int64 * 3)(*ext), int(n_radii),
+                                               float(volume), _device.ptr(counts), _device.ptr(curve),
+                                               _device.stream()))
+    return curve.cpu().numpy()
+
+
 def saturation_curve(subdomain, pm_t: torch.Tensor, code_map: torch.Tensor, n_radii: int) -> np.ndarray:
     """Wetting saturation after each radius of a sweep:
     S_w(r_i) = (count(own pore & not invaded at radii[0..i]) / V) / (count(own pore) / V),
-    in float64 in that order (multiphase.py:56-95, porousmedia.py:56-66)."""
-    shape = tuple(pm_t.shape)
+    in float64 in that order (multiphase.py:56-95, porousmedia.py:56-66),
+    V = the domain's voxel count."""
     own = subdomain.get_own_voxels()
-    lo = (C.c_int64 * 3)(int(own[0]), int(own[2]), int(own[4]))
-    ext = (C.c_int64 * 3)(int(own[1] - own[0]), int(own[3] - own[2]), int(own[5] - own[4]))
-    counts = torch.empty(n_radii + 1, dtype=torch.int64, device=pm_t.device)
-    _lib.check(_lib.lib().pmx_code_histogram(_device.ptr(pm_t), _device.ptr(code_map), (C.c_int64 * 3)(*shape),
-                                             lo, ext, n_radii, _device.ptr(counts), _device.stream()))
-    c = counts.cpu().numpy()
-    vol = float(np.prod(subdomain.domain.voxels))
-    pore = int(c.sum())
-    invaded = np.cumsum(c[1:])
-    wet = pore - invaded
-    porosity = pore / vol
-    return np.array([(w / vol) / porosity for w in wet], dtype=np.float64)
+    lo = (int(own[0]), int(own[2]), int(own[4]))
+    ext = (int(own[1] - own[0]), int(own[3] - own[2]), int(own[5] - own[4]))
+    return saturation_curve_box(pm_t, code_map, lo, ext, n_radii, float(np.prod(subdomain.domain.voxels)))

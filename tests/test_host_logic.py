@@ -54,8 +54,9 @@ def test_radius_threshold_negative():
 
 def test_sweep_box_cuts_only_all_solid_wall_pads():
     """The fused sweep's box: a WALL face's pad layer is cut off (and the face
-    marked WALL) only when it holds no pore voxel; END and periodic faces and
-    pads with pore voxels stay in the box as END faces."""
+    marked WALL) only when it holds no pore voxel; END faces and pads with
+    pore voxels stay in the box as END faces; periodic axes keep their own
+    voxels with wrap."""
     import torch
 
     import pmmoto_amd
@@ -76,7 +77,8 @@ def test_sweep_box_cuts_only_all_solid_wall_pads():
     per = pmmoto_amd.initialize((10, 12, 14), box=((0, 10), (0, 12), (0, 14)),
                                 boundary_types=((BT.PERIODIC, BT.PERIODIC),) * 3)
     origin, n, bc = PO._sweep_box(per, torch.zeros(per.voxels, dtype=torch.uint8))
-    assert origin == [0, 0, 0] and n == list(per.voxels) and bc == [_lib.BC_END] * 6
+    # periodic axes: the own voxels with wrap (pads are copies, filled afterwards)
+    assert origin == [1, 1, 1] and n == [10, 12, 14] and bc == [_lib.BC_PERIODIC] * 6
 
 
 def test_probe_radius_and_pc_known_answers():

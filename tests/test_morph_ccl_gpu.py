@@ -390,3 +390,31 @@ def test_fused_sweep_periodic_axes(bts, inlet):
     got = np.where(fused > 0, np.array([0.0] + [float(r) for r in radii])[fused], 0.0)
     assert np.count_nonzero(got) > 0
     np.testing.assert_array_equal(got, want)
+
+
+def test_saturation_curve_kernel_matches_oracle_arithmetic():
+    """pmx_saturation_curve: the code histogram of the own box and
+    (wet / V) / (pore / V) in float64, bit for bit against the oracle's
+    evaluation of the reference's arithmetic, on a padded subdomain and on a
+    batched domain (no pads)."""
+    import torch
+
+    from pmmoto_amd.filters import porosimetry as PO
+    from pmmoto_amd.filters.batch import batch_saturation
+    rng = np.random.default_rng(8)
+    own = rand_binary((30, 28, 26), 0.4, 8)
+    img, sd, box = padded_case(own, BT_CASES["mixed"])
+    codes = np.where(img == 1, rng.integers(0, 13, size=img.shape), rng.integers(0, 13, size=img.shape))
+    codes = codes.astype(np.int32)
+    got = PO.saturation_curve(sd, torch.from_numpy(img).cuda(), torch.from_numpy(codes).cuda(), 12)
+    want = O.saturation_from_codes(box, img, codes.astype(np.uint8), 12, own.shape)
+    np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64))
+    bbox = O.Box(bc=((O.BC_END, O.BC_END),) * 3, pad=((0, 0),) * 3)
+    c2 = codes[:20, :20, :20].copy()
+    i2 = img[:20, :20, :20].copy()
+    got = batch_saturation(torch.from_numpy(c2).cuda(), torch.from_numpy(i2).cuda(), 12)
+    want = O.saturation_from_codes(bbox, i2, c2.astype(np.uint8), 12, i2.shape)
+    np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64))
+    none = batch_saturation(torch.zeros((4, 4, 4), dtype=torch.int32, device="cuda"),
+                            torch.zeros((4, 4, 4), dtype=torch.uint8, device="cuda"), 3)
+    assert np.all(np.isnan(none))  # 0 / 0, as the reference's numpy division gives

@@ -651,6 +651,9 @@ struct Sq16 {
     uint16_t v;
 };
 
+// two 16-bit values in one register (v_pk_* integer operations)
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
 // Input element -> clamped squared distance: pass Y reads pass Z's uint16
 // distances (kNoSolid: none), pass X the squared distances of pass Y (int32,
 // or Sq16 when the cap fits in 16 bits).
@@ -828,6 +831,9 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
     // column stays inside the row (columns >= nz are dropped)
     const int c4 = min(4 * q, (int)(g.pitch - 4 - z0));
     const E *base = in + (int64_t)outer * g.os + z0;
+    // pass X with the int32 output only: two neighbouring columns go out as one 8-byte store
+    // when every pair starts 8-byte aligned (even strides, aligned array)
+    const bool d2_pairs = D2ONLY && o.d2 && ((o.sx | o.sy) & 1) == 0 && ((uintptr_t)o.d2 & 7) == 0;
 
     int4 acc = make_int4(kBig, kBig, kBig, kBig);
     int r0 = chunk * g.tpb * T;
@@ -850,10 +856,19 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
         }
         if (lane < 16) *reinterpret_cast<int4 *>(&part[wv][4 * lane]) = acc;
         __syncthreads();
-        // every tile row of this column is >= cm: a scan can stop once cm + k^2 >= best
-        int cm = part[0][lane];
+        // every tile row of a column is >= its minimum cm: a scan can stop once cm + k^2 >= best
+        int cm = kBig, cm_lo = kBig, cm_hi = kBig;
+        if constexpr (kNarrow) {
 #pragma unroll
-        for (int w = 1; w < kWaves; ++w) cm = min(cm, part[w][lane]);
+            for (int w = 0; w < kWaves; ++w) {
+                cm_lo = min(cm_lo, part[w][2 * (lane & 31)]);
+                cm_hi = min(cm_hi, part[w][2 * (lane & 31) + 1]);
+            }
+        } else {
+            cm = part[0][lane];
+#pragma unroll
+            for (int w = 1; w < kWaves; ++w) cm = min(cm, part[w][lane]);
+        }
         const bool more = rt + 1 < rt_end;
         if (more) issue_rows<T / 16>(base, g, r0 + T + W, row0, c4, xm, xr);
 
@@ -901,29 +916,127 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
                 emit_x(o, y, outer, z, nx, ny, g.nz, best >= kBig ? kInf : best);
             }
         };
-        // two rows per step (j and j + kWaves): their scans run in one loop, so a wave keeps
-        // the LDS reads of both in flight; the loop ends when neither can improve
-        for (int j = wv; j < jend; j += 2 * kWaves) {
-            const int j2 = j + kWaves;
-            const bool two = j2 < jend;  // wave-uniform
-            const L *c = col + (W + j) * kTileZ;
-            const L *c2 = col + (W + (two ? j2 : j)) * kTileZ;
-            int best = c[0], best2 = c2[0];
-            // blocks of four offsets while any lane of the wave can still improve; a lane that
-            // is done only folds in candidates >= k^2 >= best, which leave it unchanged, so the
-            // lanes need no exec mask and the loop exit is one wave-wide ballot
+        // The rest of a lane's two columns once their tile scan is done (wave-uniform calls;
+        // y: the row of this lane's half).  The common case -- every column settled within
+        // the tile -- costs one ballot; the global-memory scan and the deferral are the rare
+        // path, column by column.
+        auto finish_pair = [&](int y, int zp, bool act, u16x2 best) {
+            int bv0 = best.x, bv1 = best.y;
+            bool ok0 = true, ok1 = true;
+            constexpr int kTileReach = (W + 1) * (W + 1);
+            if (__builtin_amdgcn_ballot_w64(act && max(bv0, bv1) > kTileReach) != 0) {
 #pragma unroll
-            for (int k = 1; k <= W; k += 4) {
-                if (__builtin_amdgcn_ballot_w64(cm + k * k < max(best, best2)) == 0) break;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int kk = k + u;
-                    best = min(best, min(c[-kk * kTileZ], c[kk * kTileZ]) + kk * kk);
-                    best2 = min(best2, min(c2[-kk * kTileZ], c2[kk * kTileZ]) + kk * kk);
+                for (int c = 0; c < 2; ++c) {
+                    int &bv = c ? bv1 : bv0;
+                    bool &ok = c ? ok1 : ok0;
+                    const int zc = zp + c;
+                    if (!act || zc >= g.nz || bv <= kTileReach) continue;
+                    const int64_t ln_id = (int64_t)outer * g.nz + zc;
+                    const uint32_t bit = 1u << (ln_id & 31);
+                    int k = W + 1;
+                    ok = false;
+                    if (!(line_bits[ln_id >> 5] & bit)) {
+                        const E *ln = in + (int64_t)outer * g.os + zc;
+                        const int kmax = g.bc_lo == PMX_BC_PERIODIC ? g.n / 2 : max(y + 1, g.n - y);
+                        for (; k <= kReach && k <= kmax; ++k) {
+                            if (k * k >= bv) break;
+                            bv = min(bv, min(fetch(ln, g, y - k), fetch(ln, g, y + k)) + k * k);
+                        }
+                        ok = k * k >= bv || k > kmax;
+                    }
+                    if (!ok && !(atomicOr(line_bits + (ln_id >> 5), bit) & bit))
+                        ovf_list[atomicAdd(ovf_count, 1u)] = (uint32_t)ln_id;
                 }
             }
-            finish_row(j, best);
-            if (two) finish_row(j2, best2);
+            if constexpr (kY) {
+                // 16-bit output of both columns in one store (a deferred column's value is
+                // rewritten by the fallback; columns past nz land in the row padding); a
+                // settled value of 65535 sends the EDT to its int32 passes
+                const bool sat = act && ((ok0 && bv0 >= 65535 && zp < g.nz) || (ok1 && bv1 >= 65535 && zp + 1 < g.nz));
+                if (__builtin_amdgcn_ballot_w64(sat) != 0 && lane == 0) {
+                    ovf_count[2] = 1u;
+                    ovf_count[3] = 0u;
+                }
+                if (act && zp < g.pitch)
+                    *reinterpret_cast<uint32_t *>(out + (int64_t)outer * g.out_os + (int64_t)y * g.out_rs + zp) =
+                        (uint32_t)min(bv0, 65535) | ((uint32_t)min(bv1, 65535) << 16);
+            } else if constexpr (D2ONLY) {
+                // every value of this path is below 65535: no "none" marker can occur
+                int32_t *d = o.d2 + (int64_t)y * o.sx + (int64_t)outer * o.sy + zp;
+                if (act && ok0 && ok1 && zp + 1 < g.nz) {
+                    if (d2_pairs)
+                        *reinterpret_cast<int2 *>(d) = make_int2(bv0, bv1);
+                    else {
+                        d[0] = bv0;
+                        d[1] = bv1;
+                    }
+                } else {
+                    if (act && ok0 && zp < g.nz) d[0] = bv0;
+                    if (act && ok1 && zp + 1 < g.nz) d[1] = bv1;
+                }
+            } else {
+                if (act && ok0 && zp < g.nz) emit_x(o, y, outer, zp, nx, ny, g.nz, bv0 >= kBig ? kInf : bv0);
+                if (act && ok1 && zp + 1 < g.nz) emit_x(o, y, outer, zp + 1, nx, ny, g.nz, bv1 >= kBig ? kInf : bv1);
+            }
+        };
+        if constexpr (kNarrow) {
+            // Packed 16-bit scan: every tile value is at most 65535, so a lane takes two
+            // neighbouring columns (2p, 2p + 1) in one 32-bit register: one LDS read and one
+            // v_pk_min_u16 / v_pk_add_u16 (saturating) per offset serve both.  Lanes 0..31 take
+            // row j, lanes 32..63 row j + kWaves (the rows the scalar scan pairs).  A candidate
+            // min(f, 65535) + k^2 saturating at 65535 leaves the minimum unchanged below 65535
+            // and at 65535 (flagged) otherwise, as the uint16 tile does.
+            const int p = lane & 31, h = lane >> 5;
+            const int zp = z0 + 2 * p;  // first column of this lane's pair
+            const uint32_t *pcol = reinterpret_cast<const uint32_t *>(tile) + p;
+            const u16x2 cmv = {(unsigned short)min(cm_lo, 65535), (unsigned short)min(cm_hi, 65535)};
+            for (int j = wv; j < jend; j += 2 * kWaves) {
+                const bool act = h == 0 || j + kWaves < jend;
+                const int jr = act ? j + kWaves * h : j;
+                const uint32_t *c = pcol + (W + jr) * (kTileZ / 2);
+                // an idle half (the last row of an odd count) scans nothing: best 0
+                u16x2 best = __builtin_bit_cast(u16x2, act ? c[0] : 0u);
+#pragma unroll
+                for (int k = 1; k <= W; k += 4) {
+                    // a column can still improve while best - cm > k^2
+                    const u16x2 gap = __builtin_elementwise_sub_sat(best, cmv);
+                    if (__builtin_amdgcn_ballot_w64(max((int)gap.x, (int)gap.y) > k * k) == 0) break;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int kk = k + u;
+                        const u16x2 a = __builtin_bit_cast(u16x2, c[-kk * (kTileZ / 2)]);
+                        const u16x2 b = __builtin_bit_cast(u16x2, c[kk * (kTileZ / 2)]);
+                        const u16x2 k2 = {(unsigned short)(kk * kk), (unsigned short)(kk * kk)};
+                        best = __builtin_elementwise_min(best, __builtin_elementwise_add_sat(__builtin_elementwise_min(a, b), k2));
+                    }
+                }
+                finish_pair(r0 + jr, zp, act, best);
+            }
+        } else {
+            // two rows per step (j and j + kWaves): their scans run in one loop, so a wave keeps
+            // the LDS reads of both in flight; the loop ends when neither can improve
+            for (int j = wv; j < jend; j += 2 * kWaves) {
+                const int j2 = j + kWaves;
+                const bool two = j2 < jend;  // wave-uniform
+                const L *c = col + (W + j) * kTileZ;
+                const L *c2 = col + (W + (two ? j2 : j)) * kTileZ;
+                int best = c[0], best2 = c2[0];
+                // blocks of four offsets while any lane of the wave can still improve; a lane that
+                // is done only folds in candidates >= k^2 >= best, which leave it unchanged, so the
+                // lanes need no exec mask and the loop exit is one wave-wide ballot
+    #pragma unroll
+                for (int k = 1; k <= W; k += 4) {
+                    if (__builtin_amdgcn_ballot_w64(cm + k * k < max(best, best2)) == 0) break;
+    #pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int kk = k + u;
+                        best = min(best, min(c[-kk * kTileZ], c[kk * kTileZ]) + kk * kk);
+                        best2 = min(best2, min(c2[-kk * kTileZ], c2[kk * kTileZ]) + kk * kk);
+                    }
+                }
+                finish_row(j, best);
+                if (two) finish_row(j2, best2);
+            }
         }
         if (!more) break;
         // slide: rows T .. T+2W-1 become rows 0 .. 2W-1, the new rows follow

@@ -56,9 +56,15 @@ constexpr int kHullCap = kNumHull + 1; // members returned per face
 #ifndef PMX_REF_LDS_CAP
 #define PMX_REF_LDS_CAP 64
 #endif
-// LDS stack entries per lane: 64 leave about 0.1 % of the lines of a
-// porosity-0.5 sphere pack to the fallback (32: 11 %, 48: 1 %)
+// LDS stack entries per lane of the boundary-hull kernel
 constexpr int kLdsCap = PMX_REF_LDS_CAP;
+#ifndef PMX_REF_RING
+#define PMX_REF_RING 32
+#endif
+// LDS ring entries per lane of the streaming parabolic pass
+constexpr int kRingSlots = PMX_REF_RING;
+constexpr int kChunk = 16;       // input positions staged per lane at a time
+constexpr int kEvalPerStep = 2;  // positions evaluated per input position
 constexpr int kFbMaxThreads = 32768;   // persistent lanes of the fallback kernels, at most
 
 struct RefHull {
@@ -91,18 +97,54 @@ struct Skips {
     int padded;
 };
 
+// Calls f(i, p[i * st]) for i = i0, i0 + step, ... while i is in [lo, hi)
+// (step +1 from i0 = lo, or -1 from i0 = hi - 1), stopping when f returns
+// false.  One lane walks its line in order, so the loads run kScan values
+// ahead of the calls: a line pays one memory latency per kScan values
+// instead of one per value.
+constexpr int kScan = 16;
+template <int kStep, class T, class F>
+__host__ __device__ __forceinline__ void stream_values(const T *p, int64_t st, int lo, int hi, F &&f)
+{
+    static_assert(kStep == 1 || kStep == -1, "unit steps");
+    const int count = hi - lo;
+    if (count <= 0) return;
+    const int i0 = kStep > 0 ? lo : hi - 1;
+    auto load = [&](T (&buf)[kScan], int j0) {
+#pragma unroll
+        for (int c = 0; c < kScan; ++c)
+            buf[c] = j0 + c < count ? p[(int64_t)(i0 + kStep * (j0 + c)) * st] : T();
+    };
+    T cur[kScan], nxt[kScan];
+    load(cur, 0);
+    for (int j0 = 0; j0 < count; j0 += kScan) {
+        const bool more = j0 + kScan < count;
+        if (more) load(nxt, j0 + kScan);
+#pragma unroll
+        for (int c = 0; c < kScan; ++c)
+            if (j0 + c < count && !f(i0 + kStep * (j0 + c), cur[c])) return;
+        if (more) {
+#pragma unroll
+            for (int c = 0; c < kScan; ++c) cur[c] = nxt[c];
+        }
+    }
+}
+
 // first (forward) / last index of value 0 within [lo_skip, n - hi_skip), or -1
 // (_voxels.hpp:60-79 through _voxels.pyx:231-283)
 __device__ int nearest_zero(const uint8_t *img, int n, int64_t stride, bool forward, int lo_skip, int hi_skip)
 {
-    if (forward) {
-        for (int i = lo_skip; i < n - hi_skip; ++i)
-            if (img[(int64_t)i * stride] == 0) return i;
-    } else {
-        for (int i = n - hi_skip - 1; i >= lo_skip; --i)
-            if (img[(int64_t)i * stride] == 0) return i;
-    }
-    return -1;
+    int found = -1;
+    auto hit = [&](int i, uint8_t v) {
+        if (v != 0) return true;
+        found = i;
+        return false;
+    };
+    if (forward)
+        stream_values<1>(img, stride, lo_skip, n - hi_skip, hit);
+    else
+        stream_values<-1>(img, stride, lo_skip, n - hi_skip, hit);
+    return found;
 }
 
 // pass 0 along g.dim from the image into out (squared, INF -> FLT_MAX)
@@ -140,8 +182,7 @@ __global__ __launch_bounds__(256) void k_ref_pass0(const uint8_t *__restrict__ i
     float prev = seg == 0 ? 0.0f : lo * res;
     o[0] = prev;
     uint8_t vprev = seg;
-    for (int i = 1; i < n; ++i) {
-        const uint8_t v = im[(int64_t)i * ist];
+    stream_values<1>(im, ist, 1, n, [&](int i, uint8_t v) {
         float cur;
         if (v == 0) {
             cur = 0.0f;
@@ -155,7 +196,8 @@ __global__ __launch_bounds__(256) void k_ref_pass0(const uint8_t *__restrict__ i
         o[(int64_t)i * ost] = cur;
         prev = cur;
         vprev = v;
-    }
+        return true;
+    });
     // cap, backward min, square, to_finite (:168-181)
     float next = o[(int64_t)(n - 1) * ost];
     if (next > hi * res) next = hi * res;
@@ -163,19 +205,28 @@ __global__ __launch_bounds__(256) void k_ref_pass0(const uint8_t *__restrict__ i
         const float sq = next * next;
         o[(int64_t)(n - 1) * ost] = sq == INFINITY ? kFltMax : sq;
     }
-    for (int i = n - 2; i >= 0; --i) {
-        const float cur = fminf(o[(int64_t)i * ost], next + res);
+    stream_values<-1>(o, ost, 0, n - 1, [&](int i, float v) {
+        const float cur = fminf(v, next + res);
         const float sq = cur * cur;
         o[(int64_t)i * ost] = sq == INFINITY ? kFltMax : sq;
         next = cur;
-    }
+        return true;
+    });
 }
 
-__device__ __forceinline__ float intersect(float f_r, float r, float f_q, float q, float aniso)
+// intersect_parabolas divides the float numerator by 2.0 * factor1 in
+// double and rounds to float.  A quotient of two floats rounded to double
+// (53 bits) and then to float (24 bits) equals the correctly rounded float
+// quotient, since 53 >= 2 * 24 + 2 (double rounding is innocuous for
+// division at that margin), and 2 * factor1 is exact in float below 2^127; so
+// the float division (correctly rounded in HIP) gives the same bits, with the
+// double route kept for a factor1 near the top of the float range.
+__host__ __device__ __forceinline__ float intersect(float f_r, float r, float f_q, float q, float aniso)
 {
     const float factor1 = (r - q) * aniso;
     const float factor2 = r + q;
     const float num = f_r - f_q + factor1 * factor2;
+    if (fabsf(factor1) < 0x1p125f) return num / (2.0f * factor1);
     return (float)((double)num / (2.0 * (double)factor1));
 }
 
@@ -185,13 +236,13 @@ struct Stack {
     int *hv;
     float *hh, *rg;
     int64_t step;
-    __device__ __forceinline__ int &V(int k) const { return hv[(int64_t)k * step]; }
-    __device__ __forceinline__ float &H(int k) const { return hh[(int64_t)k * step]; }
-    __device__ __forceinline__ float &R(int k) const { return rg[(int64_t)k * step]; }
+    __host__ __device__ __forceinline__ int &V(int k) const { return hv[(int64_t)k * step]; }
+    __host__ __device__ __forceinline__ float &H(int k) const { return hh[(int64_t)k * step]; }
+    __host__ __device__ __forceinline__ float &R(int k) const { return rg[(int64_t)k * step]; }
 };
 
 // update_hull (_distance.hpp:89-112); false when entry k+1 would reach cap
-__device__ __forceinline__ bool update_hull(const Stack &S, int &k, int i, float fv, float aniso, int cap)
+__host__ __device__ __forceinline__ bool update_hull(const Stack &S, int &k, int i, float fv, float aniso, int cap)
 {
     float s = intersect(fv, (float)i, S.H(k), (float)S.V(k), aniso);
     while (k > 0 && s <= S.R(k)) {
@@ -212,7 +263,7 @@ __device__ __forceinline__ bool update_hull(const Stack &S, int &k, int i, float
 // the envelope outgrows cap.  A one-voxel segment with no finite value reads
 // past the reference's vectors there (undefined behaviour); this path then
 // returns the zero-initialised slot, as the oracle does.
-__device__ int boundary_hull(const float *p, int n, int64_t st, float res, int index_corrector, bool forward,
+__host__ __device__ int boundary_hull(const float *p, int n, int64_t st, float res, int index_corrector, bool forward,
                              const Stack &S, int cap, RefHull *out)
 {
     if (n == 1 && p[0] < kFltMax) {
@@ -227,8 +278,9 @@ __device__ int boundary_hull(const float *p, int n, int64_t st, float res, int i
     S.R(0) = -INFINITY;
     S.R(1) = INFINITY;
     int k = 1;
-    for (int i = 1; i < n; ++i)
-        if (!update_hull(S, k, i, p[(int64_t)i * st], aniso, cap)) return -1;
+    bool fits = true;
+    stream_values<1>(p, st, 1, n, [&](int i, float v) { return fits = update_hull(S, k, i, v, aniso, cap); });
+    if (!fits) return -1;
     const int num_hull = min(kNumHull, k);
     int cnt = 0, kk = forward ? 0 : k;
     const int step = forward ? 1 : -1;
@@ -297,18 +349,36 @@ __global__ __launch_bounds__(kWave) void k_ref_hulls(const uint8_t *__restrict__
         ovf[atomicAdd(n_ovf, 1)] = (int)line;
 }
 
-__global__ __launch_bounds__(256) void k_ref_hulls_fallback(const uint8_t *__restrict__ og,
-                                                            const float *__restrict__ f, LineGeom g, float res,
-                                                            Skips sk, RefHull *__restrict__ hulls,
-                                                            int *__restrict__ counts, const int *__restrict__ ovf,
-                                                            const int *__restrict__ n_ovf, int *stk, int cap)
+// A lane's stack of cap entries for the run and hull fallbacks: dynamic LDS
+// of [cap][lanes of the workgroup] ints, floats, floats (kLds), else the
+// workspace `gstk` as [cap][lanes of the grid].
+template <bool kLds>
+__device__ __forceinline__ Stack lane_stack(int *dyn, int *gstk, int cap)
 {
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (kLds) {
+        const int lanes = blockDim.x;
+        float *f = reinterpret_cast<float *>(dyn);
+        return Stack{dyn + threadIdx.x, f + (size_t)cap * lanes + threadIdx.x,
+                     f + (size_t)2 * cap * lanes + threadIdx.x, lanes};
+    }
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, T = (int64_t)gridDim.x * blockDim.x;
+    float *f = reinterpret_cast<float *>(gstk);
+    return Stack{gstk + tid, f + (int64_t)cap * T + tid, f + (int64_t)2 * cap * T + tid, T};
+}
+
+// the overflow list of k_ref_hulls, stacks of cap entries (a whole line)
+template <bool kLds>
+__global__ __launch_bounds__(kWave) void k_ref_hulls_fallback(const uint8_t *__restrict__ og,
+                                                              const float *__restrict__ f, LineGeom g, float res,
+                                                              Skips sk, RefHull *__restrict__ hulls,
+                                                              int *__restrict__ counts, const int *__restrict__ ovf,
+                                                              const int *__restrict__ n_ovf, int *gstk, int cap)
+{
+    extern __shared__ int dyn[];
+    const Stack S = lane_stack<kLds>(dyn, gstk, cap);
     const int64_t T = (int64_t)gridDim.x * blockDim.x;
-    Stack S{stk + tid, reinterpret_cast<float *>(stk) + (int64_t)cap * T + tid,
-            reinterpret_cast<float *>(stk) + (int64_t)2 * cap * T + tid, T};
     const int m = *n_ovf;
-    for (int64_t j = tid; j < m; j += T) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += T) {
         const int64_t line = ovf[j];
         int64_t ib, fb, ob;
         g.base(line, &ib, &fb, &ob);
@@ -316,21 +386,232 @@ __global__ __launch_bounds__(256) void k_ref_hulls_fallback(const uint8_t *__res
     }
 }
 
-// squared_edt_1d_parabolic (_distance.hpp:184-272) on one line, from fin
-// to fout, with the injected hulls (lower pushed in reverse, upper at
-// vertex + n); flushes the final members before every zero-height
-// parabola.  Returns false when the stack outgrows cap (fout is then
-// partly written; the caller recomputes the line).
-__device__ bool parabolic_line(const float *fin, float *fout, int n, int64_t ist, int64_t ost, float aniso,
-                               const RefHull *lower, int nl, const RefHull *upper, int nu, const Stack &S, int cap,
-                               bool take_sqrt)
+// A lane's stack for the streaming pass: entry k lives at slot (k & mask),
+// slot s at base[s * lanes]; with a power-of-two ring (mask = slots - 1)
+// entries below the evaluation cursor are overwritten as the line advances.
+struct RingStack {
+    int *v;
+    float *h, *r;
+    int lanes;
+    unsigned mask;
+    __host__ __device__ __forceinline__ int at(int k) const { return (int)((unsigned)k & mask) * lanes; }
+};
+
+// Positions [p1, p2) of a line, between zero-height parabolas at p1 and p2;
+// p1 = -1 starts at the line start (lower hull or position 0), p2 = n runs
+// to the line end (upper hull).
+//
+// Such a run is a problem of its own.  The zero at p1 is never popped (a
+// later parabola r meets it at s >= (r + p1) / 2 > p1 >= its range), every
+// entry below it has a range that ends before p1, and update_hull never
+// reaches below it; so the envelope over [p1, p2] and the values there are
+// those of a stack that starts with the zero at p1 alone.
+struct Chord {
+    int line, p1, p2;
+};
+
+// squared_edt_1d_parabolic (_distance.hpp:184-272) on one line, streaming:
+//
+//   * the envelope's top entry stays in registers; an entry is written to
+//     the ring when pushed and read back only when the entries above it are
+//     popped;
+//   * a position is evaluated once a zero-height parabola lies after it
+//     (its covering entry and that entry's range are then final), at most
+//     kEvalPerStep positions per input position, so the lanes of a wave do
+//     the same work per step; the rest is evaluated after the upper hull;
+//   * the input arrives kChunk positions at a time through `stage` (this
+//     lane's column of an LDS tile, stride `lanes`), the next chunk's loads
+//     in flight while the current one is processed;
+//   * a run whose live entries outgrow the ring (after everything final is
+//     evaluated) is handed on whole, emit(p1, p2), and the lane resumes with
+//     a fresh stack at the zero that ends it.  emit returns false when it
+//     took the whole line instead; the lane then stops.
+//
+// The covering entry of position i is the first kk with R(kk + 1) >= i, and
+// its value aniso * (i - v)^2 + h, as in the reference, so each value is
+// bit-identical.  All lanes of a wave run the call together (the chunk loop
+// is uniform); a lane with alive = false follows along without effect.
+template <bool kSqrt, class Emit>
+__host__ __device__ void parabolic_stream(const float *fin, float *fout, int n, int64_t ist, int64_t ost,
+                                          float aniso, const RefHull *lower, int nl, const RefHull *upper, int nu,
+                                          const RingStack &S, int cap, bool alive, float *stage, int lanes,
+                                          Emit &&emit)
 {
     int k = 0, loop_start = 1;
+    int tv = 0;  // top entry k: vertex, height, range
+    float th = 0.0f, tr = -INFINITY;
+    if (alive) {
+        if (nl > 0) {
+            loop_start = 0;
+            for (int m = nl - 1, j = 0; m >= 0; --m, ++j) {
+                const int o = S.at(j);
+                S.v[o] = lower[m].vertex;
+                S.h[o] = lower[m].height;
+                S.r[o] = lower[m].range;
+            }
+            k = nl - 1;
+            tv = lower[0].vertex;
+            th = lower[0].height;
+            tr = lower[0].range;
+        } else {
+            th = fin[0];
+            const int o = S.at(0);
+            S.v[o] = 0;
+            S.h[o] = th;
+            S.r[o] = -INFINITY;
+        }
+    }
+    // evaluation cursor: next position pe (output op), its covering
+    // candidate kk (ev, eh) and R(kk + 1) in er; positions below zl (the
+    // last zero) are final
+    int kk = 0, pe = 0, zl = -1;
+    int ev = nl > 0 ? lower[nl - 1].vertex : 0;
+    float eh = nl > 0 ? lower[nl - 1].height : th, er = INFINITY;
+    float *op = fout;
+    auto eval = [&]() {
+        while (er < (float)pe) {
+            ++kk;
+            const int o = S.at(kk);
+            ev = S.v[o];
+            eh = S.h[o];
+            er = kk < k ? S.r[S.at(kk + 1)] : INFINITY;
+        }
+        const float di = (float)(pe - ev);
+        const float val = aniso * (di * di) + eh;
+        *op = kSqrt ? sqrtf(val) : val;
+        op += ost;
+        ++pe;
+    };
+    // update_hull (_distance.hpp:89-112); false when the ring is full
+    auto push = [&](int i, float fv) -> bool {
+        float s = intersect(fv, (float)i, th, (float)tv, aniso);
+        while (k > 0 && s <= tr) {
+            --k;
+            const int o = S.at(k);
+            tv = S.v[o];
+            th = S.h[o];
+            tr = S.r[o];
+            s = intersect(fv, (float)i, th, (float)tv, aniso);
+        }
+        if (k + 1 - kk >= cap) {
+            while (pe < zl && k + 1 - kk >= cap) eval();
+            if (k + 1 - kk >= cap) return false;
+        }
+        ++k;
+        tv = i;
+        th = fv;
+        tr = s;
+        const int o = S.at(k);
+        S.v[o] = i;
+        S.h[o] = fv;
+        S.r[o] = s;
+        return true;
+    };
+    // a fresh stack holding the zero at i alone
+    auto restart = [&](int i) {
+        k = kk = 0;
+        const int o = S.at(0);
+        S.v[o] = i;
+        S.h[o] = 0.0f;
+        S.r[o] = -INFINITY;
+        tv = ev = i;
+        th = eh = 0.0f;
+        tr = -INFINITY;
+        er = INFINITY;
+        pe = zl = i;
+        op = fout + (int64_t)i * ost;
+    };
+    bool skip = false;  // inside a run handed on, which starts at p1
+    int p1 = -1;
+    float pre[kChunk];
+#pragma unroll
+    for (int c = 0; c < kChunk; ++c) pre[c] = c < n ? fin[(int64_t)c * ist] : 0.0f;
+    for (int i0 = 0; i0 < n; i0 += kChunk) {
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) stage[c * lanes] = pre[c];
+        if (i0 + kChunk < n) {
+#pragma unroll
+            for (int c = 0; c < kChunk; ++c) {
+                const int i = i0 + kChunk + c;
+                pre[c] = i < n ? fin[(int64_t)i * ist] : 0.0f;
+            }
+        }
+        const int m = n - i0 < kChunk ? n - i0 : kChunk;
+        for (int c = 0; c < m; ++c) {
+            const int i = i0 + c;
+            const float fv = stage[c * lanes];
+            if (alive && i >= loop_start) {
+                if (skip) {
+                    if (fv == 0.0f) {
+                        skip = false;
+                        alive = emit(p1, i);
+                        restart(i);
+                    }
+                } else if (!push(i, fv)) {
+                    if (fv == 0.0f) {
+                        alive = emit(zl, i);
+                        restart(i);
+                    } else {
+                        skip = true;
+                        p1 = zl;
+                    }
+                } else if (fv == 0.0f) {
+                    // entries 0..k are final; R(kk + 1) may have changed
+                    // since it was read (kk + 1 > the previous zero's entry)
+                    zl = i;
+                    er = kk < k ? S.r[S.at(kk + 1)] : INFINITY;
+                }
+            }
+            if (alive && !skip) {
+#pragma unroll
+                for (int e = 0; e < kEvalPerStep; ++e)
+                    if (pe < zl) eval();
+            }
+        }
+    }
+    if (alive && !skip) {
+        for (int m = 0; m < nu; ++m) {
+            if (!push(upper[m].vertex + n, upper[m].height)) {
+                skip = true;
+                p1 = zl;
+                break;
+            }
+        }
+    }
+    if (alive && skip) {
+        emit(p1, n);
+    } else if (alive) {
+        zl = n;
+        er = kk < k ? S.r[S.at(kk + 1)] : INFINITY;
+        while (pe < n) eval();
+    }
+}
+
+// Stack entries a run can need: its positions, the hull entries at the line
+// ends, and update_hull's range sentinel.
+__host__ __device__ __forceinline__ int chord_entries(const Chord &c, int n)
+{
+    const int first = c.p1 < 0 ? 0 : c.p1, last = c.p2 < n ? c.p2 : n - 1;
+    return last - first + 1 + 2 * kHullCap + 2;
+}
+
+// One run on its own (see Chord): the envelope over its positions (and the
+// hulls at the line ends), then its positions evaluated against it.  Returns
+// false when the stack outgrows cap.
+template <bool kSqrt>
+__host__ __device__ bool chord_line(const float *fin, float *fout, int n, int64_t ist, int64_t ost, float aniso,
+                                    const RefHull *lower, int nl, const RefHull *upper, int nu, Chord c,
+                                    const Stack &S, int cap)
+{
+    int k = 0, i0;
     S.R(0) = -INFINITY;
     S.R(1) = INFINITY;
-    if (nl > 0) {
+    if (c.p1 >= 0) {
+        S.V(0) = c.p1;
+        S.H(0) = 0.0f;
+        i0 = c.p1 + 1;
+    } else if (nl > 0) {
         if (nl + 1 >= cap) return false;
-        loop_start = 0;
         for (int m = nl - 1; m >= 0; --m) {
             S.V(k) = lower[m].vertex;
             S.H(k) = lower[m].height;
@@ -339,83 +620,116 @@ __device__ bool parabolic_line(const float *fin, float *fout, int n, int64_t ist
             ++k;
         }
         --k;
+        i0 = 0;
     } else {
         S.V(0) = 0;
         S.H(0) = fin[0];
+        i0 = 1;
     }
-    int from = 0;  // first position not yet written
-    auto flush = [&](int to) {
-        int kk = 0;
-        for (int i = from; i < to; ++i) {
-            while (S.R(kk + 1) < (float)i) ++kk;
-            const float di = (float)(i - S.V(kk));
-            const float v = aniso * (di * di) + S.H(kk);
-            fout[(int64_t)i * ost] = take_sqrt ? sqrtf(v) : v;
-        }
-        from = to;
-    };
-    for (int i = loop_start; i < n; ++i) {
-        const float fi = fin[(int64_t)i * ist];
-        if (!update_hull(S, k, i, fi, aniso, cap)) return false;
-        if (fi == 0.0f && k > 0) {
-            // entries 0..k are final for every position before i
-            flush(i);
-            S.V(0) = S.V(k);
-            S.H(0) = S.H(k);
-            S.R(0) = S.R(k);
-            S.R(1) = INFINITY;
-            k = 0;
-        }
+    const int last = c.p2 < n ? c.p2 : n - 1;
+    bool fits = true;
+    stream_values<1>(fin, ist, i0, last + 1, [&](int i, float v) { return fits = update_hull(S, k, i, v, aniso, cap); });
+    if (!fits) return false;
+    if (c.p2 >= n)
+        for (int m = 0; m < nu; ++m)
+            if (!update_hull(S, k, upper[m].vertex + n, upper[m].height, aniso, cap)) return false;
+    const int e0 = c.p1 < 0 ? 0 : c.p1, e1 = c.p2 < n ? c.p2 : n;
+    float *op = fout + (int64_t)e0 * ost;
+    int kk = 0;
+    for (int i = e0; i < e1; ++i, op += ost) {
+        while (S.R(kk + 1) < (float)i) ++kk;
+        const float di = (float)(i - S.V(kk));
+        const float val = aniso * (di * di) + S.H(kk);
+        *op = kSqrt ? sqrtf(val) : val;
     }
-    for (int m = 0; m < nu; ++m)
-        if (!update_hull(S, k, upper[m].vertex + n, upper[m].height, aniso, cap)) return false;
-    flush(n);
     return true;
 }
 
-__global__ __launch_bounds__(kWave) void k_ref_parabolic(const float *__restrict__ fin, float *__restrict__ fout,
-                                                         LineGeom g, float res, const RefHull *__restrict__ hulls,
-                                                         const int *__restrict__ counts, int take_sqrt,
-                                                         int *__restrict__ ovf, int *__restrict__ n_ovf)
+struct LineHulls {
+    const RefHull *lower, *upper;
+    int nl, nu;
+};
+
+__device__ __forceinline__ LineHulls hulls_of(const RefHull *hulls, const int *counts, int64_t line)
 {
-    __shared__ int sv[kLdsCap * kWave];
-    __shared__ float sh[kLdsCap * kWave], sr[kLdsCap * kWave];
-    const int64_t line = (int64_t)blockIdx.x * kWave + threadIdx.x;
-    if (line >= g.lines()) return;
-    Stack S{sv + threadIdx.x, sh + threadIdx.x, sr + threadIdx.x, kWave};
-    int64_t ib, fb, ob;
-    g.base(line, &ib, &fb, &ob);
-    const RefHull *lower = hulls ? hulls + (line * 2 + 1) * kHullCap : nullptr;
-    const RefHull *upper = hulls ? hulls + (line * 2 + 0) * kHullCap : nullptr;
-    const int nl = hulls ? counts[line * 2 + 1] : 0;
-    const int nu = hulls ? counts[line * 2 + 0] : 0;
-    if (!parabolic_line(fin + fb, fout + ob, (int)g.n[g.dim], g.fs[g.dim], g.os[g.dim], res * res, lower, nl, upper,
-                        nu, S, kLdsCap, take_sqrt != 0))
-        ovf[atomicAdd(n_ovf, 1)] = (int)line;
+    LineHulls h{nullptr, nullptr, 0, 0};
+    if (hulls) {
+        h.lower = hulls + (line * 2 + 1) * kHullCap;
+        h.upper = hulls + (line * 2 + 0) * kHullCap;
+        h.nl = counts[line * 2 + 1];
+        h.nu = counts[line * 2 + 0];
+    }
+    return h;
 }
 
-__global__ __launch_bounds__(256) void k_ref_parabolic_fallback(const float *__restrict__ fin,
-                                                                float *__restrict__ fout, LineGeom g, float res,
-                                                                const RefHull *__restrict__ hulls,
-                                                                const int *__restrict__ counts, int take_sqrt,
-                                                                const int *__restrict__ ovf,
-                                                                const int *__restrict__ n_ovf, int *stk, int cap)
+// Run lists between the parabolic kernels: `a` from the ring pass (runs that
+// outgrew the ring), `c` for the last kernel (runs too long for the second
+// and whole lines once `a` is full); c_cap = lines + a_cap, so `c` cannot
+// fill up.
+struct ChordLists {
+    Chord *a, *c;
+    int *n_a, *n_c;
+    int a_cap, c_cap;
+};
+
+// one wave per workgroup, one line per lane, stacks in an LDS ring of
+// kRingSlots entries per lane
+template <int kRingSlots, bool kSqrt>
+__global__ __launch_bounds__(kWave) void k_ref_parabolic(const float *__restrict__ fin, float *__restrict__ fout,
+                                                         LineGeom g, float res, const RefHull *__restrict__ hulls,
+                                                         const int *__restrict__ counts, ChordLists L)
 {
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    static_assert((kRingSlots & (kRingSlots - 1)) == 0, "ring slots must be a power of two");
+    __shared__ int sv[kRingSlots * kWave];
+    __shared__ float sh[kRingSlots * kWave], sr[kRingSlots * kWave], st[kChunk * kWave];
+    const int lane = threadIdx.x;
+    const int64_t lines = g.lines();
+    int64_t line = (int64_t)blockIdx.x * kWave + lane;
+    const bool real = line < lines;
+    if (!real) line = lines - 1;  // keeps the tail lanes' loads in bounds
+    int64_t ib, fb, ob;
+    g.base(line, &ib, &fb, &ob);
+    const LineHulls h = hulls_of(hulls, counts, line);
+    const int n = (int)g.n[g.dim];
+    const RingStack S{sv + lane, sh + lane, sr + lane, kWave, (unsigned)(kRingSlots - 1)};
+    auto emit = [&](int p1, int p2) -> bool {
+        const int j = atomicAdd(L.n_a, 1);
+        if (j < L.a_cap) {
+            L.a[j] = Chord{(int)line, p1, p2};
+            return true;
+        }
+        L.c[atomicAdd(L.n_c, 1)] = Chord{(int)line, -1, n};
+        return false;
+    };
+    parabolic_stream<kSqrt>(fin + fb, fout + ob, n, g.fs[g.dim], g.os[g.dim], res * res, h.lower, h.nl, h.upper,
+                            h.nu, S, kRingSlots, real, st + lane, kWave, emit);
+}
+
+// The runs of a list, one per lane, each with a stack of `cap` entries (see
+// lane_stack).  Runs that need more than cap entries go to `out` (non-null
+// whenever cap is below a whole line's).
+template <bool kSqrt, bool kLds>
+__global__ __launch_bounds__(kWave) void k_ref_chords(const float *__restrict__ fin, float *__restrict__ fout,
+                                                      LineGeom g, float res, const RefHull *__restrict__ hulls,
+                                                      const int *__restrict__ counts, const Chord *__restrict__ in,
+                                                      const int *__restrict__ n_in, int in_cap, Chord *out,
+                                                      int *n_out, int cap, int *gstk)
+{
+    extern __shared__ int dyn[];
+    const Stack S = lane_stack<kLds>(dyn, gstk, cap);
     const int64_t T = (int64_t)gridDim.x * blockDim.x;
-    Stack S{stk + tid, reinterpret_cast<float *>(stk) + (int64_t)cap * T + tid,
-            reinterpret_cast<float *>(stk) + (int64_t)2 * cap * T + tid, T};
-    const int m = *n_ovf;
-    for (int64_t j = tid; j < m; j += T) {
-        const int64_t line = ovf[j];
+    const int n = (int)g.n[g.dim];
+    const int m = min(*n_in, in_cap);
+    const float aniso = res * res;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += T) {
+        const Chord c = in[j];
         int64_t ib, fb, ob;
-        g.base(line, &ib, &fb, &ob);
-        const RefHull *lower = hulls ? hulls + (line * 2 + 1) * kHullCap : nullptr;
-        const RefHull *upper = hulls ? hulls + (line * 2 + 0) * kHullCap : nullptr;
-        const int nl = hulls ? counts[line * 2 + 1] : 0;
-        const int nu = hulls ? counts[line * 2 + 0] : 0;
-        parabolic_line(fin + fb, fout + ob, (int)g.n[g.dim], g.fs[g.dim], g.os[g.dim], res * res, lower, nl, upper,
-                       nu, S, cap, take_sqrt != 0);
+        g.base(c.line, &ib, &fb, &ob);
+        const LineHulls h = hulls_of(hulls, counts, c.line);
+        if (chord_entries(c, n) > cap ||
+            !chord_line<kSqrt>(fin + fb, fout + ob, n, g.fs[g.dim], g.os[g.dim], aniso, h.lower, h.nl, h.upper,
+                               h.nu, c, S, cap))
+            if (out) out[atomicAdd(n_out, 1)] = c;  // cap below a whole line only with an out list
     }
 }
 
@@ -457,11 +771,22 @@ int64_t max_lines(const int64_t *n)
 // fallback lanes: one per line up to kFbMaxThreads, whole blocks of 256
 int fb_threads(const int64_t *n) { return (int)std::min<int64_t>(kFbMaxThreads, (max_lines(n) + 255) / 256 * 256); }
 
+// Runs per lane: stacks of cap entries (12 bytes each) in kChordLdsBytes
+// of LDS per workgroup, as many lanes (up to a wave) as fit, or 0
+constexpr size_t kChordLdsBytes = 48 * 1024;
+size_t chord_lds_bytes(int cap, int lanes) { return (size_t)lanes * (size_t)cap * 12; }
+int chord_lds_lanes(int cap) { return (int)std::min<size_t>(kWave, kChordLdsBytes / chord_lds_bytes(cap, 1)); }
+constexpr int kChordCap = 128;       // stack entries per lane of the second parabolic kernel
+constexpr int kChordsPerLine = 4;    // room in the ring pass's run list, per line
+constexpr int kChordMaxBlocks = 4096;
+
 struct RefWs {
     float *tmp;
     int *stk;
     RefHull *hulls;
     int *counts, *ovf, *n_ovf;
+    Chord *ca, *cc;
+    int a_cap, c_cap;
 };
 
 RefWs carve_ref(Carver &c, const int64_t *n)
@@ -475,6 +800,10 @@ RefWs carve_ref(Carver &c, const int64_t *n)
     w.counts = c.take<int>((size_t)lines * 2);
     w.ovf = c.take<int>((size_t)lines);
     w.n_ovf = c.take<int>(4);
+    w.a_cap = (int)std::min<int64_t>(lines * kChordsPerLine, INT32_MAX / 2);
+    w.c_cap = (int)(lines + w.a_cap);
+    w.ca = c.take<Chord>((size_t)w.a_cap);
+    w.cc = c.take<Chord>((size_t)w.c_cap);
     return w;
 }
 
@@ -557,7 +886,7 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
         const int64_t lines = g.lines();
         const unsigned nb = (unsigned)((lines + kWave - 1) / kWave);
         const int sq = last && take_sqrt ? 1 : 0;
-        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, w.n_ovf, 2);
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, w.n_ovf, 4);
         if (periodic[dim]) {
             {
                 ProfScope ps("edt_ref_hulls", s);
@@ -567,24 +896,67 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
             if (int e = check_launch("reference hulls")) return e;
             {
                 ProfScope ps("edt_ref_hulls_fallback", s);
-                hipLaunchKernelGGL(k_ref_hulls_fallback, dim3(fb_threads(n) / 256), dim3(256), 0, s, img, fin, g,
-                                   resolution[dim], skips(dim), w.hulls, w.counts, w.ovf, w.n_ovf, w.stk, cap);
+                const int lanes = chord_lds_lanes(cap);
+                if (lanes > 0) {
+                    const int64_t blocks = std::min<int64_t>((lines + lanes - 1) / lanes, kChordMaxBlocks);
+                    hipLaunchKernelGGL(k_ref_hulls_fallback<true>, dim3((unsigned)blocks), dim3(lanes),
+                                       chord_lds_bytes(cap, lanes), s, img, fin, g, resolution[dim], skips(dim),
+                                       w.hulls, w.counts, w.ovf, w.n_ovf, nullptr, cap);
+                } else {
+                    hipLaunchKernelGGL(k_ref_hulls_fallback<false>, dim3(fb_threads(n) / kWave), dim3(kWave), 0, s,
+                                       img, fin, g, resolution[dim], skips(dim), w.hulls, w.counts, w.ovf, w.n_ovf,
+                                       w.stk, cap);
+                }
             }
-            if (int e = check_launch("reference hulls fallback")) return e;
+            if (int e = check_launch("reference hulls")) return e;
         }
         const RefHull *hl = periodic[dim] ? w.hulls : nullptr;
+        const float r = resolution[dim];
+        const ChordLists L{w.ca, w.cc, w.n_ovf + 1, w.n_ovf + 2, w.a_cap, w.c_cap};
         {
             ProfScope ps(dim == 2 ? "edt_ref_parabolic_rows" : "edt_ref_parabolic", s);
-            hipLaunchKernelGGL(k_ref_parabolic, dim3(nb), dim3(kWave), 0, s, fin, fout, g, resolution[dim], hl,
-                               w.counts, sq, w.ovf, w.n_ovf + 1);
+            if (sq)
+                hipLaunchKernelGGL((k_ref_parabolic<kRingSlots, true>), dim3(nb), dim3(kWave), 0, s, fin, fout, g, r,
+                                   hl, w.counts, L);
+            else
+                hipLaunchKernelGGL((k_ref_parabolic<kRingSlots, false>), dim3(nb), dim3(kWave), 0, s, fin, fout, g,
+                                   r, hl, w.counts, L);
         }
         if (int e = check_launch("reference parabolic")) return e;
+        // runs that outgrew the ring: stacks of kChordCap entries, longer
+        // runs (and whole lines from a full list) with stacks of a line
+        auto chords = [&](const Chord *in, const int *n_in, int in_cap, Chord *out, int *n_out, int ccap) {
+            const int lanes = chord_lds_lanes(ccap);
+            if (lanes > 0) {
+                const int64_t blocks = std::min<int64_t>((in_cap + lanes - 1) / lanes, kChordMaxBlocks);
+                const size_t lds = chord_lds_bytes(ccap, lanes);
+                if (sq)
+                    hipLaunchKernelGGL((k_ref_chords<true, true>), dim3((unsigned)blocks), dim3(lanes), lds, s, fin,
+                                       fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap, nullptr);
+                else
+                    hipLaunchKernelGGL((k_ref_chords<false, true>), dim3((unsigned)blocks), dim3(lanes), lds, s, fin,
+                                       fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap, nullptr);
+            } else {
+                // fb_threads(n) lanes of the grid share w.stk (stacks of a line)
+                const int64_t blocks = fb_threads(n) / kWave;
+                if (sq)
+                    hipLaunchKernelGGL((k_ref_chords<true, false>), dim3((unsigned)blocks), dim3(kWave), 0, s, fin,
+                                       fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap, w.stk);
+                else
+                    hipLaunchKernelGGL((k_ref_chords<false, false>), dim3((unsigned)blocks), dim3(kWave), 0, s, fin,
+                                       fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap, w.stk);
+            }
+        };
         {
-            ProfScope ps("edt_ref_parabolic_fallback", s);
-            hipLaunchKernelGGL(k_ref_parabolic_fallback, dim3(fb_threads(n) / 256), dim3(256), 0, s, fin, fout, g,
-                               resolution[dim], hl, w.counts, sq, w.ovf, w.n_ovf + 1, w.stk, cap);
+            ProfScope ps("edt_ref_chords", s);
+            chords(L.a, L.n_a, L.a_cap, L.c, L.n_c, std::min(kChordCap, cap));
         }
-        if (int e = check_launch("reference parabolic fallback")) return e;
+        if (int e = check_launch("reference parabolic runs")) return e;
+        {
+            ProfScope ps("edt_ref_chords_long", s);
+            chords(L.c, L.n_c, L.c_cap, nullptr, nullptr, cap);
+        }
+        if (int e = check_launch("reference parabolic long runs")) return e;
         ++step;
     }
     return PMX_OK;

@@ -63,7 +63,7 @@ constexpr int kLdsCap = PMX_REF_LDS_CAP;
 #endif
 // LDS ring entries per lane of the streaming parabolic pass
 constexpr int kRingSlots = PMX_REF_RING;
-constexpr int kChunk = 16;       // input positions staged per lane at a time
+constexpr int kChunk = 8;        // input positions staged per lane at a time
 constexpr int kEvalPerStep = 2;  // positions evaluated per input position
 constexpr int kFbMaxThreads = 32768;   // persistent lanes of the fallback kernels, at most
 
@@ -221,12 +221,15 @@ __global__ __launch_bounds__(256) void k_ref_pass0(const uint8_t *__restrict__ i
 // division at that margin), and 2 * factor1 is exact in float below 2^127; so
 // the float division (correctly rounded in HIP) gives the same bits, with the
 // double route kept for a factor1 near the top of the float range.
+// kFloatDiv: the caller knows |factor1| < 2^125 (vertices less than 3n apart,
+// 3n * aniso below it), so the check is left out.
+template <bool kFloatDiv = false>
 __host__ __device__ __forceinline__ float intersect(float f_r, float r, float f_q, float q, float aniso)
 {
     const float factor1 = (r - q) * aniso;
     const float factor2 = r + q;
     const float num = f_r - f_q + factor1 * factor2;
-    if (fabsf(factor1) < 0x1p125f) return num / (2.0f * factor1);
+    if (kFloatDiv || fabsf(factor1) < 0x1p125f) return num / (2.0f * factor1);
     return (float)((double)num / (2.0 * (double)factor1));
 }
 
@@ -244,10 +247,10 @@ struct Stack {
 // update_hull (_distance.hpp:89-112); false when entry k+1 would reach cap
 __host__ __device__ __forceinline__ bool update_hull(const Stack &S, int &k, int i, float fv, float aniso, int cap)
 {
-    float s = intersect(fv, (float)i, S.H(k), (float)S.V(k), aniso);
+    float s = intersect<>(fv, (float)i, S.H(k), (float)S.V(k), aniso);
     while (k > 0 && s <= S.R(k)) {
         --k;
-        s = intersect(fv, (float)i, S.H(k), (float)S.V(k), aniso);
+        s = intersect<>(fv, (float)i, S.H(k), (float)S.V(k), aniso);
     }
     ++k;
     if (k + 1 >= cap) return false;
@@ -389,12 +392,22 @@ __global__ __launch_bounds__(kWave) void k_ref_hulls_fallback(const uint8_t *__r
 // A lane's stack for the streaming pass: entry k lives at slot (k & mask),
 // slot s at base[s * lanes]; with a power-of-two ring (mask = slots - 1)
 // entries below the evaluation cursor are overwritten as the line advances.
+// Vertices are held as v + voff in 16 bits (voff = n: lower-hull vertices
+// lie in [-n, 0), upper-hull ones in [n, 2n), so v + n < 3n < 2^16).
 struct RingStack {
-    int *v;
+    unsigned short *v;
     float *h, *r;
     int lanes;
     unsigned mask;
+    int voff;
     __host__ __device__ __forceinline__ int at(int k) const { return (int)((unsigned)k & mask) * lanes; }
+    __host__ __device__ __forceinline__ int V(int o) const { return (int)v[o] - voff; }
+    __host__ __device__ __forceinline__ void set(int o, int vert, float hv, float rv) const
+    {
+        v[o] = (unsigned short)(vert + voff);
+        h[o] = hv;
+        r[o] = rv;
+    }
 };
 
 // Positions [p1, p2) of a line, between zero-height parabolas at p1 and p2;
@@ -431,34 +444,31 @@ struct Chord {
 // its value aniso * (i - v)^2 + h, as in the reference, so each value is
 // bit-identical.  All lanes of a wave run the call together (the chunk loop
 // is uniform); a lane with alive = false follows along without effect.
-template <bool kSqrt, class Emit>
+template <bool kSqrt, bool kFloatDiv, class Emit>
 __host__ __device__ void parabolic_stream(const float *fin, float *fout, int n, int64_t ist, int64_t ost,
                                           float aniso, const RefHull *lower, int nl, const RefHull *upper, int nu,
                                           const RingStack &S, int cap, bool alive, float *stage, int lanes,
                                           Emit &&emit)
 {
     int k = 0, loop_start = 1;
-    int tv = 0;  // top entry k: vertex, height, range
-    float th = 0.0f, tr = -INFINITY;
+    int tv = 0, bv = 0;  // top entry k and the entry k - 1 below it
+    float th = 0.0f, tr = -INFINITY, bh = 0.0f, br = -INFINITY;
     if (alive) {
         if (nl > 0) {
             loop_start = 0;
-            for (int m = nl - 1, j = 0; m >= 0; --m, ++j) {
-                const int o = S.at(j);
-                S.v[o] = lower[m].vertex;
-                S.h[o] = lower[m].height;
-                S.r[o] = lower[m].range;
-            }
+            for (int m = nl - 1, j = 0; m >= 0; --m, ++j) S.set(S.at(j), lower[m].vertex, lower[m].height, lower[m].range);
             k = nl - 1;
             tv = lower[0].vertex;
             th = lower[0].height;
             tr = lower[0].range;
+            if (nl > 1) {
+                bv = lower[1].vertex;
+                bh = lower[1].height;
+                br = lower[1].range;
+            }
         } else {
             th = fin[0];
-            const int o = S.at(0);
-            S.v[o] = 0;
-            S.h[o] = th;
-            S.r[o] = -INFINITY;
+            S.set(S.at(0), 0, th, -INFINITY);
         }
     }
     // evaluation cursor: next position pe (output op), its covering
@@ -472,7 +482,7 @@ __host__ __device__ void parabolic_stream(const float *fin, float *fout, int n, 
         while (er < (float)pe) {
             ++kk;
             const int o = S.at(kk);
-            ev = S.v[o];
+            ev = S.V(o);
             eh = S.h[o];
             er = kk < k ? S.r[S.at(kk + 1)] : INFINITY;
         }
@@ -482,38 +492,53 @@ __host__ __device__ void parabolic_stream(const float *fin, float *fout, int n, 
         op += ost;
         ++pe;
     };
-    // update_hull (_distance.hpp:89-112); false when the ring is full
-    auto push = [&](int i, float fv) -> bool {
-        float s = intersect(fv, (float)i, th, (float)tv, aniso);
-        while (k > 0 && s <= tr) {
-            --k;
-            const int o = S.at(k);
-            tv = S.v[o];
-            th = S.h[o];
-            tr = S.r[o];
-            s = intersect(fv, (float)i, th, (float)tv, aniso);
+    // update_hull (_distance.hpp:89-112).  A pop takes the entry below from
+    // registers and refills that copy from the ring, so the next pop test
+    // does not wait for the ring.  (The copy can hold an overwritten slot
+    // only when the top is the last zero's entry, which is never popped.)
+    auto pop = [&]() {
+        --k;
+        tv = bv;
+        th = bh;
+        tr = br;
+        if (k > 0) {
+            const int o = S.at(k - 1);
+            bv = S.V(o);
+            bh = S.h[o];
+            br = S.r[o];
         }
-        if (k + 1 - kk >= cap) {
+    };
+    // room for one more entry, after evaluating what is final if needed
+    auto room = [&]() {
+        if (k + 1 - kk >= cap)
             while (pe < zl && k + 1 - kk >= cap) eval();
-            if (k + 1 - kk >= cap) return false;
-        }
+        return k + 1 - kk < cap;
+    };
+    auto push = [&](int i, float fv, float s) {
         ++k;
+        bv = tv;
+        bh = th;
+        br = tr;
         tv = i;
         th = fv;
         tr = s;
-        const int o = S.at(k);
-        S.v[o] = i;
-        S.h[o] = fv;
-        S.r[o] = s;
+        S.set(S.at(k), i, fv, s);
+    };
+    // false when the ring is full
+    auto insert = [&](int i, float fv) -> bool {
+        float s = intersect<kFloatDiv>(fv, (float)i, th, (float)tv, aniso);
+        while (k > 0 && s <= tr) {
+            pop();
+            s = intersect<kFloatDiv>(fv, (float)i, th, (float)tv, aniso);
+        }
+        if (!room()) return false;
+        push(i, fv, s);
         return true;
     };
     // a fresh stack holding the zero at i alone
     auto restart = [&](int i) {
         k = kk = 0;
-        const int o = S.at(0);
-        S.v[o] = i;
-        S.h[o] = 0.0f;
-        S.r[o] = -INFINITY;
+        S.set(S.at(0), i, 0.0f, -INFINITY);
         tv = ev = i;
         th = eh = 0.0f;
         tr = -INFINITY;
@@ -537,9 +562,11 @@ __host__ __device__ void parabolic_stream(const float *fin, float *fout, int n, 
             }
         }
         const int m = n - i0 < kChunk ? n - i0 : kChunk;
+        float fnext = stage[0];
         for (int c = 0; c < m; ++c) {
             const int i = i0 + c;
-            const float fv = stage[c * lanes];
+            const float fv = fnext;
+            if (c + 1 < m) fnext = stage[(c + 1) * lanes];
             if (alive && i >= loop_start) {
                 if (skip) {
                     if (fv == 0.0f) {
@@ -547,7 +574,7 @@ __host__ __device__ void parabolic_stream(const float *fin, float *fout, int n, 
                         alive = emit(p1, i);
                         restart(i);
                     }
-                } else if (!push(i, fv)) {
+                } else if (!insert(i, fv)) {
                     if (fv == 0.0f) {
                         alive = emit(zl, i);
                         restart(i);
@@ -571,7 +598,7 @@ __host__ __device__ void parabolic_stream(const float *fin, float *fout, int n, 
     }
     if (alive && !skip) {
         for (int m = 0; m < nu; ++m) {
-            if (!push(upper[m].vertex + n, upper[m].height)) {
+            if (!insert(upper[m].vertex + n, upper[m].height)) {
                 skip = true;
                 p1 = zl;
                 break;
@@ -662,25 +689,29 @@ __device__ __forceinline__ LineHulls hulls_of(const RefHull *hulls, const int *c
     return h;
 }
 
-// Run lists between the parabolic kernels: `a` from the ring pass (runs that
-// outgrew the ring), `c` for the last kernel (runs too long for the second
-// and whole lines once `a` is full); c_cap = lines + a_cap, so `c` cannot
-// fill up.
+// Run lists between the parabolic kernels.  Each workgroup of the ring pass
+// collects its runs in its own segment of kSegChords slots (counted in LDS,
+// so the lanes do not contend for one counter), then reserves a range of the
+// dense list `a` with one atomic and copies them there.  `c` takes, with one
+// counter, the runs too long for the second kernel and the whole lines of a
+// workgroup whose segment is full; c_cap = lines + every slot of `a`, so `c`
+// cannot fill up.
+constexpr int kSegChords = kWave * 4;
 struct ChordLists {
-    Chord *a, *c;
+    Chord *seg, *a, *c;
     int *n_a, *n_c;
     int a_cap, c_cap;
 };
 
 // one wave per workgroup, one line per lane, stacks in an LDS ring of
 // kRingSlots entries per lane
-template <int kRingSlots, bool kSqrt>
+template <int kRingSlots, bool kSqrt, bool kFloatDiv>
 __global__ __launch_bounds__(kWave) void k_ref_parabolic(const float *__restrict__ fin, float *__restrict__ fout,
                                                          LineGeom g, float res, const RefHull *__restrict__ hulls,
                                                          const int *__restrict__ counts, ChordLists L)
 {
     static_assert((kRingSlots & (kRingSlots - 1)) == 0, "ring slots must be a power of two");
-    __shared__ int sv[kRingSlots * kWave];
+    __shared__ unsigned short sv[kRingSlots * kWave];
     __shared__ float sh[kRingSlots * kWave], sr[kRingSlots * kWave], st[kChunk * kWave];
     const int lane = threadIdx.x;
     const int64_t lines = g.lines();
@@ -691,23 +722,33 @@ __global__ __launch_bounds__(kWave) void k_ref_parabolic(const float *__restrict
     g.base(line, &ib, &fb, &ob);
     const LineHulls h = hulls_of(hulls, counts, line);
     const int n = (int)g.n[g.dim];
-    const RingStack S{sv + lane, sh + lane, sr + lane, kWave, (unsigned)(kRingSlots - 1)};
+    const RingStack S{sv + lane, sh + lane, sr + lane, kWave, (unsigned)(kRingSlots - 1), n};
+    __shared__ int seg_n;
+    if (lane == 0) seg_n = 0;
+    __syncthreads();
+    Chord *seg = L.seg + (size_t)blockIdx.x * kSegChords;
     auto emit = [&](int p1, int p2) -> bool {
-        const int j = atomicAdd(L.n_a, 1);
-        if (j < L.a_cap) {
-            L.a[j] = Chord{(int)line, p1, p2};
+        const int j = atomicAdd(&seg_n, 1);
+        if (j < kSegChords) {
+            seg[j] = Chord{(int)line, p1, p2};
             return true;
         }
         L.c[atomicAdd(L.n_c, 1)] = Chord{(int)line, -1, n};
         return false;
     };
-    parabolic_stream<kSqrt>(fin + fb, fout + ob, n, g.fs[g.dim], g.os[g.dim], res * res, h.lower, h.nl, h.upper,
+    parabolic_stream<kSqrt, kFloatDiv>(fin + fb, fout + ob, n, g.fs[g.dim], g.os[g.dim], res * res, h.lower, h.nl, h.upper,
                             h.nu, S, kRingSlots, real, st + lane, kWave, emit);
+    __syncthreads();  // the segment's writes are visible to the workgroup
+    __shared__ int base;
+    const int cnt = min(seg_n, kSegChords);
+    if (lane == 0) base = cnt ? atomicAdd(L.n_a, cnt) : 0;
+    __syncthreads();
+    for (int t = lane; t < cnt; t += kWave) L.a[base + t] = seg[t];
 }
 
-// The runs of a list, one per lane, each with a stack of `cap` entries (see
-// lane_stack).  Runs that need more than cap entries go to `out` (non-null
-// whenever cap is below a whole line's).
+// The runs of a list (n_in[0] of them, at most in_cap), one per lane, each
+// with a stack of `cap` entries (see lane_stack).  Runs that need more than
+// cap entries go to `out` (non-null whenever cap is below a whole line's).
 template <bool kSqrt, bool kLds>
 __global__ __launch_bounds__(kWave) void k_ref_chords(const float *__restrict__ fin, float *__restrict__ fout,
                                                       LineGeom g, float res, const RefHull *__restrict__ hulls,
@@ -777,7 +818,6 @@ constexpr size_t kChordLdsBytes = 48 * 1024;
 size_t chord_lds_bytes(int cap, int lanes) { return (size_t)lanes * (size_t)cap * 12; }
 int chord_lds_lanes(int cap) { return (int)std::min<size_t>(kWave, kChordLdsBytes / chord_lds_bytes(cap, 1)); }
 constexpr int kChordCap = 128;       // stack entries per lane of the second parabolic kernel
-constexpr int kChordsPerLine = 4;    // room in the ring pass's run list, per line
 constexpr int kChordMaxBlocks = 4096;
 
 struct RefWs {
@@ -785,7 +825,7 @@ struct RefWs {
     int *stk;
     RefHull *hulls;
     int *counts, *ovf, *n_ovf;
-    Chord *ca, *cc;
+    Chord *cseg, *ca, *cc;
     int a_cap, c_cap;
 };
 
@@ -800,8 +840,9 @@ RefWs carve_ref(Carver &c, const int64_t *n)
     w.counts = c.take<int>((size_t)lines * 2);
     w.ovf = c.take<int>((size_t)lines);
     w.n_ovf = c.take<int>(4);
-    w.a_cap = (int)std::min<int64_t>(lines * kChordsPerLine, INT32_MAX / 2);
+    w.a_cap = (int)((lines + kWave - 1) / kWave * kSegChords);
     w.c_cap = (int)(lines + w.a_cap);
+    w.cseg = c.take<Chord>((size_t)w.a_cap);
     w.ca = c.take<Chord>((size_t)w.a_cap);
     w.cc = c.take<Chord>((size_t)w.c_cap);
     return w;
@@ -912,39 +953,48 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
         }
         const RefHull *hl = periodic[dim] ? w.hulls : nullptr;
         const float r = resolution[dim];
-        const ChordLists L{w.ca, w.cc, w.n_ovf + 1, w.n_ovf + 2, w.a_cap, w.c_cap};
+        const ChordLists L{w.cseg, w.ca, w.cc, w.n_ovf + 1, w.n_ovf + 2, w.a_cap, w.c_cap};
         {
             ProfScope ps(dim == 2 ? "edt_ref_parabolic_rows" : "edt_ref_parabolic", s);
+            // vertices lie in [-n, 2n): float division needs 3n * r^2 < 2^125
+            const bool fdiv = 3.0 * (double)g.n[dim] * (double)(r * r) < 0x1p125;
+            auto launch = [&](auto kernel) {
+                hipLaunchKernelGGL(kernel, dim3(nb), dim3(kWave), 0, s, fin, fout, g, r, hl, w.counts, L);
+            };
             if (sq)
-                hipLaunchKernelGGL((k_ref_parabolic<kRingSlots, true>), dim3(nb), dim3(kWave), 0, s, fin, fout, g, r,
-                                   hl, w.counts, L);
+                fdiv ? launch(k_ref_parabolic<kRingSlots, true, true>) : launch(k_ref_parabolic<kRingSlots, true, false>);
             else
-                hipLaunchKernelGGL((k_ref_parabolic<kRingSlots, false>), dim3(nb), dim3(kWave), 0, s, fin, fout, g,
-                                   r, hl, w.counts, L);
+                fdiv ? launch(k_ref_parabolic<kRingSlots, false, true>)
+                     : launch(k_ref_parabolic<kRingSlots, false, false>);
         }
         if (int e = check_launch("reference parabolic")) return e;
         // runs that outgrew the ring: stacks of kChordCap entries, longer
         // runs (and whole lines from a full list) with stacks of a line
         auto chords = [&](const Chord *in, const int *n_in, int in_cap, Chord *out, int *n_out, int ccap) {
             const int lanes = chord_lds_lanes(ccap);
+            const int64_t slots = in_cap;
             if (lanes > 0) {
-                const int64_t blocks = std::min<int64_t>((in_cap + lanes - 1) / lanes, kChordMaxBlocks);
+                const int64_t blocks = std::min<int64_t>((slots + lanes - 1) / lanes, kChordMaxBlocks);
                 const size_t lds = chord_lds_bytes(ccap, lanes);
                 if (sq)
                     hipLaunchKernelGGL((k_ref_chords<true, true>), dim3((unsigned)blocks), dim3(lanes), lds, s, fin,
-                                       fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap, nullptr);
+                                       fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap,
+                                       nullptr);
                 else
                     hipLaunchKernelGGL((k_ref_chords<false, true>), dim3((unsigned)blocks), dim3(lanes), lds, s, fin,
-                                       fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap, nullptr);
+                                       fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap,
+                                       nullptr);
             } else {
                 // fb_threads(n) lanes of the grid share w.stk (stacks of a line)
                 const int64_t blocks = fb_threads(n) / kWave;
                 if (sq)
                     hipLaunchKernelGGL((k_ref_chords<true, false>), dim3((unsigned)blocks), dim3(kWave), 0, s, fin,
-                                       fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap, w.stk);
+                                       fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap,
+                                       w.stk);
                 else
                     hipLaunchKernelGGL((k_ref_chords<false, false>), dim3((unsigned)blocks), dim3(kWave), 0, s, fin,
-                                       fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap, w.stk);
+                                       fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap,
+                                       w.stk);
             }
         };
         {

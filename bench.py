@@ -47,8 +47,21 @@ def traffic_for(kernel: str, n: int):
     return None if k is None else k["hbm_bytes_per_launch"]
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def host_threads() -> int:
-    """Host cores this process may use, at most 16 (one GPU's share of the box)."""
+    """Host cores this process may use, at most 16: one GPU's share of the
+    host on the measurement machines (a one-GPU slice of an eight-GPU node),
+    whatever the node's own core count."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -68,23 +81,27 @@ def cpu_baseline(img_t, edge: int, threads: int):
     from oracle import oracle as O
 
     use_ref = O.ref_lib() is not None
-    crop = np.ascontiguousarray(img_t[:edge, :edge, :edge].cpu().numpy())
+    # the bench domain's faces: x END, y/z WALL (a solid layer on each y, z face)
+    raw = img_t[:edge, :edge, :edge].cpu().numpy()
+    wall = ((0, 0), (1, 1), (1, 1))
+    crop = np.ascontiguousarray(np.pad(raw, wall))
     t0 = time.perf_counter()
     O.ref_edt_sq_threads(crop, threads, use_ref=use_ref)
     dt = time.perf_counter() - t0
     e1 = max(1, edge // 2)
-    crop1 = np.ascontiguousarray(crop[:e1, :e1, :e1])
+    crop1 = np.ascontiguousarray(np.pad(raw[:e1, :e1, :e1], wall))
     t0 = time.perf_counter()
     O.ref_edt_sq(crop1, use_ref=use_ref)
     dt1 = time.perf_counter() - t0
     return {
-        "value": round(crop.size / dt / 1e9, 6),
+        "value": round(raw.size / dt / 1e9, 6),
         "unit": "Gvoxel/s",
         "cores": threads,
         "kind": "reference" if use_ref else "port",
         "sample": f"squared EDT (reference pass order, float32) of the {edge}^3 corner crop of the "
-                  f"same pack, non-periodic, {dt:.2f} s on {threads} host threads",
-        "single_core_value": round(crop1.size / dt1 / 1e9, 6),
+                  f"same pack with the bench's faces (x END, y/z WALL), {dt:.2f} s on {threads} host "
+                  f"threads of a {cpu_model()}",
+        "single_core_value": round(e1 ** 3 / dt1 / 1e9, 6),
         "single_core_sample": f"{e1}^3 corner crop, {dt1:.2f} s on one host core",
     }
 
@@ -173,6 +190,36 @@ def periodic_512_leg(reps: int):
 
     res = {"workload": "512^3 overlapping-sphere pack (radii U(4,12), porosity 0.50, seed 2), periodic x/y/z"}
     vox = float(n) ** 3
+    # the drop-in edt3d on PMMoTo's unit box (resolution 1/512, periodic):
+    # the reference's float32 arithmetic with its five-parabola periodic
+    # hulls, distances (sqrt) as float32
+    dout = torch.empty((n, n, n), dtype=torch.float32, device="cuda")
+    rres = (1.0 / n,) * 3
+
+    def edt_ref():
+        E.run_edt_ref(img, (0, 0, 0), (n, n, n), 0, (True, True, True), rres, True, out=dout)
+
+    edt_ref()
+    torch.cuda.synchronize()
+    _lib.profile_begin()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        edt_ref()
+    ev1.record()
+    torch.cuda.synchronize()
+    prof = _lib.profile_end()
+    ms = ev0.elapsed_time(ev1) / reps
+    res["edt3d_reference_arithmetic"] = {
+        "what": "edt3d(periodic=(True,)*3, resolution=1/512): float32 distances bit-identical to the reference",
+        "ms": round(ms, 4),
+        "gvoxel_per_s": round(vox / (ms * 1e-3) / 1e9, 2),
+        # u8 in + f32 out of pass 0, f32 in/out of the two parabolic passes
+        "hbm_frac_at_algorithmic_bytes": round(21 * vox / (ms * 1e-3) / 8e12, 4),
+        "bound": "issue (per-line float32 envelope arithmetic, one line per lane)",
+        "kernels_ms": {k: round(t / reps, 3) for k, (t, c) in prof.items()},
+    }
+    del dout
     for name, fn, bpv in (("erode_r5", erode, 18), ("dilate_r5", dilate, 18), ("ccl26", label, 5)):
         fn()
         torch.cuda.synchronize()

@@ -20,7 +20,10 @@
 namespace pmx {
 namespace {
 
-constexpr uint32_t kNotJoined = 0xff;  // in_c: not (yet) in the inlet-connected set; levels are < 250
+// in_c of a voxel not (yet) in the inlet-connected set (levels are < 250):
+constexpr uint32_t kInactive = 0xff;  // pore voxel not yet in the eroded set
+constexpr uint32_t kNever = 0xfe;     // in no eroded set of the sweep (solid, or too thin)
+constexpr uint32_t kOpen = 0xfd;      // in the eroded set, not yet joined
 constexpr int kNoLevel = 256;          // above every level: no neighbour there
 
 __device__ __forceinline__ int ld(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
@@ -108,7 +111,7 @@ __global__ __launch_bounds__(256) void k_levels(const int32_t *__restrict__ d2, 
         }
         level[i] = (uint8_t)lo;
         flag[i] = 0;
-        in_c[i] = kNotJoined;
+        in_c[i] = lo < n_levels ? kInactive : kNever;
         if (lo < n_levels) atomicAdd(hist + lo, 1u);
     }
     __syncthreads();
@@ -163,13 +166,14 @@ __global__ __launch_bounds__(256) void k_bucket(const uint8_t *__restrict__ leve
 // read on the device, so the sweep issues a fixed sequence of launches
 // with no host synchronisation (it can be captured in a graph).
 __global__ __launch_bounds__(256) void k_activate(const int *__restrict__ order, const unsigned int *__restrict__ offsets,
-                                                  int lv, int *__restrict__ parent)
+                                                  int lv, int *__restrict__ parent, uint8_t *__restrict__ in_c)
 {
     const unsigned int begin = offsets[lv], end = offsets[lv + 1];
     for (int64_t k = (int64_t)begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < end;
          k += (int64_t)gridDim.x * blockDim.x) {
         const int v = order[k];
         parent[v] = v;
+        in_c[v] = kOpen;
     }
 }
 
@@ -273,28 +277,26 @@ __device__ __forceinline__ uint32_t eq_bytes(uint32_t w, uint32_t pat)
 }
 
 // in_c[v] = the level at which v joined the inlet-connected eroded set
-// (kNotJoined: not yet; the sets only grow).  A pore voxel left unassigned
+// (kOpen: in the eroded set, not yet joined; the sets only grow).  A pore voxel left unassigned
 // by level i-1 is farther than r_{i-1} > r_i from every older member, so
 // only the members that join at level i can invade it at level i: the
 // level's dilation takes the voxels with in_c == i as its centres (a byte
 // match in its pass Z), and no per-level mask is written.  any: set when the
 // level has a new member (otherwise its dilation is skipped on the GPU).
 //
-// 16 voxels per lane step (in_c and level are 256-byte aligned workspace
-// slices).  A member stays a member (its root stays flagged), so only
-// active voxels not yet joined (level <= lv) need a root lookup; a group of
-// 16 members skips the level read, and only groups with a new member are
-// written.
-__device__ __forceinline__ uint32_t join_byte(int64_t i, uint32_t c, uint32_t l, int lv, int *parent,
-                                              const uint8_t *flag)
+// 16 voxels per lane step (in_c is a 256-byte aligned workspace slice).  A
+// member stays a member (its root stays flagged), so only kOpen voxels need
+// a root lookup; a group of 16 with none is done after reading in_c, and
+// only groups with a new member are written.
+__device__ __forceinline__ uint32_t join_byte(int64_t i, uint32_t c, int lv, int *parent, const uint8_t *flag)
 {
-    if (c != kNotJoined || (int)l > lv) return c;
+    if (c != kOpen) return c;
     return flag[find_root<true>(parent, (int)i)] ? (uint32_t)lv : c;
 }
 
-__global__ __launch_bounds__(256) void k_mask(int64_t n, const uint8_t *__restrict__ level, int lv,
-                                              int *__restrict__ parent, const uint8_t *__restrict__ flag,
-                                              uint8_t *__restrict__ in_c, unsigned int *__restrict__ any)
+__global__ __launch_bounds__(256) void k_mask(int64_t n, int lv, int *__restrict__ parent,
+                                              const uint8_t *__restrict__ flag, uint8_t *__restrict__ in_c,
+                                              unsigned int *__restrict__ any)
 {
     __shared__ int blk_seen;
     if (threadIdx.x == 0) blk_seen = 0;
@@ -303,15 +305,14 @@ __global__ __launch_bounds__(256) void k_mask(int64_t n, const uint8_t *__restri
     const int64_t nv = n / 16, stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint4 *c4 = reinterpret_cast<uint4 *>(in_c);
-    const uint4 *l4 = reinterpret_cast<const uint4 *>(level);
+    constexpr uint32_t kOpen4 = kOpen * 0x01010101u;
     for (int64_t q = tid; q < nv; q += stride) {
         const uint4 c = c4[q];
-        const uint32_t open = eq_bytes(c.x, 0xffffffffu) | eq_bytes(c.y, 0xffffffffu) | eq_bytes(c.z, 0xffffffffu) |
-                              eq_bytes(c.w, 0xffffffffu);
-        if (!open) continue;  // all 16 joined at earlier levels
-        const uint4 l = l4[q];
-        const uint32_t cw[4] = {c.x, c.y, c.z, c.w}, lw[4] = {l.x, l.y, l.z, l.w};
-        // the group's candidates (not joined, level <= lv) walk to their roots in lockstep,
+        const uint32_t open = eq_bytes(c.x, kOpen4) | eq_bytes(c.y, kOpen4) | eq_bytes(c.z, kOpen4) |
+                              eq_bytes(c.w, kOpen4);
+        if (!open) continue;  // no voxel of the 16 waits to join
+        const uint32_t cw[4] = {c.x, c.y, c.z, c.w};
+        // the group's candidates (kOpen) walk to their roots in lockstep,
         // up to 16 independent parent loads in flight; each candidate's entry is then set to
         // its root (no union runs here, so the root is a valid ancestor), which keeps the
         // walks of the voxels that stay unjoined short at later levels
@@ -319,9 +320,9 @@ __global__ __launch_bounds__(256) void k_mask(int64_t n, const uint8_t *__restri
         uint32_t cand = 0;
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
-            const uint32_t cb = (cw[e >> 2] >> (8 * (e & 3))) & 0xffu, lb = (lw[e >> 2] >> (8 * (e & 3))) & 0xffu;
+            const uint32_t cb = (cw[e >> 2] >> (8 * (e & 3))) & 0xffu;
             r[e] = (int)(16 * q + e);
-            if (cb == kNotJoined && (int)lb <= lv) cand |= 1u << e;
+            if (cb == kOpen) cand |= 1u << e;
         }
         uint32_t walking = cand;
         while (walking) {
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(256) void k_mask(int64_t n, const uint8_t *__restri
         }
     }
     for (int64_t i = 16 * nv + tid; i < n; i += stride) {
-        const uint32_t c = in_c[i], f = join_byte(i, c, level[i], lv, parent, flag);
+        const uint32_t c = in_c[i], f = join_byte(i, c, lv, parent, flag);
         if (f != c) {
             in_c[i] = (uint8_t)f;
             seen = true;
@@ -535,7 +536,7 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
         if (has_inlet) {
             {
                 ProfScope ps("psd_union", s);
-                hipLaunchKernelGGL(k_activate, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, p.parent);
+                hipLaunchKernelGGL(k_activate, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, p.parent, p.in_c);
                 hipLaunchKernelGGL(k_merge_new, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, d, p.level,
                                    p.parent);
             }
@@ -543,8 +544,8 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
             const int64_t ni = inlet_ext[0] * inlet_ext[1] * inlet_ext[2];
             hipLaunchKernelGGL(k_flag_inlet, dim3(grid_for(ni)), dim3(256), 0, s, d, inlet_lo[0], inlet_lo[1],
                                inlet_lo[2], inlet_ext[0], inlet_ext[1], inlet_ext[2], p.level, lv, p.parent, p.flag);
-            hipLaunchKernelGGL(k_mask, dim3(grid_for(n / 16 + 1)), dim3(256), 0, s, n, p.level, lv, p.parent, p.flag,
-                               p.in_c, p.any + lv);
+            hipLaunchKernelGGL(k_mask, dim3(grid_for(n / 16 + 1)), dim3(256), 0, s, n, lv, p.parent, p.flag, p.in_c,
+                               p.any + lv);
         }
         if (int rc = check_launch("psd level")) return rc;
         // dilation of the level's new centres with the first-invasion update

@@ -294,20 +294,24 @@ def batched_sweep(n_domains: int, reps: int):
     return out
 
 
-def surrogate_leg(n: int, tau: float, reps: int):
+def surrogate_leg(n: int, targets, reps: int):
     """Config-5 style comparison on n held-out 128^3 packs (seeds 6000000 +
     i): the trained surrogate at inference batch n, with domains whose
-    predicted sigma exceeds tau recomputed by the exact sweep, against the
-    exact sweep of all n (a captured batched graph, 32 radii 32..1)."""
+    predicted sigma exceeds the threshold calibrated for each target error
+    (or whose porosity is outside the training range) recomputed by the
+    exact sweep, against the exact sweep of all n (a captured batched graph,
+    32 radii 32..1).  Also: how the per-curve largest error and sigma relate
+    on this set."""
     import numpy as np
     import torch
 
     from pmmoto_amd.filters.batch import BatchDomain, BatchSweep, batch_saturation
+    from pmmoto_amd.surrogate.calibrate import curve_errors, load_calibration, report, tau_for
     from pmmoto_amd.surrogate.data import RADII, make_dataset
     from pmmoto_amd.surrogate.train import SurrogatePorosimetry, load, mae
 
     wpath = os.path.join(ROOT, "pmmoto_amd", "surrogate", "weights_128.npz")
-    if not os.path.exists(wpath):
+    if not os.path.exists(wpath) or load_calibration() is None:
         return None
     net = load(wpath)
     x, y, bcs = make_dataset(n, 6_000_000, size=net.size)
@@ -323,34 +327,45 @@ def surrogate_leg(n: int, tau: float, reps: int):
     exact = np.stack([batch_saturation(m, x[i], len(RADII)) for i, m in enumerate(maps)])
     del sweep, maps
     torch.cuda.empty_cache()
-    # surrogate: network at batch n plus the exact fallbacks
-    sp = SurrogatePorosimetry(net, tau)
-    sp.curves(x, bcs, batch=n)  # warm-up
+    net.predict(x, batch=n)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
-        curve, sigma, fall = sp.curves(x, bcs, batch=n)
-    torch.cuda.synchronize()
-    sur_s = (time.perf_counter() - t0) / reps
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        net.predict(x, batch=n)
+        pred, sigma = net.predict(x, batch=n)
     torch.cuda.synchronize()
     net_s = (time.perf_counter() - t0) / reps
+    pred, sigma = pred.cpu().numpy(), sigma.cpu().numpy().astype(np.float64)
+    err = curve_errors(pred, exact)
+    taus = {str(t): tau_for(t) for t in targets}
     out = {
         "workload": f"{n} held-out 128^3 packs (porosity U(0.25,0.6), radii U(4,12), periodic/wall axes), "
                     "32-point saturation curve (radii 32..1)",
         "exact_curves_per_s": round(n / exact_s, 1),
-        "surrogate_curves_per_s": round(n / sur_s, 1),
-        "speedup_vs_exact": round(exact_s / sur_s, 2),
         "network_only_curves_per_s": round(n / net_s, 1),
         "network_tflops": round(n * net.flops_per_sample() / net_s / 1e12, 2),
-        "tau": tau,
-        "fallback_fraction": round(float(len(fall)) / n, 4),
-        "mae_vs_exact": round(mae(curve, exact), 5),
-        "mae_network_only": round(mae(net.predict(x, batch=n)[0].cpu().numpy(), exact), 5),
+        "mae_network_only": round(mae(pred, exact), 5),
         "labels_equal_exact": bool(np.array_equal(exact.astype(np.float32), y)),
+        "held_out": report(sigma, err, {t: (None if v == float("inf") else v) for t, v in taus.items()}),
+        "with_fallback": {},
     }
+    for t in targets:
+        sp = SurrogatePorosimetry.calibrated(net, t)
+        sp.curves(x, bcs, batch=n)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            curve, _, fall = sp.curves(x, bcs, batch=n)
+        torch.cuda.synchronize()
+        sur_s = (time.perf_counter() - t0) / reps
+        e = curve_errors(curve, exact)
+        out["with_fallback"][str(t)] = {
+            "tau": taus[str(t)] if taus[str(t)] != float("inf") else None,
+            "fallback_fraction": round(float(len(fall)) / n, 4),
+            "curves_per_s": round(n / sur_s, 1),
+            "speedup_vs_exact": round(exact_s / sur_s, 2),
+            "max_error": round(float(e.max()), 5),
+            "mae": round(mae(curve, exact), 5),
+        }
     del x, net
     torch.cuda.empty_cache()
     return out
@@ -372,7 +387,8 @@ def main() -> int:
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surrogate-n", type=int, default=512)
-    ap.add_argument("--surrogate-tau", type=float, default=0.02)
+    ap.add_argument("--surrogate-targets", default="0.02,0.05",
+                    help="largest per-point saturation errors whose calibrated fallback thresholds are timed")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy a 1 GiB device buffer 3 times (the known-size "
                          "reference kernel for HBM counter scaling under rocprofv3 --pmc)")
@@ -460,7 +476,7 @@ def main() -> int:
 
     surrogate = None
     if not args.no_surrogate:
-        surrogate = surrogate_leg(args.surrogate_n, args.surrogate_tau, 3)
+        surrogate = surrogate_leg(args.surrogate_n, [float(t) for t in args.surrogate_targets.split(",")], 3)
         print(f"[bench] surrogate: {surrogate}", file=sys.stderr, flush=True)
 
     cpu = None

@@ -16,6 +16,7 @@ import time
 import numpy as np
 import torch
 
+from .calibrate import POROSITY_RANGE, tau_for
 from .data import RADII, exact_curves, make_dataset
 from .model import SurrogateNet
 
@@ -50,18 +51,30 @@ def mae(pred: np.ndarray, want: np.ndarray) -> float:
 
 class SurrogatePorosimetry:
     """Saturation curves from the network; domains whose predicted sigma
-    exceeds tau go to the exact sweep instead (bcs gives their faces)."""
+    exceeds tau, or whose porosity lies outside the training range, go to
+    the exact sweep instead (bcs gives their faces).  ``calibrated`` takes
+    tau from the calibration of a target error (pmmoto_amd.surrogate.calibrate)."""
 
-    def __init__(self, net: SurrogateNet, tau: float, radii=RADII):
+    def __init__(self, net: SurrogateNet, tau: float, radii=RADII, porosity_range=POROSITY_RANGE):
         if len(radii) != net.q:
             raise ValueError("one radius per curve point")
         self.net, self.tau, self.radii = net, tau, tuple(radii)
+        self.porosity_range = porosity_range
+
+    @classmethod
+    def calibrated(cls, net: SurrogateNet, target: float, radii=RADII):
+        return cls(net, tau_for(target), radii)
 
     def curves(self, images, bcs, batch=512):
         curve, sigma = self.net.predict(images, batch=batch)
         curve = curve.cpu().numpy()
         sigma = sigma.cpu().numpy()
-        fall = np.nonzero(sigma > self.tau)[0]
+        out = np.zeros(len(sigma), bool)
+        if self.porosity_range is not None:
+            lo, hi = self.porosity_range
+            por = np.array([float(images[i].float().mean()) for i in range(len(sigma))])
+            out = (por < lo) | (por > hi)
+        fall = np.nonzero((sigma > self.tau) | out)[0]
         if fall.size:
             exact = exact_curves([images[i] for i in fall], [bcs[i] for i in fall], self.radii)
             curve[fall] = exact

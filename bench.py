@@ -230,8 +230,15 @@ def periodic_512_leg(reps: int):
         ev1.record()
         torch.cuda.synchronize()
         ms = ev0.elapsed_time(ev1) / reps
+        # the per-phase split from HIP events in a separate run (the events add a little)
+        _lib.profile_begin()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        prof = _lib.profile_end()
         res[name] = {"ms": round(ms, 4), "gvoxel_per_s": round(vox / (ms * 1e-3) / 1e9, 2),
-                     "hbm_frac_at_algorithmic_bytes": round(bpv * vox / (ms * 1e-3) / 8e12, 4)}
+                     "hbm_frac_at_algorithmic_bytes": round(bpv * vox / (ms * 1e-3) / 8e12, 4),
+                     "kernels_ms": {k: round(t / reps, 4) for k, (t, c) in prof.items()}}
     res["ccl26"]["components"] = int(count.item())
     del img, out, labels
     torch.cuda.empty_cache()

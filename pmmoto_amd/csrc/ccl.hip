@@ -136,12 +136,65 @@ __device__ __forceinline__ void lunite(int *lp, int a, int b)
     }
 }
 
+// 2 x 2 x 2 voxel blocks.  The voxels of a block are pairwise 26-adjacent,
+// so its voxels of one value belong to one component: each is linked to the
+// block's first voxel of that value (in C-order, so the smallest index: every
+// component's smallest voxel stays its root), and the union-find only joins
+// these representatives across the faces, edges and corners of neighbouring
+// blocks.  Tile origins are multiples of 2 on every axis, so blocks never
+// straddle a tile; a block at the high end of an odd extent is partial (its
+// missing voxels read 0).  Bit k = 4 dx + 2 dy + dz of a block mask is its
+// voxel (dx, dy, dz); byte k of a block word is that voxel's value.
+constexpr int kBX = kTX / 2, kBY = kTY / 2, kBZ = kTZ / 2, kTBlk = kBX * kBY * kBZ;
+
+// the block's voxels whose coordinate on `axis` is c
+__device__ __forceinline__ uint32_t layer_bits(int axis, int c)
+{
+    constexpr uint32_t kLayer[3][2] = {{0x0fu, 0xf0u}, {0x33u, 0xccu}, {0x55u, 0xaau}};
+    return kLayer[axis][c];
+}
+
+// The voxels of block A and of its neighbour B = A + d (d in {-1, 0, 1}^3)
+// that face each other: per axis, d = -1 takes A's layer 0 and B's layer 1,
+// d = +1 the reverse, d = 0 both layers.  Every voxel of A's set is
+// 26-adjacent to every voxel of B's set.
+__device__ __forceinline__ void facing(int dx, int dy, int dz, uint32_t &fa, uint32_t &fb)
+{
+    fa = fb = 0xffu;
+    const int d[3] = {dx, dy, dz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (d[a] == 0) continue;
+        fa &= layer_bits(a, d[a] < 0 ? 0 : 1);
+        fb &= layer_bits(a, d[a] < 0 ? 1 : 0);
+    }
+}
+
+// bit k set where byte k of w equals v (v < 256; exact, no carry crosses a byte)
+__device__ __forceinline__ uint32_t eq_mask8(uint64_t w, uint32_t v)
+{
+    constexpr uint64_t k7f = 0x7f7f7f7f7f7f7f7full;
+    const uint64_t x = w ^ (0x0101010101010101ull * v);
+    const uint64_t t = ~(((x & k7f) + k7f) | x | k7f);  // 0x80 in the bytes of x that are 0
+    return (uint32_t)(((t >> 7) * 0x0102040810204080ull) >> 56);
+}
+
+__device__ __forceinline__ uint32_t byte_of(uint64_t w, int k) { return (uint32_t)(w >> (8 * k)) & 0xffu; }
+
+// tile-local index of voxel k of the block whose first voxel is l0
+__device__ __forceinline__ int blk_voxel(int l0, int k)
+{
+    return l0 + ((k >> 2) & 1) * (kTY * kTZ) + ((k >> 1) & 1) * kTZ + (k & 1);
+}
+
 __global__ __launch_bounds__(256) void k_local(const uint8_t *__restrict__ img, Box b, TileGeom tg,
                                                int vec16, int *__restrict__ parent)
 {
     static_assert(kTVox == 256 * 16, "one 16-byte quarter row per thread");
+    static_assert(kTBlk == 2 * 256, "two blocks per thread");
     __shared__ int lp[kTVox];
     __shared__ __align__(16) uint8_t lv[kTVox];
+    __shared__ uint64_t bv[kTBlk];  // each block's eight values
     int x0, y0, z0, ex, ey, ez;
     tile_of(b, tg, blockIdx.x, x0, y0, z0, ex, ey, ez);
     {
@@ -161,25 +214,72 @@ __global__ __launch_bounds__(256) void k_local(const uint8_t *__restrict__ img, 
         }
         *reinterpret_cast<uint4 *>(lv + r * kTZ + c0) = q;
     }
-    for (int l = threadIdx.x; l < kTVox; l += blockDim.x) lp[l] = l;
     __syncthreads();
-    for (int l = threadIdx.x; l < kTVox; l += blockDim.x) {
-        const uint8_t v = lv[l];
-        if (!v) continue;
-        const int lx = l / (kTY * kTZ), ly = (l / kTZ) % kTY, lz = l % kTZ;
-        int last = -1;  // neighbours pointing at the node already joined add nothing
+    // blocks: values, and every voxel linked to its block's first voxel of the same value
+    const int ebx = (ex + 1) >> 1, eby = (ey + 1) >> 1, ebz = (ez + 1) >> 1;  // blocks of the tile in the box
+    uint64_t mv[2];
+    int ml0[2];
 #pragma unroll
-        for (int k = 0; k < 13; ++k) {
+    for (int s = 0; s < 2; ++s) {
+        const int bi = threadIdx.x + 256 * s;
+        const int bx = bi / (kBY * kBZ), by = (bi / kBZ) % kBY, bz = bi % kBZ;
+        const int l0 = ((2 * bx) * kTY + 2 * by) * kTZ + 2 * bz;
+        const uint16_t *h = reinterpret_cast<const uint16_t *>(lv);
+        const uint64_t w = (uint64_t)h[l0 >> 1] | ((uint64_t)h[(l0 + kTZ) >> 1] << 16) |
+                           ((uint64_t)h[(l0 + kTY * kTZ) >> 1] << 32) |
+                           ((uint64_t)h[(l0 + kTY * kTZ + kTZ) >> 1] << 48);
+        bv[bi] = w;
+        mv[s] = w;
+        ml0[s] = l0;
+        const uint32_t fg = ~eq_mask8(w, 0u) & 0xffu;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int l = blk_voxel(l0, k);
+            int r = l;
+            if ((fg >> k) & 1u) r = blk_voxel(l0, __builtin_ctz(eq_mask8(w, byte_of(w, k))));
+            lp[l] = r;
+        }
+    }
+    __syncthreads();
+    // each block joins its 13 backward neighbour blocks inside the tile: one
+    // union per value the facing voxels share (others cross the tile: phase 2)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int bi = threadIdx.x + 256 * s;
+        const int bx = bi / (kBY * kBZ), by = (bi / kBZ) % kBY, bz = bi % kBZ;
+        const uint64_t w = mv[s];
+        const uint32_t fg = ~eq_mask8(w, 0u) & 0xffu;
+        if (!fg || bx >= ebx || by >= eby || bz >= ebz) continue;
+        const int l0 = ml0[s];
+        const uint32_t v0 = byte_of(w, __builtin_ctz(fg));
+        const bool uni = eq_mask8(w, v0) == fg;  // one value (every binary image's blocks)
+#pragma unroll
+        for (int j = 0; j < 13; ++j) {
             int dx, dy, dz;
-            back_nb(k, dx, dy, dz);
-            const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
-            if (nx < 0 || ny < 0 || ny >= ey || nz < 0 || nz >= ez) continue;  // leaves the tile: phase 2
-            const int m = (nx * kTY + ny) * kTZ + nz;
-            if (lv[m] != v) continue;
-            const int pm = lp[m];
-            if (pm == last) continue;
-            lunite(lp, l, m);
-            last = pm;
+            back_nb(j, dx, dy, dz);
+            const int nbx = bx + dx, nby = by + dy, nbz = bz + dz;
+            if (nbx < 0 || nby < 0 || nby >= eby || nbz < 0 || nbz >= ebz) continue;
+            uint32_t fa, fb;
+            facing(dx, dy, dz, fa, fb);
+            const uint32_t sa = fg & fa;
+            if (!sa) continue;
+            const int bj = (nbx * kBY + nby) * kBZ + nbz;
+            const uint64_t wb = bv[bj];
+            const int lb0 = ((2 * nbx) * kTY + 2 * nby) * kTZ + 2 * nbz;
+            if (uni) {
+                const uint32_t mb = eq_mask8(wb, v0);
+                if (mb & fb) lunite(lp, blk_voxel(l0, __builtin_ctz(fg)), blk_voxel(lb0, __builtin_ctz(mb)));
+                continue;
+            }
+            uint32_t todo = sa;
+            while (todo) {
+                const int k = __builtin_ctz(todo);
+                const uint32_t v = byte_of(w, k);
+                const uint32_t ma = eq_mask8(w, v);
+                todo &= ~ma;  // one union per shared value
+                const uint32_t mb = eq_mask8(wb, v);
+                if (mb & fb) lunite(lp, blk_voxel(l0, __builtin_ctz(ma)), blk_voxel(lb0, __builtin_ctz(mb)));
+            }
         }
     }
     __syncthreads();
@@ -197,31 +297,15 @@ __global__ __launch_bounds__(256) void k_local(const uint8_t *__restrict__ img, 
     }
 }
 
-// coordinate c + d on an axis of n voxels: wrapped on a periodic axis,
-// else -1 outside the box
-__device__ __forceinline__ int step(int c, int d, int n, int per)
-{
-    int r = c + d;
-    if (r < 0 || r >= n) {
-        if (!per) return -1;
-        r = r < 0 ? r + n : r - n;
-    }
-    return r;
-}
-
-// One candidate pair per lane (pj < 0: none).  A lane whose (parent of i,
-// parent of j) pair equals the previous lane's adds nothing (that lane, or
-// one before it, joins them), nor does one whose pj it has just joined;
-// the rest go through the block's set of joined parent pairs: a pair
-// already there was joined by the lane that inserted it, and a full probe
-// falls back to joining.  Every lane of the wave calls this (shuffles).
-__device__ __forceinline__ void join_pair(int *parent, unsigned long long *seen, int pi, int pj, int &last, int lane)
+// A pair of parents to join (pj < 0: none), at most once per block: a pj
+// this lane has just joined adds nothing, and the rest go through the
+// block's set of joined parent pairs (a pair already there was joined by the
+// lane that inserted it; a full probe falls back to joining).
+__device__ __forceinline__ void join_pair(int *parent, unsigned long long *seen, int pi, int pj, int &last)
 {
     constexpr int kProbe = 8;
     constexpr unsigned long long kEmpty = ~0ull;
-    const int ppi = __shfl_up(pi, 1), ppj = __shfl_up(pj, 1);
-    const bool dup = lane > 0 && ppj >= 0 && ppi == pi && ppj == pj;
-    if (pj < 0 || dup || pj == last) return;
+    if (pj < 0 || pj == pi || pj == last) return;
     last = pj;
     const unsigned long long key = ((unsigned long long)(uint32_t)pi << 32) | (uint32_t)pj;
     const uint32_t h = ((uint32_t)pi * 0x9e3779b1u) ^ ((uint32_t)pj * 0x85ebca6bu);
@@ -233,84 +317,168 @@ __device__ __forceinline__ void join_pair(int *parent, unsigned long long *seen,
     unite(parent, pi, pj);
 }
 
-// Phase 2: the backward pairs k_local did not see, in two disjoint groups.
-//  (a) Face rows: a z-row of the tile with a backward row R (offsets
-//      (-1,-1), (-1,0), (-1,+1), (0,-1) in x, y) outside the tile's x/y
-//      footprint takes every dz of that R (wrapped on periodic axes).  A
-//      wave holds the row, lane = z: R is loaded once, with one halo voxel
-//      at each end, and dz = -1 / +1 come from lane shuffles.
-//  (b) Z faces: for backward rows inside the footprint (and the voxel's own
-//      row for dz = -1), the pairs whose dz leaves the tile's z range: the
-//      lz = 0 voxels with dz = -1 and the lz = ez-1 voxels with dz = +1, one
-//      lane per row.
+// join_pair called by every lane of the wave at once: a lane whose pair
+// equals the previous lane's adds nothing (that lane, or one before it,
+// joins it), which keeps lanes with one pair off the same hash slot.
+__device__ __forceinline__ void join_pair_wave(int *parent, unsigned long long *seen, int pi, int pj, int &last,
+                                               int lane)
+{
+    const int ppi = __shfl_up(pi, 1), ppj = __shfl_up(pj, 1);
+    if (lane > 0 && ppj >= 0 && ppi == pi && ppj == pj) return;
+    join_pair(parent, seen, pi, pj, last);
+}
+
+// coordinate c (possibly outside [0, n)) on an axis of n voxels: wrapped on a
+// periodic axis, else -1 outside the box
+__device__ __forceinline__ int wrap_coord(int c, int n, int per)
+{
+    if (c >= 0 && c < n) return c;
+    if (!per) return -1;
+    c %= n;
+    return c < 0 ? c + n : c;
+}
+
+// Phase 2: the backward pairs k_local did not see -- those whose neighbour
+// leaves the tile or wraps -- by blocks, like k_local.  The tile and a
+// one-voxel shell around it (tile coordinates -1 .. kT on each axis; wrapped
+// on periodic axes, 0 outside the box) are staged in LDS.  For a block A
+// and a backward direction d, the neighbour voxels V are, per axis, A's
+// range where d = 0 and the layer just below (d = -1) or above (d = +1) A's
+// range; every voxel of V is adjacent to every voxel of A's facing set, and
+// V lies in one 2 x 2 x 2 block of one tile (its same-valued voxels share a
+// tile-local root).  Where V leaves the tile, each value A's facing set
+// shares with V joins A's parent with the parent of V's voxel of that value.
 // Parent reads go through L2: a stale parent is still an ancestor, and a
 // union acting on a stale root fails its CAS and retries.
-__global__ __launch_bounds__(256) void k_border(const uint8_t *__restrict__ img, Box b, TileGeom tg,
+// The shell box in LDS: rows (x, y) for tile coordinates -1 .. kT, each row
+// kHP bytes with z = -1 .. kTZ at byte z + 16 (z = 0 .. kTZ-1 16-byte aligned).
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHP = kTZ + 32;
+
+__global__ __launch_bounds__(256) void k_border(const uint8_t *__restrict__ img, Box b, TileGeom tg, int vec16,
                                                 int *__restrict__ parent)
 {
-    static_assert(kTX * kTY == 64 && kTZ == 64, "a z-row per wave, a z face per wave");
     constexpr unsigned long long kEmpty = ~0ull;
     __shared__ unsigned long long seen[kBorderHash];
+    __shared__ __align__(16) uint8_t hv[kHX * kHY * kHP];
     for (int h = threadIdx.x; h < kBorderHash; h += blockDim.x) seen[h] = kEmpty;
-    __syncthreads();
     int x0, y0, z0, ex, ey, ez;
     tile_of(b, tg, blockIdx.x, x0, y0, z0, ex, ey, ez);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-
-    // (a) face rows
-    for (int row = wv; row < ex * ey; row += 4) {
-        const int lx = row / ey, ly = row - lx * ey;
-        if (lx != 0 && ly != 0 && ly != ey - 1) continue;  // every backward row inside the footprint
-        const int x = x0 + lx, y = y0 + ly, z = z0 + lane;
-        const bool in_z = lane < ez;
-        const uint8_t v = in_z ? img[(int64_t)x * b.is0 + (int64_t)y * b.is1 + z] : (uint8_t)0;
-        const int pi = v ? parent[((int64_t)x * b.n1 + y) * b.n2 + z] : -1;
-        const int zlo = lane == 0 ? step(z0, -1, b.n2, b.per2) : -1;
-        const int zhi = lane == ez - 1 ? step(z0 + ez - 1, 1, b.n2, b.per2) : -1;
-        int last = -1;
-        for (int r = 0; r < 4; ++r) {
-            const int dx = r < 3 ? -1 : 0, dy = r < 3 ? r - 1 : -1;
-            if (lx + dx >= 0 && ly + dy >= 0 && ly + dy < ey) continue;  // k_local or the z faces
-            const int nx = step(x, dx, b.n0, b.per0), ny = step(y, dy, b.n1, b.per1);
-            if (nx < 0 || ny < 0) continue;
-            const uint8_t *irow = img + (int64_t)nx * b.is0 + (int64_t)ny * b.is1;
-            const int *prow = parent + ((int64_t)nx * b.n1 + ny) * b.n2;
-            const uint8_t cv = in_z ? irow[z] : (uint8_t)0;
-            const int cp = cv ? prow[z] : -1;
-            int lov = __shfl_up((int)cv, 1), lop = __shfl_up(cp, 1);
-            int hiv = __shfl_down((int)cv, 1), hip = __shfl_down(cp, 1);
-            if (lane == 0) {
-                lov = zlo >= 0 ? irow[zlo] : 0;
-                lop = lov ? prow[zlo] : -1;
+    {
+        // four threads per row: 16 interior bytes each (one aligned load on a full z tile with
+        // 16-byte aligned rows), the first and the last also the byte just outside the tile
+        const int q = threadIdx.x & 3;
+        const int zlo = wrap_coord(z0 - 1, b.n2, b.per2), zhi = wrap_coord(z0 + kTZ, b.n2, b.per2);
+        for (int r = threadIdx.x >> 2; r < kHX * kHY; r += 64) {
+            const int hx = r / kHY, hy = r % kHY;
+            const int gx = wrap_coord(x0 + hx - 1, b.n0, b.per0), gy = wrap_coord(y0 + hy - 1, b.n1, b.per1);
+            uint8_t *dst = hv + r * kHP + 16;
+            if (gx < 0 || gy < 0) {
+                *reinterpret_cast<uint4 *>(dst + 16 * q) = make_uint4(0u, 0u, 0u, 0u);
+                if (q == 0) dst[-1] = 0;
+                if (q == 3) dst[kTZ] = 0;
+                continue;
             }
-            if (lane == ez - 1) {
-                hiv = zhi >= 0 ? irow[zhi] : 0;
-                hip = hiv ? prow[zhi] : -1;
+            const uint8_t *src = img + (int64_t)gx * b.is0 + (int64_t)gy * b.is1;
+            if (vec16 && ez == kTZ) {
+                *reinterpret_cast<uint4 *>(dst + 16 * q) = *reinterpret_cast<const uint4 *>(src + z0 + 16 * q);
+            } else {
+                // a partial z tile: byte by byte, the positions past its end wrapped like the shell
+                for (int k = 16 * q; k < 16 * q + 16; ++k) {
+                    const int gz = wrap_coord(z0 + k, b.n2, b.per2);
+                    dst[k] = gz >= 0 ? src[gz] : (uint8_t)0;
+                }
             }
-            join_pair(parent, seen, pi, v && lov == v ? lop : -2, last, lane);
-            join_pair(parent, seen, pi, v && cv == v ? cp : -2, last, lane);
-            join_pair(parent, seen, pi, v && hiv == v ? hip : -2, last, lane);
+            if (q == 0) dst[-1] = zlo >= 0 ? src[zlo] : (uint8_t)0;
+            if (q == 3) dst[kTZ] = zhi >= 0 ? src[zhi] : (uint8_t)0;
         }
     }
-
-    // (b) z faces: wave 2 the lz = 0 face (dz = -1), wave 3 the lz = ez-1 face (dz = +1)
-    if (wv >= 2) {
-        const bool lo = wv == 2;
-        const int lx = lane / kTY, ly = lane % kTY;
-        const int x = x0 + lx, y = y0 + ly, z = z0 + (lo ? 0 : ez - 1);
-        const int nz = step(z, lo ? -1 : 1, b.n2, b.per2);  // the same for every lane
-        if (nz < 0) return;
-        const uint8_t v = lx < ex && ly < ey ? img[(int64_t)x * b.is0 + (int64_t)y * b.is1 + z] : (uint8_t)0;
-        const int pi = v ? parent[((int64_t)x * b.n1 + y) * b.n2 + z] : -1;
+    __syncthreads();
+    const int e[3] = {ex, ey, ez}, o[3] = {x0, y0, z0}, n[3] = {b.n0, b.n1, b.n2}, per[3] = {b.per0, b.per1, b.per2};
+    auto hidx = [](int x, int y, int z) { return ((x + 1) * kHY + (y + 1)) * kHP + z + 16; };
+    // global index of the voxel at tile coordinates c (inside the box or wrapped into it)
+    auto gidx = [&](const int (&c)[3]) -> int {
+        int gc[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) gc[a] = wrap_coord(o[a] + c[a], n[a], per[a]);
+        return (gc[0] * b.n1 + gc[1]) * b.n2 + gc[2];
+    };
+    // V's first voxel of value v: its parent (-1: none)
+    auto find_v = [&](const int (&vlo)[3], const int (&vhi)[3], uint32_t v) -> int {
+        for (int cx = vlo[0]; cx <= vhi[0]; ++cx)
+            for (int cy = vlo[1]; cy <= vhi[1]; ++cy)
+                for (int cz = vlo[2]; cz <= vhi[2]; ++cz)
+                    if (hv[hidx(cx, cy, cz)] == v) {
+                        const int c[3] = {cx, cy, cz};
+                        return parent[gidx(c)];
+                    }
+        return -1;
+    };
+    const int lane = threadIdx.x & 63;
+    // every lane runs every step below (join_pair_wave exchanges pairs between lanes)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int bi = threadIdx.x + 256 * s;
+        const int lo[3] = {2 * (bi / (kBY * kBZ)), 2 * ((bi / kBZ) % kBY), 2 * (bi % kBZ)};
+        int hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) hi[a] = min(lo[a] + 1, e[a] - 1);
+        const bool in_box = lo[0] < ex && lo[1] < ey && lo[2] < ez;
+        // only blocks with a backward neighbour layer outside the tile
+        const bool border = in_box && (lo[0] == 0 || lo[1] == 0 || hi[1] + 1 >= ey || lo[2] == 0 || hi[2] + 1 >= ez);
+        uint64_t w = 0;
+        if (border) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int c[3] = {lo[0] + ((k >> 2) & 1), lo[1] + ((k >> 1) & 1), lo[2] + (k & 1)};
+                if (c[0] <= hi[0] && c[1] <= hi[1] && c[2] <= hi[2])
+                    w |= (uint64_t)hv[hidx(c[0], c[1], c[2])] << (8 * k);
+            }
+        }
+        const uint32_t fg = ~eq_mask8(w, 0u) & 0xffu;  // 0 unless a border block
+        // A's first value and its parent: the only value of every binary image's blocks
+        const int k0 = fg ? __builtin_ctz(fg) : 0;
+        const uint32_t v0 = byte_of(w, k0);
+        const uint32_t m0 = eq_mask8(w, v0) & fg;
+        int pi0 = -1;
+        if (fg) {
+            const int c0[3] = {lo[0] + ((k0 >> 2) & 1), lo[1] + ((k0 >> 1) & 1), lo[2] + (k0 & 1)};
+            pi0 = parent[gidx(c0)];
+        }
         int last = -1;
-        for (int r = 0; r < 5; ++r) {
-            const int dx = r < 3 ? -1 : 0, dy = r < 3 ? r - 1 : (r == 3 ? -1 : 0);
-            // rows outside the footprint are the face rows' (a); the own row only for dz = -1
-            const bool inside = lx + dx >= 0 && ly + dy >= 0 && ly + dy < ey && (r < 4 || lo);
-            const int64_t m = (int64_t)(x + dx) * b.is0 + (int64_t)(y + dy) * b.is1 + nz;
-            const uint8_t nv = v && inside ? img[m] : (uint8_t)0;
-            const int pj = v && nv == v ? parent[((int64_t)(x + dx) * b.n1 + (y + dy)) * b.n2 + nz] : -2;
-            join_pair(parent, seen, pi, pj, last, lane);
+#pragma unroll
+        for (int j = 0; j < 13; ++j) {
+            int d[3];
+            back_nb(j, d[0], d[1], d[2]);
+            int vlo[3], vhi[3];
+            bool out = false;
+            uint32_t fa = 0xffu;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (d[a] == 0) {
+                    vlo[a] = lo[a];
+                    vhi[a] = hi[a];
+                } else {
+                    vlo[a] = vhi[a] = d[a] < 0 ? lo[a] - 1 : hi[a] + 1;
+                    out |= vlo[a] < 0 || vlo[a] >= e[a];
+                    fa &= layer_bits(a, d[a] < 0 ? 0 : hi[a] - lo[a]);
+                }
+            }
+            // pairs inside the tile were joined by k_local
+            const int pj = (out && (m0 & fa)) ? find_v(vlo, vhi, v0) : -1;
+            join_pair_wave(parent, seen, pi0, pj, last, lane);
+            // the other values of blocks holding several
+            uint32_t todo = out ? fg & fa & ~m0 : 0u;
+            while (todo) {
+                const int k = __builtin_ctz(todo);
+                const uint32_t v = byte_of(w, k);
+                const uint32_t ma = eq_mask8(w, v);
+                todo &= ~ma;  // one union per shared value
+                const int pv = find_v(vlo, vhi, v);
+                if (pv < 0) continue;
+                const int ka = __builtin_ctz(ma);
+                const int ca[3] = {lo[0] + ((ka >> 2) & 1), lo[1] + ((ka >> 1) & 1), lo[2] + (ka & 1)};
+                join_pair(parent, seen, parent[gidx(ca)], pv, last);
+            }
         }
     }
 }
@@ -605,15 +773,15 @@ int pmx_ccl26(const uint8_t *img, const pmx_geom *g, int32_t *labels32, uint64_t
     const TileGeom tg{(b.n0 + kTX - 1) / kTX, (b.n1 + kTY - 1) / kTY, (b.n2 + kTZ - 1) / kTZ};
     const int64_t ntile = (int64_t)tg.tx * tg.ty * tg.tz;
     if (ntile > UINT32_MAX / 2) return fail(PMX_ERR_INVALID, "too many tiles");
+    const int vec16 = ((uintptr_t)img % 16 == 0 && g->in_s[0] % 16 == 0 && g->in_s[1] % 16 == 0) ? 1 : 0;
     {
         ProfScope ps("ccl_init", s);
-        const int vec16 = ((uintptr_t)img % 16 == 0 && g->in_s[0] % 16 == 0 && g->in_s[1] % 16 == 0) ? 1 : 0;
         hipLaunchKernelGGL(k_local, dim3((unsigned)ntile), dim3(256), 0, s, img, b, tg, vec16, p.parent);
     }
     if (int e = check_launch("ccl local")) return e;
     {
         ProfScope ps("ccl_merge", s);
-        hipLaunchKernelGGL(k_border, dim3((unsigned)ntile), dim3(256), 0, s, img, b, tg, p.parent);
+        hipLaunchKernelGGL(k_border, dim3((unsigned)ntile), dim3(256), 0, s, img, b, tg, vec16, p.parent);
     }
     if (int e = check_launch("ccl merge")) return e;
     {

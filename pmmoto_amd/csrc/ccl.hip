@@ -5,20 +5,25 @@
 // faces, the boundary-label matching of connect_subdomain_boundaries
 // (:65-101, core/voxels.py:167-397): the wrap is part of the neighbourhood.
 //
-//   local     per 8 x 8 x 64 tile, in LDS: each voxel unions with its 13
-//             backward 26-neighbours of the same value inside the tile;
+//   local     per 8 x 8 x 64 tile, in LDS, by 2 x 2 x 2 voxel blocks: the
+//             voxels of one value in a block are linked to the block's
+//             first such voxel, and these representatives unite across
+//             the faces, edges and corners of neighbouring blocks;
 //             parent[i] = global index of the tile-local root (background
-//             marked)
-//   border    the backward pairs that leave a tile or wrap on periodic
-//             axes, unioned globally; lock-free union by atomicCAS hooking
-//             the larger root under the smaller, so every component's root
-//             is its smallest C-order index
-//   compress  parent[i] = root(i); per 4096-voxel rank tile, its roots as a
-//             bitmap and their count
+//             marked); the tile-local roots enter a root bitmap (one bit per
+//             voxel)
+//   border    the backward block pairs that leave a tile or wrap on
+//             periodic axes, unioned globally; lock-free union by atomicCAS
+//             hooking the larger root under the smaller, so every
+//             component's root is its smallest C-order index
+//   settle    the tile-local roots that were hooked leave the bitmap and
+//             point straight at their root; roots counted per 4096-voxel
+//             rank tile
 //   rank      tile counts scanned; from the bitmap, each root's rank is
 //             written over its parent entry as -(rank+1)
-//   label     label[i] = rank of root(i) + 1: components numbered 1..N in
-//             C-order of their first voxel (the canonical numbering)
+//   label     label[i] = rank of root(i) + 1 (a walk up the parents):
+//             components numbered 1..N in C-order of their first voxel (the
+//             canonical numbering)
 #include "common.hpp"
 
 namespace pmx {
@@ -188,7 +193,7 @@ __device__ __forceinline__ int blk_voxel(int l0, int k)
 }
 
 __global__ __launch_bounds__(256) void k_local(const uint8_t *__restrict__ img, Box b, TileGeom tg,
-                                               int vec16, int *__restrict__ parent)
+                                               int vec16, int *__restrict__ parent, uint64_t *__restrict__ root_bits)
 {
     static_assert(kTVox == 256 * 16, "one 16-byte quarter row per thread");
     static_assert(kTBlk == 2 * 256, "two blocks per thread");
@@ -283,17 +288,44 @@ __global__ __launch_bounds__(256) void k_local(const uint8_t *__restrict__ img, 
         }
     }
     __syncthreads();
-    for (int l = threadIdx.x; l < kTVox; l += blockDim.x) {
-        const int lx = l / (kTY * kTZ), ly = (l / kTZ) % kTY, lz = l % kTZ;
-        if (lx >= ex || ly >= ey || lz >= ez) continue;
-        const int64_t i = ((int64_t)(x0 + lx) * b.n1 + (y0 + ly)) * b.n2 + z0 + lz;
-        if (!lv[l]) {
-            parent[i] = kBg;
-            continue;
-        }
-        const int r = lfind(lp, l);
+    // each block's voxels get the global index of their group's tile-local
+    // root (one root lookup per value); a group whose representative is that
+    // root enters the root bitmap
+    auto tile_to_global = [&](int r) {
         const int rx = r / (kTY * kTZ), ry = (r / kTZ) % kTY, rz = r % kTZ;
-        parent[i] = (int)(((int64_t)(x0 + rx) * b.n1 + (y0 + ry)) * b.n2 + z0 + rz);
+        return (int)(((int64_t)(x0 + rx) * b.n1 + (y0 + ry)) * b.n2 + z0 + rz);
+    };
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int bi = threadIdx.x + 256 * s;
+        const int bx = bi / (kBY * kBZ), by = (bi / kBZ) % kBY, bz = bi % kBZ;
+        if (2 * bx >= ex || 2 * by >= ey || 2 * bz >= ez) continue;
+        const uint64_t w = mv[s];
+        const int l0 = ml0[s];
+        uint32_t todo = ~eq_mask8(w, 0u) & 0xffu;
+        int out[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) out[k] = kBg;
+        while (todo) {
+            const uint32_t ma = eq_mask8(w, byte_of(w, __builtin_ctz(todo)));
+            todo &= ~ma;
+            const int rep = blk_voxel(l0, __builtin_ctz(ma));
+            const int r = lfind(lp, rep);
+            const int gr = tile_to_global(r);
+            if (r == rep) atomicOr(reinterpret_cast<unsigned long long *>(root_bits + (gr >> 6)), 1ull << (gr & 63));
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if ((ma >> k) & 1u) out[k] = gr;
+        }
+        // rows (dx, dy) of the block, two voxels each
+#pragma unroll
+        for (int rw = 0; rw < 4; ++rw) {
+            const int lx = 2 * bx + (rw >> 1), ly = 2 * by + (rw & 1);
+            if (lx >= ex || ly >= ey) continue;
+            const int64_t i = ((int64_t)(x0 + lx) * b.n1 + (y0 + ly)) * b.n2 + z0 + 2 * bz;
+            parent[i] = out[2 * rw];
+            if (2 * bz + 1 < ez) parent[i + 1] = out[2 * rw + 1];
+        }
     }
 }
 
@@ -354,6 +386,106 @@ __device__ __forceinline__ int wrap_coord(int c, int n, int per)
 // kHP bytes with z = -1 .. kTZ at byte z + 16 (z = 0 .. kTZ-1 16-byte aligned).
 constexpr int kHX = kTX + 2, kHY = kTY + 2, kHP = kTZ + 32;
 
+__device__ __forceinline__ int hoff(int x, int y, int z) { return ((x + 1) * kHY + (y + 1)) * kHP + z + 16; }
+
+struct BorderCtx {
+    const uint8_t *hv;               // the shell box
+    int *parent;
+    unsigned long long *seen;        // the block's joined parent pairs
+    int e[3], o[3], n[3], per[3];    // tile extents in the box, tile origin, box extents, periodic axes
+    int lane;
+};
+
+// a tile block A: its range per axis (lo .. hi: one voxel on the high side
+// of an odd extent), values, voxel mask, first value and its voxels, and the
+// parent of that value's voxels
+struct BorderBlock {
+    int lo[3], hi[3];
+    uint64_t w;
+    uint32_t fg, v0, m0;
+    int pi0;
+};
+
+// global index of the voxel at tile coordinates (x, y, z), inside the box or
+// wrapped into it
+__device__ __forceinline__ int gindex(const BorderCtx &c, int x, int y, int z)
+{
+    const int gx = wrap_coord(c.o[0] + x, c.n[0], c.per[0]), gy = wrap_coord(c.o[1] + y, c.n[1], c.per[1]),
+              gz = wrap_coord(c.o[2] + z, c.n[2], c.per[2]);
+    return (gx * c.n[1] + gy) * c.n[2] + gz;
+}
+
+// a voxel of value v in V = vlo .. vhi (the axes with d = 0 span A's range,
+// the others one layer): its global index, or -1
+template <int DX, int DY, int DZ>
+__device__ __forceinline__ int find_in(const BorderCtx &c, const int (&vlo)[3], const int (&vhi)[3], uint32_t v)
+{
+    // (shell coordinates start at -1, so a flag, not a coordinate, says "found")
+    bool found = false;
+    int fx = 0, fy = 0, fz = 0;
+#pragma unroll
+    for (int qx = (DX == 0 ? 1 : 0); qx >= 0; --qx)
+#pragma unroll
+        for (int qy = (DY == 0 ? 1 : 0); qy >= 0; --qy)
+#pragma unroll
+            for (int qz = (DZ == 0 ? 1 : 0); qz >= 0; --qz) {
+                const int x = vlo[0] + qx, y = vlo[1] + qy, z = vlo[2] + qz;
+                if (x <= vhi[0] && y <= vhi[1] && z <= vhi[2] && c.hv[hoff(x, y, z)] == v) {
+                    found = true;
+                    fx = x;
+                    fy = y;
+                    fz = z;
+                }
+            }
+    return found ? gindex(c, fx, fy, fz) : -1;
+}
+
+// Backward direction (DX, DY, DZ) of block A: V and A's facing voxels as
+// above; when V leaves the tile, one union per value they share.  Every
+// lane of the wave calls this; a wave none of whose lanes has such a V
+// returns at once.
+template <int DX, int DY, int DZ>
+__device__ __forceinline__ void border_dir(const BorderCtx &c, const BorderBlock &a, int &last)
+{
+    constexpr int d[3] = {DX, DY, DZ};
+    int vlo[3], vhi[3];
+    bool out = false;
+    uint32_t fa = 0xffu;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (d[q] == 0) {
+            vlo[q] = a.lo[q];
+            vhi[q] = a.hi[q];
+        } else {
+            vlo[q] = vhi[q] = d[q] < 0 ? a.lo[q] - 1 : a.hi[q] + 1;
+            out |= d[q] < 0 ? vlo[q] < 0 : vlo[q] >= c.e[q];
+            fa &= layer_bits(q, d[q] < 0 ? 0 : a.hi[q] - a.lo[q]);
+        }
+    }
+    // pairs inside the tile were joined by k_local
+    const bool first = out && (a.m0 & fa) != 0;
+    const uint32_t rest = out ? a.fg & fa & ~a.m0 : 0u;  // the other values of blocks holding several
+    if (__builtin_amdgcn_ballot_w64(first || rest != 0) == 0) return;
+    int pj = -1;
+    if (first) {
+        const int j = find_in<DX, DY, DZ>(c, vlo, vhi, a.v0);
+        if (j >= 0) pj = c.parent[j];
+    }
+    join_pair_wave(c.parent, c.seen, a.pi0, pj, last, c.lane);
+    uint32_t todo = rest;
+    while (todo) {
+        const int k = __builtin_ctz(todo);
+        const uint32_t v = byte_of(a.w, k);
+        const uint32_t ma = eq_mask8(a.w, v);
+        todo &= ~ma;  // one union per shared value
+        const int j = find_in<DX, DY, DZ>(c, vlo, vhi, v);
+        if (j < 0) continue;
+        const int ka = __builtin_ctz(ma);
+        const int pa = c.parent[gindex(c, a.lo[0] + ((ka >> 2) & 1), a.lo[1] + ((ka >> 1) & 1), a.lo[2] + (ka & 1))];
+        join_pair(c.parent, c.seen, pa, c.parent[j], last);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_border(const uint8_t *__restrict__ img, Box b, TileGeom tg, int vec16,
                                                 int *__restrict__ parent)
 {
@@ -393,156 +525,92 @@ __global__ __launch_bounds__(256) void k_border(const uint8_t *__restrict__ img,
         }
     }
     __syncthreads();
-    const int e[3] = {ex, ey, ez}, o[3] = {x0, y0, z0}, n[3] = {b.n0, b.n1, b.n2}, per[3] = {b.per0, b.per1, b.per2};
-    auto hidx = [](int x, int y, int z) { return ((x + 1) * kHY + (y + 1)) * kHP + z + 16; };
-    // global index of the voxel at tile coordinates c (inside the box or wrapped into it)
-    auto gidx = [&](const int (&c)[3]) -> int {
-        int gc[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) gc[a] = wrap_coord(o[a] + c[a], n[a], per[a]);
-        return (gc[0] * b.n1 + gc[1]) * b.n2 + gc[2];
-    };
-    // V's first voxel of value v: its parent (-1: none)
-    auto find_v = [&](const int (&vlo)[3], const int (&vhi)[3], uint32_t v) -> int {
-        for (int cx = vlo[0]; cx <= vhi[0]; ++cx)
-            for (int cy = vlo[1]; cy <= vhi[1]; ++cy)
-                for (int cz = vlo[2]; cz <= vhi[2]; ++cz)
-                    if (hv[hidx(cx, cy, cz)] == v) {
-                        const int c[3] = {cx, cy, cz};
-                        return parent[gidx(c)];
-                    }
-        return -1;
-    };
-    const int lane = threadIdx.x & 63;
+    const BorderCtx c{hv, parent, seen, {ex, ey, ez}, {x0, y0, z0}, {b.n0, b.n1, b.n2}, {b.per0, b.per1, b.per2},
+                      (int)(threadIdx.x & 63)};
     // every lane runs every step below (join_pair_wave exchanges pairs between lanes)
-#pragma unroll
+#pragma unroll 1
     for (int s = 0; s < 2; ++s) {
         const int bi = threadIdx.x + 256 * s;
-        const int lo[3] = {2 * (bi / (kBY * kBZ)), 2 * ((bi / kBZ) % kBY), 2 * (bi % kBZ)};
-        int hi[3];
+        BorderBlock a;
+        a.lo[0] = 2 * (bi / (kBY * kBZ));
+        a.lo[1] = 2 * ((bi / kBZ) % kBY);
+        a.lo[2] = 2 * (bi % kBZ);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) hi[a] = min(lo[a] + 1, e[a] - 1);
-        const bool in_box = lo[0] < ex && lo[1] < ey && lo[2] < ez;
+        for (int q = 0; q < 3; ++q) a.hi[q] = min(a.lo[q] + 1, c.e[q] - 1);
+        const bool in_box = a.lo[0] < ex && a.lo[1] < ey && a.lo[2] < ez;
         // only blocks with a backward neighbour layer outside the tile
-        const bool border = in_box && (lo[0] == 0 || lo[1] == 0 || hi[1] + 1 >= ey || lo[2] == 0 || hi[2] + 1 >= ez);
-        uint64_t w = 0;
+        const bool border =
+            in_box && (a.lo[0] == 0 || a.lo[1] == 0 || a.hi[1] + 1 >= ey || a.lo[2] == 0 || a.hi[2] + 1 >= ez);
+        a.w = 0;
         if (border) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                const int c[3] = {lo[0] + ((k >> 2) & 1), lo[1] + ((k >> 1) & 1), lo[2] + (k & 1)};
-                if (c[0] <= hi[0] && c[1] <= hi[1] && c[2] <= hi[2])
-                    w |= (uint64_t)hv[hidx(c[0], c[1], c[2])] << (8 * k);
+                const int cx = a.lo[0] + ((k >> 2) & 1), cy = a.lo[1] + ((k >> 1) & 1), cz = a.lo[2] + (k & 1);
+                if (cx <= a.hi[0] && cy <= a.hi[1] && cz <= a.hi[2]) a.w |= (uint64_t)hv[hoff(cx, cy, cz)] << (8 * k);
             }
         }
-        const uint32_t fg = ~eq_mask8(w, 0u) & 0xffu;  // 0 unless a border block
+        a.fg = ~eq_mask8(a.w, 0u) & 0xffu;  // 0 unless a border block
         // A's first value and its parent: the only value of every binary image's blocks
-        const int k0 = fg ? __builtin_ctz(fg) : 0;
-        const uint32_t v0 = byte_of(w, k0);
-        const uint32_t m0 = eq_mask8(w, v0) & fg;
-        int pi0 = -1;
-        if (fg) {
-            const int c0[3] = {lo[0] + ((k0 >> 2) & 1), lo[1] + ((k0 >> 1) & 1), lo[2] + (k0 & 1)};
-            pi0 = parent[gidx(c0)];
-        }
+        const int k0 = a.fg ? __builtin_ctz(a.fg) : 0;
+        a.v0 = byte_of(a.w, k0);
+        a.m0 = eq_mask8(a.w, a.v0) & a.fg;
+        a.pi0 = -1;
+        if (a.fg)
+            a.pi0 = parent[gindex(c, a.lo[0] + ((k0 >> 2) & 1), a.lo[1] + ((k0 >> 1) & 1), a.lo[2] + (k0 & 1))];
         int last = -1;
-#pragma unroll
-        for (int j = 0; j < 13; ++j) {
-            int d[3];
-            back_nb(j, d[0], d[1], d[2]);
-            int vlo[3], vhi[3];
-            bool out = false;
-            uint32_t fa = 0xffu;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                if (d[a] == 0) {
-                    vlo[a] = lo[a];
-                    vhi[a] = hi[a];
-                } else {
-                    vlo[a] = vhi[a] = d[a] < 0 ? lo[a] - 1 : hi[a] + 1;
-                    out |= vlo[a] < 0 || vlo[a] >= e[a];
-                    fa &= layer_bits(a, d[a] < 0 ? 0 : hi[a] - lo[a]);
-                }
-            }
-            // pairs inside the tile were joined by k_local
-            const int pj = (out && (m0 & fa)) ? find_v(vlo, vhi, v0) : -1;
-            join_pair_wave(parent, seen, pi0, pj, last, lane);
-            // the other values of blocks holding several
-            uint32_t todo = out ? fg & fa & ~m0 : 0u;
-            while (todo) {
-                const int k = __builtin_ctz(todo);
-                const uint32_t v = byte_of(w, k);
-                const uint32_t ma = eq_mask8(w, v);
-                todo &= ~ma;  // one union per shared value
-                const int pv = find_v(vlo, vhi, v);
-                if (pv < 0) continue;
-                const int ka = __builtin_ctz(ma);
-                const int ca[3] = {lo[0] + ((ka >> 2) & 1), lo[1] + ((ka >> 1) & 1), lo[2] + (ka & 1)};
-                join_pair(parent, seen, parent[gidx(ca)], pv, last);
-            }
-        }
+        border_dir<-1, -1, -1>(c, a, last);
+        border_dir<-1, -1, 0>(c, a, last);
+        border_dir<-1, -1, 1>(c, a, last);
+        border_dir<-1, 0, -1>(c, a, last);
+        border_dir<-1, 0, 0>(c, a, last);
+        border_dir<-1, 0, 1>(c, a, last);
+        border_dir<-1, 1, -1>(c, a, last);
+        border_dir<-1, 1, 0>(c, a, last);
+        border_dir<-1, 1, 1>(c, a, last);
+        border_dir<0, -1, -1>(c, a, last);
+        border_dir<0, -1, 0>(c, a, last);
+        border_dir<0, -1, 1>(c, a, last);
+        border_dir<0, 0, -1>(c, a, last);
     }
 }
 
-// The root walks here only read.  Compression must not halve other voxels'
-// paths: a halving write computed from a stale read could overwrite a
-// pointer another lane has just set to the root with a non-root ancestor,
-// and the label gather relies on every parent being a root.  No union runs
-// any more, so plain (L2) reads are enough: an entry only ever changes to
-// its root.
-//
-// One block per rank tile of kTile voxels, in chunks of 256 consecutive
-// voxels: parent[i] = root(i) (each lane writes only its own entry, so roots
-// and pointers already set to a root stay as written), the tile's roots as
-// one bit per voxel (a ballot per 64 voxels), and its root count.
-__global__ __launch_bounds__(256) void k_compress(int *__restrict__ parent, int64_t n,
-                                                  uint64_t *__restrict__ root_bits, int *__restrict__ tile_counts)
+// the root bitmap starts empty (k_local enters its tile-local roots)
+__global__ __launch_bounds__(256) void k_clear_roots(uint64_t *__restrict__ bits, int64_t n_words)
 {
-    __shared__ int sh[4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t base = (int64_t)blockIdx.x * kTile;
-    int c = 0;
-    // four chunks at a time: their four root walks advance in lockstep, so a
-    // lane keeps four independent parent loads in flight
-    constexpr int kWays = 4;
-    for (int k = 0; k < kTile / 256; k += kWays) {
-        int64_t i[kWays];
-        int p[kWays], r[kWays];
-        bool root[kWays];
-#pragma unroll
-        for (int s = 0; s < kWays; ++s) {
-            i[s] = base + 256 * (k + s) + threadIdx.x;
-            p[s] = i[s] < n ? parent[i[s]] : kBg;
-            root[s] = p[s] == (int)i[s];
-            r[s] = p[s];
-        }
-        // r[s] walks from p[s] to its root (the entry of a root is itself); background and
-        // roots need no walk
-        bool open[kWays];
-#pragma unroll
-        for (int s = 0; s < kWays; ++s) open[s] = p[s] != kBg && !root[s];
-        while (open[0] || open[1] || open[2] || open[3]) {
-            int q[kWays];
-#pragma unroll
-            for (int s = 0; s < kWays; ++s) q[s] = open[s] ? parent[r[s]] : r[s];
-#pragma unroll
-            for (int s = 0; s < kWays; ++s) {
-                if (q[s] == r[s]) open[s] = false;
-                r[s] = q[s];
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < kWays; ++s) {
-            if (p[s] != kBg && !root[s] && r[s] != p[s]) parent[i[s]] = r[s];
-            const uint64_t m = __ballot(root[s]);
-            if (lane == 0) root_bits[(base + 256 * (k + s)) / 64 + wv] = m;
-            c += root[s];
-        }
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = tid; i < n_words; i += stride) bits[i] = 0ull;
+}
+
+// One wave per rank tile of kTile voxels (one bitmap word per lane).  The
+// bitmap holds the tile-local roots; those k_border hooked under another
+// root leave it, and their entry is pointed straight at their root (a short
+// read-only walk: no union runs any more, and an entry only ever changes to
+// an ancestor), so every voxel is at most two entries from its root.  The
+// tile's count of remaining roots goes to tile_counts.
+__global__ __launch_bounds__(256) void k_settle_roots(int *__restrict__ parent, int64_t ntiles,
+                                                      uint64_t *__restrict__ root_bits, int *__restrict__ tile_counts)
+{
+    static_assert(kTile == 64 * 64, "one root word per lane");
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= ntiles) return;
+    const int64_t wi = t * 64 + lane;
+    uint64_t m = root_bits[wi], keep = m;
+    while (m) {
+        const int bit = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        const int i = (int)(wi * 64 + bit);
+        int r = parent[i];
+        if (r == i) continue;
+        for (int q = parent[r]; q != r; q = parent[r]) r = q;
+        parent[i] = r;
+        keep &= ~(1ull << bit);
     }
+    if (keep != root_bits[wi]) root_bits[wi] = keep;
+    int c = __popcll(keep);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-    if (lane == 0) sh[wv] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) tile_counts[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    if (lane == 0) tile_counts[t] = c;
 }
 
 // single block: exclusive scan of tile counts in place, total to *count.
@@ -630,9 +698,14 @@ __global__ __launch_bounds__(256) void k_rank_roots(int *__restrict__ parent, in
     }
 }
 
+// label of a voxel whose parent entry is p: roots hold -(rank + 1) after
+// k_rank_roots, other entries point at an ancestor (a smaller index), so the
+// walk ends at the voxel's root
 __device__ __forceinline__ int64_t label_of(const int *parent, int p)
 {
-    return p == kBg ? 0 : (p < 0 ? -(int64_t)p : -(int64_t)parent[p]);
+    if (p == kBg) return 0;
+    while (p >= 0) p = parent[p];
+    return -(int64_t)p;
 }
 
 template <class T>
@@ -776,7 +849,9 @@ int pmx_ccl26(const uint8_t *img, const pmx_geom *g, int32_t *labels32, uint64_t
     const int vec16 = ((uintptr_t)img % 16 == 0 && g->in_s[0] % 16 == 0 && g->in_s[1] % 16 == 0) ? 1 : 0;
     {
         ProfScope ps("ccl_init", s);
-        hipLaunchKernelGGL(k_local, dim3((unsigned)ntile), dim3(256), 0, s, img, b, tg, vec16, p.parent);
+        hipLaunchKernelGGL(k_clear_roots, dim3((unsigned)std::min<int64_t>((p.ntiles * (kTile / 64) + 255) / 256, 4096)),
+                           dim3(256), 0, s, p.root_bits, p.ntiles * (kTile / 64));
+        hipLaunchKernelGGL(k_local, dim3((unsigned)ntile), dim3(256), 0, s, img, b, tg, vec16, p.parent, p.root_bits);
     }
     if (int e = check_launch("ccl local")) return e;
     {
@@ -785,10 +860,11 @@ int pmx_ccl26(const uint8_t *img, const pmx_geom *g, int32_t *labels32, uint64_t
     }
     if (int e = check_launch("ccl merge")) return e;
     {
-        ProfScope ps("ccl_compress", s);
-        hipLaunchKernelGGL(k_compress, dim3((unsigned)p.ntiles), dim3(256), 0, s, p.parent, n, p.root_bits, p.tiles);
+        ProfScope ps("ccl_settle", s);
+        hipLaunchKernelGGL(k_settle_roots, dim3((unsigned)((p.ntiles + 3) / 4)), dim3(256), 0, s, p.parent, p.ntiles,
+                           p.root_bits, p.tiles);
     }
-    if (int e = check_launch("ccl compress")) return e;
+    if (int e = check_launch("ccl settle")) return e;
     {
         ProfScope ps("ccl_rank", s);
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, p.tiles, p.ntiles, count);

@@ -11,20 +11,24 @@
 //    erosion (Te decreases with i, so the eroded sets grow), and a single
 //    union-find over the sweep absorbs each level's new voxels: unions only
 //    ever merge, so the labelling of level i is incremental, and inlet
-//    connectivity is monotone (a root flagged once stays flagged);
-//  * per level: inlet-slab voxels flag their roots, the inlet-connected
-//    set is masked, and one exact EDT of the mask (distance to the set)
-//    feeds the first-invasion update directly from its last pass.
+//    connectivity is monotone (a root flagged once stays flagged).  Its
+//    nodes are 2 x 2 x 2 voxel blocks: the voxels of a block are pairwise
+//    26-adjacent, so a block's active voxels are always in one component;
+//  * per level: inlet-slab voxels flag their roots, blocks whose root is
+//    flagged join the inlet-connected set (their active voxels, and later
+//    ones as they activate, take the level in in_c), and one exact EDT of
+//    the level's joiners (distance to them) feeds the first-invasion update
+//    directly from its last pass.
 #include "psd_internal.hpp"
 
 namespace pmx {
 namespace {
 
 // in_c of a voxel not (yet) in the inlet-connected set (levels are < 250):
-constexpr uint32_t kInactive = 0xff;  // pore voxel not yet in the eroded set
+constexpr uint32_t kInactive = 0xff;  // pore voxel not yet joined
 constexpr uint32_t kNever = 0xfe;     // in no eroded set of the sweep (solid, or too thin)
-constexpr uint32_t kOpen = 0xfd;      // in the eroded set, not yet joined
 constexpr int kNoLevel = 256;          // above every level: no neighbour there
+constexpr uint8_t kNoJoin = 0xff;      // bjoin of a block not yet in the inlet-connected set
 
 __device__ __forceinline__ int ld(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
@@ -74,7 +78,13 @@ __device__ __forceinline__ int unite(int *parent, int a, int b)
 struct Dims {
     int n0, n1, n2;
     int per0, per1, per2;
+    int nb0, nb1, nb2;  // 2 x 2 x 2 blocks per axis (the last one partial on an odd extent)
 };
+
+__device__ __forceinline__ int block_of(const Dims &d, int x, int y, int z)
+{
+    return ((x >> 1) * d.nb1 + (y >> 1)) * d.nb2 + (z >> 1);
+}
 
 __device__ __forceinline__ int64_t n_of(const Dims &d) { return (int64_t)d.n0 * d.n1 * d.n2; }
 
@@ -83,13 +93,10 @@ constexpr int kBucketItems = 16;  // voxels per thread between bucket reservatio
 
 // level[v] = first i with d2 > te[i] (the erosion at radius i keeps v),
 // or n_levels if none (te does not increase, so a binary search finds
-// it); counts[level] += per-block totals from an LDS histogram.  The parent
-// array needs no initial value: a voxel's entry is read only once the level
-// map says it is active (level <= the current level), after k_activate.
+// it); counts[level] += per-block totals from an LDS histogram.
 __global__ __launch_bounds__(256) void k_levels(const int32_t *__restrict__ d2, int64_t n, const int64_t *__restrict__ te,
                                                 int n_levels, uint8_t *__restrict__ level,
-                                                unsigned int *__restrict__ counts, uint8_t *__restrict__ flag,
-                                                uint8_t *__restrict__ in_c)
+                                                unsigned int *__restrict__ counts, uint8_t *__restrict__ in_c)
 {
     __shared__ int64_t ste[kMaxLevels];
     __shared__ unsigned int hist[kMaxLevels];
@@ -110,7 +117,6 @@ __global__ __launch_bounds__(256) void k_levels(const int32_t *__restrict__ d2, 
                 lo = mid + 1;
         }
         level[i] = (uint8_t)lo;
-        flag[i] = 0;
         in_c[i] = lo < n_levels ? kInactive : kNever;
         if (lo < n_levels) atomicAdd(hist + lo, 1u);
     }
@@ -162,39 +168,73 @@ __global__ __launch_bounds__(256) void k_bucket(const uint8_t *__restrict__ leve
     }
 }
 
-// Level lv's voxels are order[offsets[lv] .. offsets[lv+1]): the range is
-// read on the device, so the sweep issues a fixed sequence of launches
-// with no host synchronisation (it can be captured in a graph).
-__global__ __launch_bounds__(256) void k_activate(const int *__restrict__ order, const unsigned int *__restrict__ offsets,
-                                                  int lv, int *__restrict__ parent, uint8_t *__restrict__ in_c)
+// Per 2 x 2 x 2 block: blevel = the smallest level of its voxels (the level
+// at which the block becomes a node of the union-find; n_levels: never),
+// bjoin = kNoJoin, bflag = 0.  The parent array needs no initial value: a
+// block's entry is read only once blevel says it is active, after
+// k_activate.
+__global__ __launch_bounds__(256) void k_block_init(const uint8_t *__restrict__ level, Dims d, int n_levels,
+                                                    uint8_t *__restrict__ blevel, uint8_t *__restrict__ bjoin,
+                                                    uint8_t *__restrict__ bflag)
 {
-    const unsigned int begin = offsets[lv], end = offsets[lv + 1];
-    for (int64_t k = (int64_t)begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < end;
-         k += (int64_t)gridDim.x * blockDim.x) {
-        const int v = order[k];
-        parent[v] = v;
-        in_c[v] = kOpen;
+    const int64_t nb = (int64_t)d.nb0 * d.nb1 * d.nb2;
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += (int64_t)gridDim.x * blockDim.x) {
+        const int bz = (int)(b % d.nb2);
+        const int64_t t = b / d.nb2;
+        const int by = (int)(t % d.nb1), bx = (int)(t / d.nb1);
+        int lo = n_levels;
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) {
+                const int x = 2 * bx + i, y = 2 * by + j, z = 2 * bz;
+                if (x >= d.n0 || y >= d.n1) continue;
+                const int64_t v = ((int64_t)x * d.n1 + y) * d.n2 + z;
+                lo = min(lo, (int)level[v]);
+                if (z + 1 < d.n2) lo = min(lo, (int)level[v + 1]);
+            }
+        blevel[b] = (uint8_t)lo;
+        bjoin[b] = kNoJoin;
+        bflag[b] = 0;
     }
 }
 
-// each new voxel unions with its active 26-neighbours (wrapping on periodic
-// axes).  Activity comes from the level map (active at level lv <=> level
-// <= lv; k_activate has run for this level), a read-only byte array that
-// stays in L2, rather than from the parent array.  A pair of new voxels is
-// joined once, by the one with the larger index.  Each union starts from the
-// root the previous one returned (an element of v's set), not from v, and
-// from u's parent; a neighbour whose parent is that root, or the parent just
-// joined, is already in v's set.
-__global__ __launch_bounds__(256) void k_merge_new(const int *__restrict__ order, const unsigned int *__restrict__ offsets,
-                                                   int lv, Dims d, const uint8_t *__restrict__ level,
-                                                   int *__restrict__ parent)
+// Level lv's voxels are order[offsets[lv] .. offsets[lv+1]): the range is
+// read on the device, so the sweep issues a fixed sequence of launches
+// with no host synchronisation (it can be captured in a graph).  A block
+// whose first voxels arrive at this level becomes a node (every such voxel
+// writes the same entry).
+__global__ __launch_bounds__(256) void k_activate(const int *__restrict__ order, const unsigned int *__restrict__ offsets,
+                                                  int lv, Dims d, const uint8_t *__restrict__ blevel,
+                                                  int *__restrict__ bparent)
 {
     const unsigned int begin = offsets[lv], end = offsets[lv + 1];
     for (int64_t k = (int64_t)begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < end;
          k += (int64_t)gridDim.x * blockDim.x) {
         const int v = order[k];
-        int rv = v, last = -1;
         const int z = v % d.n2, t = v / d.n2, y = t % d.n1, x = t / d.n1;
+        const int b = block_of(d, x, y, z);
+        if (blevel[b] == lv) bparent[b] = b;
+    }
+}
+
+// each new voxel joins its block with the blocks of its active 26-neighbours
+// (wrapping on periodic axes).  Activity comes from the level map (active
+// at level lv <=> level <= lv), a read-only byte array that stays in L2.  A
+// pair of new voxels is joined once, by the one with the larger index.
+// Each union starts from the root the previous one returned (an element of
+// the block's set), not from the block, and from the neighbour block's
+// parent; a neighbour whose block is the voxel's own, or whose parent is
+// that root or the parent just joined, is already in the set.
+__global__ __launch_bounds__(256) void k_merge_new(const int *__restrict__ order, const unsigned int *__restrict__ offsets,
+                                                   int lv, Dims d, const uint8_t *__restrict__ level,
+                                                   int *__restrict__ bparent)
+{
+    const unsigned int begin = offsets[lv], end = offsets[lv + 1];
+    for (int64_t k = (int64_t)begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < end;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int v = order[k];
+        const int z = v % d.n2, t = v / d.n2, y = t % d.n1, x = t / d.n1;
+        const int bv = block_of(d, x, y, z);
+        int rv = bv, last = -1;
         for (int dx = -1; dx <= 1; ++dx) {
             int nx = x + dx;
             if (nx < 0 || nx >= d.n0) {
@@ -211,59 +251,61 @@ __global__ __launch_bounds__(256) void k_merge_new(const int *__restrict__ order
                 // the levels at z-1, z, z+1: away from the z faces one aligned 8-byte load
                 // holds all three (two when they straddle an 8-byte boundary; the level map is
                 // a 256-byte aligned workspace slice with slack after it)
-                int lz[3] = {kNoLevel, kNoLevel, kNoLevel}, uz[3] = {-1, -1, -1};
+                int lz[3] = {kNoLevel, kNoLevel, kNoLevel}, nzs[3] = {-1, -1, -1};
                 if (z >= 1 && z + 1 < d.n2) {
                     const int a = row + z - 1, off = a & 7;
                     const uint64_t *q = reinterpret_cast<const uint64_t *>(level + (a - off));
                     uint64_t w = q[0] >> (8 * off);
                     if (off > 5) w |= q[1] << (64 - 8 * off);
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        lz[k] = (int)((w >> (8 * k)) & 0xffu);
-                        uz[k] = a + k;
+                    for (int kk = 0; kk < 3; ++kk) {
+                        lz[kk] = (int)((w >> (8 * kk)) & 0xffu);
+                        nzs[kk] = z + kk - 1;
                     }
                 } else {
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        int nz = z + k - 1;
+                    for (int kk = 0; kk < 3; ++kk) {
+                        int nz = z + kk - 1;
                         if (nz < 0 || nz >= d.n2) {
                             if (!d.per2) continue;
                             nz = nz < 0 ? nz + d.n2 : nz - d.n2;
                         }
-                        uz[k] = row + nz;
-                        lz[k] = level[uz[k]];
+                        nzs[kk] = nz;
+                        lz[kk] = level[row + nz];
                     }
                 }
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    if (!dx && !dy && k == 1) continue;
-                    const int u = uz[k], lu = lz[k];
-                    if (u < 0 || lu > lv || (lu == lv && u > v)) continue;
-                    const int pu = parent[u];
+                for (int kk = 0; kk < 3; ++kk) {
+                    if (!dx && !dy && kk == 1) continue;
+                    const int nz = nzs[kk], lu = lz[kk];
+                    if (nz < 0 || lu > lv || (lu == lv && row + nz > v)) continue;
+                    const int bu = block_of(d, nx, ny, nz);
+                    if (bu == bv) continue;
+                    const int pu = bparent[bu];
                     if (pu == rv || pu == last) continue;
                     last = pu;
-                    rv = unite<true>(parent, rv, pu);
+                    rv = unite<true>(bparent, rv, pu);
                 }
             }
         }
     }
 }
 
-// active voxels of the inlet slab flag their current roots.  Here and in
-// k_mask no union runs, so the forest only changes by path halving (a
-// pointer moves to an ancestor) and its reads go through L2.
+// active voxels of the inlet slab flag their blocks' current roots.  Here
+// and in k_join_blocks no union runs, so the forest only changes by path
+// halving (a pointer moves to an ancestor) and its reads go through L2.
 __global__ __launch_bounds__(256) void k_flag_inlet(Dims d, int64_t lo0, int64_t lo1, int64_t lo2, int64_t e0,
                                                     int64_t e1, int64_t e2, const uint8_t *__restrict__ level, int lv,
-                                                    int *__restrict__ parent, uint8_t *__restrict__ flag)
+                                                    int *__restrict__ bparent, uint8_t *__restrict__ bflag)
 {
     const int64_t n = e0 * e1 * e2;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t z = i % e2, t = i / e2, y = t % e1, x = t / e1;
-        const int v = (int)(((lo0 + x) * d.n1 + lo1 + y) * d.n2 + lo2 + z);
-        if (level[v] <= lv) {
-            const int r = find_root<true>(parent, v);
-            if (!flag[r]) flag[r] = 1;  // read first: a large component's voxels share one root
+        const int gx = (int)(lo0 + x), gy = (int)(lo1 + y), gz = (int)(lo2 + z);
+        if (level[((int64_t)gx * d.n1 + gy) * d.n2 + gz] <= lv) {
+            const int r = find_root<true>(bparent, block_of(d, gx, gy, gz));
+            if (!bflag[r]) bflag[r] = 1;  // read first: a large component's blocks share one root
         }
     }
 }
@@ -276,83 +318,94 @@ __device__ __forceinline__ uint32_t eq_bytes(uint32_t w, uint32_t pat)
     return ~(((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x | 0x7f7f7f7fu) >> 7;
 }
 
-// in_c[v] = the level at which v joined the inlet-connected eroded set
-// (kOpen: in the eroded set, not yet joined; the sets only grow).  A pore voxel left unassigned
-// by level i-1 is farther than r_{i-1} > r_i from every older member, so
-// only the members that join at level i can invade it at level i: the
-// level's dilation takes the voxels with in_c == i as its centres (a byte
-// match in its pass Z), and no per-level mask is written.  any: set when the
-// level has a new member (otherwise its dilation is skipped on the GPU).
-//
-// 16 voxels per lane step (in_c is a 256-byte aligned workspace slice).  A
-// member stays a member (its root stays flagged), so only kOpen voxels need
-// a root lookup; a group of 16 with none is done after reading in_c, and
-// only groups with a new member are written.
-__device__ __forceinline__ uint32_t join_byte(int64_t i, uint32_t c, int lv, int *parent, const uint8_t *flag)
+// 1 in each byte of w that is <= the byte in pat (unsigned).  Per byte, with
+// x = pat and y = w: where the top bits differ, x >= y iff x's top bit is
+// set; where they agree, iff the low seven bits compare so, which is the top
+// bit of (x | 0x80) - (y & 0x7f) (at least 1 per byte: no borrow crosses)
+__device__ __forceinline__ uint32_t le_bytes(uint32_t w, uint32_t pat)
 {
-    if (c != kOpen) return c;
-    return flag[find_root<true>(parent, (int)i)] ? (uint32_t)lv : c;
+    constexpr uint32_t kH = 0x80808080u;
+    const uint32_t low = (pat | kH) - (w & ~kH);
+    return (((pat & ~w) | (~(pat ^ w) & low)) & kH) >> 7;
 }
 
-__global__ __launch_bounds__(256) void k_mask(int64_t n, int lv, int *__restrict__ parent,
-                                              const uint8_t *__restrict__ flag, uint8_t *__restrict__ in_c,
-                                              unsigned int *__restrict__ any)
+// The inlet-connected set at level lv.  A block joins when its root is
+// flagged (bjoin = lv): its active voxels (level <= lv) take in_c = lv.  A
+// member stays a member, so only active blocks not yet joined look up their
+// root; 16 blocks per lane step (bjoin and blevel are 256-byte aligned
+// workspace slices, levels < 128), and the candidates of a group walk to
+// their roots in lockstep, each entry then set to its root (no union runs
+// here, so the root is a valid ancestor).  any: set when the level has a
+// new member (otherwise its dilation is skipped on the GPU).
+__global__ __launch_bounds__(256) void k_join_blocks(Dims d, int lv, int *__restrict__ bparent,
+                                                     const uint8_t *__restrict__ bflag,
+                                                     const uint8_t *__restrict__ blevel, uint8_t *__restrict__ bjoin,
+                                                     const uint8_t *__restrict__ level, uint8_t *__restrict__ in_c,
+                                                     unsigned int *__restrict__ any)
 {
     __shared__ int blk_seen;
     if (threadIdx.x == 0) blk_seen = 0;
     __syncthreads();
     bool seen = false;
-    const int64_t nv = n / 16, stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nb = (int64_t)d.nb0 * d.nb1 * d.nb2;
+    const int64_t ng = (nb + 15) / 16, stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint4 *c4 = reinterpret_cast<uint4 *>(in_c);
-    constexpr uint32_t kOpen4 = kOpen * 0x01010101u;
-    for (int64_t q = tid; q < nv; q += stride) {
-        const uint4 c = c4[q];
-        const uint32_t open = eq_bytes(c.x, kOpen4) | eq_bytes(c.y, kOpen4) | eq_bytes(c.z, kOpen4) |
-                              eq_bytes(c.w, kOpen4);
-        if (!open) continue;  // no voxel of the 16 waits to join
-        const uint32_t cw[4] = {c.x, c.y, c.z, c.w};
-        // the group's candidates (kOpen) walk to their roots in lockstep,
-        // up to 16 independent parent loads in flight; each candidate's entry is then set to
-        // its root (no union runs here, so the root is a valid ancestor), which keeps the
-        // walks of the voxels that stay unjoined short at later levels
-        int r[16];
+    const uint32_t none4 = kNoJoin * 0x01010101u, lv4 = (uint32_t)lv * 0x01010101u;
+    // writes the level into in_c for block b's voxels of level <= lv
+    auto join_voxels = [&](int64_t b) {
+        const int bz = (int)(b % d.nb2);
+        const int64_t t = b / d.nb2;
+        const int by = (int)(t % d.nb1), bx = (int)(t / d.nb1);
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) {
+                const int x = 2 * bx + i, y = 2 * by + j, z = 2 * bz;
+                if (x >= d.n0 || y >= d.n1) continue;
+                const int64_t v = ((int64_t)x * d.n1 + y) * d.n2 + z;
+                if (level[v] <= lv) in_c[v] = (uint8_t)lv;
+                if (z + 1 < d.n2 && level[v + 1] <= lv) in_c[v + 1] = (uint8_t)lv;
+            }
+    };
+    for (int64_t q = tid; q < ng; q += stride) {
         uint32_t cand = 0;
+        if (16 * q + 16 <= nb) {
+            const uint4 jw = reinterpret_cast<const uint4 *>(bjoin)[q];
+            const uint4 lw = reinterpret_cast<const uint4 *>(blevel)[q];
+            const uint32_t jj[4] = {jw.x, jw.y, jw.z, jw.w}, ll[4] = {lw.x, lw.y, lw.z, lw.w};
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const uint32_t cb = (cw[e >> 2] >> (8 * (e & 3))) & 0xffu;
-            r[e] = (int)(16 * q + e);
-            if (cb == kOpen) cand |= 1u << e;
+            for (int h = 0; h < 4; ++h) {
+                const uint32_t m = eq_bytes(jj[h], none4) & le_bytes(ll[h], lv4);
+                // one bit per byte: bits 0, 8, 16, 24 -> 4 consecutive bits
+                cand |= ((m | (m >> 7) | (m >> 14) | (m >> 21)) & 0xfu) << (4 * h);
+            }
+        } else {
+            for (int e = 0; 16 * q + e < nb; ++e)
+                if (bjoin[16 * q + e] == kNoJoin && blevel[16 * q + e] <= lv) cand |= 1u << e;
         }
+        if (!cand) continue;
+        int r[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) r[e] = (int)(16 * q + e);
         uint32_t walking = cand;
         while (walking) {
             int nx[16];
 #pragma unroll
-            for (int e = 0; e < 16; ++e) nx[e] = (walking >> e) & 1u ? parent[r[e]] : r[e];
+            for (int e = 0; e < 16; ++e) nx[e] = (walking >> e) & 1u ? bparent[r[e]] : r[e];
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
                 if (nx[e] == r[e]) walking &= ~(1u << e);
                 r[e] = nx[e];
             }
         }
-        uint32_t nw[4] = {cw[0], cw[1], cw[2], cw[3]};
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             if (!((cand >> e) & 1u)) continue;
-            const int64_t i = 16 * q + e;
-            if (parent[i] != r[e]) parent[i] = r[e];
-            if (flag[r[e]]) nw[e >> 2] = (nw[e >> 2] & ~(0xffu << (8 * (e & 3)))) | ((uint32_t)lv << (8 * (e & 3)));
-        }
-        if (nw[0] != cw[0] || nw[1] != cw[1] || nw[2] != cw[2] || nw[3] != cw[3]) {
-            c4[q] = make_uint4(nw[0], nw[1], nw[2], nw[3]);
-            seen = true;
-        }
-    }
-    for (int64_t i = 16 * nv + tid; i < n; i += stride) {
-        const uint32_t c = in_c[i], f = join_byte(i, c, lv, parent, flag);
-        if (f != c) {
-            in_c[i] = (uint8_t)f;
-            seen = true;
+            const int64_t b = 16 * q + e;
+            if (bparent[b] != r[e]) bparent[b] = r[e];
+            if (bflag[r[e]]) {
+                bjoin[b] = (uint8_t)lv;
+                join_voxels(b);
+                seen = true;
+            }
         }
     }
     if (__any(seen) && (threadIdx.x & 63) == 0) blk_seen = 1;
@@ -360,6 +413,26 @@ __global__ __launch_bounds__(256) void k_mask(int64_t n, int lv, int *__restrict
     // one lane per block, and only while the flag still reads 0: the shared word is
     // read at device scope (past the per-XCD L2), so few reads and fewer atomics
     if (threadIdx.x == 0 && blk_seen && ld(reinterpret_cast<const int *>(any)) == 0) atomicOr(any, 1u);
+}
+
+// the level's new voxels in blocks that joined at an earlier level join now
+__global__ __launch_bounds__(256) void k_join_new(const int *__restrict__ order, const unsigned int *__restrict__ offsets,
+                                                  int lv, Dims d, const uint8_t *__restrict__ bjoin,
+                                                  uint8_t *__restrict__ in_c, unsigned int *__restrict__ any)
+{
+    const unsigned int begin = offsets[lv], end = offsets[lv + 1];
+    bool seen = false;
+    for (int64_t k = (int64_t)begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < end;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int v = order[k];
+        const int z = v % d.n2, t = v / d.n2, y = t % d.n1, x = t / d.n1;
+        const uint32_t j = bjoin[block_of(d, x, y, z)];
+        if (j < (uint32_t)lv) {
+            in_c[v] = (uint8_t)lv;
+            seen = true;
+        }
+    }
+    if (__any(seen) && (threadIdx.x & 63) == 0 && ld(reinterpret_cast<const int *>(any)) == 0) atomicOr(any, 1u);
 }
 
 // the erosion thresholds travel as a kernel argument (a graph-capturable
@@ -403,8 +476,9 @@ __global__ __launch_bounds__(256) void k_zero_map(int32_t *__restrict__ map, con
 
 struct PsdPlan {
     int32_t *d2 = nullptr;
-    uint8_t *level = nullptr, *flag = nullptr, *in_c = nullptr, *state = nullptr;
-    int *parent = nullptr, *order = nullptr;
+    uint8_t *level = nullptr, *in_c = nullptr, *state = nullptr;
+    uint8_t *blevel = nullptr, *bjoin = nullptr, *bflag = nullptr;  // per 2 x 2 x 2 block
+    int *bparent = nullptr, *order = nullptr;
     unsigned int *counts = nullptr, *offsets = nullptr, *cursor = nullptr, *any = nullptr;
     int64_t *te = nullptr;
     char *edt_ws = nullptr;
@@ -416,11 +490,14 @@ size_t plan_psd(const pmx_geom *g, void *ws, PsdPlan *p)
     Carver c(ws, SIZE_MAX);
     const size_t n = (size_t)(g->n[0] * g->n[1] * g->n[2]);
     p->d2 = c.take<int32_t>(n);
+    const size_t nb = (size_t)((g->n[0] + 1) / 2) * ((g->n[1] + 1) / 2) * ((g->n[2] + 1) / 2);
     p->level = c.take<uint8_t>(n);
-    p->flag = c.take<uint8_t>(n);
     p->in_c = c.take<uint8_t>(n);
     p->state = c.take<uint8_t>(n);
-    p->parent = c.take<int>(n);
+    p->blevel = c.take<uint8_t>(nb);
+    p->bjoin = c.take<uint8_t>(nb);
+    p->bflag = c.take<uint8_t>(nb);
+    p->bparent = c.take<int>(nb);
     p->order = c.take<int>(n);
     p->counts = c.take<unsigned int>(kMaxLevels + 1);
     p->offsets = c.take<unsigned int>(kMaxLevels + 1);
@@ -478,7 +555,8 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
         return fail(PMX_ERR_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, need);
     plan_psd(g, workspace, &p);
     Dims d{(int)g->n[0], (int)g->n[1], (int)g->n[2], g->bc[0] == PMX_BC_PERIODIC, g->bc[2] == PMX_BC_PERIODIC,
-           g->bc[4] == PMX_BC_PERIODIC};
+           g->bc[4] == PMX_BC_PERIODIC, (int)(g->n[0] + 1) / 2, (int)(g->n[1] + 1) / 2, (int)(g->n[2] + 1) / 2};
+    const int64_t nblk = (int64_t)d.nb0 * d.nb1 * d.nb2;
     const unsigned gr = grid_for(n);
     // the erosion EDT reads the image with its strides and writes the
     // internal contiguous d2; the dilations read the internal contiguous mask
@@ -517,12 +595,14 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
         hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(256), 0, s, p.any, kMaxLevels);
         // few blocks: each adds its per-level totals with one global atomic per level
         hipLaunchKernelGGL(k_levels, dim3(std::min<unsigned>(gr, 2048u)), dim3(256), 0, s, p.d2, n, p.te, n_levels,
-                           p.level, p.counts, p.flag, p.in_c);
+                           p.level, p.counts, p.in_c);
         if (!has_inlet) hipLaunchKernelGGL(k_any_from_counts, dim3(1), dim3(256), 0, s, p.counts, n_levels, p.any);
         if (has_inlet) {  // the union-find absorbs each level's voxels from these buckets
             hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, s, p.counts, n_levels, p.offsets, p.cursor);
             hipLaunchKernelGGL(k_bucket, dim3(grid_for((n + kBucketItems - 1) / kBucketItems)), dim3(256), 0, s,
                                p.level, n, n_levels, p.cursor, p.order);
+            hipLaunchKernelGGL(k_block_init, dim3(grid_for(nblk)), dim3(256), 0, s, p.level, d, n_levels, p.blevel,
+                               p.bjoin, p.bflag);
         }
         hipLaunchKernelGGL(k_zero_map, dim3(gr), dim3(256), 0, s, code_map, img, p.state, d, g->out_s[0],
                            g->out_s[1]);
@@ -536,15 +616,18 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
         if (has_inlet) {
             {
                 ProfScope ps("psd_union", s);
-                hipLaunchKernelGGL(k_activate, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, p.parent, p.in_c);
+                hipLaunchKernelGGL(k_activate, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, d, p.blevel,
+                                   p.bparent);
                 hipLaunchKernelGGL(k_merge_new, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, d, p.level,
-                                   p.parent);
+                                   p.bparent);
             }
             ProfScope ps("psd_mask", s);
             const int64_t ni = inlet_ext[0] * inlet_ext[1] * inlet_ext[2];
             hipLaunchKernelGGL(k_flag_inlet, dim3(grid_for(ni)), dim3(256), 0, s, d, inlet_lo[0], inlet_lo[1],
-                               inlet_lo[2], inlet_ext[0], inlet_ext[1], inlet_ext[2], p.level, lv, p.parent, p.flag);
-            hipLaunchKernelGGL(k_mask, dim3(grid_for(n / 16 + 1)), dim3(256), 0, s, n, lv, p.parent, p.flag, p.in_c,
+                               inlet_lo[2], inlet_ext[0], inlet_ext[1], inlet_ext[2], p.level, lv, p.bparent, p.bflag);
+            hipLaunchKernelGGL(k_join_blocks, dim3(grid_for(nblk / 16 + 1)), dim3(256), 0, s, d, lv, p.bparent,
+                               p.bflag, p.blevel, p.bjoin, p.level, p.in_c, p.any + lv);
+            hipLaunchKernelGGL(k_join_new, dim3(gu), dim3(256), 0, s, p.order, p.offsets, lv, d, p.bjoin, p.in_c,
                                p.any + lv);
         }
         if (int rc = check_launch("psd level")) return rc;

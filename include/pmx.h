@@ -297,6 +297,30 @@ int pmx_psd_sweep_batch(int n_domains, const pmx_psd_domain *domains, int n_leve
                         const int64_t *t_erode, const int64_t *t_dilate, int n_lanes,
                         const hipStream_t *lanes, void *workspace, size_t workspace_bytes);
 
+/* The fused batch (no counterpart in the reference; for callers that sweep
+ * hundreds of domains at once): every pass of every level is one launch
+ * over all domains, each block finding its domain in a table of per-domain
+ * arguments, and the erosion EDT is each domain's full exact EDT (written
+ * to edt_d2[i], an int32 C-contiguous box, when edt_d2 and edt_d2[i] are
+ * given; pmx_edt's squared distances).  Codes and distances are bit-equal to
+ * per-domain pmx_psd_sweep / pmx_edt calls.  Domains without an inlet
+ * requirement only, thresholds t_dilate below 128^2 (no dilation defers a
+ * line).  pmx_psd_batch_create checks the domains, lays out the workspace and
+ * uploads the launch tables (synchronising with the stream);
+ * pmx_psd_batch_run then only launches, with no host synchronisation (it
+ * can be captured in a graph; the plan, the workspace and the domains'
+ * arrays must outlive every replay). */
+typedef struct pmx_psd_batch pmx_psd_batch;
+int pmx_psd_batch_plan_size(int n_domains, const pmx_psd_domain *domains, int32_t *const *edt_d2,
+                            int n_levels, const int64_t *t_erode, const int64_t *t_dilate,
+                            size_t *bytes);
+int pmx_psd_batch_create(int n_domains, const pmx_psd_domain *domains, int32_t *const *edt_d2,
+                         int n_levels, const int64_t *t_erode, const int64_t *t_dilate,
+                         void *workspace, size_t workspace_bytes, hipStream_t stream,
+                         pmx_psd_batch **out);
+int pmx_psd_batch_run(const pmx_psd_batch *batch, hipStream_t stream);
+void pmx_psd_batch_destroy(pmx_psd_batch *batch);
+
 /* Tile autotuner (no counterpart in the reference): times the windowed EDT
  * passes of img with 32- and 64-row tiles (a threshold EDT clamped at cap;
  * invert as for pmx_morph) and returns the faster in *tile_rows.  The
@@ -307,6 +331,13 @@ int pmx_edt_autotune_workspace_size(const pmx_geom *g, size_t *bytes);
 int pmx_edt_autotune(const uint8_t *img, const pmx_geom *g, int64_t cap, int invert,
                      int32_t *tile_rows, void *workspace, size_t workspace_bytes,
                      hipStream_t stream);
+/* The autotuner's choices on disk (no counterpart in the reference): save
+ * writes every choice of this process to a text file (one line per key:
+ * shape, faces, invert, clamp class, tile rows); load adds a file's choices,
+ * so pmx_edt_autotune returns them without timing.  A missing file loads
+ * nothing; a malformed line is an error. */
+int pmx_tune_cache_save(const char *path);
+int pmx_tune_cache_load(const char *path);
 
 /* Learned-surrogate kernels (no counterpart in the reference, which has no
  * learned component; for the porosimetry surrogate).  BF16 operands are

@@ -73,6 +73,12 @@ SIGNATURES = {
     "pmx_psd_sweep_batch": (C.c_int, [C.c_int, P, C.c_int, P, P, C.c_int, P, P, SZ]),
     "pmx_edt_autotune_workspace_size": (C.c_int, [P, P]),
     "pmx_edt_autotune": (C.c_int, [P, P, I64, C.c_int, P, P, SZ, P]),
+    "pmx_tune_cache_save": (C.c_int, [C.c_char_p]),
+    "pmx_tune_cache_load": (C.c_int, [C.c_char_p]),
+    "pmx_psd_batch_plan_size": (C.c_int, [C.c_int, P, P, C.c_int, P, P, P]),
+    "pmx_psd_batch_create": (C.c_int, [C.c_int, P, P, C.c_int, P, P, P, SZ, P, P]),
+    "pmx_psd_batch_run": (C.c_int, [P, P]),
+    "pmx_psd_batch_destroy": (None, [P]),
     "pmx_psd_update": (C.c_int, [P, P, I64, I32, P, P]),
     "pmx_codes_to_values": (C.c_int, [P, I64, P, I32, P, P]),
     "pmx_code_histogram": (C.c_int, [P, P, P, P, P, I32, P, P]),

@@ -1,11 +1,17 @@
 """Pore-size-distribution sweeps of many unevenly sized domains at once.
 
 The reference runs one domain per (MPI) job; here a batch of domains runs
-as one captured graph of launches spread over a few HIP streams
-(``pmx_psd_sweep_batch``), with the EDT tile size of each domain shape
-chosen by timing (``pmx_edt_autotune``).  Each domain is its own region
-(no pads) with per-axis PERIODIC, WALL or END faces and gives exactly the
-codes of the single-domain sweep (``pmx_psd_sweep``) on the same input.
+as one captured graph.  Without inlet requirements it is the fused batch
+(``pmx_psd_batch_create`` / ``pmx_psd_batch_run``): every pass of every
+level is one launch over all domains, and each domain's full exact EDT is
+computed on the way (``edts``).  Domains with an inlet requirement run the
+per-domain sweeps spread over a few HIP streams (``pmx_psd_sweep_batch``).
+The EDT tile size of each domain shape is chosen by timing
+(``pmx_edt_autotune``), and the choices can be kept in a file across
+processes (``tune_cache``, or the ``PMX_TUNE_CACHE`` environment variable).
+Each domain is its own region (no pads) with per-axis PERIODIC, WALL or END
+faces and gives exactly the codes of the single-domain sweep
+(``pmx_psd_sweep``) and the distances of ``pmx_edt`` on the same input.
 
     sweep = BatchSweep(domains, radii=range(24, 0, -1))
     maps = sweep.run()          # first call tunes and captures, later calls replay
@@ -15,6 +21,7 @@ codes of the single-domain sweep (``pmx_psd_sweep``) on the same input.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -57,11 +64,16 @@ def _geom(shape, bc):
 
 class BatchSweep:
     """radii: descending radii in voxels (unit resolution, "distance" mode:
-    erosion d <= r by float32 sqrt, as porosimetry's EDT mode).  lanes: HIP
-    streams the domains are spread over.  graph: capture the batch once and
-    replay it."""
+    erosion d <= r by float32 sqrt, as porosimetry's EDT mode).  graph:
+    capture the batch once and replay it.  fused: one launch per pass over
+    all domains (only without inlet requirements; otherwise, or with
+    fused=False, the per-domain sweeps spread over `lanes` HIP streams).
+    edt: keep each domain's full squared EDT (int32, ``edts``; fused batch
+    only).  tune_cache: a file of tile choices read before and written after
+    autotuning (default: the PMX_TUNE_CACHE environment variable, if set)."""
 
-    def __init__(self, domains, radii, lanes: int = 4, autotune: bool = True, graph: bool = True):
+    def __init__(self, domains, radii, lanes: int = 4, autotune: bool = True, graph: bool = True,
+                 fused: bool = True, edt: bool = False, tune_cache: str | None = None):
         _device.require_gpu()
         radii = [float(r) for r in radii]
         if any(radii[i] < radii[i + 1] for i in range(len(radii) - 1)):
@@ -99,21 +111,52 @@ class BatchSweep:
             e.tile_rows = 0
         self.arr = arr
         self.n_lanes = max(1, min(int(lanes), len(self.domains)))
+        self.fused = bool(fused) and all(d.inlet is None for d in self.domains)
+        if edt and not self.fused:
+            raise ValueError("edt=True needs the fused batch (no inlet requirements, fused=True)")
         L = _lib.lib()
+        self.tune_cache = tune_cache if tune_cache is not None else os.environ.get("PMX_TUNE_CACHE")
         if autotune:
             self._autotune()
-        nbytes = C.c_size_t()
-        _lib.check(L.pmx_psd_batch_workspace_size(len(self.domains), arr, self.n_lanes, C.byref(nbytes)))
-        self.ws_bytes = nbytes.value
-        self.ws = _device.workspace(self.ws_bytes)
-        self.streams = [torch.cuda.Stream() for _ in range(self.n_lanes)]
         self.graph = graph
         self._g = None
+        self._plan = None
+        self.edts = []
+        if self.fused:
+            if edt:
+                self.edts = [torch.empty(tuple(d.img.shape), dtype=torch.int32, device=d.img.device)
+                             for d in self.domains]
+            self._edt_ptrs = (C.c_void_p * len(self.domains))(*[_device.ptr(t) for t in self.edts]) if edt else None
+            nbytes = C.c_size_t()
+            _lib.check(L.pmx_psd_batch_plan_size(len(self.domains), arr, self._edt_ptrs, len(self.radii), self.te,
+                                                 self.td, C.byref(nbytes)))
+            self.ws_bytes = nbytes.value
+            self.ws = _device.workspace(self.ws_bytes)
+            plan = C.c_void_p()
+            _lib.check(L.pmx_psd_batch_create(len(self.domains), arr, self._edt_ptrs, len(self.radii), self.te,
+                                              self.td, _device.ptr(self.ws), self.ws_bytes, _device.stream(),
+                                              C.byref(plan)))
+            self._plan = plan
+            self.streams = []
+        else:
+            nbytes = C.c_size_t()
+            _lib.check(L.pmx_psd_batch_workspace_size(len(self.domains), arr, self.n_lanes, C.byref(nbytes)))
+            self.ws_bytes = nbytes.value
+            self.ws = _device.workspace(self.ws_bytes)
+            self.streams = [torch.cuda.Stream() for _ in range(self.n_lanes)]
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        if plan is not None and plan.value:
+            _lib.lib().pmx_psd_batch_destroy(plan)
+            self._plan = None
 
     def _autotune(self):
         """Tile rows per domain shape: the faster of 32 and 64 on the
         domain's own erosion EDT (clamped at the largest threshold)."""
         L = _lib.lib()
+        if self.tune_cache:
+            _lib.check(L.pmx_tune_cache_load(os.fsencode(self.tune_cache)))
         need = C.c_size_t()
         big = 0
         for e in self.arr:
@@ -126,11 +169,18 @@ class BatchSweep:
                                           big, _device.stream()))
             e.tile_rows = tr.value
         del ws
+        if self.tune_cache:
+            d = os.path.dirname(os.path.abspath(self.tune_cache))
+            os.makedirs(d, exist_ok=True)
+            _lib.check(L.pmx_tune_cache_save(os.fsencode(self.tune_cache)))
 
     def tile_rows(self):
         return [e.tile_rows for e in self.arr]
 
     def _launch(self):
+        if self.fused:
+            _lib.check(_lib.lib().pmx_psd_batch_run(self._plan, _device.stream()))
+            return
         cur = torch.cuda.current_stream()
         for st in self.streams:
             st.wait_stream(cur)

@@ -171,7 +171,7 @@ def test_config4_batch_256_equals_single_domain_sweeps():
 
     from pmmoto_amd.domain_generation import pack_radii, sphere_pack
     from pmmoto_amd.filters.batch import BatchDomain, BatchSweep
-    from tests.test_batch_gpu import single_sweep
+    from tests.test_batch_gpu import single_edt, single_sweep
     doms = []
     for d in range(256):
         rng = np.random.default_rng(4000 + d)
@@ -182,14 +182,16 @@ def test_config4_batch_256_equals_single_domain_sweeps():
                              periodic=tuple(b == "periodic" for b in bc))
         doms.append(BatchDomain(img, bc, None))
     radii = list(range(24, 0, -1))
-    print("[batch] 256 domains, 24 radii, captured graph", flush=True)
-    sweep = BatchSweep(doms, radii, lanes=4, autotune=True, graph=True)
+    print("[batch] 256 domains, 24 radii, fused batch with full EDTs, captured graph", flush=True)
+    sweep = BatchSweep(doms, radii, autotune=True, graph=True, edt=True)
+    assert sweep.fused
     maps = sweep.run()
     maps = sweep.run()  # a replay
     torch.cuda.synchronize()
     for i, d in enumerate(doms):
         want = single_sweep(d.img, d.bc, radii, None)
         assert torch.equal(maps[i], want), f"domain {i} {tuple(d.img.shape)} {d.bc}"
+        assert torch.equal(sweep.edts[i], single_edt(d.img, d.bc)), f"edt {i} {tuple(d.img.shape)} {d.bc}"
 
 
 def test_config5_surrogate_512_heldout():

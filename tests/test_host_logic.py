@@ -100,3 +100,25 @@ def test_plain_subdomain_has_no_pads_or_reservoir():
                                inlet=((True, False), (False, False), (False, False)))
     assert not sd.padded and sd.voxels == (5, 6, 7)
     assert sd.reservoir_pad == ((0, 0),) * 3
+
+
+def test_tile_choice_cache_round_trip(tmp_path):
+    """pmx_tune_cache_load / pmx_tune_cache_save (host only): a saved file
+    loads back to the same choices, a missing file loads nothing, and a
+    malformed line is an error."""
+    import os
+
+    from pmmoto_amd import _lib
+    L = _lib.lib()
+    src = tmp_path / "in.txt"
+    src.write_text("# choices\n40 33 50 1 1 1 1 2 2 0 1 32\n7 8 9 0 0 2 2 1 1 1 0 64\n")
+    _lib.check(L.pmx_tune_cache_load(os.fsencode(src)))
+    out = tmp_path / "out.txt"
+    _lib.check(L.pmx_tune_cache_save(os.fsencode(out)))
+    lines = set(out.read_text().splitlines()[1:])
+    assert {"40 33 50 1 1 1 1 2 2 0 1 32", "7 8 9 0 0 2 2 1 1 1 0 64"} <= lines
+    _lib.check(L.pmx_tune_cache_load(os.fsencode(tmp_path / "missing.txt")))
+    bad = tmp_path / "bad.txt"
+    bad.write_text("40 33 50 1 1 1 1 2 2 0 1 48\n")
+    with pytest.raises(ValueError, match="not a tile choice"):
+        _lib.check(L.pmx_tune_cache_load(os.fsencode(bad)))

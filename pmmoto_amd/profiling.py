@@ -3,7 +3,12 @@
     rocprofv3 --kernel-trace --stats -d DIR_STATS -o run -- python3 bench.py --no-cpu --no-psd --steps 5 --warmup 1
     rocprofv3 --pmc FETCH_SIZE -d DIR_FETCH -o run -- python3 bench.py --no-cpu --no-psd --steps 2 --warmup 1 --calibrate-copy
     rocprofv3 --pmc WRITE_SIZE -d DIR_WRITE -o run -- python3 bench.py --no-cpu --no-psd --steps 2 --warmup 1 --calibrate-copy
-    python -m pmmoto_amd.profiling DIR_STATS DIR_FETCH DIR_WRITE OUT.json OUT.md
+    python -m pmmoto_amd.profiling DIR_STATS DIR_FETCH DIR_WRITE OUT.json OUT.md [--runs TEXT]
+
+    python -m pmmoto_amd.profiling table DIR_STATS OUT.md --runs TEXT [--note TEXT] [--pmc PMC_DIR ...]
+
+(the second form lists every kernel of a trace with the counters of any
+--pmc runs, e.g. the SQ instruction counters).
 
 (or the same three runs of ``python3 -m pmmoto_amd.surrogate.infer --calibrate-copy``
 for the surrogate's inference kernels).
@@ -125,15 +130,100 @@ def to_markdown(s: dict, command: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+def all_counters(pmc_dir: str) -> dict:
+    """kernel symbol -> {counter: mean over its dispatches} for every counter of a --pmc run."""
+    con = sqlite3.connect(_db(pmc_dir))
+    acc = defaultdict(lambda: defaultdict(list))
+    for k, c, v in con.execute("select kernel_name, counter_name, value from counters_collection"):
+        acc[_short(k)][c].append(v)
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in acc.items()}
+
+
+def table(argv) -> int:
+    """python -m pmmoto_amd.profiling table STATS_DIR OUT.md --runs TEXT [--note TEXT] [--match REGEX]
+    [--pmc PMC_DIR ...]: a kernel table (launches, mean and total time from the kernel trace) with the
+    per-dispatch mean of every counter the PMC runs recorded; FETCH_SIZE / WRITE_SIZE become HBM
+    bytes when the runs hold the 1 GiB calibration copy (bench.py --calibrate-copy)."""
+    import argparse
+    ap = argparse.ArgumentParser(prog="pmmoto_amd.profiling table")
+    ap.add_argument("stats")
+    ap.add_argument("out")
+    ap.add_argument("--runs", required=True, help="the rocprofv3 commands that made the directories")
+    ap.add_argument("--note", default="")
+    ap.add_argument("--match", default=".", help="kernels to list (regular expression on the symbol)")
+    ap.add_argument("--pmc", nargs="*", default=[], help="directories of rocprofv3 --pmc runs")
+    a = ap.parse_args(argv)
+    times = kernel_times(a.stats)
+    ctr = defaultdict(dict)
+    scale = {}
+    for d in a.pmc:
+        for k, cs in all_counters(d).items():
+            ctr[k].update(cs)
+        for name in ("FETCH_SIZE", "WRITE_SIZE"):
+            copies = [v for k, v in counter(d, name, max).items() if COPY.search(k)] if _has(d, name) else []
+            if copies:
+                scale[name] = GIB / (copies[0] * 1024.0)
+    rx = re.compile(a.match)
+    names = sorted((k for k in times if rx.search(k)), key=lambda k: -times[k]["avg_us"] * times[k]["calls"])
+    cols = sorted({c for k in names for c in ctr.get(k, {})})
+    head = ["kernel", "launches", "avg us", "total ms"] + cols
+    if "FETCH_SIZE" in scale and "WRITE_SIZE" in scale:
+        head += ["HBM GB/launch", "HBM GB/s"]
+    shown, skip = [], 0
+    for i, t in enumerate(argv):  # the --runs / --note texts are shown above and below, not twice
+        if skip:
+            skip -= 1
+            continue
+        if t in ("--runs", "--note"):
+            shown.append(f"{t} '…'")
+            skip = 1
+        else:
+            shown.append(t)
+    lines = [f"Runs (from /tmp, TMPDIR=/tmp): {a.runs}", "",
+             f"Command: `python -m pmmoto_amd.profiling table {' '.join(shown)}`", ""]
+    if a.note:
+        lines += [a.note, ""]
+    if scale:
+        lines += ["Counter scale (bytes per reported KiB, from the 1 GiB copy): " +
+                  ", ".join(f"{k} x{v / 1024 * 1024:.1f}" for k, v in sorted(scale.items())), ""]
+    lines += ["| " + " | ".join(head) + " |", "|" + "---|" * len(head)]
+    for k in names:
+        t = times[k]
+        row = [f"`{k}`", str(t["calls"]), f"{t['avg_us']:.1f}", f"{t['avg_us'] * t['calls'] / 1e3:.2f}"]
+        cs = ctr.get(k, {})
+        row += [f"{cs[c]:.4g}" if c in cs else "" for c in cols]
+        if "FETCH_SIZE" in scale and "WRITE_SIZE" in scale:
+            if "FETCH_SIZE" in cs and "WRITE_SIZE" in cs:
+                gb = (cs["FETCH_SIZE"] * 1024 * scale["FETCH_SIZE"] + cs["WRITE_SIZE"] * 1024 * scale["WRITE_SIZE"]) / 1e9
+                row += [f"{gb:.3f}", f"{gb / (t['avg_us'] * 1e-6):.0f}"]
+            else:
+                row += ["", ""]
+        lines.append("| " + " | ".join(row) + " |")
+    with open(a.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return 0
+
+
+def _has(pmc_dir: str, name: str) -> bool:
+    con = sqlite3.connect(_db(pmc_dir))
+    return con.execute("select count(*) from counters_collection where counter_name = ?", (name,)).fetchone()[0] > 0
+
+
 def main(argv) -> int:
-    if len(argv) != 6:
+    if len(argv) > 1 and argv[1] == "table":
+        return table(argv[2:])
+    runs = None
+    if len(argv) == 8 and argv[6] == "--runs":
+        runs = argv[7]
+    elif len(argv) != 6:
         print(__doc__)
         return 2
     s = summarize(argv[1], argv[2], argv[3])
     with open(argv[4], "w") as f:
         json.dump(s, f, indent=1)
+    md = to_markdown(s, "python -m pmmoto_amd.profiling " + " ".join(argv[1:6]) + (" --runs '…'" if runs else ""))
     with open(argv[5], "w") as f:
-        f.write(to_markdown(s, "python -m pmmoto_amd.profiling " + " ".join(argv[1:])))
+        f.write((f"Runs (from /tmp, TMPDIR=/tmp): {runs}\n\n" if runs else "") + md)
     return 0
 
 

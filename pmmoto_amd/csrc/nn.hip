@@ -511,6 +511,136 @@ __global__ __launch_bounds__(256) void k_conv_first(const uint8_t *__restrict__ 
     }
 }
 
+// The strided first layer with the block's input slab in LDS (stride 2, w a
+// multiple of 16).  A block's 16 output rows x 64 output columns of one
+// output plane read 3 input planes x 33 rows x 129 bytes; they are staged
+// once with 16-byte loads (zeros outside the box), and each lane then takes
+// its three bytes per tap row from LDS (two aligned dwords and alignbyte)
+// instead of two buffer loads per tap row and column block: the direct form
+// above issues 24 vector memory loads per lane per output row, mostly for
+// bytes its neighbours also load.  The product and the epilogue are the
+// same.
+constexpr int kSlabRows = 2 * kFirstRows + 1;   // input rows per staged plane
+constexpr int kSlabPitch = 16 + 2 * kFirstCols;  // 15 lead bytes, column -1, 128 columns
+
+template <int MB>
+__global__ __launch_bounds__(256) void k_conv_first_lds(const uint8_t *__restrict__ x, uint32_t x_bytes, ConvDims cd,
+                                                        int gy, int gx, int co, const uint16_t *__restrict__ w,
+                                                        const float *__restrict__ bias, uint16_t *__restrict__ out,
+                                                        int out_pad)
+{
+    // + 16: the dword pair of the last column of the last row reads 4 bytes past the slab
+    __shared__ __align__(16) uint8_t slab[3 * kSlabRows * kSlabPitch + 16];
+    const int lane = threadIdx.x & 63, g = lane >> 4, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    unsigned r = (unsigned)xcd_tile(blockIdx.x, gridDim.x);
+    const int bx = (int)(r % (unsigned)gx);
+    r /= (unsigned)gx;
+    const int by = (int)(r % (unsigned)gy);
+    r /= (unsigned)gy;
+    const int oz = (int)(r % (unsigned)cd.od);
+    const int b = (int)(r / (unsigned)cd.od);
+    const int oyb = by * kFirstRows;
+    const __amdgpu_buffer_rsrc_t xb = buffer(x, x_bytes);
+    const int plane = cd.h * cd.w;
+    const int sample_off = b * cd.d * plane;
+    const int ix0 = 2 * kFirstCols * bx;  // input column of slab byte 16
+    // staging: per slab row, eight 16-byte chunks and the byte of column ix0 - 1
+    for (int t = threadIdx.x; t < 3 * kSlabRows * 9; t += 256) {
+        const int row = t / 9, part = t - 9 * row;
+        const int pz = row / kSlabRows, py = row - pz * kSlabRows;
+        const int iz = oz * 2 - 1 + pz, iy = oyb * 2 - 1 + py;
+        const bool in = iz >= 0 && iz < cd.d && iy >= 0 && iy < cd.h;
+        const uint32_t rowoff = (uint32_t)(sample_off + iz * plane + iy * cd.w);
+        uint8_t *dst = slab + row * kSlabPitch;
+        if (part < 8) {
+            const int ix = ix0 + 16 * part;
+            const uint32_t o = in && ix < cd.w ? rowoff + (uint32_t)ix : kOob;
+            *reinterpret_cast<uint4 *>(dst + 16 + 16 * part) =
+                __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xb, o, 0, 0));
+        } else {
+            const uint32_t o = in && ix0 > 0 ? rowoff + (uint32_t)(ix0 - 1) : kOob;
+            dst[15] = (uint8_t)__builtin_amdgcn_raw_buffer_load_b8(xb, o, 0, 0);
+        }
+    }
+    // weights: A[ch = l % 16 + 16 mb][slot 16 j + 4 g + kx]
+    bf16x4 wa[MB][3];
+    float bs[MB][4];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+        const int ch = 16 * mb + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int row = 4 * j + g;
+            short e[4] = {0, 0, 0, 0};
+            if (ch < co && row < 9) {
+#pragma unroll
+                for (int kx = 0; kx < 3; ++kx) e[kx] = (short)w[ch * 27 + row * 3 + kx];
+            }
+            wa[mb][j] = bf16x4{e[0], e[1], e[2], e[3]};
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = 16 * mb + 4 * g + v;
+            bs[mb][v] = c < co ? bias[c] : 0.f;
+        }
+    }
+    // this lane's tap rows for the three MFMA steps (rows 9..11 hold zero)
+    int tz[3], ty[3];
+    bool tap[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int row = 4 * j + g;
+        tap[j] = row < 9;
+        tz[j] = tap[j] ? row / 3 : 0;
+        ty[j] = tap[j] ? row - 3 * (row / 3) : 0;
+    }
+    __syncthreads();
+    const int oy0 = oyb + wv * kFirstWaveRows;
+    if (oy0 >= cd.oh) return;  // wave-uniform, after the block's barrier
+    const int cl = 2 * (lane & 15) + 15;  // slab byte of this lane's first input column (column block 0)
+    const int odp = cd.od + 2 * out_pad, ohp = cd.oh + 2 * out_pad, owp = cd.ow + 2 * out_pad;
+    const int ox_l = bx * kFirstCols + (lane & 15);
+    for (int rr = 0; rr < kFirstWaveRows; ++rr) {
+        const int oy = oy0 + rr;
+        if (oy >= cd.oh) break;  // wave-uniform
+        bf16x4 tp[4][3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int rowb = (tz[j] * kSlabRows + 2 * (oy - oyb) + ty[j]) * kSlabPitch;
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const int o = rowb + cl + 32 * nb;
+                const uint32_t *d = reinterpret_cast<const uint32_t *>(slab + (o & ~3));
+                uint32_t v = __builtin_amdgcn_alignbyte(d[1], d[0], (uint32_t)(o & 3));
+                v = tap[j] ? v : 0u;
+                // byte -> float -> bf16 (exact for 0..255); perm takes the high halves
+                const float f0 = (float)(v & 0xffu), f1 = (float)((v >> 8) & 0xffu), f2 = (float)((v >> 16) & 0xffu);
+                const uint32_t p01 = __builtin_amdgcn_perm(__float_as_uint(f1), __float_as_uint(f0), 0x07060302u);
+                const uint32_t p2 = __float_as_uint(f2) >> 16;
+                tp[nb][j] = bf16x4{(short)(p01 & 0xffffu), (short)(p01 >> 16), (short)p2, (short)0};
+            }
+        }
+        uint16_t *orow = out + ((((int64_t)b * odp + oz + out_pad) * ohp + oy + out_pad) * owp + out_pad + ox_l) * co;
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            const int ox = ox_l + 16 * nb;
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                floatx4 acc = {bs[mb][0], bs[mb][1], bs[mb][2], bs[mb][3]};  // the bias rides in the accumulator
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[mb][j], tp[nb][j], acc, 0, 0, 0);
+                const int c0 = 16 * mb + 4 * g;
+                if (ox < cd.ow && c0 < co) {
+                    const uint32_t q0 = pack_bf16(relu_bits(acc[0]), relu_bits(acc[1]));
+                    const uint32_t q1 = pack_bf16(relu_bits(acc[2]), relu_bits(acc[3]));
+                    *reinterpret_cast<uint2 *>(orow + 16 * nb * co + c0) = make_uint2(q0, q1);
+                }
+            }
+        }
+    }
+}
+
 // Implicit-GEMM convolution for c % 8 == 0 on the matrix cores, computed as
 // out^T (channels x positions) = W (channels x K) . P^T (K x positions) with
 // k = tap * c + ci.  Wave = 16 P output positions x 16 CB output channels;
@@ -848,6 +978,7 @@ int pmx_conv3d_first(const uint8_t *x, const int32_t *in_shape, int stride, int 
     if (sample >= pmx::kOob) return pmx::fail(PMX_ERR_INVALID, "one sample exceeds 2 GiB");
     const int64_t chunk = std::max<int64_t>(1, (int64_t)(pmx::kOob - 1) / sample);
     const bool fast = stride == 2 && cd.w % 4 == 0;
+    const bool slab = stride == 2 && cd.w % 16 == 0;  // the LDS-staged form
     const int gy = (cd.oh + pmx::kFirstRows - 1) / pmx::kFirstRows, gx = (cd.ow + pmx::kFirstCols - 1) / pmx::kFirstCols;
     const int64_t out_sample = (int64_t)(cd.od + 2 * out_pad) * (cd.oh + 2 * out_pad) * (cd.ow + 2 * out_pad) * co;
     for (int64_t n0 = 0; n0 < cd.n; n0 += chunk) {
@@ -861,12 +992,18 @@ int pmx_conv3d_first(const uint8_t *x, const int32_t *in_shape, int stride, int 
         const dim3 grid((unsigned)blocks);
 #define PMX_FIRST(MB, F) \
     hipLaunchKernelGGL((pmx::k_conv_first<MB, F>), grid, dim3(256), 0, s, xc, xbytes, c, gy, gx, co, w_bf16, bias, oc, out_pad)
-        if (co <= 16) {
+#define PMX_FIRST_LDS(MB)                                                                                  \
+    hipLaunchKernelGGL((pmx::k_conv_first_lds<MB>), grid, dim3(256), 0, s, xc, xbytes, c, gy, gx, co, w_bf16, bias, \
+                       oc, out_pad)
+        if (slab) {
+            if (co <= 16) PMX_FIRST_LDS(1); else PMX_FIRST_LDS(2);
+        } else if (co <= 16) {
             if (fast) PMX_FIRST(1, true); else PMX_FIRST(1, false);
         } else {
             if (fast) PMX_FIRST(2, true); else PMX_FIRST(2, false);
         }
 #undef PMX_FIRST
+#undef PMX_FIRST_LDS
         if (int e = pmx::check_launch("conv3d_first")) return e;
     }
     return PMX_OK;

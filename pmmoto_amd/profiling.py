@@ -43,7 +43,7 @@ BENCH_NAMES = (
     (re.compile(r"k_fallback_y<int>"), "edt_fallback_y32"),
     (re.compile(r"k_fallback_x<[\w:() ]*Sq16>"), "edt_fallback_x"),
     (re.compile(r"k_fallback_x<int>"), "edt_fallback_x32"),
-    (re.compile(r"k_conv_first<"), None),  # surrogate inference kernels: reported under their symbol
+    (re.compile(r"k_conv_first"), None),  # surrogate inference kernels: reported under their symbol
     (re.compile(r"k_conv_igemm<"), None),
 )
 COPY = re.compile(r"copyBuffer")

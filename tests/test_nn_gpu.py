@@ -68,7 +68,10 @@ def _pad(a):
 
 @pytest.mark.parametrize("co,s,shape", [(8, 2, (2, 9, 7, 10)), (16, 1, (1, 5, 6, 7)), (32, 2, (3, 4, 4, 4)),
                                         (8, 2, (2, 6, 19, 16)), (16, 2, (1, 3, 5, 136)), (8, 1, (1, 3, 4, 130)),
-                                        (12, 2, (1, 4, 6, 8)), (24, 1, (2, 3, 3, 5))])
+                                        (12, 2, (1, 4, 6, 8)), (24, 1, (2, 3, 3, 5)),
+                                        # the LDS-staged form (stride 2, w % 16 == 0): two column blocks,
+                                        # a partial row block, odd depth, one and two channel blocks
+                                        (8, 2, (2, 7, 35, 256)), (24, 2, (1, 5, 40, 48)), (12, 2, (3, 3, 2, 32))])
 @pytest.mark.parametrize("out_pad", [0, 1])
 def test_first_layer_conv_matches_oracle(co, s, shape, out_pad):
     """uint8 first-layer convolution on the matrix cores, bias + ReLU + BF16 fused."""
@@ -233,3 +236,24 @@ def test_training_beats_the_mean_curve_and_fallback_is_exact():
     got, sig, fall = sp.curves(xt, bt)
     assert fall.size > 0
     np.testing.assert_array_equal(got[fall], exact_curves([xt[i] for i in fall], [bt[i] for i in fall], radii))
+
+
+def test_graphed_inference_equals_eager_on_every_replay():
+    """predict() through captured graphs equals the eager launches bit for bit:
+    two inputs, a partial last batch (another batch shape), eager work
+    between replays, and the first input again after the shape changed."""
+    import torch
+
+    from pmmoto_amd.surrogate import SurrogateNet
+    net = SurrogateNet(size=32, channels=(8, 16, 32, 32), hidden=32, q=8, seed=3)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    xa = (torch.rand((40, 32, 32, 32), generator=g, device="cuda") < 0.45).to(torch.uint8)
+    xb = (torch.rand((40, 32, 32, 32), generator=g, device="cuda") < 0.55).to(torch.uint8)
+    want = {k: [t.clone() for t in net.predict(x, batch=16, graph=False)] for k, x in (("a", xa), ("b", xb))}
+    for rep in range(3):
+        for k, x in (("a", xa), ("b", xb)):
+            print(f"[graphed inference] replay {rep} input {k}", flush=True)
+            curve, sigma = net.predict(x, batch=16, graph=True)
+            assert torch.equal(curve, want[k][0]) and torch.equal(sigma, want[k][1]), (rep, k)
+            torch.zeros(1 << 20, device="cuda").add_(1.0)  # eager work between replays
+        torch.cuda.synchronize()

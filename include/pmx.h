@@ -147,6 +147,22 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
                 const float *resolution, int take_sqrt, void *workspace, size_t workspace_bytes,
                 hipStream_t stream);
 
+/* No counterpart in the reference: introspection of pmx_edt_ref.  For edt3d
+ * (pads NULL, first_axis 0) at one power-of-two resolution on every axis and
+ * extents up to 1024, pmx_edt_ref first runs the same arithmetic in integer
+ * form (every float the reference forms there is an exact multiple of
+ * res^2; only the boundary hulls depend on rounding, and they are computed
+ * in float), and falls back to the float kernels, on the device, when the
+ * image or its distances leave that form's range.  pmx_edt_ref_int_form
+ * switches the integer form on or off for the process (for comparisons) and
+ * returns the previous setting.  pmx_edt_ref_last_form waits for `stream` and
+ * reports which path produced the last result written with this workspace
+ * (box n): 0 the integer form, 1 the float kernels after the integer form
+ * set its result aside, 2 the float kernels alone. */
+int pmx_edt_ref_int_form(int enable);
+int pmx_edt_ref_last_form(const int64_t *n, const void *workspace, size_t workspace_bytes,
+                          hipStream_t stream, int *form);
+
 /* The reference's halo-image morphology at any resolution
  * (morphological_operators.py:103-228): the padded uint8 array (shape[3],
  * pad widths pad[6]) is extended by halo_width[3] voxels per side (periodic

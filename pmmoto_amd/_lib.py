@@ -59,6 +59,8 @@ SIGNATURES = {
     "pmx_edt_noalloc": (C.c_int, [P, P, P, P, P, P, P]),
     "pmx_edt_ref_workspace_size": (C.c_int, [P, P]),
     "pmx_edt_ref": (C.c_int, [P, P, P, P, P, C.c_int, P, P, P, C.c_int, P, SZ, P]),
+    "pmx_edt_ref_int_form": (C.c_int, [C.c_int]),
+    "pmx_edt_ref_last_form": (C.c_int, [P, P, SZ, P, P]),
     "pmx_halo_morph_workspace_size": (C.c_int, [P, P, P]),
     "pmx_halo_morph": (C.c_int, [P, P, P, P, P, C.c_int, C.c_double, C.c_int, U8, U8, U32, P, P, SZ, P]),
     "pmx_morph": (C.c_int, [P, P, C.c_int, I64, U8, U8, U32, P, P, SZ, P]),

@@ -618,7 +618,24 @@ struct WinGeom {
     int64_t pitch;  // input row length in memory: a multiple of 8 >= nz, 16-byte aligned rows
     int64_t out_os, out_rs;  // output strides of the outer index and of a row (pass Y)
     int tpb = kTilesPerBlock;  // row tiles per block (a power of two <= kTilesPerBlock)
+    // truncated periodic wrap (the windowed kernels' kHull form): per line,
+    // the wrapped rows that may serve as candidates (see hull_row_kept)
+    const uint32_t *hmask = nullptr;
 };
+
+// Truncated wrap of a periodic line: a row r < 0 or r >= n is a candidate
+// only if hmask marks it.  Words 0..3 of a line's eight mark the rows -1 - b
+// (bit b), words 4..7 the rows n + b, b < kHullMaskRows; rows farther out are
+// never candidates (a scan stops at kReach rows, so none is read).
+constexpr int kHullMaskRows = 128;
+constexpr int kHullMaskWords = 8;
+__device__ __forceinline__ bool hull_row_kept(const uint32_t *__restrict__ m, int64_t line, int r, int n)
+{
+    const int b = r < 0 ? -1 - r : r - n;
+    if (b >= kHullMaskRows) return false;
+    const uint32_t w = m[line * kHullMaskWords + (r < 0 ? 0 : 4) + (b >> 5)];
+    return (w >> (b & 31)) & 1u;
+}
 
 // blocks of a windowed pass with T-row tiles and tpb tiles per block
 __host__ __device__ __forceinline__ int64_t window_blocks(const WinGeom &g, int T, int tpb)
@@ -659,10 +676,12 @@ __device__ __forceinline__ int decode(uint16_t d, int cap)
 __device__ __forceinline__ int decode(int32_t v, int cap) { return min(v, cap); }
 __device__ __forceinline__ int decode(Sq16 v, int cap) { return min((int)v.v, cap); }
 
-// input at row r of the line starting at `line`, under the boundary rule
-template <class E>
-__device__ __forceinline__ int fetch(const E *line, const WinGeom &g, int r)
+// input at row r of the line starting at `line` (number line_id), under the
+// boundary rule; kHull: wrapped rows only where the line's mask keeps them
+template <bool kHull, class E>
+__device__ __forceinline__ int fetch(const E *line, const WinGeom &g, int r, int64_t line_id)
 {
+    if (kHull && (r < 0 || r >= g.n) && !hull_row_kept(g.hmask, line_id, r, g.n)) return g.cap;
     const int m = map_row(r, g.n, g.bc_lo, g.bc_hi);
     if (m >= 0) return decode(line[(int64_t)m * g.rs], g.cap);
     return m == kRowWall ? 0 : g.cap;
@@ -733,10 +752,13 @@ template <> struct TileIO<uint16_t> {
 
 // clamp (columns past nz hold 0, settled at once; rows beyond the line the
 // boundary value) and store at tile rows dst + row0 + 16 i; acc keeps the
-// per-column minimum
-template <class E, int N, class V, class L>
+// per-column minimum.  first: the line row of the first staged row (as
+// issue_rows takes it); line0: the line number of this thread's first
+// column (kHull: wrapped rows go through the line masks).
+template <bool kHull, class E, int N, class V, class L>
 __device__ __forceinline__ void commit_rows(L *tile, const WinGeom &g, int dst, int row0, int q, int zl,
-                                            const int (&mrow)[N], const V (&raw)[N], int4 &acc, bool inside)
+                                            const int (&mrow)[N], const V (&raw)[N], int4 &acc, bool inside,
+                                            int first, int64_t line0)
 {
     // inside (wave-uniform): every staged row lies in the line; with a full quad of columns
     // the values go to the tile as loaded
@@ -756,7 +778,16 @@ __device__ __forceinline__ void commit_rows(L *tile, const WinGeom &g, int dst, 
     for (int i = 0; i < N; ++i) {
         const bool in_line = mrow[i] >= 0;
         const int c = mrow[i] == kRowWall ? 0 : g.cap;
-        const int4 u = unpack<E>(raw[i], g.cap);
+        int4 u = unpack<E>(raw[i], g.cap);
+        if constexpr (kHull) {
+            const int r = first + row0 + 16 * i;
+            if (r < 0 || r >= g.n) {
+                if (zl >= 0 && !hull_row_kept(g.hmask, line0 + 0, r, g.n)) u.x = g.cap;
+                if (zl >= 1 && !hull_row_kept(g.hmask, line0 + 1, r, g.n)) u.y = g.cap;
+                if (zl >= 2 && !hull_row_kept(g.hmask, line0 + 2, r, g.n)) u.z = g.cap;
+                if (zl >= 3 && !hull_row_kept(g.hmask, line0 + 3, r, g.n)) u.w = g.cap;
+            }
+        }
         int4 v;
         v.x = in_line ? (zl >= 0 ? u.x : 0) : c;
         v.y = in_line ? (zl >= 1 ? u.y : 0) : c;
@@ -779,7 +810,7 @@ __device__ __forceinline__ void commit_rows(L *tile, const WinGeom &g, int dst, 
 // is scanned), so each row is loaded once per block.  D2ONLY (pass X):
 // the int32 squared distance is the only output, written without the
 // per-row tests of the other outputs.
-template <int T, int W, class E, class O, bool D2ONLY>
+template <int T, int W, class E, class O, bool D2ONLY, bool kHull = false>
 __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in, O *__restrict__ out,
                                             const PassXOut &o, const WinGeom &g, int nx, int ny,
                                             uint32_t *__restrict__ line_bits, uint32_t *__restrict__ ovf_count,
@@ -832,7 +863,8 @@ __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in,
         int fm[kRows / 16];
         V fr[kRows / 16];
         issue_rows<kRows / 16>(base, g, r0 - W, row0, c4, fm, fr);
-        commit_rows<E, kRows / 16>(tile, g, 0, row0, q, zl, fm, fr, acc, r0 - W >= 0 && r0 - W + kRows <= g.n);
+        commit_rows<kHull, E, kRows / 16>(tile, g, 0, row0, q, zl, fm, fr, acc, r0 - W >= 0 && r0 - W + kRows <= g.n,
+                                          r0 - W, (int64_t)outer * g.nz + z0 + 4 * q);
     }
     for (int rt = chunk * g.tpb; rt < rt_end; ++rt, r0 += T) {
 #pragma unroll
@@ -874,10 +906,11 @@ __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in,
                 const uint32_t bit = 1u << (line & 31);
                 if (!(line_bits[line >> 5] & bit)) {
                     const E *ln = in + (int64_t)outer * g.os + z;
-                    const int kmax = g.bc_lo == PMX_BC_PERIODIC ? g.n / 2 : max(y + 1, g.n - y);
+                    // kHull: candidates lie in [-n, 2n), no shorter way round
+                    const int kmax = kHull ? 2 * g.n : g.bc_lo == PMX_BC_PERIODIC ? g.n / 2 : max(y + 1, g.n - y);
                     for (; k <= kReach && k <= kmax; ++k) {
                         if (k * k >= best) break;
-                        best = min(best, min(fetch(ln, g, y - k), fetch(ln, g, y + k)) + k * k);
+                        best = min(best, min(fetch<kHull>(ln, g, y - k, line), fetch<kHull>(ln, g, y + k, line)) + k * k);
                     }
                     ok = k * k >= best || k > kmax;
                 }
@@ -925,10 +958,10 @@ __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in,
                     ok = false;
                     if (!(line_bits[ln_id >> 5] & bit)) {
                         const E *ln = in + (int64_t)outer * g.os + zc;
-                        const int kmax = g.bc_lo == PMX_BC_PERIODIC ? g.n / 2 : max(y + 1, g.n - y);
+                        const int kmax = kHull ? 2 * g.n : g.bc_lo == PMX_BC_PERIODIC ? g.n / 2 : max(y + 1, g.n - y);
                         for (; k <= kReach && k <= kmax; ++k) {
                             if (k * k >= bv) break;
-                            bv = min(bv, min(fetch(ln, g, y - k), fetch(ln, g, y + k)) + k * k);
+                            bv = min(bv, min(fetch<kHull>(ln, g, y - k, ln_id), fetch<kHull>(ln, g, y + k, ln_id)) + k * k);
                         }
                         ok = k * k >= bv || k > kmax;
                     }
@@ -1041,11 +1074,13 @@ __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in,
             acc.z = min(acc.z, h[i].z);
             acc.w = min(acc.w, h[i].w);
         }
-        commit_rows<E, T / 16>(tile, g, 2 * W, row0, q, zl, xm, xr, acc, r0 + T + W >= 0 && r0 + 2 * T + W <= g.n);
+        commit_rows<kHull, E, T / 16>(tile, g, 2 * W, row0, q, zl, xm, xr, acc,
+                                      r0 + T + W >= 0 && r0 + 2 * T + W <= g.n, r0 + T + W,
+                                      (int64_t)outer * g.nz + z0 + 4 * q);
     }
 }
 
-template <int T, int W, class E, class O, bool D2ONLY = false>
+template <int T, int W, class E, class O, bool D2ONLY = false, bool kHull = false>
 __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in, O *__restrict__ out,
                                                         PassXOut o, WinGeom g, int nx, int ny,
                                                         uint32_t *__restrict__ line_bits,
@@ -1054,7 +1089,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
                                                         const unsigned int *__restrict__ proceed)
 {
     if (proceed && *proceed == 0u) return;
-    window_body<T, W, E, O, D2ONLY>(xcd_tile(blockIdx.x, window_blocks(g, T, g.tpb)), in, out, o, g, nx, ny,
+    window_body<T, W, E, O, D2ONLY, kHull>(xcd_tile(blockIdx.x, window_blocks(g, T, g.tpb)), in, out, o, g, nx, ny,
                                     line_bits, ovf_count, ovf_list);
 }
 

@@ -45,7 +45,7 @@
 //
 // Compiled with contraction off (the reference's x86-64 build has no FMA),
 // so a*b+c rounds twice.
-#include "common.hpp"
+#include "edt_kernels.hpp"
 
 namespace pmx {
 namespace {
@@ -63,7 +63,7 @@ constexpr int kLdsCap = PMX_REF_LDS_CAP;
 #endif
 // LDS ring entries per lane of the streaming parabolic pass
 constexpr int kRingSlots = PMX_REF_RING;
-constexpr int kChunk = 8;        // input positions staged per lane at a time
+constexpr int kStage = 8;        // input positions staged per lane at a time
 constexpr int kEvalPerStep = 2;  // positions evaluated per input position
 constexpr int kFbMaxThreads = 32768;   // persistent lanes of the fallback kernels, at most
 
@@ -149,8 +149,10 @@ __device__ int nearest_zero(const uint8_t *img, int n, int64_t stride, bool forw
 
 // pass 0 along g.dim from the image into out (squared, INF -> FLT_MAX)
 __global__ __launch_bounds__(256) void k_ref_pass0(const uint8_t *__restrict__ img, float *__restrict__ out,
-                                                   LineGeom g, float res, int periodic, Skips sk)
+                                                   LineGeom g, float res, int periodic, Skips sk,
+                                                   const unsigned *__restrict__ gate)
 {
+    if (gate && *gate == 0u) return;
     const int64_t line = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (line >= g.lines()) return;
     const int n = (int)g.n[g.dim];
@@ -235,7 +237,7 @@ __host__ __device__ __forceinline__ float intersect(float f_r, float r, float f_
 
 // A lane's stack: entry k at base[k * step] (LDS: step = lanes of the
 // wave; workspace: step = lanes of the grid).
-struct Stack {
+struct LineStack {
     int *hv;
     float *hh, *rg;
     int64_t step;
@@ -245,7 +247,7 @@ struct Stack {
 };
 
 // update_hull (_distance.hpp:89-112); false when entry k+1 would reach cap
-__host__ __device__ __forceinline__ bool update_hull(const Stack &S, int &k, int i, float fv, float aniso, int cap)
+__host__ __device__ __forceinline__ bool update_hull(const LineStack &S, int &k, int i, float fv, float aniso, int cap)
 {
     float s = intersect<>(fv, (float)i, S.H(k), (float)S.V(k), aniso);
     while (k > 0 && s <= S.R(k)) {
@@ -261,28 +263,51 @@ __host__ __device__ __forceinline__ bool update_hull(const Stack &S, int &k, int
     return true;
 }
 
+// Pass inputs as the reference's float32 heights: the float array itself,
+// or (the integer form of the reference arithmetic at a power-of-two
+// resolution) a 1-D distance c or a squared distance m in voxel units, whose
+// float heights (c * res)^2 = aniso * c^2 and aniso * m are exact.  A line
+// with no zero along the first axis holds kNoSolid, the reference's
+// infinity, which get_initial_envelope turns into FLT_MAX (_distance.pyx:351).
+struct AsFloat {
+    __host__ __device__ __forceinline__ float operator()(float v) const { return v; }
+};
+struct FromDist {
+    float aniso;
+    __device__ __forceinline__ float operator()(uint16_t c) const
+    {
+        return c == kNoSolid ? kFltMax : aniso * (float)((uint32_t)c * c);
+    }
+};
+struct FromSq {
+    float aniso;
+    __device__ __forceinline__ float operator()(uint16_t m) const { return aniso * (float)m; }
+};
+
 // return_boundary_hull (_distance.hpp:296-356) on n values at p (stride
 // st); writes <= kHullCap members to out and returns the count, or -1 when
 // the envelope outgrows cap.  A one-voxel segment with no finite value reads
 // past the reference's vectors there (undefined behaviour); this path then
 // returns the zero-initialised slot, as the oracle does.
-__host__ __device__ int boundary_hull(const float *p, int n, int64_t st, float res, int index_corrector, bool forward,
-                             const Stack &S, int cap, RefHull *out)
+template <class T, class Cv>
+__host__ __device__ int boundary_hull(const T *p, Cv cv, int n, int64_t st, float res, int index_corrector,
+                                      bool forward, const LineStack &S, int cap, RefHull *out)
 {
-    if (n == 1 && p[0] < kFltMax) {
-        out[0] = {index_corrector, p[0], INFINITY};
+    const float p0 = cv(p[0]);
+    if (n == 1 && p0 < kFltMax) {
+        out[0] = {index_corrector, p0, INFINITY};
         return 1;
     }
     const float aniso = res * res;
     S.V(0) = 0;
-    S.H(0) = p[0];
+    S.H(0) = p0;
     S.V(1) = 0;
     S.H(1) = 0.0f;
     S.R(0) = -INFINITY;
     S.R(1) = INFINITY;
     int k = 1;
     bool fits = true;
-    stream_values<1>(p, st, 1, n, [&](int i, float v) { return fits = update_hull(S, k, i, v, aniso, cap); });
+    stream_values<1>(p, st, 1, n, [&](int i, T v) { return fits = update_hull(S, k, i, cv(v), aniso, cap); });
     if (!fits) return -1;
     const int num_hull = min(kNumHull, k);
     int cnt = 0, kk = forward ? 0 : k;
@@ -300,8 +325,9 @@ __host__ __device__ int boundary_hull(const float *p, int n, int64_t st, float r
 // Both face hulls of one line (get_boundary_hull, _distance.pyx:544-618):
 // side 0 forward (from the lower face to the first zero), side 1 backward.
 // Returns false when a segment outgrows cap.
-__device__ bool line_hulls(const uint8_t *og, const float *f, const LineGeom &g, float res, Skips sk,
-                           const Stack &S, int cap, RefHull *hulls, int *counts)
+template <class T, class Cv>
+__device__ bool line_hulls(const uint8_t *og, const T *f, Cv cv, const LineGeom &g, float res, Skips sk,
+                           const LineStack &S, int cap, RefHull *hulls, int *counts)
 {
     const int n = (int)g.n[g.dim];
     const int64_t st = g.fs[g.dim], ist = g.is[g.dim];
@@ -326,7 +352,7 @@ __device__ bool line_hulls(const uint8_t *og, const float *f, const LineGeom &g,
         }
         int c = 0;
         if (nv > 0) {
-            c = boundary_hull(f + (int64_t)start * st, nv, st, res, ic, fw, S, cap, hulls + side * kHullCap);
+            c = boundary_hull(f + (int64_t)start * st, cv, nv, st, res, ic, fw, S, cap, hulls + side * kHullCap);
             if (c < 0) return false;
         }
         counts[side] = c;
@@ -334,58 +360,94 @@ __device__ bool line_hulls(const uint8_t *og, const float *f, const LineGeom &g,
     return true;
 }
 
+// The wrapped rows a line's hulls keep, as the windowed kernels read them
+// (hull_row_kept): lower-hull vertices v < 0 (side 1) mark row v, upper-hull
+// vertices v >= 0 (side 0, pushed at v + n) row n + v.
+__device__ __forceinline__ void write_hull_mask(const RefHull *h, const int *counts, uint32_t *m)
+{
+    uint32_t w[kHullMaskWords] = {};
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int c = counts[side];
+#pragma unroll
+        for (int j = 0; j < kHullCap; ++j) {
+            if (j >= c) break;
+            const int v = h[side * kHullCap + j].vertex;
+            const int b = side ? -1 - v : v;
+            if (b < 0 || b >= kHullMaskRows) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if ((b >> 5) == q) w[(side ? 0 : 4) + q] |= 1u << (b & 31);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kHullMaskWords; ++q) m[q] = w[q];
+}
+
 // hulls[line][side][kHullCap], counts[line][side]; lines whose segments
-// outgrow the LDS stack go to the overflow list
-__global__ __launch_bounds__(kWave) void k_ref_hulls(const uint8_t *__restrict__ og, const float *__restrict__ f,
+// outgrow the LDS stack go to the overflow list.  masks: also the wrapped
+// rows each line keeps (write_hull_mask).  Nothing runs while *gate == 0.
+template <class T, class Cv>
+__global__ __launch_bounds__(kWave) void k_ref_hulls(const uint8_t *__restrict__ og, const T *__restrict__ f, Cv cv,
                                                      LineGeom g, float res, Skips sk, RefHull *__restrict__ hulls,
                                                      int *__restrict__ counts, int *__restrict__ ovf,
-                                                     int *__restrict__ n_ovf)
+                                                     int *__restrict__ n_ovf, uint32_t *__restrict__ masks,
+                                                     const unsigned *__restrict__ gate)
 {
+    if (gate && *gate == 0u) return;
     __shared__ int sv[kLdsCap * kWave];
     __shared__ float sh[kLdsCap * kWave], sr[kLdsCap * kWave];
     const int64_t line = (int64_t)blockIdx.x * kWave + threadIdx.x;
     if (line >= g.lines()) return;
-    Stack S{sv + threadIdx.x, sh + threadIdx.x, sr + threadIdx.x, kWave};
+    LineStack S{sv + threadIdx.x, sh + threadIdx.x, sr + threadIdx.x, kWave};
     int64_t ib, fb, ob;
     g.base(line, &ib, &fb, &ob);
-    if (!line_hulls(og + ib, f + fb, g, res, sk, S, kLdsCap, hulls + line * 2 * kHullCap, counts + line * 2))
+    RefHull *h = hulls + line * 2 * kHullCap;
+    if (!line_hulls(og + ib, f + fb, cv, g, res, sk, S, kLdsCap, h, counts + line * 2))
         ovf[atomicAdd(n_ovf, 1)] = (int)line;
+    else if (masks)
+        write_hull_mask(h, counts + line * 2, masks + line * kHullMaskWords);
 }
 
 // A lane's stack of cap entries for the run and hull fallbacks: dynamic LDS
 // of [cap][lanes of the workgroup] ints, floats, floats (kLds), else the
 // workspace `gstk` as [cap][lanes of the grid].
 template <bool kLds>
-__device__ __forceinline__ Stack lane_stack(int *dyn, int *gstk, int cap)
+__device__ __forceinline__ LineStack lane_stack(int *dyn, int *gstk, int cap)
 {
     if (kLds) {
         const int lanes = blockDim.x;
         float *f = reinterpret_cast<float *>(dyn);
-        return Stack{dyn + threadIdx.x, f + (size_t)cap * lanes + threadIdx.x,
+        return LineStack{dyn + threadIdx.x, f + (size_t)cap * lanes + threadIdx.x,
                      f + (size_t)2 * cap * lanes + threadIdx.x, lanes};
     }
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, T = (int64_t)gridDim.x * blockDim.x;
     float *f = reinterpret_cast<float *>(gstk);
-    return Stack{gstk + tid, f + (int64_t)cap * T + tid, f + (int64_t)2 * cap * T + tid, T};
+    return LineStack{gstk + tid, f + (int64_t)cap * T + tid, f + (int64_t)2 * cap * T + tid, T};
 }
 
 // the overflow list of k_ref_hulls, stacks of cap entries (a whole line)
-template <bool kLds>
-__global__ __launch_bounds__(kWave) void k_ref_hulls_fallback(const uint8_t *__restrict__ og,
-                                                              const float *__restrict__ f, LineGeom g, float res,
-                                                              Skips sk, RefHull *__restrict__ hulls,
-                                                              int *__restrict__ counts, const int *__restrict__ ovf,
-                                                              const int *__restrict__ n_ovf, int *gstk, int cap)
+template <bool kLds, class T, class Cv>
+__global__ __launch_bounds__(kWave) void k_ref_hulls_fallback(const uint8_t *__restrict__ og, const T *__restrict__ f,
+                                                              Cv cv, LineGeom g, float res, Skips sk,
+                                                              RefHull *__restrict__ hulls, int *__restrict__ counts,
+                                                              const int *__restrict__ ovf,
+                                                              const int *__restrict__ n_ovf, int *gstk, int cap,
+                                                              uint32_t *__restrict__ masks,
+                                                              const unsigned *__restrict__ gate)
 {
+    if (gate && *gate == 0u) return;
     extern __shared__ int dyn[];
-    const Stack S = lane_stack<kLds>(dyn, gstk, cap);
-    const int64_t T = (int64_t)gridDim.x * blockDim.x;
+    const LineStack S = lane_stack<kLds>(dyn, gstk, cap);
+    const int64_t nt = (int64_t)gridDim.x * blockDim.x;
     const int m = *n_ovf;
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += T) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += nt) {
         const int64_t line = ovf[j];
         int64_t ib, fb, ob;
         g.base(line, &ib, &fb, &ob);
-        line_hulls(og + ib, f + fb, g, res, sk, S, cap, hulls + line * 2 * kHullCap, counts + line * 2);
+        RefHull *h = hulls + line * 2 * kHullCap;
+        line_hulls(og + ib, f + fb, cv, g, res, sk, S, cap, h, counts + line * 2);
+        if (masks) write_hull_mask(h, counts + line * 2, masks + line * kHullMaskWords);
     }
 }
 
@@ -432,7 +494,7 @@ struct Chord {
 //     (its covering entry and that entry's range are then final), at most
 //     kEvalPerStep positions per input position, so the lanes of a wave do
 //     the same work per step; the rest is evaluated after the upper hull;
-//   * the input arrives kChunk positions at a time through `stage` (this
+//   * the input arrives kStage positions at a time through `stage` (this
 //     lane's column of an LDS tile, stride `lanes`), the next chunk's loads
 //     in flight while the current one is processed;
 //   * a run whose live entries outgrow the ring (after everything final is
@@ -548,20 +610,20 @@ __host__ __device__ void parabolic_stream(const float *fin, float *fout, int n, 
     };
     bool skip = false;  // inside a run handed on, which starts at p1
     int p1 = -1;
-    float pre[kChunk];
+    float pre[kStage];
 #pragma unroll
-    for (int c = 0; c < kChunk; ++c) pre[c] = c < n ? fin[(int64_t)c * ist] : 0.0f;
-    for (int i0 = 0; i0 < n; i0 += kChunk) {
+    for (int c = 0; c < kStage; ++c) pre[c] = c < n ? fin[(int64_t)c * ist] : 0.0f;
+    for (int i0 = 0; i0 < n; i0 += kStage) {
 #pragma unroll
-        for (int c = 0; c < kChunk; ++c) stage[c * lanes] = pre[c];
-        if (i0 + kChunk < n) {
+        for (int c = 0; c < kStage; ++c) stage[c * lanes] = pre[c];
+        if (i0 + kStage < n) {
 #pragma unroll
-            for (int c = 0; c < kChunk; ++c) {
-                const int i = i0 + kChunk + c;
+            for (int c = 0; c < kStage; ++c) {
+                const int i = i0 + kStage + c;
                 pre[c] = i < n ? fin[(int64_t)i * ist] : 0.0f;
             }
         }
-        const int m = n - i0 < kChunk ? n - i0 : kChunk;
+        const int m = n - i0 < kStage ? n - i0 : kStage;
         float fnext = stage[0];
         for (int c = 0; c < m; ++c) {
             const int i = i0 + c;
@@ -628,7 +690,7 @@ __host__ __device__ __forceinline__ int chord_entries(const Chord &c, int n)
 template <bool kSqrt>
 __host__ __device__ bool chord_line(const float *fin, float *fout, int n, int64_t ist, int64_t ost, float aniso,
                                     const RefHull *lower, int nl, const RefHull *upper, int nu, Chord c,
-                                    const Stack &S, int cap)
+                                    const LineStack &S, int cap)
 {
     int k = 0, i0;
     S.R(0) = -INFINITY;
@@ -708,11 +770,13 @@ struct ChordLists {
 template <int kRingSlots, bool kSqrt, bool kFloatDiv>
 __global__ __launch_bounds__(kWave) void k_ref_parabolic(const float *__restrict__ fin, float *__restrict__ fout,
                                                          LineGeom g, float res, const RefHull *__restrict__ hulls,
-                                                         const int *__restrict__ counts, ChordLists L)
+                                                         const int *__restrict__ counts, ChordLists L,
+                                                         const unsigned *__restrict__ gate)
 {
+    if (gate && *gate == 0u) return;
     static_assert((kRingSlots & (kRingSlots - 1)) == 0, "ring slots must be a power of two");
     __shared__ unsigned short sv[kRingSlots * kWave];
-    __shared__ float sh[kRingSlots * kWave], sr[kRingSlots * kWave], st[kChunk * kWave];
+    __shared__ float sh[kRingSlots * kWave], sr[kRingSlots * kWave], st[kStage * kWave];
     const int lane = threadIdx.x;
     const int64_t lines = g.lines();
     int64_t line = (int64_t)blockIdx.x * kWave + lane;
@@ -754,10 +818,12 @@ __global__ __launch_bounds__(kWave) void k_ref_chords(const float *__restrict__ 
                                                       LineGeom g, float res, const RefHull *__restrict__ hulls,
                                                       const int *__restrict__ counts, const Chord *__restrict__ in,
                                                       const int *__restrict__ n_in, int in_cap, Chord *out,
-                                                      int *n_out, int cap, int *gstk)
+                                                      int *n_out, int cap, int *gstk,
+                                                      const unsigned *__restrict__ gate)
 {
+    if (gate && *gate == 0u) return;
     extern __shared__ int dyn[];
-    const Stack S = lane_stack<kLds>(dyn, gstk, cap);
+    const LineStack S = lane_stack<kLds>(dyn, gstk, cap);
     const int64_t T = (int64_t)gridDim.x * blockDim.x;
     const int n = (int)g.n[g.dim];
     const int m = min(*n_in, in_cap);
@@ -774,10 +840,193 @@ __global__ __launch_bounds__(kWave) void k_ref_chords(const float *__restrict__ 
     }
 }
 
-__global__ void k_zero_ints(int *p, int n)
+__global__ void k_zero_ints(int *p, int n, const unsigned *__restrict__ gate)
 {
+    if (gate && *gate == 0u) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// The integer form of the reference arithmetic (edt3d at a power-of-two
+// resolution)
+// ---------------------------------------------------------------------------
+//
+// Let res = 2^e on every axis, aniso = res^2, and every extent be at most
+// kIntMaxN.  Then every float of edt3d is aniso times an integer below 2^24,
+// computed exactly: pass 0's d += res and d * d (distances c * res, c <= n),
+// intersect_parabolas' numerator aniso * (m_r - m_q + (r - q)(r + q)) with
+// vertices in [-n, 2n), and the evaluation aniso * (j^2 + m).  Only the
+// intersections s = (m_r - m_q + r^2 - q^2) / (2 (r - q)) round.  Such an s,
+// when not an integer, lies at least 1 / (6n) from every integer, farther
+// than its rounding error (at most 2^-23 n for |s| <= 2n, twice that for a
+// hull range shifted by its index corrector, whose vertices lie less than n
+// apart: 1 / (2n) there); n <= 1024 keeps both below the gap.  So no range
+// rounds across an integer position, and a float pop decision that differs
+// from the exact one only moves envelope breakpoints inside one float's
+// rounding interval, which holds no position: at every position the
+// reference selects a parabola of least value, and its output is aniso times
+// the exact minimum over the parabolas it is given.
+//
+// Which parabolas those are does depend on the rounding: on a periodic axis
+// the line gets the first / last kHullCap members of the float envelope of
+// its head / tail segment (return_boundary_hull).  Those are computed with
+// the reference's float operations (k_ref_hulls over the integer pass input)
+// and kept as masks of the wrapped rows they occupy.  The passes are then
+// integer minima: pass 0 the 1-D distance, pass 1 the windowed kernel over
+// y with the masks (the exact EDT's pass Y), pass 2 a scan along z with the
+// masks, which writes aniso * d2 (or its sqrtf).
+//
+// A line along x with no zero holds the reference's infinity (FLT_MAX after
+// get_initial_envelope), which every finite parabola of a later line beats
+// in the float envelope too (its intersections come out as +-huge or
+// infinite, so it is popped or never selected).  A case outside the form (a
+// byte above 1, a squared distance above 65534 after pass 1, which includes
+// a line along y with no finite height, a scan past kIntWrap rows) raises
+// ctl[kCtlBail] or the windowed pass's flags; the float kernels then run in
+// full, gated on ctl[kCtlGate].
+constexpr int kIntMaxN = 1024;
+constexpr int kIntWrap = kHullMaskRows;  // rows past a line end a scan may read
+// control words: the bail flag, the windowed pass's counters (deferred
+// lines [0], saturation [2]), the gate of the float kernels (0: the integer
+// result stands, 1: it is recomputed, 2: the integer form did not apply),
+// then the windowed pass's deferred-line bitmap
+constexpr int kCtlBail = 0, kCtlWin = 4, kCtlGate = 8, kCtlBits = 16;
+bool g_int_form = true;
+
+// pass 0 (axis 0) of edt3d in integers: the 1-D distance c to the nearest
+// zero along x (wrapping on a periodic axis) at dz[x][y][z] (row pitch
+// `pitch`), one lane per (y, z) with its column's zeros as a bit mask in LDS
+__global__ __launch_bounds__(256) void k_int_pass0(const uint8_t *__restrict__ img, int64_t is0, int64_t is1, int nx,
+                                                   int ny, int nz, int periodic, uint16_t *__restrict__ dz,
+                                                   int64_t pitch, uint32_t *__restrict__ ctl)
+{
+    __shared__ uint32_t bits[(kIntMaxN / 32) * 256];
+    const int t = threadIdx.x;
+    const int64_t col = (int64_t)blockIdx.x * 256 + t;
+    if (col >= (int64_t)ny * nz) return;  // no barriers below: each lane owns its LDS column
+    const int y = (int)(col / nz), z = (int)(col - (int64_t)y * nz);
+    const uint8_t *p = img + (int64_t)y * is1 + z;
+    const int nw = (nx + 31) >> 5;
+    uint32_t bad = 0;
+    for (int x0 = 0; x0 < nx; x0 += 32) {
+        uint8_t v[32];
+#pragma unroll
+        for (int j = 0; j < 32; ++j) v[j] = x0 + j < nx ? p[(int64_t)(x0 + j) * is0] : (uint8_t)1;
+        uint32_t word = 0;
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            word |= (uint32_t)(v[j] == 0) << j;
+            bad |= v[j] > 1;
+        }
+        bits[(x0 >> 5) * 256 + t] = word;
+    }
+    int first = -1, last = -1;
+    for (int q = 0; q < nw; ++q) {
+        const uint32_t wq = bits[q * 256 + t];
+        if (wq) {
+            if (first < 0) first = q * 32 + __builtin_ctz(wq);
+            last = q * 32 + 31 - __builtin_clz(wq);
+        }
+    }
+    if (bad) ctl[kCtlBail] = 1u;
+    uint16_t *o = dz + (int64_t)y * pitch + z;
+    const int64_t xs = (int64_t)ny * pitch;
+    if (first < 0) {
+        // no zero: the reference's infinity (FLT_MAX in its later passes),
+        // which loses to every finite parabola of a line
+        for (int x = 0; x < nx; ++x) o[(int64_t)x * xs] = kNoSolid;
+        return;
+    }
+    constexpr int kFar = 1 << 20;
+    int prev = periodic ? last - nx : -kFar, next = first;
+    for (int x = 0; x < nx; ++x) {
+        int d;
+        if (x == next) {
+            d = 0;
+            prev = x;
+            int nn = -1;
+            if (x + 1 < nx) {
+                int q = (x + 1) >> 5;
+                uint32_t wq = bits[q * 256 + t] & (~0u << ((x + 1) & 31));
+                while (true) {
+                    if (wq) {
+                        nn = q * 32 + __builtin_ctz(wq);
+                        break;
+                    }
+                    if (++q >= nw) break;
+                    wq = bits[q * 256 + t];
+                }
+            }
+            next = nn >= 0 ? nn : (periodic ? first + nx : kFar);
+        } else {
+            d = min(x - prev, next - x);
+        }
+        o[(int64_t)x * xs] = (uint16_t)d;
+    }
+}
+
+// pass 2 (axis 2, contiguous) in integers: per voxel the minimum over rows q
+// of (z - q)^2 + m(q), scanning outward until k^2 reaches the best value, on
+// a line staged in LDS with kIntWrap rows past each end (the wrapped rows
+// the line's hulls keep on a periodic axis, none otherwise); written as the
+// float aniso * d2, or its sqrtf.  One wave per line (x, y) of b[y][x][z].
+template <bool kSqrt>
+__global__ __launch_bounds__(256) void k_int_pass2(const uint16_t *__restrict__ b, int nx, int ny, int nz,
+                                                   int64_t pitch, int periodic, const uint32_t *__restrict__ masks,
+                                                   float *__restrict__ out, int64_t os0, int64_t os1, float aniso,
+                                                   uint32_t *__restrict__ ctl)
+{
+    constexpr int kExt = kIntMaxN + 2 * kIntWrap;
+    __shared__ int ext[4][kExt];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t lines = (int64_t)nx * ny;
+    const int64_t line = (int64_t)blockIdx.x * 4 + w;
+    const bool live = line < lines;
+    const int x = live ? (int)(line / ny) : 0, y = live ? (int)(line - (int64_t)x * ny) : 0;
+    const uint16_t *src = b + (int64_t)y * nx * pitch + (int64_t)x * pitch;
+    int *e = ext[w];
+    if (live) {
+        for (int j = lane; j < nz + 2 * kIntWrap; j += kWave) {
+            const int r = j - kIntWrap;
+            int v = kBig;
+            if (r >= 0 && r < nz) {
+                v = src[r];
+            } else if (periodic && hull_row_kept(masks, line, r, nz)) {
+                int m = r % nz;
+                v = src[m < 0 ? m + nz : m];
+            }
+            e[j] = v;
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    bool over = false;
+    float *o = out + (int64_t)x * os0 + (int64_t)y * os1;
+    for (int z = lane; z < nz; z += kWave) {
+        const int *c = e + kIntWrap + z;
+        int best = c[0], k = 1;
+        for (; k <= kIntWrap; ++k) {
+            if (k * k >= best) break;
+            best = min(best, min(c[-k], c[k]) + k * k);
+        }
+        over |= k * k < best;
+        const float v = aniso * (float)best;
+        o[z] = kSqrt ? sqrtf(v) : v;
+    }
+    if (over) ctl[kCtlBail] = 1u;
+}
+
+// the float kernels' gate from the integer passes' flags
+__global__ void k_int_decide(uint32_t *ctl)
+{
+    if (threadIdx.x == 0)
+        ctl[kCtlGate] = (ctl[kCtlBail] | ctl[kCtlWin + 0] | ctl[kCtlWin + 2]) != 0u ? 1u : 0u;
+}
+
+__global__ void k_int_mark(uint32_t *ctl, uint32_t v)
+{
+    if (threadIdx.x == 0) ctl[kCtlGate] = v;
 }
 
 LineGeom make_line_geom(const int64_t *n, const int64_t *is, const int64_t *fs, const int64_t *os, int dim)
@@ -827,6 +1076,12 @@ struct RefWs {
     int *counts, *ovf, *n_ovf;
     Chord *cseg, *ca, *cc;
     int a_cap, c_cap;
+    // the integer form: its two uint16 arrays share tmp (the float kernels
+    // run only after the integer result is set aside), masks, control words
+    uint16_t *dz, *b;
+    int64_t pitch;
+    uint32_t *masks, *ctl, *wlist;
+    int64_t ctl_words;
 };
 
 RefWs carve_ref(Carver &c, const int64_t *n)
@@ -834,7 +1089,11 @@ RefWs carve_ref(Carver &c, const int64_t *n)
     const int64_t nmax = std::max(n[0], std::max(n[1], n[2]));
     const int64_t lines = max_lines(n);
     RefWs w;
-    w.tmp = c.take<float>((size_t)(n[0] * n[1] * n[2]));
+    w.pitch = (n[2] + 7) / 8 * 8;
+    const size_t int_words = (size_t)(n[0] * n[1] * w.pitch);  // two uint16 arrays = int_words floats
+    w.tmp = c.take<float>(std::max((size_t)(n[0] * n[1] * n[2]), int_words));
+    w.dz = reinterpret_cast<uint16_t *>(w.tmp);
+    w.b = w.dz + int_words;
     w.stk = c.take<int>((size_t)3 * fb_cap(nmax) * fb_threads(n));
     w.hulls = c.take<RefHull>((size_t)lines * 2 * kHullCap);
     w.counts = c.take<int>((size_t)lines * 2);
@@ -845,7 +1104,24 @@ RefWs carve_ref(Carver &c, const int64_t *n)
     w.cseg = c.take<Chord>((size_t)w.a_cap);
     w.ca = c.take<Chord>((size_t)w.a_cap);
     w.cc = c.take<Chord>((size_t)w.c_cap);
+    w.masks = c.take<uint32_t>((size_t)lines * kHullMaskWords);
+    w.ctl_words = kCtlBits + (lines + 31) / 32;
+    w.ctl = c.take<uint32_t>((size_t)w.ctl_words);
+    w.wlist = c.take<uint32_t>((size_t)lines);
     return w;
+}
+
+// whether edt3d's integer form covers these arguments (the data-dependent
+// conditions are checked on the device)
+bool int_form_applies(const int64_t *n, int first_axis, const int64_t *pads, const float *res)
+{
+    if (!g_int_form || pads || first_axis != 0) return false;
+    if (!(res[0] == res[1] && res[1] == res[2])) return false;
+    int e = 0;
+    if (std::frexp(res[0], &e) != 0.5f || e < -40 || e > 40) return false;
+    for (int a = 0; a < 3; ++a)
+        if (n[a] > kIntMaxN) return false;
+    return true;
 }
 
 }  // namespace
@@ -904,6 +1180,101 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
         }
         return k;
     };
+    // boundary hulls of the lines along g.dim (float path and integer form):
+    // the LDS kernel, then the lines whose segments outgrew it
+    auto hulls = [&](const auto *f, auto cv, const LineGeom &g, float r, Skips sk, uint32_t *masks,
+                     const unsigned *gate) -> int {
+        using T = std::remove_cv_t<std::remove_pointer_t<decltype(f)>>;
+        using Cv = decltype(cv);
+        const int64_t lines = g.lines();
+        {
+            ProfScope ps("edt_ref_hulls", s);
+            hipLaunchKernelGGL((k_ref_hulls<T, Cv>), dim3((unsigned)((lines + kWave - 1) / kWave)), dim3(kWave), 0, s,
+                               img, f, cv, g, r, sk, w.hulls, w.counts, w.ovf, w.n_ovf, masks, gate);
+        }
+        if (int e = check_launch("reference hulls")) return e;
+        {
+            ProfScope ps("edt_ref_hulls_fallback", s);
+            const int lanes = chord_lds_lanes(cap);
+            if (lanes > 0) {
+                const int64_t blocks = std::min<int64_t>((lines + lanes - 1) / lanes, kChordMaxBlocks);
+                hipLaunchKernelGGL((k_ref_hulls_fallback<true, T, Cv>), dim3((unsigned)blocks), dim3(lanes),
+                                   chord_lds_bytes(cap, lanes), s, img, f, cv, g, r, sk, w.hulls, w.counts, w.ovf,
+                                   w.n_ovf, nullptr, cap, masks, gate);
+            } else {
+                hipLaunchKernelGGL((k_ref_hulls_fallback<false, T, Cv>), dim3(fb_threads(n) / kWave), dim3(kWave), 0,
+                                   s, img, f, cv, g, r, sk, w.hulls, w.counts, w.ovf, w.n_ovf, w.stk, cap, masks,
+                                   gate);
+            }
+        }
+        return check_launch("reference hulls");
+    };
+
+    // edt3d's integer form where it applies; the float kernels below then
+    // run only if it set its result aside (ctl[kCtlGate])
+    const unsigned *gate = nullptr;
+    if (int_form_applies(n, first_axis, pads, resolution)) {
+        const int nx = (int)n[0], ny = (int)n[1], nz = (int)n[2];
+        const float aniso = resolution[0] * resolution[0];
+        const int64_t pitch = w.pitch;
+        hipLaunchKernelGGL(k_clear_words, dim3((unsigned)std::min<int64_t>((w.ctl_words + 255) / 256, 1024)),
+                           dim3(256), 0, s, w.ctl, w.ctl_words);
+        {
+            ProfScope ps("edt_int_pass0", s);
+            hipLaunchKernelGGL(k_int_pass0, dim3((unsigned)(((int64_t)ny * nz + 255) / 256)), dim3(256), 0, s, img,
+                               is[0], is[1], nx, ny, nz, periodic[0] ? 1 : 0, w.dz, pitch, w.ctl);
+        }
+        if (int e = check_launch("integer pass 0")) return e;
+        // pass 1 over y: dz [x][y][z] -> b [y][x][z] (uint16 squared distances)
+        const int64_t fs_dz[3] = {ny * pitch, pitch, 1}, fs_b[3] = {pitch, nx * pitch, 1};
+        if (periodic[1]) {
+            hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, w.n_ovf, 4, nullptr);
+            if (int e = hulls(w.dz, FromDist{aniso}, make_line_geom(n, is, fs_dz, fs_dz, 1), resolution[1],
+                              Skips{0, 0, 0}, w.masks, nullptr))
+                return e;
+        }
+        {
+            ProfScope ps("edt_int_pass1", s);
+            const int bc = periodic[1] ? PMX_BC_PERIODIC : PMX_BC_END;
+            WinGeom wg{ny, pitch, ny * pitch, nx, nz, bc, bc, kBig, pitch, pitch, nx * pitch};
+            wg.hmask = w.masks;
+            wg.tpb = tiles_per_block(wg, 64);
+            const unsigned blocks = (unsigned)window_blocks(wg, 64, wg.tpb);
+            Sq16 *b16 = reinterpret_cast<Sq16 *>(w.b);
+            if (periodic[1])
+                hipLaunchKernelGGL((k_window<64, 16, uint16_t, Sq16, false, true>), dim3(blocks), dim3(kWinThreads),
+                                   0, s, w.dz, b16, PassXOut{}, wg, nx, ny, w.ctl + kCtlBits, w.ctl + kCtlWin,
+                                   w.wlist, nullptr);
+            else
+                hipLaunchKernelGGL((k_window<64, 16, uint16_t, Sq16, false, false>), dim3(blocks), dim3(kWinThreads),
+                                   0, s, w.dz, b16, PassXOut{}, wg, nx, ny, w.ctl + kCtlBits, w.ctl + kCtlWin,
+                                   w.wlist, nullptr);
+        }
+        if (int e = check_launch("integer pass 1")) return e;
+        if (periodic[2]) {
+            hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, w.n_ovf, 4, nullptr);
+            if (int e = hulls(w.b, FromSq{aniso}, make_line_geom(n, is, fs_b, fs_b, 2), resolution[2],
+                              Skips{0, 0, 0}, w.masks, nullptr))
+                return e;
+        }
+        {
+            ProfScope ps("edt_int_pass2", s);
+            const unsigned blocks = (unsigned)(((int64_t)nx * ny + 3) / 4);
+            if (take_sqrt)
+                hipLaunchKernelGGL(k_int_pass2<true>, dim3(blocks), dim3(256), 0, s, w.b, nx, ny, nz, pitch,
+                                   periodic[2] ? 1 : 0, w.masks, d, os[0], os[1], aniso, w.ctl);
+            else
+                hipLaunchKernelGGL(k_int_pass2<false>, dim3(blocks), dim3(256), 0, s, w.b, nx, ny, nz, pitch,
+                                   periodic[2] ? 1 : 0, w.masks, d, os[0], os[1], aniso, w.ctl);
+        }
+        if (int e = check_launch("integer pass 2")) return e;
+        hipLaunchKernelGGL(k_int_decide, dim3(1), dim3(64), 0, s, w.ctl);
+        gate = w.ctl + kCtlGate;
+    } else {
+        hipLaunchKernelGGL(k_int_mark, dim3(1), dim3(64), 0, s, w.ctl, 2u);
+    }
+    if (int e = check_launch("integer form")) return e;
+
     // pass order: first_axis with the 1-D sweep, then the remaining axes in
     // ascending order (edt3d: 0 then 1, 2; edt2d on a (1, X, Y) box: 2 then
     // 1, 0); pass 0 writes d, the parabolic passes go d -> tmp -> d
@@ -912,7 +1283,7 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
         LineGeom g = make_line_geom(n, is, os, os, first_axis);
         const int64_t nb = (g.lines() + 255) / 256;
         hipLaunchKernelGGL(k_ref_pass0, dim3((unsigned)nb), dim3(256), 0, s, img, d, g, resolution[first_axis],
-                           periodic[first_axis] ? 1 : 0, skips(first_axis));
+                           periodic[first_axis] ? 1 : 0, skips(first_axis), gate);
         if (int e = check_launch("reference pass 0")) return e;
     }
     int step = 0;
@@ -927,30 +1298,9 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
         const int64_t lines = g.lines();
         const unsigned nb = (unsigned)((lines + kWave - 1) / kWave);
         const int sq = last && take_sqrt ? 1 : 0;
-        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, w.n_ovf, 4);
-        if (periodic[dim]) {
-            {
-                ProfScope ps("edt_ref_hulls", s);
-                hipLaunchKernelGGL(k_ref_hulls, dim3(nb), dim3(kWave), 0, s, img, fin, g, resolution[dim],
-                                   skips(dim), w.hulls, w.counts, w.ovf, w.n_ovf);
-            }
-            if (int e = check_launch("reference hulls")) return e;
-            {
-                ProfScope ps("edt_ref_hulls_fallback", s);
-                const int lanes = chord_lds_lanes(cap);
-                if (lanes > 0) {
-                    const int64_t blocks = std::min<int64_t>((lines + lanes - 1) / lanes, kChordMaxBlocks);
-                    hipLaunchKernelGGL(k_ref_hulls_fallback<true>, dim3((unsigned)blocks), dim3(lanes),
-                                       chord_lds_bytes(cap, lanes), s, img, fin, g, resolution[dim], skips(dim),
-                                       w.hulls, w.counts, w.ovf, w.n_ovf, nullptr, cap);
-                } else {
-                    hipLaunchKernelGGL(k_ref_hulls_fallback<false>, dim3(fb_threads(n) / kWave), dim3(kWave), 0, s,
-                                       img, fin, g, resolution[dim], skips(dim), w.hulls, w.counts, w.ovf, w.n_ovf,
-                                       w.stk, cap);
-                }
-            }
-            if (int e = check_launch("reference hulls")) return e;
-        }
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, w.n_ovf, 4, gate);
+        if (periodic[dim])
+            if (int e = hulls(fin, AsFloat{}, g, resolution[dim], skips(dim), nullptr, gate)) return e;
         const RefHull *hl = periodic[dim] ? w.hulls : nullptr;
         const float r = resolution[dim];
         const ChordLists L{w.cseg, w.ca, w.cc, w.n_ovf + 1, w.n_ovf + 2, w.a_cap, w.c_cap};
@@ -959,7 +1309,7 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
             // vertices lie in [-n, 2n): float division needs 3n * r^2 < 2^125
             const bool fdiv = 3.0 * (double)g.n[dim] * (double)(r * r) < 0x1p125;
             auto launch = [&](auto kernel) {
-                hipLaunchKernelGGL(kernel, dim3(nb), dim3(kWave), 0, s, fin, fout, g, r, hl, w.counts, L);
+                hipLaunchKernelGGL(kernel, dim3(nb), dim3(kWave), 0, s, fin, fout, g, r, hl, w.counts, L, gate);
             };
             if (sq)
                 fdiv ? launch(k_ref_parabolic<kRingSlots, true, true>) : launch(k_ref_parabolic<kRingSlots, true, false>);
@@ -979,22 +1329,22 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
                 if (sq)
                     hipLaunchKernelGGL((k_ref_chords<true, true>), dim3((unsigned)blocks), dim3(lanes), lds, s, fin,
                                        fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap,
-                                       nullptr);
+                                       nullptr, gate);
                 else
                     hipLaunchKernelGGL((k_ref_chords<false, true>), dim3((unsigned)blocks), dim3(lanes), lds, s, fin,
                                        fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap,
-                                       nullptr);
+                                       nullptr, gate);
             } else {
                 // fb_threads(n) lanes of the grid share w.stk (stacks of a line)
                 const int64_t blocks = fb_threads(n) / kWave;
                 if (sq)
                     hipLaunchKernelGGL((k_ref_chords<true, false>), dim3((unsigned)blocks), dim3(kWave), 0, s, fin,
                                        fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap,
-                                       w.stk);
+                                       w.stk, gate);
                 else
                     hipLaunchKernelGGL((k_ref_chords<false, false>), dim3((unsigned)blocks), dim3(kWave), 0, s, fin,
                                        fout, g, r, hl, w.counts, in, n_in, in_cap, out, n_out, ccap,
-                                       w.stk);
+                                       w.stk, gate);
             }
         };
         {
@@ -1009,6 +1359,30 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
         if (int e = check_launch("reference parabolic long runs")) return e;
         ++step;
     }
+    return PMX_OK;
+}
+
+int pmx_edt_ref_int_form(int enable)
+{
+    const int prev = pmx::g_int_form ? 1 : 0;
+    pmx::g_int_form = enable != 0;
+    return prev;
+}
+
+int pmx_edt_ref_last_form(const int64_t *n, const void *workspace, size_t workspace_bytes, hipStream_t s, int *form)
+{
+    using namespace pmx;
+    clear_error();
+    if (!n || !workspace || !form) return fail(PMX_ERR_INVALID, "NULL argument");
+    size_t need = 0;
+    if (int e = pmx_edt_ref_workspace_size(n, &need)) return e;
+    if (workspace_bytes < need) return fail(PMX_ERR_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, need);
+    Carver c(const_cast<void *>(workspace), workspace_bytes);
+    const RefWs w = carve_ref(c, n);
+    uint32_t v = 0;
+    if (hipStreamSynchronize(s) != hipSuccess || hipMemcpy(&v, w.ctl + kCtlGate, 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(PMX_ERR_HIP, "reading the form flag failed");
+    *form = (int)v;
     return PMX_OK;
 }
 

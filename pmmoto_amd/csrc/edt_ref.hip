@@ -1017,6 +1017,180 @@ __global__ __launch_bounds__(256) void k_int_pass2(const uint16_t *__restrict__ 
     if (over) ctl[kCtlBail] = 1u;
 }
 
+// Boundary hulls of the integer form: return_boundary_hull's float
+// envelope over a line's head segment (position 0 to its first zero) and
+// tail segment (last zero to the end), one lane per line, written as the
+// line's wrapped-row mask (write_hull_mask's layout).  The pass input is in
+// integers, so a stack entry is one word: the exact height m (aniso * m is
+// the float height; kIntInfM for the reference's FLT_MAX) in the low 22
+// bits, the segment-local vertex in the high 10.  Ranges are not stored: an
+// entry's range is intersect(entry, entry below), recomputed with the same
+// float operations when the entry below the top is popped, so it has the
+// bits it had when pushed.  The first update pops the reference's
+// placeholder entry 1 (vertex 0, height 0, range +inf) unconditionally, so
+// the stack here starts from entry 0 alone.  Lines whose envelope outgrows
+// kIntHullCap entries go to the overflow list of k_ref_hulls_fallback.
+// Stack entries per lane: a first kernel with a small stack keeps many
+// waves in flight; the lines it cannot hold go to a second one with a large
+// stack (few lines, so few waves), and the rare lines beyond that to the
+// whole-line float fallback.
+#ifndef PMX_INT_HULL_CAP
+#define PMX_INT_HULL_CAP 16
+#endif
+constexpr int kIntHullCap = PMX_INT_HULL_CAP;
+constexpr int kIntHullCapBig = 128;
+constexpr int kIntHullListWaves = 1024;  // persistent waves of the second kernel
+constexpr uint32_t kIntInfM = (1u << 22) - 1;
+
+// both hulls of one line into its mask; false when the envelope outgrows
+// kCap entries (then nothing is written)
+template <bool kDist, int kCap>
+__device__ __forceinline__ bool int_hull_line(const uint16_t *__restrict__ f, const LineGeom &g, float aniso,
+                                              int64_t line, uint32_t *S, uint32_t *__restrict__ masks,
+                                              uint32_t *__restrict__ ctl)
+{
+    int64_t ib, fb, ob;
+    g.base(line, &ib, &fb, &ob);
+    const int n = (int)g.n[g.dim];
+    const int64_t st = g.fs[g.dim];
+    const uint16_t *p = f + fb;
+    auto mval = [](uint16_t v) -> uint32_t {
+        return kDist ? (v == kNoSolid ? kIntInfM : (uint32_t)v * v) : (uint32_t)v;
+    };
+    auto hf = [aniso](uint32_t m) { return m == kIntInfM ? kFltMax : aniso * (float)m; };
+    auto ent = [](int v, uint32_t m) { return ((uint32_t)v << 22) | m; };
+    // the envelope of the segment's values, calls of add(i, m) in order
+    // (i from 0); top entry and the one below it kept in registers
+    int k = 0, tv = 0, bv = 0;
+    uint32_t tm = 0, bm = 0;
+    float th = 0.0f, tr = -INFINITY, bh = 0.0f;
+    bool fits = true;
+    auto add = [&](int i, uint32_t m) {
+        const float h = hf(m);
+        if (i == 0) {
+            k = 0;
+            tv = 0;
+            tm = m;
+            th = h;
+            tr = -INFINITY;
+            S[0] = ent(0, m);
+            return;
+        }
+        float sv = intersect<true>(h, (float)i, th, (float)tv, aniso);
+        while (k > 0 && sv <= tr) {
+            // pop: the entry below becomes the top, its range recomputed from
+            // the entry below it
+            --k;
+            tv = bv;
+            tm = bm;
+            th = bh;
+            if (k > 0) {
+                const uint32_t e = S[(k - 1) * kWave];
+                bv = (int)(e >> 22);
+                bm = e & kIntInfM;
+                bh = hf(bm);
+                tr = intersect<true>(th, (float)tv, bh, (float)bv, aniso);
+            } else {
+                tr = -INFINITY;
+            }
+            sv = intersect<true>(h, (float)i, th, (float)tv, aniso);
+        }
+        ++k;
+        if (k + 1 >= kCap) {
+            fits = false;
+            return;
+        }
+        S[k * kWave] = ent(i, m);
+        bv = tv;
+        bm = tm;
+        bh = th;
+        tv = i;
+        tm = m;
+        th = h;
+        tr = sv;
+    };
+    uint32_t w[kHullMaskWords] = {};
+    // the reference's selection: from the start (forward) or the top, the
+    // finite entries, at most min(kNumHull, k) + 1 of them; mark row
+    // bit(local vertex)
+    auto pick = [&](bool forward, int nv) {
+        const int num = min(kNumHull, k);
+        int cnt = 0;
+        for (int j = 0; j <= k && cnt <= num; ++j) {
+            const uint32_t e = S[(forward ? j : k - j) * kWave];
+            if ((e & kIntInfM) == kIntInfM) continue;
+            ++cnt;
+            const int v = (int)(e >> 22);
+            const int b = forward ? v : nv - 1 - v;  // rows n + v, or -1 - (v - nv)
+            if (b >= kHullMaskRows) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if ((b >> 5) == q) w[(forward ? 4 : 0) + q] |= 1u << (b & 31);
+        }
+    };
+    // head: position 0 through the first zero (the whole line if none)
+    int first = -1;
+    stream_values<1>(p, st, 0, n, [&](int i, uint16_t v) {
+        add(i, mval(v));
+        if (v == 0) {
+            first = i;
+            return false;
+        }
+        return fits;
+    });
+    if (!fits) return false;
+    if (n == 1 && tm == kIntInfM) {
+        // a one-voxel line with no finite height: the reference's selection
+        // reads its placeholder entry there; the float kernels reproduce that
+        ctl[kCtlBail] = 1u;
+        return true;
+    }
+    pick(true, first < 0 ? n : first + 1);
+    if (first < 0) {
+        // no zero: the tail segment is the same whole line
+        pick(false, n);
+    } else {
+        int last = -1;
+        stream_values<-1>(p, st, 0, n, [&](int i, uint16_t v) {
+            if (v != 0) return true;
+            last = i;
+            return false;
+        });
+        stream_values<1>(p + (int64_t)last * st, st, 0, n - last, [&](int i, uint16_t v) {
+            add(i, mval(v));
+            return fits;
+        });
+        if (!fits) return false;
+        pick(false, n - last);
+    }
+    uint32_t *m = masks + line * kHullMaskWords;
+#pragma unroll
+    for (int q = 0; q < kHullMaskWords; ++q) m[q] = w[q];
+    return true;
+}
+
+// The lines (all of them, or those of `list`), one per lane; a line whose
+// envelope outgrows kCap entries goes to `out`.
+template <bool kDist, int kCap>
+__global__ __launch_bounds__(kWave) void k_int_hulls(const uint16_t *__restrict__ f, LineGeom g, float aniso,
+                                                     uint32_t *__restrict__ masks, const int *__restrict__ list,
+                                                     const int *__restrict__ list_n, int *__restrict__ out,
+                                                     int *__restrict__ out_n, uint32_t *__restrict__ ctl)
+{
+    __shared__ uint32_t stk[kCap * kWave];
+    uint32_t *S = stk + threadIdx.x;
+    // no barriers below: each lane owns its LDS column
+    if (!list) {
+        const int64_t line = (int64_t)blockIdx.x * kWave + threadIdx.x;
+        if (line < g.lines() && !int_hull_line<kDist, kCap>(f, g, aniso, line, S, masks, ctl))
+            out[atomicAdd(out_n, 1)] = (int)line;
+        return;
+    }
+    const int m = *list_n;
+    for (int j = blockIdx.x * kWave + threadIdx.x; j < m; j += gridDim.x * kWave)
+        if (!int_hull_line<kDist, kCap>(f, g, aniso, list[j], S, masks, ctl)) out[atomicAdd(out_n, 1)] = list[j];
+}
+
 // the float kernels' gate from the integer passes' flags
 __global__ void k_int_decide(uint32_t *ctl)
 {
@@ -1068,12 +1242,13 @@ size_t chord_lds_bytes(int cap, int lanes) { return (size_t)lanes * (size_t)cap 
 int chord_lds_lanes(int cap) { return (int)std::min<size_t>(kWave, kChordLdsBytes / chord_lds_bytes(cap, 1)); }
 constexpr int kChordCap = 128;       // stack entries per lane of the second parabolic kernel
 constexpr int kChordMaxBlocks = 4096;
+constexpr int kIntFbBlocks = 256;   // persistent blocks of the integer form's hull fallback
 
 struct RefWs {
     float *tmp;
     int *stk;
     RefHull *hulls;
-    int *counts, *ovf, *n_ovf;
+    int *counts, *ovf, *n_ovf, *ovf2;
     Chord *cseg, *ca, *cc;
     int a_cap, c_cap;
     // the integer form: its two uint16 arrays share tmp (the float kernels
@@ -1105,6 +1280,7 @@ RefWs carve_ref(Carver &c, const int64_t *n)
     w.ca = c.take<Chord>((size_t)w.a_cap);
     w.cc = c.take<Chord>((size_t)w.c_cap);
     w.masks = c.take<uint32_t>((size_t)lines * kHullMaskWords);
+    w.ovf2 = c.take<int>((size_t)lines);
     w.ctl_words = kCtlBits + (lines + 31) / 32;
     w.ctl = c.take<uint32_t>((size_t)w.ctl_words);
     w.wlist = c.take<uint32_t>((size_t)lines);
@@ -1225,14 +1401,45 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
                                is[0], is[1], nx, ny, nz, periodic[0] ? 1 : 0, w.dz, pitch, w.ctl);
         }
         if (int e = check_launch("integer pass 0")) return e;
+        // the hull masks of the lines along dim, then the lines whose
+        // envelope outgrew the packed stack (few: whole-line stacks)
+        auto int_hulls = [&](const uint16_t *f, auto cv, int dim, const int64_t *fs) -> int {
+            using Cv = decltype(cv);
+            constexpr bool kDist = std::is_same<Cv, FromDist>::value;
+            const LineGeom g = make_line_geom(n, is, fs, fs, dim);
+            const int64_t lines = g.lines();
+            hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, w.n_ovf, 4, nullptr);
+            {
+                // small stacks for every line, large ones for the lines that
+                // outgrew them (listed in w.ovf2), the rest into w.ovf
+                ProfScope ps("edt_int_hulls", s);
+                hipLaunchKernelGGL((k_int_hulls<kDist, kIntHullCap>), dim3((unsigned)((lines + kWave - 1) / kWave)),
+                                   dim3(kWave), 0, s, f, g, aniso, w.masks, nullptr, nullptr, w.ovf2, w.n_ovf + 1,
+                                   w.ctl);
+                hipLaunchKernelGGL((k_int_hulls<kDist, kIntHullCapBig>),
+                                   dim3((unsigned)std::min<int64_t>(kIntHullListWaves, (lines + kWave - 1) / kWave)),
+                                   dim3(kWave), 0, s, f, g, aniso, w.masks, w.ovf2, w.n_ovf + 1, w.ovf, w.n_ovf,
+                                   w.ctl);
+            }
+            if (int e = check_launch("integer hulls")) return e;
+            ProfScope ps("edt_int_hulls_fallback", s);
+            const int lanes = chord_lds_lanes(cap);
+            if (lanes > 0) {
+                const int64_t blocks = std::min<int64_t>((lines + lanes - 1) / lanes, kIntFbBlocks);
+                hipLaunchKernelGGL((k_ref_hulls_fallback<true, uint16_t, Cv>), dim3((unsigned)blocks), dim3(lanes),
+                                   chord_lds_bytes(cap, lanes), s, img, f, cv, g, resolution[dim], Skips{0, 0, 0},
+                                   w.hulls, w.counts, w.ovf, w.n_ovf, nullptr, cap, w.masks, nullptr);
+            } else {
+                hipLaunchKernelGGL((k_ref_hulls_fallback<false, uint16_t, Cv>), dim3(fb_threads(n) / kWave),
+                                   dim3(kWave), 0, s, img, f, cv, g, resolution[dim], Skips{0, 0, 0}, w.hulls,
+                                   w.counts, w.ovf, w.n_ovf, w.stk, cap, w.masks, nullptr);
+            }
+            return check_launch("integer hulls");
+        };
         // pass 1 over y: dz [x][y][z] -> b [y][x][z] (uint16 squared distances)
         const int64_t fs_dz[3] = {ny * pitch, pitch, 1}, fs_b[3] = {pitch, nx * pitch, 1};
-        if (periodic[1]) {
-            hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, w.n_ovf, 4, nullptr);
-            if (int e = hulls(w.dz, FromDist{aniso}, make_line_geom(n, is, fs_dz, fs_dz, 1), resolution[1],
-                              Skips{0, 0, 0}, w.masks, nullptr))
-                return e;
-        }
+        if (periodic[1])
+            if (int e = int_hulls(w.dz, FromDist{aniso}, 1, fs_dz)) return e;
         {
             ProfScope ps("edt_int_pass1", s);
             const int bc = periodic[1] ? PMX_BC_PERIODIC : PMX_BC_END;
@@ -1251,12 +1458,8 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
                                    w.wlist, nullptr);
         }
         if (int e = check_launch("integer pass 1")) return e;
-        if (periodic[2]) {
-            hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, w.n_ovf, 4, nullptr);
-            if (int e = hulls(w.b, FromSq{aniso}, make_line_geom(n, is, fs_b, fs_b, 2), resolution[2],
-                              Skips{0, 0, 0}, w.masks, nullptr))
-                return e;
-        }
+        if (periodic[2])
+            if (int e = int_hulls(w.b, FromSq{aniso}, 2, fs_b)) return e;
         {
             ProfScope ps("edt_int_pass2", s);
             const unsigned blocks = (unsigned)(((int64_t)nx * ny + 3) / 4);

@@ -968,9 +968,13 @@ __global__ __launch_bounds__(256) void k_int_pass0(const uint8_t *__restrict__ i
 
 // pass 2 (axis 2, contiguous) in integers: per voxel the minimum over rows q
 // of (z - q)^2 + m(q), scanning outward until k^2 reaches the best value, on
-// a line staged in LDS with kIntWrap rows past each end (the wrapped rows
-// the line's hulls keep on a periodic axis, none otherwise); written as the
-// float aniso * d2, or its sqrtf.  One wave per line (x, y) of b[y][x][z].
+// lines staged in LDS with kIntWrap rows past each end (the wrapped rows the
+// line's hulls keep on a periodic axis, none otherwise); written as the float
+// aniso * d2, or its sqrtf.  One wave per two lines of b[y][x][z] (line
+// numbers 2p, 2p + 1 of x * ny + y), held as pairs of uint16 in one word, so
+// one LDS read and packed minimum / saturating add serve both (values are at
+// most 65534, a missing row reads 65535, and a sum saturating at 65535 never
+// undercuts a best value).
 template <bool kSqrt>
 __global__ __launch_bounds__(256) void k_int_pass2(const uint16_t *__restrict__ b, int nx, int ny, int nz,
                                                    int64_t pitch, int periodic, const uint32_t *__restrict__ masks,
@@ -978,58 +982,90 @@ __global__ __launch_bounds__(256) void k_int_pass2(const uint16_t *__restrict__ 
                                                    uint32_t *__restrict__ ctl)
 {
     constexpr int kExt = kIntMaxN + 2 * kIntWrap;
-    __shared__ int ext[4][kExt];
+    __shared__ __align__(16) uint32_t ext[4][kExt];
+    __shared__ uint32_t msk[4][2 * kHullMaskWords];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t lines = (int64_t)nx * ny;
-    const int64_t line = (int64_t)blockIdx.x * 4 + w;
-    const bool live = line < lines;
-    const int x = live ? (int)(line / ny) : 0, y = live ? (int)(line - (int64_t)x * ny) : 0;
-    const uint16_t *src = b + (int64_t)y * nx * pitch + (int64_t)x * pitch;
-    int *e = ext[w];
-    if (live) {
-        for (int j = lane; j < nz + 2 * kIntWrap; j += kWave) {
-            const int r = j - kIntWrap;
-            int v = kBig;
-            if (r >= 0 && r < nz) {
-                v = src[r];
-            } else if (periodic && hull_row_kept(masks, line, r, nz)) {
-                int m = r % nz;
-                v = src[m < 0 ? m + nz : m];
+    const int64_t l0 = ((int64_t)blockIdx.x * 4 + w) * 2, l1 = l0 + 1;
+    const bool live0 = l0 < lines, live1 = l1 < lines;
+    const int x0 = live0 ? (int)(l0 / ny) : 0, y0 = live0 ? (int)(l0 - (int64_t)x0 * ny) : 0;
+    const int x1 = live1 ? (int)(l1 / ny) : x0, y1 = live1 ? (int)(l1 - (int64_t)x1 * ny) : y0;
+    const uint16_t *s0 = b + (int64_t)y0 * nx * pitch + (int64_t)x0 * pitch;
+    const uint16_t *s1 = b + (int64_t)y1 * nx * pitch + (int64_t)x1 * pitch;
+    uint32_t *e = ext[w];
+    if (live0) {
+        if (periodic && lane < 2 * kHullMaskWords) {
+            const int64_t ln = lane < kHullMaskWords ? l0 : (live1 ? l1 : l0);
+            msk[w][lane] = masks[ln * kHullMaskWords + (lane & (kHullMaskWords - 1))];
+        }
+        // 8 values of each line per lane (rows are 16-byte aligned and padded
+        // to the pitch; values past nz are overwritten by the wrap rows below)
+        for (int j = lane * 8; j < nz; j += kWave * 8) {
+            const uint4 a = *reinterpret_cast<const uint4 *>(s0 + j);
+            const uint4 c = live1 ? *reinterpret_cast<const uint4 *>(s1 + j) : make_uint4(0, 0, 0, 0);
+            const uint32_t av[4] = {a.x, a.y, a.z, a.w}, cv[4] = {c.x, c.y, c.z, c.w};
+            uint32_t pr[8];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                pr[2 * t] = (av[t] & 0xffffu) | (cv[t] << 16);
+                pr[2 * t + 1] = (av[t] >> 16) | (cv[t] & 0xffff0000u);
             }
-            e[j] = v;
+            *reinterpret_cast<uint4 *>(e + kIntWrap + j) = make_uint4(pr[0], pr[1], pr[2], pr[3]);
+            *reinterpret_cast<uint4 *>(e + kIntWrap + j + 4) = make_uint4(pr[4], pr[5], pr[6], pr[7]);
         }
     }
     __syncthreads();
-    if (!live) return;
-    bool over = false;
-    float *o = out + (int64_t)x * os0 + (int64_t)y * os1;
-    for (int z = lane; z < nz; z += kWave) {
-        const int *c = e + kIntWrap + z;
-        int best = c[0], k = 1;
-        for (; k <= kIntWrap; ++k) {
-            if (k * k >= best) break;
-            best = min(best, min(c[-k], c[k]) + k * k);
+    if (live0) {
+        // rows -1 - j and nz + j: the wrapped copies the masks keep, else 65535
+        for (int j = lane; j < kIntWrap; j += kWave) {
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                uint32_t v = 0xffffffffu;
+                if (periodic) {
+                    const int r = side ? nz + j : -1 - j;
+                    int m = r % nz;
+                    m = m < 0 ? m + nz : m;
+                    const bool inb = j < kHullMaskRows;
+                    const int wi = (side ? 4 : 0) + (j >> 5);
+                    const bool k0 = inb && ((msk[w][wi] >> (j & 31)) & 1u);
+                    const bool k1 = inb && ((msk[w][kHullMaskWords + wi] >> (j & 31)) & 1u);
+                    const uint32_t own = e[kIntWrap + m];
+                    v = (k0 ? (own & 0xffffu) : 0xffffu) | (k1 ? (own & 0xffff0000u) : 0xffff0000u);
+                }
+                if (side) e[kIntWrap + nz + j] = v;
+                else e[kIntWrap - 1 - j] = v;
+            }
         }
-        over |= k * k < best;
-        const float v = aniso * (float)best;
-        o[z] = kSqrt ? sqrtf(v) : v;
+    }
+    __syncthreads();
+    if (!live0) return;
+    bool over = false;
+    float *o0 = out + (int64_t)x0 * os0 + (int64_t)y0 * os1;
+    float *o1 = out + (int64_t)x1 * os0 + (int64_t)y1 * os1;
+    for (int z = lane; z < nz; z += kWave) {
+        const uint32_t *c = e + kIntWrap + z;
+        u16x2 best = __builtin_bit_cast(u16x2, c[0]);
+        if (!live1) best.y = 0;
+        int k = 1;
+        for (; k <= kIntWrap; k += 4) {
+            if (__builtin_amdgcn_ballot_w64(max((int)best.x, (int)best.y) > k * k) == 0) break;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int kk = k + u;
+                const u16x2 lo = __builtin_bit_cast(u16x2, c[-kk]), hi = __builtin_bit_cast(u16x2, c[kk]);
+                const u16x2 k2 = {(unsigned short)(kk * kk), (unsigned short)(kk * kk)};
+                best = __builtin_elementwise_min(best, __builtin_elementwise_add_sat(__builtin_elementwise_min(lo, hi), k2));
+            }
+        }
+        // offsets up to kIntWrap are in; a value above (kIntWrap + 1)^2 may need more
+        over |= max((int)best.x, (int)best.y) > (kIntWrap + 1) * (kIntWrap + 1);
+        const float v0 = aniso * (float)best.x, v1 = aniso * (float)best.y;
+        o0[z] = kSqrt ? sqrtf(v0) : v0;
+        if (live1) o1[z] = kSqrt ? sqrtf(v1) : v1;
     }
     if (over) ctl[kCtlBail] = 1u;
 }
 
-// Boundary hulls of the integer form: return_boundary_hull's float
-// envelope over a line's head segment (position 0 to its first zero) and
-// tail segment (last zero to the end), one lane per line, written as the
-// line's wrapped-row mask (write_hull_mask's layout).  The pass input is in
-// integers, so a stack entry is one word: the exact height m (aniso * m is
-// the float height; kIntInfM for the reference's FLT_MAX) in the low 22
-// bits, the segment-local vertex in the high 10.  Ranges are not stored: an
-// entry's range is intersect(entry, entry below), recomputed with the same
-// float operations when the entry below the top is popped, so it has the
-// bits it had when pushed.  The first update pops the reference's
-// placeholder entry 1 (vertex 0, height 0, range +inf) unconditionally, so
-// the stack here starts from entry 0 alone.  Lines whose envelope outgrows
-// kIntHullCap entries go to the overflow list of k_ref_hulls_fallback.
 // Stack entries per lane: a first kernel with a small stack keeps many
 // waves in flight; the lines it cannot hold go to a second one with a large
 // stack (few lines, so few waves), and the rare lines beyond that to the
@@ -1462,7 +1498,7 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
             if (int e = int_hulls(w.b, FromSq{aniso}, 2, fs_b)) return e;
         {
             ProfScope ps("edt_int_pass2", s);
-            const unsigned blocks = (unsigned)(((int64_t)nx * ny + 3) / 4);
+            const unsigned blocks = (unsigned)(((int64_t)nx * ny + 7) / 8);
             if (take_sqrt)
                 hipLaunchKernelGGL(k_int_pass2<true>, dim3(blocks), dim3(256), 0, s, w.b, nx, ny, nz, pitch,
                                    periodic[2] ? 1 : 0, w.masks, d, os[0], os[1], aniso, w.ctl);

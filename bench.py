@@ -196,30 +196,55 @@ def periodic_512_leg(reps: int):
     dout = torch.empty((n, n, n), dtype=torch.float32, device="cuda")
     rres = (1.0 / n,) * 3
 
-    def edt_ref():
-        E.run_edt_ref(img, (0, 0, 0), (n, n, n), 0, (True, True, True), rres, True, out=dout)
+    nn3 = (C.c_int64 * 3)(n, n, n)
+    rb = C.c_size_t()
+    _lib.check(L.pmx_edt_ref_workspace_size(nn3, C.byref(rb)))
+    rws = _device.workspace(rb.value)
+    st2 = (C.c_int64 * 2)(n * n, n)
+    per3 = (C.c_int32 * 3)(1, 1, 1)
+    rr3 = (C.c_float * 3)(*rres)
 
-    edt_ref()
-    torch.cuda.synchronize()
-    _lib.profile_begin()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(reps):
-        edt_ref()
-    ev1.record()
-    torch.cuda.synchronize()
-    prof = _lib.profile_end()
-    ms = ev0.elapsed_time(ev1) / reps
+    def edt_ref():
+        _lib.check(L.pmx_edt_ref(_device.ptr(img), nn3, st2, _device.ptr(dout), st2, 0, per3, None, rr3, 1,
+                                 _device.ptr(rws), rb.value, _device.stream()))
+
+    def ref_timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(reps):
+            fn()
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        # the per-phase split from HIP events in a separate run (the events add a little)
+        _lib.profile_begin()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        prof = _lib.profile_end()
+        return ms, {k: round(t / reps, 4) for k, (t, c) in prof.items()}
+
+    ms, kms = ref_timed(edt_ref)
+    form = C.c_int()
+    _lib.check(L.pmx_edt_ref_last_form(nn3, _device.ptr(rws), rb.value, _device.stream(), C.byref(form)))
+    prev = L.pmx_edt_ref_int_form(0)
+    try:
+        ms_float, kms_float = ref_timed(edt_ref)
+    finally:
+        L.pmx_edt_ref_int_form(prev)
     res["edt3d_reference_arithmetic"] = {
         "what": "edt3d(periodic=(True,)*3, resolution=1/512): float32 distances bit-identical to the reference",
+        "path": {0: "integer form", 1: "float kernels after the integer form", 2: "float kernels"}[form.value],
         "ms": round(ms, 4),
         "gvoxel_per_s": round(vox / (ms * 1e-3) / 1e9, 2),
-        # u8 in + f32 out of pass 0, f32 in/out of the two parabolic passes
-        "hbm_frac_at_algorithmic_bytes": round(21 * vox / (ms * 1e-3) / 8e12, 4),
-        "bound": "issue (per-line float32 envelope arithmetic, one line per lane)",
-        "kernels_ms": {k: round(t / reps, 3) for k, (t, c) in prof.items()},
+        # u8 in + f32 out, and the u16 arrays between the three passes (2 + 2 B out, 2 + 2 B in)
+        "hbm_frac_at_algorithmic_bytes": round(13 * vox / (ms * 1e-3) / 8e12, 4),
+        "kernels_ms": kms,
+        "float_kernels_alone": {"ms": round(ms_float, 4), "kernels_ms": kms_float},
     }
-    del dout
+    del dout, rws
     for name, fn, bpv in (("erode_r5", erode, 18), ("dilate_r5", dilate, 18), ("ccl26", label, 5)):
         fn()
         torch.cuda.synchronize()

@@ -147,10 +147,10 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
                 const float *resolution, int take_sqrt, void *workspace, size_t workspace_bytes,
                 hipStream_t stream);
 
-/* No counterpart in the reference: introspection of pmx_edt_ref.  For edt3d
- * (pads NULL, first_axis 0) at one power-of-two resolution on every axis and
- * extents up to 1024, pmx_edt_ref first runs the same arithmetic in integer
- * form (every float the reference forms there is an exact multiple of
+/* No counterpart in the reference: introspection of pmx_edt_ref.  For the
+ * 3-D modes (first_axis 0: edt3d, or corrected_edt with pads) at one
+ * power-of-two resolution on every axis and extents up to 1024, pmx_edt_ref
+ * first runs the same arithmetic in integer form (every float the reference forms there is an exact multiple of
  * res^2; only the boundary hulls depend on rounding, and they are computed
  * in float), and falls back to the float kernels, on the device, when the
  * image or its distances leave that form's range.  pmx_edt_ref_int_form

@@ -619,8 +619,12 @@ struct WinGeom {
     int64_t out_os, out_rs;  // output strides of the outer index and of a row (pass Y)
     int tpb = kTilesPerBlock;  // row tiles per block (a power of two <= kTilesPerBlock)
     // truncated periodic wrap (the windowed kernels' kHull form): per line,
-    // the wrapped rows that may serve as candidates (see hull_row_kept)
+    // the wrapped rows that may serve as candidates (see hull_row_kept); a
+    // wrapped row r < 0 reads row r + wrap_lo, a row r >= n row r - wrap_hi
+    // (both n for a whole periodic line; less on a padded subdomain, whose
+    // period skips the pads)
     const uint32_t *hmask = nullptr;
+    int wrap_lo = 0, wrap_hi = 0;
 };
 
 // Truncated wrap of a periodic line: a row r < 0 or r >= n is a candidate
@@ -629,6 +633,17 @@ struct WinGeom {
 // never candidates (a scan stops at kReach rows, so none is read).
 constexpr int kHullMaskRows = 128;
 constexpr int kHullMaskWords = 8;
+
+// map_row, or (kHull) the row a wrapped row reads under the truncated wrap
+template <bool kHull>
+__device__ __forceinline__ int map_row_g(int r, const WinGeom &g)
+{
+    if (kHull && (r < 0 || r >= g.n)) {
+        const int m = r < 0 ? r + g.wrap_lo : r - g.wrap_hi;
+        return m >= 0 && m < g.n ? m : kRowNone;
+    }
+    return map_row(r, g.n, g.bc_lo, g.bc_hi);
+}
 __device__ __forceinline__ bool hull_row_kept(const uint32_t *__restrict__ m, int64_t line, int r, int n)
 {
     const int b = r < 0 ? -1 - r : r - n;
@@ -682,7 +697,7 @@ template <bool kHull, class E>
 __device__ __forceinline__ int fetch(const E *line, const WinGeom &g, int r, int64_t line_id)
 {
     if (kHull && (r < 0 || r >= g.n) && !hull_row_kept(g.hmask, line_id, r, g.n)) return g.cap;
-    const int m = map_row(r, g.n, g.bc_lo, g.bc_hi);
+    const int m = map_row_g<kHull>(r, g);
     if (m >= 0) return decode(line[(int64_t)m * g.rs], g.cap);
     return m == kRowWall ? 0 : g.cap;
 }
@@ -696,13 +711,13 @@ template <> struct Quad<uint16_t> { using type = uint2; };
 template <> struct Quad<int32_t> { using type = int4; };
 template <> struct Quad<Sq16> { using type = uint2; };
 
-template <int N, class E>
+template <int N, class E, bool kHull = false>
 __device__ __forceinline__ void issue_rows(const E *__restrict__ base, const WinGeom &g, int first, int row0,
                                            int c4, int (&mrow)[N], typename Quad<E>::type (&raw)[N])
 {
     using V = typename Quad<E>::type;
 #pragma unroll
-    for (int i = 0; i < N; ++i) mrow[i] = map_row(first + row0 + 16 * i, g.n, g.bc_lo, g.bc_hi);
+    for (int i = 0; i < N; ++i) mrow[i] = map_row_g<kHull>(first + row0 + 16 * i, g);
 #pragma unroll
     for (int i = 0; i < N; ++i) raw[i] = *reinterpret_cast<const V *>(base + (int64_t)max(mrow[i], 0) * g.rs + c4);
 }
@@ -862,7 +877,7 @@ __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in,
         // the first tile: all kRows rows
         int fm[kRows / 16];
         V fr[kRows / 16];
-        issue_rows<kRows / 16>(base, g, r0 - W, row0, c4, fm, fr);
+        issue_rows<kRows / 16, E, kHull>(base, g, r0 - W, row0, c4, fm, fr);
         commit_rows<kHull, E, kRows / 16>(tile, g, 0, row0, q, zl, fm, fr, acc, r0 - W >= 0 && r0 - W + kRows <= g.n,
                                           r0 - W, (int64_t)outer * g.nz + z0 + 4 * q);
     }
@@ -890,7 +905,7 @@ __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in,
             for (int w = 1; w < kWaves; ++w) cm = min(cm, part[w][lane]);
         }
         const bool more = rt + 1 < rt_end;
-        if (more) issue_rows<T / 16>(base, g, r0 + T + W, row0, c4, xm, xr);
+        if (more) issue_rows<T / 16, E, kHull>(base, g, r0 + T + W, row0, c4, xm, xr);
 
         const int jend = min(T, g.n - r0);  // wave-uniform: the row loop needs no exec mask
         // the rest of a row once its tile scan is done (wave-uniform calls: it holds a ballot)

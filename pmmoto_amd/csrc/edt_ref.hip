@@ -848,8 +848,8 @@ __global__ void k_zero_ints(int *p, int n, const unsigned *__restrict__ gate)
 }
 
 // ---------------------------------------------------------------------------
-// The integer form of the reference arithmetic (edt3d at a power-of-two
-// resolution)
+// The integer form of the reference arithmetic (edt3d, or corrected_edt on a
+// padded subdomain, at a power-of-two resolution)
 // ---------------------------------------------------------------------------
 //
 // Let res = 2^e on every axis, aniso = res^2, and every extent be at most
@@ -894,12 +894,18 @@ constexpr int kIntWrap = kHullMaskRows;  // rows past a line end a scan may read
 constexpr int kCtlBail = 0, kCtlWin = 4, kCtlGate = 8, kCtlBits = 16;
 bool g_int_form = true;
 
-// pass 0 (axis 0) of edt3d in integers: the 1-D distance c to the nearest
-// zero along x (wrapping on a periodic axis) at dz[x][y][z] (row pitch
-// `pitch`), one lane per (y, z) with its column's zeros as a bit mask in LDS
+// pass 0 (axis 0) in integers: the 1-D distance c to the nearest zero along
+// x at dz[x][y][z] (row pitch `pitch`), one lane per (y, z) with its
+// column's zeros as a bit mask in LDS.  Past the line's last zero the next
+// one is vnext, before its first the previous one vprev: edt3d's wrap
+// (last - n, first + n; _distance.pyx:240-279) with wrap == 1, corrected_edt's
+// pad-skipping face distances with wrap == 2 (first zero f at or after
+// 2 p_lo and last zero l before n - 2 p_hi, l >= 2 p_lo: l + p_hi + 1 - n and
+// n - 1 + f - p_lo, the virtual zeros of its correctors; :304-447), none
+// (END) with wrap == 0.
 __global__ __launch_bounds__(256) void k_int_pass0(const uint8_t *__restrict__ img, int64_t is0, int64_t is1, int nx,
-                                                   int ny, int nz, int periodic, uint16_t *__restrict__ dz,
-                                                   int64_t pitch, uint32_t *__restrict__ ctl)
+                                                   int ny, int nz, int wrap, int plo, int phi,
+                                                   uint16_t *__restrict__ dz, int64_t pitch, uint32_t *__restrict__ ctl)
 {
     __shared__ uint32_t bits[(kIntMaxN / 32) * 256];
     const int t = threadIdx.x;
@@ -921,17 +927,31 @@ __global__ __launch_bounds__(256) void k_int_pass0(const uint8_t *__restrict__ i
         }
         bits[(x0 >> 5) * 256 + t] = word;
     }
-    int first = -1, last = -1;
-    for (int q = 0; q < nw; ++q) {
-        const uint32_t wq = bits[q * 256 + t];
-        if (wq) {
-            if (first < 0) first = q * 32 + __builtin_ctz(wq);
-            last = q * 32 + 31 - __builtin_clz(wq);
+    // first zero at or after a / last zero before b, or -1
+    auto first_from = [&](int a) -> int {
+        if (a >= nx) return -1;
+        int q = a >> 5;
+        uint32_t wq = bits[q * 256 + t] & (~0u << (a & 31));
+        while (true) {
+            if (wq) return q * 32 + __builtin_ctz(wq);
+            if (++q >= nw) return -1;
+            wq = bits[q * 256 + t];
         }
-    }
+    };
+    auto last_before = [&](int b) -> int {
+        if (b <= 0) return -1;
+        int q = (b - 1) >> 5;
+        uint32_t wq = bits[q * 256 + t] & (0xffffffffu >> (31 - ((b - 1) & 31)));
+        while (true) {
+            if (wq) return q * 32 + 31 - __builtin_clz(wq);
+            if (--q < 0) return -1;
+            wq = bits[q * 256 + t];
+        }
+    };
     if (bad) ctl[kCtlBail] = 1u;
     uint16_t *o = dz + (int64_t)y * pitch + z;
     const int64_t xs = (int64_t)ny * pitch;
+    const int first = first_from(0);
     if (first < 0) {
         // no zero: the reference's infinity (FLT_MAX in its later passes),
         // which loses to every finite parabola of a line
@@ -939,37 +959,35 @@ __global__ __launch_bounds__(256) void k_int_pass0(const uint8_t *__restrict__ i
         return;
     }
     constexpr int kFar = 1 << 20;
-    int prev = periodic ? last - nx : -kFar, next = first;
+    int vprev = -kFar, vnext = kFar;
+    if (wrap == 1) {
+        vprev = last_before(nx) - nx;
+        vnext = first + nx;
+    } else if (wrap == 2) {
+        const int f = first_from(2 * plo), l = last_before(nx - 2 * phi);
+        if (l >= 2 * plo) vprev = l + phi + 1 - nx;
+        if (f >= 0) vnext = nx - 1 + f - plo;
+    }
+    int prev = vprev, next = first;
     for (int x = 0; x < nx; ++x) {
         int d;
         if (x == next) {
             d = 0;
             prev = x;
-            int nn = -1;
-            if (x + 1 < nx) {
-                int q = (x + 1) >> 5;
-                uint32_t wq = bits[q * 256 + t] & (~0u << ((x + 1) & 31));
-                while (true) {
-                    if (wq) {
-                        nn = q * 32 + __builtin_ctz(wq);
-                        break;
-                    }
-                    if (++q >= nw) break;
-                    wq = bits[q * 256 + t];
-                }
-            }
-            next = nn >= 0 ? nn : (periodic ? first + nx : kFar);
+            const int nn = first_from(x + 1);
+            next = nn >= 0 ? nn : vnext;
         } else {
             d = min(x - prev, next - x);
         }
-        o[(int64_t)x * xs] = (uint16_t)d;
+        o[(int64_t)x * xs] = (uint16_t)min(d, 0xfffe);
     }
 }
 
 // pass 2 (axis 2, contiguous) in integers: per voxel the minimum over rows q
 // of (z - q)^2 + m(q), scanning outward until k^2 reaches the best value, on
 // lines staged in LDS with kIntWrap rows past each end (the wrapped rows the
-// line's hulls keep on a periodic axis, none otherwise); written as the float
+// line's hulls keep on a periodic axis, none otherwise: row r < 0 reads
+// r + wrap_lo, row r >= nz reads r - wrap_hi); written as the float
 // aniso * d2, or its sqrtf.  One wave per two lines of b[y][x][z] (line
 // numbers 2p, 2p + 1 of x * ny + y), held as pairs of uint16 in one word, so
 // one LDS read and packed minimum / saturating add serve both (values are at
@@ -977,7 +995,8 @@ __global__ __launch_bounds__(256) void k_int_pass0(const uint8_t *__restrict__ i
 // undercuts a best value).
 template <bool kSqrt>
 __global__ __launch_bounds__(256) void k_int_pass2(const uint16_t *__restrict__ b, int nx, int ny, int nz,
-                                                   int64_t pitch, int periodic, const uint32_t *__restrict__ masks,
+                                                   int64_t pitch, int periodic, int wrap_lo, int wrap_hi,
+                                                   const uint32_t *__restrict__ masks,
                                                    float *__restrict__ out, int64_t os0, int64_t os1, float aniso,
                                                    uint32_t *__restrict__ ctl)
 {
@@ -1021,10 +1040,9 @@ __global__ __launch_bounds__(256) void k_int_pass2(const uint16_t *__restrict__ 
 #pragma unroll
             for (int side = 0; side < 2; ++side) {
                 uint32_t v = 0xffffffffu;
-                if (periodic) {
-                    const int r = side ? nz + j : -1 - j;
-                    int m = r % nz;
-                    m = m < 0 ? m + nz : m;
+                const int r = side ? nz + j : -1 - j;
+                const int m = side ? r - wrap_hi : r + wrap_lo;  // the row a wrapped row reads
+                if (periodic && m >= 0 && m < nz) {
                     const bool inb = j < kHullMaskRows;
                     const int wi = (side ? 4 : 0) + (j >> 5);
                     const bool k0 = inb && ((msk[w][wi] >> (j & 31)) & 1u);
@@ -1082,8 +1100,8 @@ constexpr uint32_t kIntInfM = (1u << 22) - 1;
 // kCap entries (then nothing is written)
 template <bool kDist, int kCap>
 __device__ __forceinline__ bool int_hull_line(const uint16_t *__restrict__ f, const LineGeom &g, float aniso,
-                                              int64_t line, uint32_t *S, uint32_t *__restrict__ masks,
-                                              uint32_t *__restrict__ ctl)
+                                              int hs, int te, int64_t line, uint32_t *S,
+                                              uint32_t *__restrict__ masks, uint32_t *__restrict__ ctl)
 {
     int64_t ib, fb, ob;
     g.base(line, &ib, &fb, &ob);
@@ -1164,9 +1182,12 @@ __device__ __forceinline__ bool int_hull_line(const uint16_t *__restrict__ f, co
                 if ((b >> 5) == q) w[(forward ? 4 : 0) + q] |= 1u << (b & 31);
         }
     };
-    // head: position 0 through the first zero (the whole line if none)
+    // head: from hs through the first zero (to the end if none); tail: from
+    // the last zero before te (from 0 if none) to te.  edt3d: hs = 0, te = n;
+    // corrected_edt skips twice the pads: hs = 2 p_lo, te = n - 2 p_hi
+    // (get_boundary_hull's lower_skip / upper_skip, _distance.pyx:587-616)
     int first = -1;
-    stream_values<1>(p, st, 0, n, [&](int i, uint16_t v) {
+    stream_values<1>(p + (int64_t)hs * st, st, 0, n - hs, [&](int i, uint16_t v) {
         add(i, mval(v));
         if (v == 0) {
             first = i;
@@ -1175,29 +1196,36 @@ __device__ __forceinline__ bool int_hull_line(const uint16_t *__restrict__ f, co
         return fits;
     });
     if (!fits) return false;
-    if (n == 1 && tm == kIntInfM) {
-        // a one-voxel line with no finite height: the reference's selection
-        // reads its placeholder entry there; the float kernels reproduce that
+    const int nvh = first < 0 ? n - hs : first + 1;
+    if (nvh == 1 && tm == kIntInfM) {
+        // a one-voxel segment with no finite height: the reference's
+        // selection reads its placeholder entry there; the float kernels
+        // reproduce that
         ctl[kCtlBail] = 1u;
         return true;
     }
-    pick(true, first < 0 ? n : first + 1);
-    if (first < 0) {
+    pick(true, nvh);
+    if (first < 0 && hs == 0 && te == n) {
         // no zero: the tail segment is the same whole line
         pick(false, n);
     } else {
         int last = -1;
-        stream_values<-1>(p, st, 0, n, [&](int i, uint16_t v) {
+        stream_values<-1>(p, st, 0, te, [&](int i, uint16_t v) {
             if (v != 0) return true;
             last = i;
             return false;
         });
-        stream_values<1>(p + (int64_t)last * st, st, 0, n - last, [&](int i, uint16_t v) {
+        const int t0 = last < 0 ? 0 : last, nvt = te - t0;
+        stream_values<1>(p + (int64_t)t0 * st, st, 0, nvt, [&](int i, uint16_t v) {
             add(i, mval(v));
             return fits;
         });
         if (!fits) return false;
-        pick(false, n - last);
+        if (nvt == 1 && tm == kIntInfM) {
+            ctl[kCtlBail] = 1u;
+            return true;
+        }
+        pick(false, nvt);
     }
     uint32_t *m = masks + line * kHullMaskWords;
 #pragma unroll
@@ -1208,8 +1236,8 @@ __device__ __forceinline__ bool int_hull_line(const uint16_t *__restrict__ f, co
 // The lines (all of them, or those of `list`), one per lane; a line whose
 // envelope outgrows kCap entries goes to `out`.
 template <bool kDist, int kCap>
-__global__ __launch_bounds__(kWave) void k_int_hulls(const uint16_t *__restrict__ f, LineGeom g, float aniso,
-                                                     uint32_t *__restrict__ masks, const int *__restrict__ list,
+__global__ __launch_bounds__(kWave) void k_int_hulls(const uint16_t *__restrict__ f, LineGeom g, float aniso, int hs,
+                                                     int te, uint32_t *__restrict__ masks, const int *__restrict__ list,
                                                      const int *__restrict__ list_n, int *__restrict__ out,
                                                      int *__restrict__ out_n, uint32_t *__restrict__ ctl)
 {
@@ -1218,13 +1246,13 @@ __global__ __launch_bounds__(kWave) void k_int_hulls(const uint16_t *__restrict_
     // no barriers below: each lane owns its LDS column
     if (!list) {
         const int64_t line = (int64_t)blockIdx.x * kWave + threadIdx.x;
-        if (line < g.lines() && !int_hull_line<kDist, kCap>(f, g, aniso, line, S, masks, ctl))
+        if (line < g.lines() && !int_hull_line<kDist, kCap>(f, g, aniso, hs, te, line, S, masks, ctl))
             out[atomicAdd(out_n, 1)] = (int)line;
         return;
     }
     const int m = *list_n;
     for (int j = blockIdx.x * kWave + threadIdx.x; j < m; j += gridDim.x * kWave)
-        if (!int_hull_line<kDist, kCap>(f, g, aniso, list[j], S, masks, ctl)) out[atomicAdd(out_n, 1)] = list[j];
+        if (!int_hull_line<kDist, kCap>(f, g, aniso, hs, te, list[j], S, masks, ctl)) out[atomicAdd(out_n, 1)] = list[j];
 }
 
 // the float kernels' gate from the integer passes' flags
@@ -1323,11 +1351,11 @@ RefWs carve_ref(Carver &c, const int64_t *n)
     return w;
 }
 
-// whether edt3d's integer form covers these arguments (the data-dependent
+// whether the integer form covers these arguments (the data-dependent
 // conditions are checked on the device)
 bool int_form_applies(const int64_t *n, int first_axis, const int64_t *pads, const float *res)
 {
-    if (!g_int_form || pads || first_axis != 0) return false;
+    if (!g_int_form || first_axis != 0) return false;
     if (!(res[0] == res[1] && res[1] == res[2])) return false;
     int e = 0;
     if (std::frexp(res[0], &e) != 0.5f || e < -40 || e > 40) return false;
@@ -1422,19 +1450,22 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
         return check_launch("reference hulls");
     };
 
-    // edt3d's integer form where it applies; the float kernels below then
-    // run only if it set its result aside (ctl[kCtlGate])
+    // the integer form where it applies; the float kernels below then run
+    // only if it set its result aside (ctl[kCtlGate])
     const unsigned *gate = nullptr;
     if (int_form_applies(n, first_axis, pads, resolution)) {
         const int nx = (int)n[0], ny = (int)n[1], nz = (int)n[2];
         const float aniso = resolution[0] * resolution[0];
         const int64_t pitch = w.pitch;
+        // pad widths of the periodic axes (corrected_edt), 0 for edt3d
+        auto pad = [&](int a, int hi) { return pads && periodic[a] ? (int)pads[2 * a + hi] : 0; };
         hipLaunchKernelGGL(k_clear_words, dim3((unsigned)std::min<int64_t>((w.ctl_words + 255) / 256, 1024)),
                            dim3(256), 0, s, w.ctl, w.ctl_words);
         {
             ProfScope ps("edt_int_pass0", s);
             hipLaunchKernelGGL(k_int_pass0, dim3((unsigned)(((int64_t)ny * nz + 255) / 256)), dim3(256), 0, s, img,
-                               is[0], is[1], nx, ny, nz, periodic[0] ? 1 : 0, w.dz, pitch, w.ctl);
+                               is[0], is[1], nx, ny, nz, periodic[0] ? (pads ? 2 : 1) : 0, pad(0, 0), pad(0, 1),
+                               w.dz, pitch, w.ctl);
         }
         if (int e = check_launch("integer pass 0")) return e;
         // the hull masks of the lines along dim, then the lines whose
@@ -1444,18 +1475,19 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
             constexpr bool kDist = std::is_same<Cv, FromDist>::value;
             const LineGeom g = make_line_geom(n, is, fs, fs, dim);
             const int64_t lines = g.lines();
+            const int hs = 2 * pad(dim, 0), te = (int)n[dim] - 2 * pad(dim, 1);
             hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, w.n_ovf, 4, nullptr);
             {
                 // small stacks for every line, large ones for the lines that
                 // outgrew them (listed in w.ovf2), the rest into w.ovf
                 ProfScope ps("edt_int_hulls", s);
                 hipLaunchKernelGGL((k_int_hulls<kDist, kIntHullCap>), dim3((unsigned)((lines + kWave - 1) / kWave)),
-                                   dim3(kWave), 0, s, f, g, aniso, w.masks, nullptr, nullptr, w.ovf2, w.n_ovf + 1,
-                                   w.ctl);
+                                   dim3(kWave), 0, s, f, g, aniso, hs, te, w.masks, nullptr, nullptr, w.ovf2,
+                                   w.n_ovf + 1, w.ctl);
                 hipLaunchKernelGGL((k_int_hulls<kDist, kIntHullCapBig>),
                                    dim3((unsigned)std::min<int64_t>(kIntHullListWaves, (lines + kWave - 1) / kWave)),
-                                   dim3(kWave), 0, s, f, g, aniso, w.masks, w.ovf2, w.n_ovf + 1, w.ovf, w.n_ovf,
-                                   w.ctl);
+                                   dim3(kWave), 0, s, f, g, aniso, hs, te, w.masks, w.ovf2, w.n_ovf + 1, w.ovf,
+                                   w.n_ovf, w.ctl);
             }
             if (int e = check_launch("integer hulls")) return e;
             ProfScope ps("edt_int_hulls_fallback", s);
@@ -1463,11 +1495,11 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
             if (lanes > 0) {
                 const int64_t blocks = std::min<int64_t>((lines + lanes - 1) / lanes, kIntFbBlocks);
                 hipLaunchKernelGGL((k_ref_hulls_fallback<true, uint16_t, Cv>), dim3((unsigned)blocks), dim3(lanes),
-                                   chord_lds_bytes(cap, lanes), s, img, f, cv, g, resolution[dim], Skips{0, 0, 0},
-                                   w.hulls, w.counts, w.ovf, w.n_ovf, nullptr, cap, w.masks, nullptr);
+                                   chord_lds_bytes(cap, lanes), s, img, f, cv, g, resolution[dim], skips(dim), w.hulls,
+                                   w.counts, w.ovf, w.n_ovf, nullptr, cap, w.masks, nullptr);
             } else {
                 hipLaunchKernelGGL((k_ref_hulls_fallback<false, uint16_t, Cv>), dim3(fb_threads(n) / kWave),
-                                   dim3(kWave), 0, s, img, f, cv, g, resolution[dim], Skips{0, 0, 0}, w.hulls,
+                                   dim3(kWave), 0, s, img, f, cv, g, resolution[dim], skips(dim), w.hulls,
                                    w.counts, w.ovf, w.n_ovf, w.stk, cap, w.masks, nullptr);
             }
             return check_launch("integer hulls");
@@ -1481,6 +1513,8 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
             const int bc = periodic[1] ? PMX_BC_PERIODIC : PMX_BC_END;
             WinGeom wg{ny, pitch, ny * pitch, nx, nz, bc, bc, kBig, pitch, pitch, nx * pitch};
             wg.hmask = w.masks;
+            wg.wrap_lo = ny - 2 * pad(1, 1);
+            wg.wrap_hi = ny - 2 * pad(1, 0);
             wg.tpb = tiles_per_block(wg, 64);
             const unsigned blocks = (unsigned)window_blocks(wg, 64, wg.tpb);
             Sq16 *b16 = reinterpret_cast<Sq16 *>(w.b);
@@ -1501,10 +1535,12 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
             const unsigned blocks = (unsigned)(((int64_t)nx * ny + 7) / 8);
             if (take_sqrt)
                 hipLaunchKernelGGL(k_int_pass2<true>, dim3(blocks), dim3(256), 0, s, w.b, nx, ny, nz, pitch,
-                                   periodic[2] ? 1 : 0, w.masks, d, os[0], os[1], aniso, w.ctl);
+                                   periodic[2] ? 1 : 0, nz - 2 * pad(2, 1), nz - 2 * pad(2, 0), w.masks, d, os[0],
+                                   os[1], aniso, w.ctl);
             else
                 hipLaunchKernelGGL(k_int_pass2<false>, dim3(blocks), dim3(256), 0, s, w.b, nx, ny, nz, pitch,
-                                   periodic[2] ? 1 : 0, w.masks, d, os[0], os[1], aniso, w.ctl);
+                                   periodic[2] ? 1 : 0, nz - 2 * pad(2, 1), nz - 2 * pad(2, 0), w.masks, d, os[0],
+                                   os[1], aniso, w.ctl);
         }
         if (int e = check_launch("integer pass 2")) return e;
         hipLaunchKernelGGL(k_int_decide, dim3(1), dim3(64), 0, s, w.ctl);

@@ -21,8 +21,9 @@ from tests.helpers import rand_binary
 pytestmark = pytest.mark.gpu
 
 
-def ref_call(t, periodic, res, take_sqrt, int_form=True):
-    """pmx_edt_ref on a contiguous uint8 cuda tensor; (float32 tensor, form)."""
+def ref_call(t, periodic, res, take_sqrt, int_form=True, pads=None):
+    """pmx_edt_ref on a contiguous uint8 cuda tensor (pads: corrected_edt's
+    padded-subdomain mode); (float32 tensor, form)."""
     import torch
 
     from pmmoto_amd import _device, _lib
@@ -38,7 +39,8 @@ def ref_call(t, periodic, res, take_sqrt, int_form=True):
     out = torch.empty(shape, dtype=torch.float32, device="cuda")
     prev = L.pmx_edt_ref_int_form(1 if int_form else 0)
     try:
-        _lib.check(L.pmx_edt_ref(_device.ptr(t), nn, st, _device.ptr(out), st, 0, per, None, rr,
+        pd = None if pads is None else (C.c_int64 * 6)(*pads)
+        _lib.check(L.pmx_edt_ref(_device.ptr(t), nn, st, _device.ptr(out), st, 0, per, pd, rr,
                                  1 if take_sqrt else 0, _device.ptr(ws), nb.value, _device.stream()))
     finally:
         L.pmx_edt_ref_int_form(prev)
@@ -47,11 +49,11 @@ def ref_call(t, periodic, res, take_sqrt, int_form=True):
     return out, form.value
 
 
-def both_forms(img, periodic, res, take_sqrt=False):
+def both_forms(img, periodic, res, take_sqrt=False, pads=None):
     import torch
     t = torch.from_numpy(np.ascontiguousarray(img)).cuda()
-    a, fa = ref_call(t, periodic, res, take_sqrt, True)
-    b, fb = ref_call(t, periodic, res, take_sqrt, False)
+    a, fa = ref_call(t, periodic, res, take_sqrt, True, pads)
+    b, fb = ref_call(t, periodic, res, take_sqrt, False, pads)
     assert fb == 2
     return a.cpu().numpy(), fa, b.cpu().numpy()
 
@@ -142,3 +144,36 @@ def test_cases_at_the_edge_of_the_integer_form(case):
     np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
     if img.size <= 200_000:
         np.testing.assert_array_equal(a.view(np.uint32), O.ref_edt_sq(img, periodic, res).view(np.uint32))
+
+
+@pytest.mark.parametrize("kind,own_shape,bts,pad,res", [
+    ("pack", (64, 64, 64), ("periodic",) * 3, (1, 1, 1), 1 / 64),
+    ("pack", (48, 40, 56), ("periodic", "wall", "periodic"), (2, 1, 3), 1.0),
+    ("noise", (40, 33, 50), ("periodic", "periodic", "end"), (2, 2, 1), 0.5),
+    ("noise", (26, 30, 22), ("wall", "periodic", "periodic"), (2, 1, 3), 1.0),
+    ("noise", (30, 31, 29), ("periodic", "periodic", "periodic"), (3, 2, 1), 2.0 ** -5),
+])
+def test_padded_subdomain_forms_agree_and_match_oracle(kind, own_shape, bts, pad, res):
+    """corrected_edt's padded-subdomain mode (periodic copies in the pads,
+    zero wall layers): the integer form against the float kernels and the
+    oracle's restatement of corrected_edt."""
+    periodic = [b == "periodic" for b in bts]
+    if kind == "pack":
+        own = pack(own_shape, 0.5, 3, 9, 51, tuple(periodic))
+    else:
+        own = rand_binary(own_shape, 0.25, 52)
+    img = own
+    pads = []
+    for a in range(3):
+        p = pad[a]
+        mode = "wrap" if periodic[a] else "constant"
+        w = (p, p) if bts[a] != "end" else (0, 0)
+        img = np.pad(img, [(0, 0)] * a + [w] + [(0, 0)] * (2 - a), mode=mode)
+        pads += list(w)
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    print(f"[int form] padded {kind} {img.shape} {bts} pads={pads} res={res}", flush=True)
+    a, form, b = both_forms(img, periodic, (res,) * 3, take_sqrt=True, pads=pads)
+    assert form == 0
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    want = np.sqrt(O.ref_corrected_edt_sq(img, periodic, pads, (res,) * 3))
+    np.testing.assert_array_equal(a.view(np.uint32), want.view(np.uint32))

@@ -1096,11 +1096,11 @@ constexpr int kIntHullCapBig = 128;
 constexpr int kIntHullListWaves = 1024;  // persistent waves of the second kernel
 constexpr uint32_t kIntInfM = (1u << 22) - 1;
 
-// both hulls of one line into its mask; false when the envelope outgrows
-// kCap entries (then nothing is written)
+// one hull of one line into its half of the line's mask; false when the
+// envelope outgrows kCap entries (then nothing is written)
 template <bool kDist, int kCap>
-__device__ __forceinline__ bool int_hull_line(const uint16_t *__restrict__ f, const LineGeom &g, float aniso,
-                                              int hs, int te, int64_t line, uint32_t *S,
+__device__ __forceinline__ bool int_hull_side(const uint16_t *__restrict__ f, const LineGeom &g, float aniso,
+                                              int hs, int te, int64_t line, int side, uint32_t *S,
                                               uint32_t *__restrict__ masks, uint32_t *__restrict__ ctl)
 {
     int64_t ib, fb, ob;
@@ -1163,7 +1163,7 @@ __device__ __forceinline__ bool int_hull_line(const uint16_t *__restrict__ f, co
         th = h;
         tr = sv;
     };
-    uint32_t w[kHullMaskWords] = {};
+    uint32_t w[4] = {};
     // the reference's selection: from the start (forward) or the top, the
     // finite entries, at most min(kNumHull, k) + 1 of them; mark row
     // bit(local vertex)
@@ -1179,35 +1179,26 @@ __device__ __forceinline__ bool int_hull_line(const uint16_t *__restrict__ f, co
             if (b >= kHullMaskRows) continue;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if ((b >> 5) == q) w[(forward ? 4 : 0) + q] |= 1u << (b & 31);
+                if ((b >> 5) == q) w[q] |= 1u << (b & 31);
         }
     };
-    // head: from hs through the first zero (to the end if none); tail: from
-    // the last zero before te (from 0 if none) to te.  edt3d: hs = 0, te = n;
-    // corrected_edt skips twice the pads: hs = 2 p_lo, te = n - 2 p_hi
-    // (get_boundary_hull's lower_skip / upper_skip, _distance.pyx:587-616)
-    int first = -1;
-    stream_values<1>(p + (int64_t)hs * st, st, 0, n - hs, [&](int i, uint16_t v) {
-        add(i, mval(v));
-        if (v == 0) {
-            first = i;
-            return false;
-        }
-        return fits;
-    });
-    if (!fits) return false;
-    const int nvh = first < 0 ? n - hs : first + 1;
-    if (nvh == 1 && tm == kIntInfM) {
-        // a one-voxel segment with no finite height: the reference's
-        // selection reads its placeholder entry there; the float kernels
-        // reproduce that
-        ctl[kCtlBail] = 1u;
-        return true;
-    }
-    pick(true, nvh);
-    if (first < 0 && hs == 0 && te == n) {
-        // no zero: the tail segment is the same whole line
-        pick(false, n);
+    // side 0, the head: from hs through the first zero (to the end if none);
+    // side 1, the tail: from the last zero before te (from 0 if none) to te.
+    // edt3d: hs = 0, te = n; corrected_edt skips twice the pads: hs = 2 p_lo,
+    // te = n - 2 p_hi (get_boundary_hull's lower_skip / upper_skip,
+    // _distance.pyx:587-616)
+    int nv;
+    if (side == 0) {
+        int first = -1;
+        stream_values<1>(p + (int64_t)hs * st, st, 0, n - hs, [&](int i, uint16_t v) {
+            add(i, mval(v));
+            if (v == 0) {
+                first = i;
+                return false;
+            }
+            return fits;
+        });
+        nv = first < 0 ? n - hs : first + 1;
     } else {
         int last = -1;
         stream_values<-1>(p, st, 0, te, [&](int i, uint16_t v) {
@@ -1215,44 +1206,56 @@ __device__ __forceinline__ bool int_hull_line(const uint16_t *__restrict__ f, co
             last = i;
             return false;
         });
-        const int t0 = last < 0 ? 0 : last, nvt = te - t0;
-        stream_values<1>(p + (int64_t)t0 * st, st, 0, nvt, [&](int i, uint16_t v) {
+        const int t0 = last < 0 ? 0 : last;
+        nv = te - t0;
+        stream_values<1>(p + (int64_t)t0 * st, st, 0, nv, [&](int i, uint16_t v) {
             add(i, mval(v));
             return fits;
         });
-        if (!fits) return false;
-        if (nvt == 1 && tm == kIntInfM) {
-            ctl[kCtlBail] = 1u;
-            return true;
-        }
-        pick(false, nvt);
     }
-    uint32_t *m = masks + line * kHullMaskWords;
+    if (!fits) return false;
+    if (nv == 1 && tm == kIntInfM) {
+        // a one-voxel segment with no finite height: the reference's
+        // selection reads its placeholder entry there; the float kernels
+        // reproduce that
+        ctl[kCtlBail] = 1u;
+        return true;
+    }
+    pick(side == 0, nv);
+    // side 0 (upper hull, rows n + b) -> words 4..7, side 1 -> words 0..3
+    uint32_t *m = masks + line * kHullMaskWords + (side == 0 ? 4 : 0);
 #pragma unroll
-    for (int q = 0; q < kHullMaskWords; ++q) m[q] = w[q];
+    for (int q = 0; q < 4; ++q) m[q] = w[q];
     return true;
 }
 
-// The lines (all of them, or those of `list`), one per lane; a line whose
-// envelope outgrows kCap entries goes to `out`.
+// Both hulls of every line, or the (line, side) items of `list` (2 line +
+// side), one per lane; an item whose envelope outgrows kCap entries goes to
+// `out`: as an item (out_items) or as a line for the whole-line fallback.
 template <bool kDist, int kCap>
 __global__ __launch_bounds__(kWave) void k_int_hulls(const uint16_t *__restrict__ f, LineGeom g, float aniso, int hs,
                                                      int te, uint32_t *__restrict__ masks, const int *__restrict__ list,
                                                      const int *__restrict__ list_n, int *__restrict__ out,
-                                                     int *__restrict__ out_n, uint32_t *__restrict__ ctl)
+                                                     int *__restrict__ out_n, int out_items,
+                                                     uint32_t *__restrict__ ctl)
 {
     __shared__ uint32_t stk[kCap * kWave];
     uint32_t *S = stk + threadIdx.x;
+    auto item = [&](int64_t line, int side) {
+        if (!int_hull_side<kDist, kCap>(f, g, aniso, hs, te, line, side, S, masks, ctl))
+            out[atomicAdd(out_n, 1)] = out_items ? (int)(2 * line + side) : (int)line;
+    };
     // no barriers below: each lane owns its LDS column
     if (!list) {
         const int64_t line = (int64_t)blockIdx.x * kWave + threadIdx.x;
-        if (line < g.lines() && !int_hull_line<kDist, kCap>(f, g, aniso, hs, te, line, S, masks, ctl))
-            out[atomicAdd(out_n, 1)] = (int)line;
+        if (line < g.lines()) {
+            item(line, 0);
+            item(line, 1);
+        }
         return;
     }
     const int m = *list_n;
-    for (int j = blockIdx.x * kWave + threadIdx.x; j < m; j += gridDim.x * kWave)
-        if (!int_hull_line<kDist, kCap>(f, g, aniso, hs, te, list[j], S, masks, ctl)) out[atomicAdd(out_n, 1)] = list[j];
+    for (int j = blockIdx.x * kWave + threadIdx.x; j < m; j += gridDim.x * kWave) item(list[j] >> 1, list[j] & 1);
 }
 
 // the float kernels' gate from the integer passes' flags
@@ -1336,7 +1339,7 @@ RefWs carve_ref(Carver &c, const int64_t *n)
     w.stk = c.take<int>((size_t)3 * fb_cap(nmax) * fb_threads(n));
     w.hulls = c.take<RefHull>((size_t)lines * 2 * kHullCap);
     w.counts = c.take<int>((size_t)lines * 2);
-    w.ovf = c.take<int>((size_t)lines);
+    w.ovf = c.take<int>((size_t)lines * 2);  // a line may be listed once per side
     w.n_ovf = c.take<int>(4);
     w.a_cap = (int)((lines + kWave - 1) / kWave * kSegChords);
     w.c_cap = (int)(lines + w.a_cap);
@@ -1344,7 +1347,7 @@ RefWs carve_ref(Carver &c, const int64_t *n)
     w.ca = c.take<Chord>((size_t)w.a_cap);
     w.cc = c.take<Chord>((size_t)w.c_cap);
     w.masks = c.take<uint32_t>((size_t)lines * kHullMaskWords);
-    w.ovf2 = c.take<int>((size_t)lines);
+    w.ovf2 = c.take<int>((size_t)lines * 2);
     w.ctl_words = kCtlBits + (lines + 31) / 32;
     w.ctl = c.take<uint32_t>((size_t)w.ctl_words);
     w.wlist = c.take<uint32_t>((size_t)lines);
@@ -1483,11 +1486,11 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
                 ProfScope ps("edt_int_hulls", s);
                 hipLaunchKernelGGL((k_int_hulls<kDist, kIntHullCap>), dim3((unsigned)((lines + kWave - 1) / kWave)),
                                    dim3(kWave), 0, s, f, g, aniso, hs, te, w.masks, nullptr, nullptr, w.ovf2,
-                                   w.n_ovf + 1, w.ctl);
+                                   w.n_ovf + 1, 1, w.ctl);
                 hipLaunchKernelGGL((k_int_hulls<kDist, kIntHullCapBig>),
                                    dim3((unsigned)std::min<int64_t>(kIntHullListWaves, (lines + kWave - 1) / kWave)),
                                    dim3(kWave), 0, s, f, g, aniso, hs, te, w.masks, w.ovf2, w.n_ovf + 1, w.ovf,
-                                   w.n_ovf, w.ctl);
+                                   w.n_ovf, 0, w.ctl);
             }
             if (int e = check_launch("integer hulls")) return e;
             ProfScope ps("edt_int_hulls_fallback", s);

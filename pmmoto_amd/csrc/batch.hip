@@ -464,6 +464,12 @@ int run_fused(const pmx_psd_batch *b, hipStream_t s)
         hipLaunchKernelGGL((k_window_batch<T, W, E, O, D2>), grid(g.total), dim3(kWinThreads), 0, s,          \
                            static_cast<const WinArgs *>(g.args), g.first, g.n, g.total, GATE, CAP, THR, CODE); \
     })
+// pass X of a dilation: the first-invasion update is its only output
+#define PMX_WBP(T, W, G, GATE, CAP, THR, CODE)                                                                \
+    launch_group(G, s, [&](const BatchGroup &g) {                                                             \
+        hipLaunchKernelGGL((k_window_batch<T, W, Sq16, int32_t, false, true>), grid(g.total), dim3(kWinThreads), \
+                           0, s, static_cast<const WinArgs *>(g.args), g.first, g.n, g.total, GATE, CAP, THR, CODE); \
+    })
         PMX_WB(32, 16, uint16_t, Sq16, false, b->y16[0], -1, -1, 0, -1);
         PMX_WB(64, 16, uint16_t, Sq16, false, b->y16[1], -1, -1, 0, -1);
         launch_group(b->fy16, s, [&](const BatchGroup &g) {
@@ -510,18 +516,19 @@ int run_fused(const pmx_psd_batch *b, hipStream_t s)
         if (wide) {
             PMX_WB(32, 32, uint16_t, Sq16, false, b->ydil[0], lv, cap, 0, -1);
             PMX_WB(64, 32, uint16_t, Sq16, false, b->ydil[1], lv, cap, 0, -1);
-            PMX_WB(32, 32, Sq16, int32_t, false, b->xdil[0], lv, cap, thr, lv + 1);
-            PMX_WB(64, 32, Sq16, int32_t, false, b->xdil[1], lv, cap, thr, lv + 1);
+            PMX_WBP(32, 32, b->xdil[0], lv, cap, thr, lv + 1);
+            PMX_WBP(64, 32, b->xdil[1], lv, cap, thr, lv + 1);
         } else {
             PMX_WB(32, 16, uint16_t, Sq16, false, b->ydil[0], lv, cap, 0, -1);
             PMX_WB(64, 16, uint16_t, Sq16, false, b->ydil[1], lv, cap, 0, -1);
-            PMX_WB(32, 16, Sq16, int32_t, false, b->xdil[0], lv, cap, thr, lv + 1);
-            PMX_WB(64, 16, Sq16, int32_t, false, b->xdil[1], lv, cap, thr, lv + 1);
+            PMX_WBP(32, 16, b->xdil[0], lv, cap, thr, lv + 1);
+            PMX_WBP(64, 16, b->xdil[1], lv, cap, thr, lv + 1);
         }
         if (int e = check_launch("batched dilation")) return e;
     }
 #undef PMX_ZB
 #undef PMX_WB
+#undef PMX_WBP
     return PMX_OK;
 }
 

@@ -50,7 +50,12 @@ void launch_window(const E *in, O *out, const PassXOut &o, const WinGeom &wg0, i
     const int64_t blocks = window_blocks(wg, T, wg.tpb);
     const bool d2only = !std::is_same<E, uint16_t>::value && o.d2 && !o.f32sq && !o.f32 && !o.f64 && !o.u8 &&
                         !o.psd_map;
-    if (d2only)
+    const bool psd_only = !std::is_same<E, uint16_t>::value && o.psd_map && !o.d2 && !o.f32sq && !o.f32 && !o.f64 &&
+                          !o.u8;
+    if (psd_only)
+        hipLaunchKernelGGL((k_window<T, W, E, O, false, false, true>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s,
+                           in, out, o, wg, nx, ny, bits, count, list, proceed);
+    else if (d2only)
         hipLaunchKernelGGL((k_window<T, W, E, O, true>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s, in, out, o,
                            wg, nx, ny, bits, count, list, proceed);
     else

@@ -535,6 +535,19 @@ __device__ bool envelope_line(int n, int bc_lo, int bc_hi, const Stack &st, bool
     return true;
 }
 
+// The first-invasion update alone (the porosimetry dilation's pass X: no
+// other output): a covered pore voxel not yet invaded (state 1) takes the
+// level's code (porosimetry.py:220-225).
+__device__ __forceinline__ void psd_update(const PassXOut &o, int64_t x, int64_t y, int64_t z, int v)
+{
+    if (v > o.t) return;
+    const int64_t si = x * o.ssx + y * o.ssy + z;
+    if (o.psd_state[si] == 1) {
+        o.psd_state[si] = 2;
+        o.psd_map[x * o.sx + y * o.sy + z] = o.psd_code;
+    }
+}
+
 __device__ __forceinline__ void emit_x(const PassXOut &o, int64_t x, int64_t y, int64_t z,
                                        int nx, int ny, int nz, int v)
 {
@@ -825,7 +838,7 @@ __device__ __forceinline__ void commit_rows(L *tile, const WinGeom &g, int dst, 
 // is scanned), so each row is loaded once per block.  D2ONLY (pass X):
 // the int32 squared distance is the only output, written without the
 // per-row tests of the other outputs.
-template <int T, int W, class E, class O, bool D2ONLY, bool kHull = false>
+template <int T, int W, class E, class O, bool D2ONLY, bool kHull = false, bool kPsd = false>
 __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in, O *__restrict__ out,
                                             const PassXOut &o, const WinGeom &g, int nx, int ny,
                                             uint32_t *__restrict__ line_bits, uint32_t *__restrict__ ovf_count,
@@ -948,6 +961,8 @@ __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in,
                 }
             } else if constexpr (D2ONLY) {
                 o.d2[(int64_t)y * o.sx + (int64_t)outer * o.sy + z] = best >= kBig ? kInf : best;
+            } else if constexpr (kPsd) {
+                psd_update(o, y, outer, z, best);
             } else {
                 emit_x(o, y, outer, z, nx, ny, g.nz, best >= kBig ? kInf : best);
             }
@@ -1010,6 +1025,9 @@ __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in,
                     if (act && ok0 && zp < g.nz) d[0] = bv0;
                     if (act && ok1 && zp + 1 < g.nz) d[1] = bv1;
                 }
+            } else if constexpr (kPsd) {
+                if (act && ok0 && zp < g.nz) psd_update(o, y, outer, zp, bv0);
+                if (act && ok1 && zp + 1 < g.nz) psd_update(o, y, outer, zp + 1, bv1);
             } else {
                 if (act && ok0 && zp < g.nz) emit_x(o, y, outer, zp, nx, ny, g.nz, bv0 >= kBig ? kInf : bv0);
                 if (act && ok1 && zp + 1 < g.nz) emit_x(o, y, outer, zp + 1, nx, ny, g.nz, bv1 >= kBig ? kInf : bv1);
@@ -1095,7 +1113,7 @@ __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in,
     }
 }
 
-template <int T, int W, class E, class O, bool D2ONLY = false, bool kHull = false>
+template <int T, int W, class E, class O, bool D2ONLY = false, bool kHull = false, bool kPsd = false>
 __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in, O *__restrict__ out,
                                                         PassXOut o, WinGeom g, int nx, int ny,
                                                         uint32_t *__restrict__ line_bits,
@@ -1104,7 +1122,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window(const E *__restrict__ in
                                                         const unsigned int *__restrict__ proceed)
 {
     if (proceed && *proceed == 0u) return;
-    window_body<T, W, E, O, D2ONLY, kHull>(xcd_tile(blockIdx.x, window_blocks(g, T, g.tpb)), in, out, o, g, nx, ny,
+    window_body<T, W, E, O, D2ONLY, kHull, kPsd>(xcd_tile(blockIdx.x, window_blocks(g, T, g.tpb)), in, out, o, g, nx, ny,
                                     line_bits, ovf_count, ovf_list);
 }
 
@@ -1289,7 +1307,7 @@ struct WinArgs {
     const unsigned int *proceed;
 };
 
-template <int T, int W, class E, class O, bool D2ONLY>
+template <int T, int W, class E, class O, bool D2ONLY, bool kPsd = false>
 __global__ __launch_bounds__(kWinThreads) void k_window_batch(const WinArgs *__restrict__ args,
                                                               const int64_t *__restrict__ first, int n,
                                                               int64_t total, int gate, int cap, int32_t thr,
@@ -1306,7 +1324,7 @@ __global__ __launch_bounds__(kWinThreads) void k_window_batch(const WinArgs *__r
         o.t = thr;
         o.psd_code = code;
     }
-    window_body<T, W, E, O, D2ONLY>(gb - first[k], static_cast<const E *>(a.in), static_cast<O *>(a.out), o, g,
+    window_body<T, W, E, O, D2ONLY, false, kPsd>(gb - first[k], static_cast<const E *>(a.in), static_cast<O *>(a.out), o, g,
                                     a.nx, a.ny, a.bits, a.count, a.list);
 }
 

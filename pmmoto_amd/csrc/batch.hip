@@ -464,10 +464,11 @@ int run_fused(const pmx_psd_batch *b, hipStream_t s)
         hipLaunchKernelGGL((k_window_batch<T, W, E, O, D2>), grid(g.total), dim3(kWinThreads), 0, s,          \
                            static_cast<const WinArgs *>(g.args), g.first, g.n, g.total, GATE, CAP, THR, CODE); \
     })
-// pass X of a dilation: the first-invasion update is its only output
-#define PMX_WBP(T, W, G, GATE, CAP, THR, CODE)                                                                \
+// pass X of a dilation (input B: Sq16, or Sq8 under a cap below 256): the
+// first-invasion update is its only output
+#define PMX_WBP(T, W, B, G, GATE, CAP, THR, CODE)                                                             \
     launch_group(G, s, [&](const BatchGroup &g) {                                                             \
-        hipLaunchKernelGGL((k_window_batch<T, W, Sq16, int32_t, false, true>), grid(g.total), dim3(kWinThreads), \
+        hipLaunchKernelGGL((k_window_batch<T, W, B, int32_t, false, true>), grid(g.total), dim3(kWinThreads),   \
                            0, s, static_cast<const WinArgs *>(g.args), g.first, g.n, g.total, GATE, CAP, THR, CODE); \
     })
         PMX_WB(32, 16, uint16_t, Sq16, false, b->y16[0], -1, -1, 0, -1);
@@ -516,13 +517,19 @@ int run_fused(const pmx_psd_batch *b, hipStream_t s)
         if (wide) {
             PMX_WB(32, 32, uint16_t, Sq16, false, b->ydil[0], lv, cap, 0, -1);
             PMX_WB(64, 32, uint16_t, Sq16, false, b->ydil[1], lv, cap, 0, -1);
-            PMX_WBP(32, 32, b->xdil[0], lv, cap, thr, lv + 1);
-            PMX_WBP(64, 32, b->xdil[1], lv, cap, thr, lv + 1);
+            PMX_WBP(32, 32, Sq16, b->xdil[0], lv, cap, thr, lv + 1);
+            PMX_WBP(64, 32, Sq16, b->xdil[1], lv, cap, thr, lv + 1);
+        } else if (cap <= 255) {
+            // 8-bit squared distances between passes Y and X
+            PMX_WB(32, 16, uint16_t, Sq8, false, b->ydil[0], lv, cap, 0, -1);
+            PMX_WB(64, 16, uint16_t, Sq8, false, b->ydil[1], lv, cap, 0, -1);
+            PMX_WBP(32, 16, Sq8, b->xdil[0], lv, cap, thr, lv + 1);
+            PMX_WBP(64, 16, Sq8, b->xdil[1], lv, cap, thr, lv + 1);
         } else {
             PMX_WB(32, 16, uint16_t, Sq16, false, b->ydil[0], lv, cap, 0, -1);
             PMX_WB(64, 16, uint16_t, Sq16, false, b->ydil[1], lv, cap, 0, -1);
-            PMX_WBP(32, 16, b->xdil[0], lv, cap, thr, lv + 1);
-            PMX_WBP(64, 16, b->xdil[1], lv, cap, thr, lv + 1);
+            PMX_WBP(32, 16, Sq16, b->xdil[0], lv, cap, thr, lv + 1);
+            PMX_WBP(64, 16, Sq16, b->xdil[1], lv, cap, thr, lv + 1);
         }
         if (int e = check_launch("batched dilation")) return e;
     }

@@ -265,7 +265,8 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     // EDT whose threshold reaches past 17^2 scans up to 32 rows in LDS
     const bool wide = ccap < kBig && ccap > 17 * 17;
     const YX yx{p, g, o, ccap, tr, wide, s};
-    // a cap that fits in 16 bits: pass Y hands pass X 16-bit squared distances
+    // a cap that fits in 8 (16) bits: pass Y hands pass X 8-bit (16-bit) squared distances
+    if (ccap <= 255) return run_yx(yx, reinterpret_cast<Sq8 *>(p.b), proceed);
     if (ccap <= 65535) return run_yx(yx, reinterpret_cast<Sq16 *>(p.b), proceed);
     // otherwise 16 bits first, int32 only where a value saturated (decided on the device)
     if (!proceed) return run_yx_narrow_first(yx);

@@ -689,6 +689,16 @@ struct Sq16 {
     uint16_t v;
 };
 
+// The same clamped to a cap <= 255, in 8 bits: the pass-Y output of a
+// dilation of radius below 16 (most levels of a pore-size sweep), which
+// halves that traffic again.
+struct Sq8 {
+    uint8_t v;
+};
+template <class T> struct IsSq : std::false_type {};
+template <> struct IsSq<Sq16> : std::true_type {};
+template <> struct IsSq<Sq8> : std::true_type {};
+
 // two 16-bit values in one register (v_pk_* integer operations)
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
@@ -703,6 +713,7 @@ __device__ __forceinline__ int decode(uint16_t d, int cap)
 }
 __device__ __forceinline__ int decode(int32_t v, int cap) { return min(v, cap); }
 __device__ __forceinline__ int decode(Sq16 v, int cap) { return min((int)v.v, cap); }
+__device__ __forceinline__ int decode(Sq8 v, int cap) { return min((int)v.v, cap); }
 
 // input at row r of the line starting at `line` (number line_id), under the
 // boundary rule; kHull: wrapped rows only where the line's mask keeps them
@@ -723,6 +734,7 @@ template <class E> struct Quad;
 template <> struct Quad<uint16_t> { using type = uint2; };
 template <> struct Quad<int32_t> { using type = int4; };
 template <> struct Quad<Sq16> { using type = uint2; };
+template <> struct Quad<Sq8> { using type = uint32_t; };
 
 template <int N, class E, bool kHull = false>
 __device__ __forceinline__ void issue_rows(const E *__restrict__ base, const WinGeom &g, int first, int row0,
@@ -746,6 +758,11 @@ template <> __device__ __forceinline__ int4 unpack<Sq16>(uint2 r, int cap)
 {
     return make_int4(min((int)(r.x & 0xffffu), cap), min((int)(r.x >> 16), cap), min((int)(r.y & 0xffffu), cap),
                      min((int)(r.y >> 16), cap));
+}
+template <> __device__ __forceinline__ int4 unpack<Sq8>(uint32_t r, int cap)
+{
+    return make_int4(min((int)(r & 0xffu), cap), min((int)((r >> 8) & 0xffu), cap), min((int)((r >> 16) & 0xffu), cap),
+                     min((int)(r >> 24), cap));
 }
 template <> __device__ __forceinline__ int4 unpack<int32_t>(int4 r, int cap)
 {
@@ -851,7 +868,7 @@ __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in,
     constexpr int kWaves = kWinThreads / kWave;
     static_assert(kWaves == 4 && kWinThreads == 256, "four waves, 16 threads per row of four columns");
     constexpr int kHalo = 2 * W / 16;  // rows of four columns per thread in the halo move
-    constexpr bool kNarrow = kY ? std::is_same<O, Sq16>::value : std::is_same<E, Sq16>::value;
+    constexpr bool kNarrow = kY ? IsSq<O>::value : IsSq<E>::value;
     using L = typename std::conditional<kNarrow, uint16_t, int>::type;
     __shared__ __align__(16) L tile[kRows * kTileZ];
     __shared__ __align__(16) int part[kWaves][kTileZ];
@@ -956,6 +973,8 @@ __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in,
                         ovf_count[3] = 0u;
                     }
                     out[(int64_t)outer * g.out_os + (int64_t)y * g.out_rs + z] = Sq16{(uint16_t)min(v, 65535)};
+                } else if constexpr (std::is_same<O, Sq8>::value) {
+                    out[(int64_t)outer * g.out_os + (int64_t)y * g.out_rs + z] = Sq8{(uint8_t)min(v, 255)};
                 } else {
                     out[(int64_t)outer * g.out_os + (int64_t)y * g.out_rs + z] = v;
                 }
@@ -999,7 +1018,12 @@ __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in,
                         ovf_list[atomicAdd(ovf_count, 1u)] = (uint32_t)ln_id;
                 }
             }
-            if constexpr (kY) {
+            if constexpr (kY && std::is_same<O, Sq8>::value) {
+                // 8-bit output of both columns (values are at most the cap, below 256)
+                if (act && zp < g.pitch)
+                    *reinterpret_cast<uint16_t *>(out + (int64_t)outer * g.out_os + (int64_t)y * g.out_rs + zp) =
+                        (uint16_t)(min(bv0, 255) | (min(bv1, 255) << 8));
+            } else if constexpr (kY) {
                 // 16-bit output of both columns in one store (a deferred column's value is
                 // rewritten by the fallback; columns past nz land in the row padding); a
                 // settled value of 65535 sends the EDT to its int32 passes
@@ -1166,6 +1190,8 @@ __device__ __forceinline__ void fallback_y_body(int64_t blk, int64_t nblk, const
                         ovf_count[3] = 0u;
                     }
                     dst[(int64_t)j * out_sy] = Sq16{(uint16_t)min(f, 65535)};
+                } else if constexpr (std::is_same<O, Sq8>::value) {
+                    dst[(int64_t)j * out_sy] = Sq8{(uint8_t)min(f, 255)};
                 } else {
                     dst[(int64_t)j * out_sy] = f;
                 }

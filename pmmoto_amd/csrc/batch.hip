@@ -451,9 +451,10 @@ int run_fused(const pmx_psd_batch *b, hipStream_t s)
             hipLaunchKernelGGL(k_clear_words_batch, grid(g.total), dim3(256), 0, s,
                                static_cast<const ClearArgs *>(g.args), g.first, g.n);
         });
-#define PMX_ZB(INV, MT, L, G, MATCH, GATE)                                                                    \
+#define PMX_ZB(INV, MT, L, G, MATCH, GATE) PMX_ZBD(INV, MT, L, uint16_t, G, MATCH, GATE)
+#define PMX_ZBD(INV, MT, L, D, G, MATCH, GATE)                                                                \
     launch_group(G, s, [&](const BatchGroup &g) {                                                             \
-        hipLaunchKernelGGL((k_pass_z_batch<INV, MT, L>), grid(g.total), dim3(256), 0, s,                      \
+        hipLaunchKernelGGL((k_pass_z_batch<INV, MT, L, D>), grid(g.total), dim3(256), 0, s,                   \
                            static_cast<const PassZArgs *>(g.args), g.first, g.n, (uint32_t)(MATCH), GATE);   \
     })
         PMX_ZB(false, false, 16, b->z_full[0], 0, -1);
@@ -511,9 +512,16 @@ int run_fused(const pmx_psd_batch *b, hipStream_t s)
         const int32_t thr = (int32_t)std::min<int64_t>(std::max<int64_t>(b->t_dilate[lv], -1), INT32_MAX - 1);
         const int cap = thr + 1;
         const bool wide = cap > 17 * 17;
-        PMX_ZB(true, true, 16, b->z_dil[0], lv, lv);
-        PMX_ZB(true, true, 32, b->z_dil[1], lv, lv);
-        PMX_ZB(true, true, 64, b->z_dil[2], lv, lv);
+        if (cap <= 255) {
+            // 8-bit distances from pass Z
+            PMX_ZBD(true, true, 16, uint8_t, b->z_dil[0], lv, lv);
+            PMX_ZBD(true, true, 32, uint8_t, b->z_dil[1], lv, lv);
+            PMX_ZBD(true, true, 64, uint8_t, b->z_dil[2], lv, lv);
+        } else {
+            PMX_ZB(true, true, 16, b->z_dil[0], lv, lv);
+            PMX_ZB(true, true, 32, b->z_dil[1], lv, lv);
+            PMX_ZB(true, true, 64, b->z_dil[2], lv, lv);
+        }
         if (wide) {
             PMX_WB(32, 32, uint16_t, Sq16, false, b->ydil[0], lv, cap, 0, -1);
             PMX_WB(64, 32, uint16_t, Sq16, false, b->ydil[1], lv, cap, 0, -1);
@@ -521,8 +529,8 @@ int run_fused(const pmx_psd_batch *b, hipStream_t s)
             PMX_WBP(64, 32, Sq16, b->xdil[1], lv, cap, thr, lv + 1);
         } else if (cap <= 255) {
             // 8-bit squared distances between passes Y and X
-            PMX_WB(32, 16, uint16_t, Sq8, false, b->ydil[0], lv, cap, 0, -1);
-            PMX_WB(64, 16, uint16_t, Sq8, false, b->ydil[1], lv, cap, 0, -1);
+            PMX_WB(32, 16, Dz8, Sq8, false, b->ydil[0], lv, cap, 0, -1);
+            PMX_WB(64, 16, Dz8, Sq8, false, b->ydil[1], lv, cap, 0, -1);
             PMX_WBP(32, 16, Sq8, b->xdil[0], lv, cap, thr, lv + 1);
             PMX_WBP(64, 16, Sq8, b->xdil[1], lv, cap, thr, lv + 1);
         } else {
@@ -534,6 +542,7 @@ int run_fused(const pmx_psd_batch *b, hipStream_t s)
         if (int e = check_launch("batched dilation")) return e;
     }
 #undef PMX_ZB
+#undef PMX_ZBD
 #undef PMX_WB
 #undef PMX_WBP
     return PMX_OK;

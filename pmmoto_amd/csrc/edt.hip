@@ -48,10 +48,9 @@ void launch_window(const E *in, O *out, const PassXOut &o, const WinGeom &wg0, i
     WinGeom wg = wg0;
     wg.tpb = tiles_per_block(wg, T);
     const int64_t blocks = window_blocks(wg, T, wg.tpb);
-    const bool d2only = !std::is_same<E, uint16_t>::value && o.d2 && !o.f32sq && !o.f32 && !o.f64 && !o.u8 &&
-                        !o.psd_map;
-    const bool psd_only = !std::is_same<E, uint16_t>::value && o.psd_map && !o.d2 && !o.f32sq && !o.f32 && !o.f64 &&
-                          !o.u8;
+    constexpr bool kX = !std::is_same<E, uint16_t>::value && !std::is_same<E, Dz8>::value;  // pass X
+    const bool d2only = kX && o.d2 && !o.f32sq && !o.f32 && !o.f64 && !o.u8 && !o.psd_map;
+    const bool psd_only = kX && o.psd_map && !o.d2 && !o.f32sq && !o.f32 && !o.f64 && !o.u8;
     if (psd_only)
         hipLaunchKernelGGL((k_window<T, W, E, O, false, false, true>), dim3((unsigned)blocks), dim3(kWinThreads), 0, s,
                            in, out, o, wg, nx, ny, bits, count, list, proceed);
@@ -129,11 +128,13 @@ struct YX {
     int64_t bsx() const { return p.pitch; }
     int64_t bsy() const { return (int64_t)nx() * p.pitch; }
 
-    template <class B> int pass_y(B *b, const unsigned int *gate, const char *name) const
+    // pass Z's array: uint16 distances, or uint8 (Dz8) under a cap below 256
+    template <class B, class Z = uint16_t> int pass_y(B *b, const unsigned int *gate, const char *name) const
     {
         ProfScope ps(name, s);
         const WinGeom wg{ny(), p.pitch, ny() * p.pitch, nx(), nz(), g->bc[2], g->bc[3], ccap, p.pitch, bsx(), bsy()};
-        dispatch_window(tr, wide, p.dz, b, o, wg, nx(), ny(), p.bits_y, p.counters + 0, p.ovf_list, gate, s);
+        dispatch_window(tr, wide, reinterpret_cast<const Z *>(p.dz), b, o, wg, nx(), ny(), p.bits_y, p.counters + 0,
+                        p.ovf_list, gate, s);
         return check_launch("pass y");
     }
     template <class B> int fallback_y(B *b, const unsigned int *gate, const char *name) const
@@ -167,11 +168,11 @@ struct YX {
 // find empty lists, are not launched.
 inline bool may_defer(int ccap) { return ccap >= (kReach + 1) * (kReach + 1); }
 
-template <class B>
+template <class B, class Z = uint16_t>
 int run_yx(const YX &yx, B *b, const unsigned int *proceed)
 {
     const bool fb = may_defer(yx.ccap);
-    if (int e = yx.pass_y(b, proceed, "edt_pass_y")) return e;
+    if (int e = yx.pass_y<B, Z>(b, proceed, "edt_pass_y")) return e;
     if (fb)
         if (int e = yx.fallback_y(b, proceed, "edt_fallback_y")) return e;
     if (int e = yx.pass_x((const B *)b, proceed, "edt_pass_x")) return e;
@@ -220,6 +221,7 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
         return fail(PMX_ERR_WORKSPACE, "workspace %zu bytes < required %zu", ws_bytes, need);
     plan_edt(g, ws, &p, max_lanes);
     const int ccap = (int)std::min<int64_t>(std::max<int64_t>(cap, 0), kBig);
+    const bool dz8 = ccap <= 255;  // 8-bit distances between passes Z and Y, and Y and X
     const int nx = (int)g->n[0], ny = (int)g->n[1], nz = (int)g->n[2];
     const int64_t rows = (int64_t)nx * ny;
     const int64_t pitch = p.pitch;
@@ -237,25 +239,33 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     const dim3 gs((unsigned)((rows + 4 * (kWave / lpr) - 1) / (4 * (kWave / lpr))));
     const uint32_t mv = match < 0 ? 0u : (uint32_t)match;
     const bool mt = match >= 0;
-#define PMX_PASS_Z_SHORT(INV, MT, L)                                                                            \
-    hipLaunchKernelGGL((k_pass_z_short<INV, MT, L>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,  \
-                       g->bc[4], g->bc[5], vec_in, mv, pitch, p.dz, p.counters, proceed, p.bits_y, (int64_t)p.n_bits)
-#define PMX_PASS_Z_LONG(INV, MT)                                                                                \
-    hipLaunchKernelGGL((k_pass_z<INV, MT>), gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows, g->bc[4], \
-                       g->bc[5], vec_in, mv, pitch, p.dz, p.counters, proceed)
-    if (lpr == 16) {
-        if (mt) PMX_PASS_Z_SHORT(true, true, 16);
-        else if (invert) PMX_PASS_Z_SHORT(true, false, 16);
-        else PMX_PASS_Z_SHORT(false, false, 16);
-    } else if (lpr == 32) {
-        if (mt) PMX_PASS_Z_SHORT(true, true, 32);
-        else if (invert) PMX_PASS_Z_SHORT(true, false, 32);
-        else PMX_PASS_Z_SHORT(false, false, 32);
-    } else {
-        if (mt) PMX_PASS_Z_LONG(true, true);
-        else if (invert) PMX_PASS_Z_LONG(true, false);
-        else PMX_PASS_Z_LONG(false, false);
+#define PMX_PASS_Z_SHORT(INV, MT, L, D)                                                                         \
+    hipLaunchKernelGGL((k_pass_z_short<INV, MT, L, D>), gs, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz,     \
+                       rows, g->bc[4], g->bc[5], vec_in, mv, pitch, reinterpret_cast<D *>(p.dz), p.counters,   \
+                       proceed, p.bits_y, (int64_t)p.n_bits)
+#define PMX_PASS_Z_LONG(INV, MT, D)                                                                             \
+    hipLaunchKernelGGL((k_pass_z<INV, MT, D>), gz, bz, 0, s, img, g->in_s[0], g->in_s[1], ny, nz, rows,        \
+                       g->bc[4], g->bc[5], vec_in, mv, pitch, reinterpret_cast<D *>(p.dz), p.counters, proceed)
+#define PMX_PASS_Z_ALL(D)                                                                                       \
+    if (lpr == 16) {                                                                                            \
+        if (mt) PMX_PASS_Z_SHORT(true, true, 16, D);                                                            \
+        else if (invert) PMX_PASS_Z_SHORT(true, false, 16, D);                                                  \
+        else PMX_PASS_Z_SHORT(false, false, 16, D);                                                             \
+    } else if (lpr == 32) {                                                                                     \
+        if (mt) PMX_PASS_Z_SHORT(true, true, 32, D);                                                            \
+        else if (invert) PMX_PASS_Z_SHORT(true, false, 32, D);                                                  \
+        else PMX_PASS_Z_SHORT(false, false, 32, D);                                                             \
+    } else {                                                                                                    \
+        if (mt) PMX_PASS_Z_LONG(true, true, D);                                                                 \
+        else if (invert) PMX_PASS_Z_LONG(true, false, D);                                                       \
+        else PMX_PASS_Z_LONG(false, false, D);                                                                  \
     }
+    if (dz8) {
+        PMX_PASS_Z_ALL(uint8_t)
+    } else {
+        PMX_PASS_Z_ALL(uint16_t)
+    }
+#undef PMX_PASS_Z_ALL
 #undef PMX_PASS_Z_SHORT
 #undef PMX_PASS_Z_LONG
     }
@@ -265,8 +275,9 @@ int run_edt(const uint8_t *img, const pmx_geom *g, bool invert, const PassXOut &
     // EDT whose threshold reaches past 17^2 scans up to 32 rows in LDS
     const bool wide = ccap < kBig && ccap > 17 * 17;
     const YX yx{p, g, o, ccap, tr, wide, s};
-    // a cap that fits in 8 (16) bits: pass Y hands pass X 8-bit (16-bit) squared distances
-    if (ccap <= 255) return run_yx(yx, reinterpret_cast<Sq8 *>(p.b), proceed);
+    // a cap that fits in 8 (16) bits: pass Y hands pass X 8-bit (16-bit) squared
+    // distances; under 8 bits pass Z hands pass Y 8-bit distances too
+    if (dz8) return run_yx<Sq8, Dz8>(yx, reinterpret_cast<Sq8 *>(p.b), proceed);
     if (ccap <= 65535) return run_yx(yx, reinterpret_cast<Sq16 *>(p.b), proceed);
     // otherwise 16 bits first, int32 only where a value saturated (decided on the device)
     if (!proceed) return run_yx_narrow_first(yx);

@@ -134,10 +134,48 @@ __device__ __forceinline__ void clear_bits(uint32_t *bits, int64_t n, int64_t bl
     for (int64_t i = base + threadIdx.x; i < n; i += nblk * blockDim.x) bits[i] = 0u;
 }
 
-template <bool kInvert, bool kMatch>
+// A lane's 16 distances (row positions e0 .. e0 + 15) into the row at dst:
+// uint16 (kNoSolid where no solid is reachable), or uint8 clamped at 255 for a
+// clamped EDT under a cap below 256, which reads every distance above 15 as
+// beyond the cap.  uint16 rows are 16-byte aligned, uint8 rows 8-byte.
+template <class D>
+__device__ __forceinline__ void store_dist16(D *__restrict__ dst, int e0, int nz, const uint32_t (&v)[16])
+{
+    if constexpr (std::is_same<D, uint16_t>::value) {
+        uint32_t dv[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) dv[b] = v[2 * b] | (v[2 * b + 1] << 16);
+        if (e0 + 16 <= nz) {
+            uint4 *q = reinterpret_cast<uint4 *>(dst + e0);
+            q[0] = make_uint4(dv[0], dv[1], dv[2], dv[3]);
+            q[1] = make_uint4(dv[4], dv[5], dv[6], dv[7]);
+        } else {
+#pragma unroll
+            for (int b = 0; b < 16; ++b)
+                if (e0 + b < nz) dst[e0 + b] = (uint16_t)v[b];
+        }
+    } else {
+        uint32_t dv[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            dv[b] = min(v[4 * b], 255u) | (min(v[4 * b + 1], 255u) << 8) | (min(v[4 * b + 2], 255u) << 16) |
+                    (min(v[4 * b + 3], 255u) << 24);
+        if (e0 + 16 <= nz) {
+            uint2 *q = reinterpret_cast<uint2 *>(dst + e0);
+            q[0] = make_uint2(dv[0], dv[1]);
+            q[1] = make_uint2(dv[2], dv[3]);
+        } else {
+#pragma unroll
+            for (int b = 0; b < 16; ++b)
+                if (e0 + b < nz) dst[e0 + b] = (uint8_t)min(v[b], 255u);
+        }
+    }
+}
+
+template <bool kInvert, bool kMatch, class D = uint16_t>
 __device__ __forceinline__ void pass_z_body(int64_t blk, const uint8_t *__restrict__ img, int64_t in_sx,
                                             int64_t in_sy, int ny, int nz, int64_t rows, int bc_lo, int bc_hi,
-                                            int vec_in, uint32_t match, int64_t pitch, uint16_t *__restrict__ out,
+                                            int vec_in, uint32_t match, int64_t pitch, D *__restrict__ out,
                                             uint32_t *__restrict__ counters)
 {
     __shared__ int sc[4][2][kSup];           // per wave: each super-chunk's last / first solid
@@ -151,7 +189,7 @@ __device__ __forceinline__ void pass_z_body(int64_t blk, const uint8_t *__restri
     if (row >= rows) return;
     const int64_t x = row / ny, y = row - x * ny;
     const uint8_t *src = img + x * in_sx + y * in_sy;
-    uint16_t *dst = out + row * pitch;
+    D *dst = out + row * pitch;
     const int nsup = (nz + 1023) / 1024;
     const bool vin = vec_in != 0;
 
@@ -217,41 +255,29 @@ __device__ __forceinline__ void pass_z_body(int64_t blk, const uint8_t *__restri
             cur = ~bm[b] & (cur + 1);
             dl[b] = cur;
         }
-        uint32_t dv[8];  // two distances per word
+        uint32_t dv[16];
         int curr = nc - (e0 + 16);
 #pragma unroll
         for (int b = 15; b >= 0; --b) {
             curr = ~bm[b] & (curr + 1);
-            const uint32_t v = (uint32_t)min(min(dl[b], curr), (int)kNoSolid);
-            if (b & 1)
-                dv[b >> 1] = v << 16;
-            else
-                dv[b >> 1] |= v;
+            dv[b] = (uint32_t)min(min(dl[b], curr), (int)kNoSolid);
         }
-        if (e0 + 16 <= nz) {
-            uint4 *q = reinterpret_cast<uint4 *>(dst + e0);  // rows are 16-byte aligned
-            q[0] = make_uint4(dv[0], dv[1], dv[2], dv[3]);
-            q[1] = make_uint4(dv[4], dv[5], dv[6], dv[7]);
-        } else {
-#pragma unroll
-            for (int b = 0; b < 16; ++b)
-                if (e0 + b < nz) dst[e0 + b] = (uint16_t)(dv[b >> 1] >> (16 * (b & 1)));
-        }
+        store_dist16(dst, e0, nz, dv);
         pcar = max(pcar, sc[w][0][s]);
     }
 }
 
-template <bool kInvert, bool kMatch>
+template <bool kInvert, bool kMatch, class D = uint16_t>
 __global__ __launch_bounds__(256) void k_pass_z(const uint8_t *__restrict__ img, int64_t in_sx,
                                                 int64_t in_sy, int ny, int nz, int64_t rows,
                                                 int bc_lo, int bc_hi, int vec_in, uint32_t match, int64_t pitch,
-                                                uint16_t *__restrict__ out,
+                                                D *__restrict__ out,
                                                 uint32_t *__restrict__ counters,
                                                 const unsigned int *__restrict__ proceed)
 {
     if (proceed && *proceed == 0u) return;
-    pass_z_body<kInvert, kMatch>(blockIdx.x, img, in_sx, in_sy, ny, nz, rows, bc_lo, bc_hi, vec_in, match, pitch,
-                                 out, counters);
+    pass_z_body<kInvert, kMatch, D>(blockIdx.x, img, in_sx, in_sy, ny, nz, rows, bc_lo, bc_hi, vec_in, match, pitch,
+                                    out, counters);
 }
 
 // Pass Z for short rows (nz <= 16 L): L lanes per row, 64 / L rows per
@@ -302,11 +328,11 @@ __device__ __forceinline__ int seg_lane(int v, int seg, int pos)
     }
 }
 
-template <bool kInvert, bool kMatch, int L>
+template <bool kInvert, bool kMatch, int L, class D = uint16_t>
 __device__ __forceinline__ void pass_z_short_body(int64_t blk, int64_t nblk, const uint8_t *__restrict__ img,
                                                   int64_t in_sx, int64_t in_sy, int ny, int nz, int64_t rows,
                                                   int bc_lo, int bc_hi, int vec_in, uint32_t match, int64_t pitch,
-                                                  uint16_t *__restrict__ out, uint32_t *__restrict__ counters,
+                                                  D *__restrict__ out, uint32_t *__restrict__ counters,
                                                   uint32_t *__restrict__ bits, int64_t n_bits)
 {
     clear_bits(bits, n_bits, blk, nblk);
@@ -354,41 +380,28 @@ __device__ __forceinline__ void pass_z_short_body(int64_t blk, int64_t nblk, con
         cur = ~bm[b] & (cur + 1);
         dl[b] = cur;
     }
-    uint32_t dv[8];
+    uint32_t dv[16];
     int curr = nc - (e0 + 16);
 #pragma unroll
     for (int b = 15; b >= 0; --b) {
         curr = ~bm[b] & (curr + 1);
-        const uint32_t v = (uint32_t)min(min(dl[b], curr), (int)kNoSolid);
-        if (b & 1)
-            dv[b >> 1] = v << 16;
-        else
-            dv[b >> 1] |= v;
+        dv[b] = (uint32_t)min(min(dl[b], curr), (int)kNoSolid);
     }
     if (!valid || e0 >= nz) return;
-    uint16_t *dst = out + row * pitch;
-    if (e0 + 16 <= nz) {
-        uint4 *q = reinterpret_cast<uint4 *>(dst + e0);  // rows are 16-byte aligned
-        q[0] = make_uint4(dv[0], dv[1], dv[2], dv[3]);
-        q[1] = make_uint4(dv[4], dv[5], dv[6], dv[7]);
-    } else {
-#pragma unroll
-        for (int b = 0; b < 16; ++b)
-            if (e0 + b < nz) dst[e0 + b] = (uint16_t)(dv[b >> 1] >> (16 * (b & 1)));
-    }
+    store_dist16(out + row * pitch, e0, nz, dv);
 }
 
-template <bool kInvert, bool kMatch, int L>
+template <bool kInvert, bool kMatch, int L, class D = uint16_t>
 __global__ __launch_bounds__(256) void k_pass_z_short(const uint8_t *__restrict__ img, int64_t in_sx,
                                                       int64_t in_sy, int ny, int nz, int64_t rows, int bc_lo,
                                                       int bc_hi, int vec_in, uint32_t match, int64_t pitch,
-                                                      uint16_t *__restrict__ out, uint32_t *__restrict__ counters,
+                                                      D *__restrict__ out, uint32_t *__restrict__ counters,
                                                       const unsigned int *__restrict__ proceed,
                                                       uint32_t *__restrict__ bits, int64_t n_bits)
 {
     if (proceed && *proceed == 0u) return;
-    pass_z_short_body<kInvert, kMatch, L>(blockIdx.x, gridDim.x, img, in_sx, in_sy, ny, nz, rows, bc_lo, bc_hi,
-                                          vec_in, match, pitch, out, counters, bits, n_bits);
+    pass_z_short_body<kInvert, kMatch, L, D>(blockIdx.x, gridDim.x, img, in_sx, in_sy, ny, nz, rows, bc_lo, bc_hi,
+                                             vec_in, match, pitch, out, counters, bits, n_bits);
 }
 
 // ---------------------------------------------------------------------------
@@ -695,6 +708,11 @@ struct Sq16 {
 struct Sq8 {
     uint8_t v;
 };
+// Pass Z's distance in 8 bits (clamped at 255) for a clamped EDT under a cap
+// below 256: any distance above 15 squares past the cap.
+struct Dz8 {
+    uint8_t v;
+};
 template <class T> struct IsSq : std::false_type {};
 template <> struct IsSq<Sq16> : std::true_type {};
 template <> struct IsSq<Sq8> : std::true_type {};
@@ -714,6 +732,7 @@ __device__ __forceinline__ int decode(uint16_t d, int cap)
 __device__ __forceinline__ int decode(int32_t v, int cap) { return min(v, cap); }
 __device__ __forceinline__ int decode(Sq16 v, int cap) { return min((int)v.v, cap); }
 __device__ __forceinline__ int decode(Sq8 v, int cap) { return min((int)v.v, cap); }
+__device__ __forceinline__ int decode(Dz8 d, int cap) { return min((int)d.v * (int)d.v, cap); }
 
 // input at row r of the line starting at `line` (number line_id), under the
 // boundary rule; kHull: wrapped rows only where the line's mask keeps them
@@ -735,6 +754,7 @@ template <> struct Quad<uint16_t> { using type = uint2; };
 template <> struct Quad<int32_t> { using type = int4; };
 template <> struct Quad<Sq16> { using type = uint2; };
 template <> struct Quad<Sq8> { using type = uint32_t; };
+template <> struct Quad<Dz8> { using type = uint32_t; };
 
 template <int N, class E, bool kHull = false>
 __device__ __forceinline__ void issue_rows(const E *__restrict__ base, const WinGeom &g, int first, int row0,
@@ -758,6 +778,11 @@ template <> __device__ __forceinline__ int4 unpack<Sq16>(uint2 r, int cap)
 {
     return make_int4(min((int)(r.x & 0xffffu), cap), min((int)(r.x >> 16), cap), min((int)(r.y & 0xffffu), cap),
                      min((int)(r.y >> 16), cap));
+}
+template <> __device__ __forceinline__ int4 unpack<Dz8>(uint32_t r, int cap)
+{
+    return make_int4(decode(Dz8{(uint8_t)r}, cap), decode(Dz8{(uint8_t)(r >> 8)}, cap),
+                     decode(Dz8{(uint8_t)(r >> 16)}, cap), decode(Dz8{(uint8_t)(r >> 24)}, cap));
 }
 template <> __device__ __forceinline__ int4 unpack<Sq8>(uint32_t r, int cap)
 {
@@ -846,7 +871,7 @@ __device__ __forceinline__ void commit_rows(L *tile, const WinGeom &g, int dst, 
     }
 }
 
-// E = uint16_t: pass Y (input: pass Z's distances, output: the array `out`
+// E = uint16_t or Dz8: pass Y (input: pass Z's distances, output: the array `out`
 // of O = int32_t, or Sq16 under a cap <= 65535); E = int32_t or Sq16: pass X
 // (input: that array, output: PassXOut).
 //
@@ -861,7 +886,7 @@ __device__ __forceinline__ void window_body(int64_t t, const E *__restrict__ in,
                                             uint32_t *__restrict__ line_bits, uint32_t *__restrict__ ovf_count,
                                             uint32_t *__restrict__ ovf_list)
 {
-    constexpr bool kY = std::is_same<E, uint16_t>::value;
+    constexpr bool kY = std::is_same<E, uint16_t>::value || std::is_same<E, Dz8>::value;
     using V = typename Quad<E>::type;
     static_assert(W % 8 == 0 && T % 16 == 0, "tile layout");
     constexpr int kRows = T + 2 * W;
@@ -1304,7 +1329,7 @@ struct PassZArgs {
 };
 
 // L = 64: one row per wave (rows longer than 512 voxels), else L lanes per row
-template <bool kInvert, bool kMatch, int L>
+template <bool kInvert, bool kMatch, int L, class D = uint16_t>
 __global__ __launch_bounds__(256) void k_pass_z_batch(const PassZArgs *__restrict__ args,
                                                       const int64_t *__restrict__ first, int n, uint32_t match,
                                                       int gate)
@@ -1314,13 +1339,14 @@ __global__ __launch_bounds__(256) void k_pass_z_batch(const PassZArgs *__restric
     const PassZArgs a = args[k];
     if (!batch_gate(a.proceed, gate)) return;
     const int64_t blk = gb - first[k], nblk = first[k + 1] - first[k];
+    D *out = reinterpret_cast<D *>(a.out);  // uint8 distances share the uint16 array's memory
     if constexpr (L == 64)
-        pass_z_body<kInvert, kMatch>(blk, a.img, a.in_sx, a.in_sy, a.ny, a.nz, a.rows, a.bc_lo, a.bc_hi, a.vec_in,
-                                     match, a.pitch, a.out, a.counters);
+        pass_z_body<kInvert, kMatch, D>(blk, a.img, a.in_sx, a.in_sy, a.ny, a.nz, a.rows, a.bc_lo, a.bc_hi,
+                                        a.vec_in, match, a.pitch, out, a.counters);
     else
-        pass_z_short_body<kInvert, kMatch, L>(blk, nblk, a.img, a.in_sx, a.in_sy, a.ny, a.nz, a.rows, a.bc_lo,
-                                              a.bc_hi, a.vec_in, match, a.pitch, a.out, a.counters, a.bits,
-                                              a.n_bits);
+        pass_z_short_body<kInvert, kMatch, L, D>(blk, nblk, a.img, a.in_sx, a.in_sy, a.ny, a.nz, a.rows, a.bc_lo,
+                                                 a.bc_hi, a.vec_in, match, a.pitch, out, a.counters, a.bits,
+                                                 a.n_bits);
 }
 
 struct WinArgs {

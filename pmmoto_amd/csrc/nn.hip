@@ -348,6 +348,8 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ w, const float
 constexpr uint32_t kOob = 0x80000000u;  // >= every buffer's size
 constexpr int kBufFlags = 0x00020000;   // raw buffer, 32-bit data format (CDNA)
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer(const void *p, uint32_t bytes)
 {
@@ -644,14 +646,14 @@ __global__ __launch_bounds__(256) void k_conv_first_lds(const uint8_t *__restric
 // Implicit-GEMM convolution for c % 8 == 0 on the matrix cores, computed as
 // out^T (channels x positions) = W (channels x K) . P^T (K x positions) with
 // k = tap * c + ci.  Wave = 16 P output positions x 16 CB output channels;
-// no LDS, no barrier: in each 16-deep step lane l gathers the four channels
-// k = 16 j + 4 (l / 16) .. + 3 of its position (one 8-byte load: a run never
-// crosses a tap) and the same k of its weight row, and after the loop holds
-// four consecutive channels of one position, stored as one 8-byte write.
-// The tap part of each step's offset is wave-uniform (with PAD a scalar
-// offset); the lanes whose four channels start in the next tap (c % 16 != 0)
-// add the difference.  PAD: the input is padded, every tap in range, no
-// masks.
+// no LDS, no barrier: in each 32-deep step lane l gathers the eight channels
+// k = 32 j + 8 (l / 16) .. + 7 of its position (one 16-byte load: with
+// c % 8 == 0 a run never crosses a tap) and the same k of its weight row,
+// for one 16x16x32 product per (position block, channel block); after the
+// loop it holds four consecutive channels of one position, stored as one
+// 8-byte write.  Each lane follows the (tap, channel) of its own run: the
+// four lane groups of a step can sit in up to four taps (c = 8).  PAD: the
+// input is padded, every tap in range, no masks.
 template <int P, int CB, bool PAD>
 __global__ __launch_bounds__(256) void k_conv_igemm(const uint16_t *__restrict__ x, uint32_t x_bytes, ConvDims cd,
                                                     int co, const uint16_t *__restrict__ w,
@@ -668,8 +670,8 @@ __global__ __launch_bounds__(256) void k_conv_igemm(const uint16_t *__restrict__
     const __amdgpu_buffer_rsrc_t wb = buffer(w, (uint32_t)co * (uint32_t)K * 2u);
     const int dp = cd.d + (PAD ? 2 : 0), hp = cd.h + (PAD ? 2 : 0), wp = cd.w + (PAD ? 2 : 0);
     const int sz = hp * wp * cd.c, sy = wp * cd.c;  // element strides of z and y
-    // per position block: the byte offset of tap (0,0,0) + this lane's 4g channels,
-    // and (unpadded input) the in-range taps per axis
+    // per position block: the byte offset of tap (0,0,0), channel 0, and
+    // (unpadded input) the in-range taps per axis
     uint32_t base[P];
     uint32_t msk[P];  // bits 0-2: z taps, 3-5: y taps, 6-8: x taps
 #pragma unroll
@@ -691,24 +693,25 @@ __global__ __launch_bounds__(256) void k_conv_igemm(const uint16_t *__restrict__
         msk[pb] = PAD ? 0x1ffu : mk;
         const int o = PAD ? (((b * dp + iz + 1) * hp + iy + 1) * wp + ix + 1) * cd.c
                           : (((b * cd.d + iz) * cd.h + iy) * cd.w + ix) * cd.c;
-        base[pb] = m < M ? (uint32_t)(2 * (o + 4 * g)) : kOob;
+        base[pb] = m < M ? (uint32_t)(2 * o) : kOob;
     }
     uint32_t wbase[CB];
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) {
         const int n = n0 + 16 * cb + (lane & 15);
-        wbase[cb] = n < co ? (uint32_t)(2 * (n * K + 4 * g)) : kOob;
+        wbase[cb] = n < co ? (uint32_t)(2 * (n * K + 8 * g)) : kOob;
     }
     floatx4 acc[P][CB];
 #pragma unroll
     for (int pb = 0; pb < P; ++pb)
 #pragma unroll
         for (int cb = 0; cb < CB; ++cb) acc[pb][cb] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const int steps = (K + 15) / 16;
-    // software pipeline: the raw operands of steps j + 2, j + 3 are loaded before the
-    // products of steps j, j + 1
-    constexpr int S = 2;
-    u32x2 rp[S][P], rw[S][CB];
+    const int steps = (K + 31) / 32;
+    // software pipeline over S operand slots: slot q holds step j + q; its
+    // products are issued, then the slot is reloaded with step j + S + q, so
+    // the loads of the other S - 1 slots stay in flight under the products
+    constexpr int S = 3;
+    u32x4 rp[S][P], rw[S][CB];
     auto tap_off = [&](int tap) {  // element offset of tap (dz, dy, dx); tap < 27
         const int dz = (tap * 29) >> 8, rem = tap - 9 * dz;
         const int dy = (rem * 11) >> 5, dx = rem - 3 * dy;
@@ -719,62 +722,49 @@ __global__ __launch_bounds__(256) void k_conv_igemm(const uint16_t *__restrict__
         const int dy = (rem * 11) >> 5, dx = rem - 3 * dy;
         return (1u << dz) | (8u << dy) | (64u << dx);
     };
-    // (tap, first channel) of the next step to issue, advanced by 16 channels per step
-    // (wave-uniform, no division)
-    int t_nx = 0, ci_nx = 0;
-    auto issue = [&](int j0) {
+    // (tap, first channel) of this lane's run in the next step to issue
+    int t_l = 0, ci_l = 8 * g;
+    while (ci_l >= cd.c) {
+        ci_l -= cd.c;
+        ++t_l;
+    }
+    auto issue = [&](int q, int j) {  // step j into slot q
+        const int ku = 32 * j;          // wave-uniform
+        const bool kin = t_l < 27;      // k = ku + 8 g < K
+        const int tap = kin ? t_l : 0;
+        const uint32_t off = (uint32_t)(2 * (tap_off(tap) + ci_l)), need = tap_bits(tap);
 #pragma unroll
-        for (int q = 0; q < S; ++q) {
-            const int ku = 16 * (j0 + q);  // wave-uniform
-            const int tap = t_nx, ci0 = ci_nx;
-            ci_nx += 16;
-            while (ci_nx >= cd.c) {
-                ci_nx -= cd.c;
-                ++t_nx;
-            }
-            const bool kin = ku + 4 * g < K;
-            const bool cross = ci0 + 4 * g >= cd.c;  // this lane's channels start in tap + 1
-            const int t_lo = tap < 27 ? tap : 0, t_hi = tap + 1 < 27 ? tap + 1 : 0;
-            const int off_lo = tap_off(t_lo) + ci0, off_hi = tap_off(t_hi) + ci0 - cd.c;
-            const uint32_t need = cross ? tap_bits(t_hi) : tap_bits(t_lo);
-            const uint32_t dlane = cross ? (uint32_t)(2 * (off_hi - off_lo)) : 0u;
+        for (int pb = 0; pb < P; ++pb) {
+            // without a border the tap (0,0,0) offset of an edge position is negative:
+            // only the sum of base and off is a byte offset
+            const bool ok = kin && (PAD || (msk[pb] & need) == need);
+            rp[q][pb] = __builtin_bit_cast(
+                u32x4, __builtin_amdgcn_raw_buffer_load_b128(xb, ok ? base[pb] + off : kOob, 0, 0));
+        }
 #pragma unroll
-            for (int pb = 0; pb < P; ++pb) {
-                // the range check covers the vector offset only: without a border the tap
-                // (0,0,0) offset of an edge position is negative, so the whole sum goes there
-                const bool ok = kin && (PAD || (msk[pb] & need) == need);
-                if (PAD)
-                    rp[q][pb] = __builtin_amdgcn_raw_buffer_load_b64(xb, ok ? base[pb] + dlane : kOob, 2 * off_lo, 0);
-                else
-                    rp[q][pb] = __builtin_amdgcn_raw_buffer_load_b64(
-                        xb, ok ? base[pb] + dlane + (uint32_t)(2 * off_lo) : kOob, 0, 0);
-            }
-#pragma unroll
-            for (int cb = 0; cb < CB; ++cb)
-                rw[q][cb] = __builtin_amdgcn_raw_buffer_load_b64(wb, kin ? wbase[cb] : kOob, 2 * ku, 0);
+        for (int cb = 0; cb < CB; ++cb)
+            rw[q][cb] = __builtin_bit_cast(
+                u32x4, __builtin_amdgcn_raw_buffer_load_b128(wb, kin ? wbase[cb] : kOob, 2 * ku, 0));
+        ci_l += 32;
+        while (ci_l >= cd.c) {
+            ci_l -= cd.c;
+            ++t_l;
         }
     };
-    auto frag = [](u32x2 v) {
-        return bf16x4{(short)(v.x & 0xffffu), (short)(v.x >> 16), (short)(v.y & 0xffffu), (short)(v.y >> 16)};
-    };
-    issue(0);
+#pragma unroll
+    for (int q = 0; q < S; ++q) issue(q, q);
     for (int j = 0; j < steps; j += S) {
-        bf16x4 pf[S][P], wf[S][CB];
 #pragma unroll
         for (int q = 0; q < S; ++q) {
-#pragma unroll
-            for (int pb = 0; pb < P; ++pb) pf[q][pb] = frag(rp[q][pb]);
-#pragma unroll
-            for (int cb = 0; cb < CB; ++cb) wf[q][cb] = frag(rw[q][cb]);
-        }
-        issue(j + S);
-#pragma unroll
-        for (int q = 0; q < S; ++q)
 #pragma unroll
             for (int pb = 0; pb < P; ++pb)
 #pragma unroll
                 for (int cb = 0; cb < CB; ++cb)
-                    acc[pb][cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wf[q][cb], pf[q][pb], acc[pb][cb], 0, 0, 0);
+                    acc[pb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, rw[q][cb]),
+                                                                          __builtin_bit_cast(bf16x8, rp[q][pb]),
+                                                                          acc[pb][cb], 0, 0, 0);
+            issue(q, j + S + q);
+        }
     }
     const int odp = cd.od + 2 * out_pad, ohp = cd.oh + 2 * out_pad, owp = cd.ow + 2 * out_pad;
 #pragma unroll

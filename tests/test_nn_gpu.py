@@ -129,6 +129,49 @@ def test_implicit_gemm_conv_matches_oracle(c, co, s, shape, relu, in_pad, out_pa
     np.testing.assert_allclose(out.float().cpu().numpy(), want, rtol=2 ** -7, atol=1e-5)
 
 
+@pytest.mark.parametrize("c,co,shape", [(8, 16, (8, 64, 64, 64)), (16, 48, (64, 32, 32, 32)),
+                                         (48, 96, (128, 16, 16, 16)), (96, 160, (256, 8, 8, 8)),
+                                         (160, 256, (512, 4, 4, 4)), (24, 64, (64, 32, 32, 32))])
+@pytest.mark.parametrize("in_pad", [1, 0])
+def test_implicit_gemm_conv_at_network_sizes_matches_patch_matrix_path(c, co, shape, in_pad):
+    """The network's layer shapes at inference batches (64 or 32 positions
+    per wave, 1..4 channel blocks): the implicit-GEMM kernel against the
+    training forward's patch matrix + GEMM + bias/ReLU on the same BF16
+    input.  Both accumulate in FP32 in different orders, so an output may
+    round to the neighbouring BF16 value."""
+    import ctypes as C
+
+    import torch
+
+    from pmmoto_amd import _device, _lib
+    from pmmoto_amd.surrogate.model import gemm
+    L = _lib.lib()
+    g = torch.Generator(device="cpu").manual_seed(c + co)
+    x = torch.relu(torch.randn(shape + (c,), generator=g)).to(torch.bfloat16).cuda()
+    w = (torch.randn((co, 27 * c), generator=g) / (27 * c) ** 0.5).to(torch.bfloat16).cuda()
+    b = torch.randn(co, generator=g).cuda()
+    n, d, h, wd = shape
+    od, oh, ow = [(v - 1) // 2 + 1 for v in (d, h, wd)]
+    m, kk = n * od * oh * ow, 27 * c
+    dims = (C.c_int32 * 5)(n, d, h, wd, c)
+    print(f"[conv_igemm] network size c={c} co={co} shape={shape} in_pad={in_pad}", flush=True)
+    cols = torch.empty((m, kk), dtype=torch.bfloat16, device="cuda")
+    _lib.check(L.pmx_im2col3d(_device.ptr(x), dims, 2, kk, _device.ptr(cols), _device.stream()))
+    y = torch.empty((m, co), dtype=torch.float32, device="cuda")
+    gemm(m, co, kk, cols, kk, 0, w, kk, 1, y, co)
+    want = torch.empty((m, co), dtype=torch.bfloat16, device="cuda")
+    _lib.check(L.pmx_bias_act(_device.ptr(y), _device.ptr(b), m, co, 1, _device.ptr(want), None, _device.stream()))
+    del cols, y
+    xin = torch.nn.functional.pad(x, (0, 0, 1, 1, 1, 1, 1, 1)).contiguous() if in_pad else x
+    out = torch.empty((n, od, oh, ow, co), dtype=torch.bfloat16, device="cuda")
+    _lib.check(L.pmx_conv3d_bf16(_device.ptr(xin), dims, in_pad, 2, co, _device.ptr(w), _device.ptr(b), 1,
+                                 _device.ptr(out), 0, _device.stream()))
+    got, ref = out.reshape(m, co).float(), want.float()
+    bad = (got - ref).abs() > 2 ** -7 * ref.abs() + 1e-6
+    assert int(bad.sum()) == 0, f"{int(bad.sum())} of {m * co} outputs differ beyond one BF16 step"
+    assert float((got == ref).float().mean()) > 0.99
+
+
 def test_fused_inference_matches_training_forward():
     import torch
 

@@ -380,7 +380,13 @@ int pmx_tune_cache_load(const char *path);
  * _first reads a one-channel uint8 image (co a multiple of 4 up to 32; ReLU
  * always; out 8-byte aligned); _bf16 reads a BF16 tensor whose C is a
  * multiple of 8 (x, w, out 8-byte aligned) on the matrix cores.  One sample
- * of the input must be below 2 GiB. */
+ * of the input must be below 2 GiB.  pmx_conv3d_first_pair: _first with
+ * co = 8 followed by _bf16 with co = 16 (both stride 2, ReLU, the first
+ * unpadded in between) in one pass that keeps the 8-channel activations on
+ * chip; w0 [8][3][3][3][1], w1 [16][3][3][3][8]; the image's width must be
+ * 128, its height a multiple of 32 and its depth a multiple of 4; the
+ * output [N][D/4][H/4][32][16] (out_pad as above) equals the two calls'
+ * bit for bit. */
 int pmx_gemm_bf16(int64_t m, int n, int k, const uint16_t *a, int64_t lda, int a_trans,
                   const uint16_t *b, int64_t ldb, int b_trans, float *c, int64_t ldc,
                   int accumulate, int k_splits, hipStream_t stream);
@@ -389,6 +395,9 @@ int pmx_conv3d_first(const uint8_t *x, const int32_t *in_shape, int stride, int 
 int pmx_conv3d_bf16(const uint16_t *x, const int32_t *in_shape, int in_pad, int stride, int co,
                     const uint16_t *w_bf16, const float *bias, int relu, uint16_t *out, int out_pad,
                     hipStream_t stream);
+int pmx_conv3d_first_pair(const uint8_t *x, const int32_t *in_shape, const uint16_t *w0_bf16, const float *b0,
+                          const uint16_t *w1_bf16, const float *b1, uint16_t *out, int out_pad,
+                          hipStream_t stream);
 int pmx_im2col3d(const uint16_t *x, const int32_t *in_shape, int stride, int64_t ldk,
                  uint16_t *cols, hipStream_t stream);
 int pmx_col2im3d(const float *dcols, const int32_t *in_shape, int stride, int64_t ldk, float *dx,

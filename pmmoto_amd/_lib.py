@@ -92,6 +92,7 @@ SIGNATURES = {
     "pmx_max_u8": (C.c_int, [P, I64, P, P]),
     "pmx_gemm_bf16": (C.c_int, [I64, C.c_int, C.c_int, P, I64, C.c_int, P, I64, C.c_int, P, I64, C.c_int, C.c_int, P]),
     "pmx_conv3d_first": (C.c_int, [P, P, C.c_int, C.c_int, P, P, P, C.c_int, P]),
+    "pmx_conv3d_first_pair": (C.c_int, [P, P, P, P, P, P, P, C.c_int, P]),
     "pmx_conv3d_bf16": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, P, C.c_int, P, C.c_int, P]),
     "pmx_im2col3d": (C.c_int, [P, P, C.c_int, I64, P, P]),
     "pmx_col2im3d": (C.c_int, [P, P, C.c_int, I64, P, P]),

@@ -643,6 +643,198 @@ __global__ __launch_bounds__(256) void k_conv_first_lds(const uint8_t *__restric
     }
 }
 
+// The first two layers of the trunk in one pass: the uint8 image (width 128)
+// -> 8 channels (stride 2) -> 16 channels (stride 2), so the 8-channel
+// activations (twice the bytes of the image) never reach HBM.  A block owns 8
+// output rows x 32 output columns of the second layer and walks kPairZ of its
+// planes.  Per plane it stages 5 input planes x 35 rows of the image in LDS
+// (the next plane's rows are loaded into registers meanwhile), computes the
+// two new first-layer planes (17 rows x 64 columns, one halo row) into a ring
+// of three planes, and reads the second layer's taps from the ring.  The
+// first-layer products and rounding are those of k_conv_first_lds (bias in
+// the accumulator, ReLU, BF16), the second layer's those of k_conv_igemm at
+// c = 8 (lane group g takes tap 4 j + g of step j, bias after the sum), so the
+// output equals the two-kernel path bit for bit.
+constexpr int kPairZ = 16;                         // second-layer planes per block
+constexpr int kPairInRows = 35;                    // image rows per staged plane
+constexpr int kPairInPitch = 144;                  // 15 lead bytes, column -1 (zero), 128 columns
+constexpr int kPairL0Rows = 17, kPairL0Cols = 65;  // ring plane: column 0 is the zero column -1
+constexpr int kPairInBytes = 5 * kPairInRows * kPairInPitch;
+constexpr int kPairL0Bytes = 3 * kPairL0Rows * kPairL0Cols * 16;
+constexpr int kPairWaves = 8;  // waves per block
+
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_conv_first_pair(const uint8_t *__restrict__ x, uint32_t x_bytes, int d,
+                                                         int h, int nzc, const uint16_t *__restrict__ w0,
+                                                         const float *__restrict__ b0,
+                                                         const uint16_t *__restrict__ w1,
+                                                         const float *__restrict__ b1, uint16_t *__restrict__ out,
+                                                         int out_pad)
+{
+    // + 16: the dword pair of the last column of the last row reads 4 bytes past the image rows
+    __shared__ __align__(16) uint8_t sin[kPairInBytes + 16];
+    __shared__ __align__(16) uint8_t ring[kPairL0Bytes];
+    constexpr int kThreads = 64 * W, kLoads = (5 * kPairInRows * 8 + kThreads - 1) / kThreads;
+    constexpr int PB = 16 / W;  // second-layer blocks of 16 positions per wave
+    const int lane = threadIdx.x & 63, g = lane >> 4, l16 = lane & 15;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int od = d / 4, oh = h / 4;
+    unsigned r = (unsigned)xcd_tile(blockIdx.x, gridDim.x);
+    const int zc = (int)(r % (unsigned)nzc);
+    r /= (unsigned)nzc;
+    const int by = (int)(r % (unsigned)(oh / 8));
+    const int b = (int)(r / (unsigned)(oh / 8));
+    const int oy0 = 8 * by, oz0 = kPairZ * zc, oz1 = min(oz0 + kPairZ, od);
+    const __amdgpu_buffer_rsrc_t xb = buffer(x, x_bytes);
+    const uint32_t sample_off = (uint32_t)b * (uint32_t)d * (uint32_t)h * 128u;
+    // the zero column -1 of the image rows and of the ring planes (never overwritten)
+    for (int t = threadIdx.x; t < 5 * kPairInRows; t += kThreads) sin[t * kPairInPitch + 15] = 0;
+    for (int t = threadIdx.x; t < 3 * kPairL0Rows; t += kThreads)
+        *reinterpret_cast<uint4 *>(ring + t * kPairL0Cols * 16) = make_uint4(0u, 0u, 0u, 0u);
+    // first layer: A[ch = l % 16][slot 16 j + 4 g + kx], channels 8..15 zero
+    bf16x4 wa[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int row = 4 * j + g;
+        short e[4] = {0, 0, 0, 0};
+        if (l16 < 8 && row < 9) {
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) e[kx] = (short)w0[l16 * 27 + row * 3 + kx];
+        }
+        wa[j] = bf16x4{e[0], e[1], e[2], e[3]};
+    }
+    float bs0[4], bs1[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        bs0[v] = g < 2 ? b0[4 * g + v] : 0.f;
+        bs1[v] = b1[4 * g + v];
+    }
+    // second layer: A[ch = l % 16][k = 32 j + 8 g .. + 7] = tap 4 j + g, channels 0..7
+    u32x4 wb1[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        uint32_t e[4] = {0u, 0u, 0u, 0u};
+        if (4 * j + g < 27) {
+            const uint16_t *src = w1 + l16 * 216 + 32 * j + 8 * g;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) e[v] = (uint32_t)src[2 * v] | ((uint32_t)src[2 * v + 1] << 16);
+        }
+        wb1[j] = u32x4{e[0], e[1], e[2], e[3]};
+    }
+    // this lane's first-layer tap rows for the three product steps (rows 9..11 hold zero)
+    int trow[3];
+    bool tap[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int row = 4 * j + g;
+        tap[j] = row < 9;
+        const int dz = tap[j] ? row / 3 : 0, dy = tap[j] ? row - 3 * (row / 3) : 0;
+        trow[j] = dz * kPairInRows + dy;  // image row of the tap, relative to (plane 2 pl, row 2 r)
+    }
+    u32x4 pre[kLoads];
+    // image planes zin0 .. zin0 + np - 1, rows 4 oy0 - 3 .. + 34, into registers
+    auto load_image = [&](int zin0, int np) {
+#pragma unroll
+        for (int i = 0; i < kLoads; ++i) {
+            const int t = threadIdx.x + kThreads * i;
+            const int pl = t / (8 * kPairInRows), rem = t - pl * 8 * kPairInRows;
+            const int iz = zin0 + pl, iy = 4 * oy0 - 3 + (rem >> 3);
+            const bool ok = pl < np && iz >= 0 && iz < d && iy >= 0 && iy < h;
+            const uint32_t o = ok ? sample_off + (uint32_t)((iz * h + iy) * 128 + 16 * (rem & 7)) : kOob;
+            pre[i] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(xb, o, 0, 0));
+        }
+    };
+    auto store_image = [&](int np) {
+#pragma unroll
+        for (int i = 0; i < kLoads; ++i) {
+            const int t = threadIdx.x + kThreads * i;
+            const int pl = t / (8 * kPairInRows), rem = t - pl * 8 * kPairInRows;
+            if (pl < np)
+                *reinterpret_cast<u32x4 *>(sin + (pl * kPairInRows + (rem >> 3)) * kPairInPitch + 16 +
+                                           16 * (rem & 7)) = pre[i];
+        }
+    };
+    // first-layer planes z0a .. z0a + np - 1 (image planes 2 z0a - 1 .. staged) into the ring;
+    // positions in the zero border (plane or row -1) are written as zeros
+    auto first_layer = [&](int z0a, int np) {
+        const int groups = np * kPairL0Rows * 4;
+#pragma unroll 2
+        for (int gi = wv; gi < groups; gi += W) {
+            const int pl = gi / (4 * kPairL0Rows), rem = gi - pl * 4 * kPairL0Rows;
+            const int rr = rem >> 2, cb = rem & 3;
+            const int z0 = z0a + pl, y0 = 2 * oy0 - 1 + rr, x0 = 16 * cb + l16;
+            const int rowb = (2 * pl * kPairInRows + 2 * rr) * kPairInPitch + 15 + 2 * x0;
+            floatx4 acc = {bs0[0], bs0[1], bs0[2], bs0[3]};
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int o = rowb + trow[j] * kPairInPitch;
+                const uint32_t *dw = reinterpret_cast<const uint32_t *>(sin + (o & ~3));
+                uint32_t v = __builtin_amdgcn_alignbyte(dw[1], dw[0], (uint32_t)(o & 3));
+                v = tap[j] ? v : 0u;
+                const float f0 = (float)(v & 0xffu), f1 = (float)((v >> 8) & 0xffu), f2 = (float)((v >> 16) & 0xffu);
+                const uint32_t p01 = __builtin_amdgcn_perm(__float_as_uint(f1), __float_as_uint(f0), 0x07060302u);
+                const uint32_t p2 = __float_as_uint(f2) >> 16;
+                const bf16x4 tp = bf16x4{(short)(p01 & 0xffffu), (short)(p01 >> 16), (short)p2, (short)0};
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[j], tp, acc, 0, 0, 0);
+            }
+            if (g < 2) {
+                const bool zero = z0 < 0 || y0 < 0;
+                const uint32_t q0 = zero ? 0u : pack_bf16(relu_bits(acc[0]), relu_bits(acc[1]));
+                const uint32_t q1 = zero ? 0u : pack_bf16(relu_bits(acc[2]), relu_bits(acc[3]));
+                const int slot = (z0 + 3) % 3;
+                *reinterpret_cast<uint2 *>(ring + ((slot * kPairL0Rows + rr) * kPairL0Cols + x0 + 1) * 16 + 8 * g) =
+                    make_uint2(q0, q1);
+            }
+        }
+    };
+    const int odp = od + 2 * out_pad, ohp = oh + 2 * out_pad, owp = 32 + 2 * out_pad;
+    // prologue: first-layer plane 2 oz0 - 1 (zeros for oz0 = 0)
+    load_image(4 * oz0 - 3, 3);
+    store_image(3);
+    load_image(4 * oz0 - 1, 5);
+    __syncthreads();
+    first_layer(2 * oz0 - 1, 1);
+    for (int oz = oz0; oz < oz1; ++oz) {
+        __syncthreads();  // the staged image and the ring's oldest planes are free
+        store_image(5);
+        if (oz + 1 < oz1) load_image(4 * oz + 3, 5);
+        __syncthreads();
+        first_layer(2 * oz, 2);
+        __syncthreads();
+        // second layer: block q = PB wv + pb of 16 positions is row q / 2, columns 16 (q % 2) ..
+        floatx4 acc[PB];
+#pragma unroll
+        for (int pb = 0; pb < PB; ++pb) acc[pb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            const int t = 4 * j + g;
+            const bool kin = t < 27;
+            const int tt = kin ? t : 0;
+            const int dz = (tt * 29) >> 8, rem = tt - 9 * dz;
+            const int dy = (rem * 11) >> 5, dx = rem - 3 * dy;
+            const int slot = (2 * oz - 1 + dz + 3) % 3;
+#pragma unroll
+            for (int pb = 0; pb < PB; ++pb) {
+                const int q = PB * wv + pb, oyl = q >> 1, ox = 16 * (q & 1) + l16;
+                u32x4 v = *reinterpret_cast<const u32x4 *>(
+                    ring + ((slot * kPairL0Rows + 2 * oyl + dy) * kPairL0Cols + 2 * ox + dx) * 16);
+                if (!kin) v = u32x4{0u, 0u, 0u, 0u};
+                acc[pb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, wb1[j]),
+                                                                  __builtin_bit_cast(bf16x8, v), acc[pb], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int pb = 0; pb < PB; ++pb) {
+            const int q = PB * wv + pb, oy = oy0 + (q >> 1), ox = 16 * (q & 1) + l16;
+            float y[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) y[v] = relu_bits(acc[pb][v] + bs1[v]);
+            const int64_t pos = (((int64_t)b * odp + oz + out_pad) * ohp + oy + out_pad) * owp + ox + out_pad;
+            *reinterpret_cast<uint2 *>(out + pos * 16 + 4 * g) = make_uint2(pack_bf16(y[0], y[1]), pack_bf16(y[2], y[3]));
+        }
+    }
+}
+
 // Implicit-GEMM convolution for c % 8 == 0 on the matrix cores, computed as
 // out^T (channels x positions) = W (channels x K) . P^T (K x positions) with
 // k = tap * c + ci.  Wave = 16 P output positions x 16 CB output channels;
@@ -995,6 +1187,37 @@ int pmx_conv3d_first(const uint8_t *x, const int32_t *in_shape, int stride, int 
 #undef PMX_FIRST
 #undef PMX_FIRST_LDS
         if (int e = pmx::check_launch("conv3d_first")) return e;
+    }
+    return PMX_OK;
+}
+
+int pmx_conv3d_first_pair(const uint8_t *x, const int32_t *in_shape, const uint16_t *w0_bf16, const float *b0,
+                          const uint16_t *w1_bf16, const float *b1, uint16_t *out, int out_pad, hipStream_t s)
+{
+    pmx::clear_error();
+    pmx::ConvDims cd;
+    if (int e = pmx::conv_dims(in_shape, 2, 27, &cd)) return e;
+    if (cd.c != 1) return pmx::fail(PMX_ERR_INVALID, "the fused first layers take one input channel");
+    if (cd.w != 128 || cd.h % 32 || cd.d % 4)
+        return pmx::fail(PMX_ERR_INVALID, "the fused first layers take width 128, height a multiple of 32 and "
+                                          "depth a multiple of 4");
+    if (out_pad != 0 && out_pad != 1) return pmx::fail(PMX_ERR_INVALID, "out_pad must be 0 or 1");
+    if (!x || !w0_bf16 || !b0 || !w1_bf16 || !b1 || !out) return pmx::fail(PMX_ERR_INVALID, "NULL buffer");
+    if ((uintptr_t)out % 8) return pmx::fail(PMX_ERR_INVALID, "out must be 8-byte aligned");
+    const int64_t sample = (int64_t)cd.d * cd.h * cd.w;
+    if (sample >= pmx::kOob) return pmx::fail(PMX_ERR_INVALID, "one sample exceeds 2 GiB");
+    const int64_t chunk = std::max<int64_t>(1, (int64_t)(pmx::kOob - 1) / sample);
+    const int od = cd.d / 4, oh = cd.h / 4, nzc = (od + pmx::kPairZ - 1) / pmx::kPairZ;
+    const int64_t out_sample = (int64_t)(od + 2 * out_pad) * (oh + 2 * out_pad) * (32 + 2 * out_pad) * 16;
+    for (int64_t n0 = 0; n0 < cd.n; n0 += chunk) {
+        const int64_t nc = std::min<int64_t>(chunk, cd.n - n0);
+        const int64_t blocks = nc * (oh / 8) * nzc;
+        if (blocks > INT32_MAX) return pmx::fail(PMX_ERR_INVALID, "input too large for one launch");
+        hipLaunchKernelGGL(pmx::k_conv_first_pair<pmx::kPairWaves>, dim3((unsigned)blocks),
+                           dim3(64 * pmx::kPairWaves), 0, s, x + n0 * sample,
+                           (uint32_t)(nc * sample), cd.d, cd.h, nzc, w0_bf16, b0, w1_bf16, b1, out + n0 * out_sample,
+                           out_pad);
+        if (int e = pmx::check_launch("conv3d_first_pair")) return e;
     }
     return PMX_OK;
 }

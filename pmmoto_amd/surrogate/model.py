@@ -226,8 +226,9 @@ class SurrogateNet:
     def infer(self, x_u8: torch.Tensor):
         """Forward pass without the training cache: the convolutions run as
         fused kernels (the first layer on the uint8 image, implicit GEMM on
-        the matrix cores after it) with bias, ReLU and BF16 rounding in their
-        epilogue, and the activations between them keep a one-voxel zero
+        the matrix cores after it; on a width-128 image with 8 and 16
+        channels the first two layers in one pass) with bias, ReLU and BF16
+        rounding in their epilogue, and the activations between them keep a one-voxel zero
         border so that the next layer's taps need no range checks.  Same
         arithmetic as forward() up to FP32 summation order.  Returns (pred
         [B, q+1] FP32, pooled, z1, p, cf)."""
@@ -239,6 +240,9 @@ class SurrogateNet:
         for i, co in enumerate(self.channels):
             kinds.append("first" if i == 0 and co % 4 == 0 and co <= 32 else "igemm" if cin % 8 == 0 else "im2col")
             cin = co
+        # the first two layers in one pass when they are 8 and 16 channels at stride 2 on a width-128 image
+        fuse = (len(self.channels) > 1 and kinds[:2] == ["first", "igemm"] and self.channels[:2] == (8, 16)
+                and self.strides[:2] == (2, 2) and shape[3] == 128 and shape[2] % 32 == 0 and shape[1] % 4 == 0)
         key = (b,) + tuple(shape[1:4])
         bufs = self._infer_bufs.get(key)
         if bufs is None:
@@ -247,7 +251,9 @@ class SurrogateNet:
             for i, co in enumerate(self.channels):
                 s = self.strides[i]
                 od, oh, ow = [(v - 1) // s + 1 for v in shp[1:4]]
-                if i + 1 < len(self.channels) and kinds[i + 1] == "igemm":
+                if fuse and i == 0:
+                    bufs.append(None)  # stays on chip
+                elif i + 1 < len(self.channels) and kinds[i + 1] == "igemm":
                     bufs.append(torch.zeros((b, od + 2, oh + 2, ow + 2, co), dtype=BF16, device=dev))
                 else:
                     bufs.append(torch.empty((b, od, oh, ow, co), dtype=BF16, device=dev))
@@ -259,10 +265,18 @@ class SurrogateNet:
             bias = self.param(f"conv{i}.b")[1]
             s = self.strides[i]
             od, oh, ow = [(v - 1) // s + 1 for v in shape[1:4]]
+            if fuse and i == 0:
+                image_dims = (C.c_int32 * 5)(*shape)
+                shape = [b, od, oh, ow, co]
+                continue
             a = bufs[i]
             a_pad = 1 if a.shape[1] == od + 2 else 0
             dims = (C.c_int32 * 5)(*shape)
-            if kinds[i] == "first":
+            if fuse and i == 1:
+                w0, b0 = self.param("conv0.w"), self.param("conv0.b")[1]
+                _lib.check(_L().pmx_conv3d_first_pair(_p(x_u8), image_dims, _p(w0[2]), _p(b0), _p(w[2]), _p(bias),
+                                                      _p(a), a_pad, _device.stream()))
+            elif kinds[i] == "first":
                 _lib.check(_L().pmx_conv3d_first(_p(x_u8), dims, s, co, _p(w[2]), _p(bias), _p(a), a_pad,
                                                  _device.stream()))
             else:

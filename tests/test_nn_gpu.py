@@ -172,6 +172,69 @@ def test_implicit_gemm_conv_at_network_sizes_matches_patch_matrix_path(c, co, sh
     assert float((got == ref).float().mean()) > 0.99
 
 
+@pytest.mark.parametrize("shape,bytes_max", [((2, 16, 32, 128), 1), ((3, 36, 64, 128), 1), ((1, 128, 128, 128), 1),
+                                              ((2, 8, 96, 128), 255)])
+@pytest.mark.parametrize("out_pad", [1, 0])
+def test_fused_first_pair_equals_the_two_layer_calls(shape, bytes_max, out_pad):
+    """pmx_conv3d_first_pair (uint8 image -> 8 -> 16 channels, both stride 2)
+    against pmx_conv3d_first followed by pmx_conv3d_bf16 on its zero-bordered
+    output: the same products in the same order, so bit for bit."""
+    import ctypes as C
+
+    import torch
+
+    from pmmoto_amd import _device, _lib
+    L = _lib.lib()
+    g = torch.Generator(device="cpu").manual_seed(sum(shape) + bytes_max)
+    x = torch.randint(0, bytes_max + 1, shape, generator=g, dtype=torch.uint8).cuda()
+    w0 = (torch.randn((8, 27), generator=g) / 27 ** 0.5).to(torch.bfloat16).cuda()
+    b0 = (0.1 * torch.randn(8, generator=g)).cuda()
+    w1 = (torch.randn((16, 216), generator=g) / 216 ** 0.5).to(torch.bfloat16).cuda()
+    b1 = (0.1 * torch.randn(16, generator=g)).cuda()
+    n, d, h, w = shape
+    print(f"[first_pair] shape={shape} bytes<={bytes_max} out_pad={out_pad}", flush=True)
+    mid = torch.zeros((n, d // 2 + 2, h // 2 + 2, w // 2 + 2, 8), dtype=torch.bfloat16, device="cuda")
+    _lib.check(L.pmx_conv3d_first(_device.ptr(x), (C.c_int32 * 5)(n, d, h, w, 1), 2, 8, _device.ptr(w0),
+                                  _device.ptr(b0), _device.ptr(mid), 1, _device.stream()))
+    p = out_pad
+    want = torch.zeros((n, d // 4 + 2 * p, h // 4 + 2 * p, w // 4 + 2 * p, 16), dtype=torch.bfloat16, device="cuda")
+    _lib.check(L.pmx_conv3d_bf16(_device.ptr(mid), (C.c_int32 * 5)(n, d // 2, h // 2, w // 2, 8), 1, 2, 16,
+                                 _device.ptr(w1), _device.ptr(b1), 1, _device.ptr(want), p, _device.stream()))
+    got = torch.zeros_like(want)
+    _lib.check(L.pmx_conv3d_first_pair(_device.ptr(x), (C.c_int32 * 5)(n, d, h, w, 1), _device.ptr(w0),
+                                       _device.ptr(b0), _device.ptr(w1), _device.ptr(b1), _device.ptr(got), p,
+                                       _device.stream()))
+    a, e = got.view(torch.int16).cpu().numpy(), want.view(torch.int16).cpu().numpy()
+    assert np.array_equal(a, e), f"{int((a != e).sum())} of {a.size} values differ"
+    assert float((want.float() > 0).float().mean()) > 0.2
+
+
+def test_fused_first_pair_rejects_shapes_it_does_not_tile():
+    import ctypes as C
+
+    from pmmoto_amd import _lib
+    L = _lib.lib()
+    for shape in [(1, 16, 32, 64, 1), (1, 16, 48, 128, 1), (1, 18, 32, 128, 1), (1, 16, 32, 128, 2)]:
+        rc = L.pmx_conv3d_first_pair(C.c_void_p(256), (C.c_int32 * 5)(*shape), C.c_void_p(256), C.c_void_p(256),
+                                     C.c_void_p(256), C.c_void_p(256), C.c_void_p(256), 1, None)
+        assert rc != 0, shape
+
+
+def test_fused_inference_at_width_128_matches_training_forward():
+    """The width-128 network (first two layers fused) against the training
+    forward's patch-matrix path."""
+    import torch
+
+    from pmmoto_amd.surrogate.model import SurrogateNet
+    net = SurrogateNet(size=128, seed=5)
+    rng = np.random.default_rng(12)
+    x = torch.from_numpy((rng.random((2, 128, 128, 128)) < 0.45).astype(np.uint8)).cuda()
+    print("[infer] size=128 default channels", flush=True)
+    want, _ = net.forward(x, keep=False)
+    got = net.infer(x)[0]
+    np.testing.assert_allclose(got.cpu().numpy(), want.cpu().numpy(), rtol=1e-2, atol=1e-2)
+
+
 def test_fused_inference_matches_training_forward():
     import torch
 

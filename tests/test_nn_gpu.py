@@ -363,3 +363,22 @@ def test_graphed_inference_equals_eager_on_every_replay():
             assert torch.equal(curve, want[k][0]) and torch.equal(sigma, want[k][1]), (rep, k)
             torch.zeros(1 << 20, device="cuda").add_(1.0)  # eager work between replays
         torch.cuda.synchronize()
+
+
+def test_graphed_inference_with_fused_first_pair_equals_eager():
+    """The width-128 network, whose first two layers run as one fused kernel,
+    through captured graphs: every replay (two batch shapes, eager work in
+    between) equals the eager launches bit for bit."""
+    import torch
+
+    from pmmoto_amd.surrogate import SurrogateNet
+    net = SurrogateNet(size=128, channels=(8, 16, 24), hidden=16, q=4, seed=9)
+    g = torch.Generator(device="cuda").manual_seed(8)
+    x = (torch.rand((6, 128, 128, 128), generator=g, device="cuda") < 0.5).to(torch.uint8)
+    want = [t.clone() for t in net.predict(x, batch=4, graph=False)]
+    for rep in range(3):
+        print(f"[graphed inference, fused first pair] replay {rep}", flush=True)
+        curve, sigma = net.predict(x, batch=4, graph=True)
+        assert torch.equal(curve, want[0]) and torch.equal(sigma, want[1]), rep
+        torch.zeros(1 << 20, device="cuda").add_(1.0)  # eager work between replays
+    torch.cuda.synchronize()

@@ -881,7 +881,8 @@ __global__ void k_zero_ints(int *p, int n, const unsigned *__restrict__ gate)
 // get_initial_envelope), which every finite parabola of a later line beats
 // in the float envelope too (its intersections come out as +-huge or
 // infinite, so it is popped or never selected).  A case outside the form (a
-// byte above 1, a squared distance above 65534 after pass 1, which includes
+// byte above 1 in corrected_edt's padded mode, a squared distance above
+// 65534 after pass 1, which includes
 // a line along y with no finite height, a scan past kIntWrap rows) raises
 // ctl[kCtlBail] or the windowed pass's flags; the float kernels then run in
 // full, gated on ctl[kCtlGate].
@@ -891,7 +892,7 @@ constexpr int kIntWrap = kHullMaskRows;  // rows past a line end a scan may read
 // lines [0], saturation [2]), the gate of the float kernels (0: the integer
 // result stands, 1: it is recomputed, 2: the integer form did not apply),
 // then the windowed pass's deferred-line bitmap
-constexpr int kCtlBail = 0, kCtlWin = 4, kCtlGate = 8, kCtlBits = 16;
+constexpr int kCtlBail = 0, kCtlLabels = 1, kCtlWin = 4, kCtlGate = 8, kCtlBits = 16;
 bool g_int_form = true;
 
 // pass 0 (axis 0) in integers: the 1-D distance c to the nearest zero along
@@ -902,12 +903,25 @@ bool g_int_form = true;
 // pad-skipping face distances with wrap == 2 (first zero f at or after
 // 2 p_lo and last zero l before n - 2 p_hi, l >= 2 p_lo: l + p_hi + 1 - n and
 // n - 1 + f - p_lo, the virtual zeros of its correctors; :304-447), none
-// (END) with wrap == 0.
+// (END) with wrap == 0.  A byte above 1 (a multi-label image) raises
+// ctl[kCtlLabels] (ctl[kCtlBail] with pads, labels_ok = 0), and the kLabels
+// instance, which returns at once unless that flag is set, writes the
+// label-aware pass: squared_edt_1d (_distance.hpp:136-182) resets to one
+// resolution step on both voxels of a change between two different non-zero
+// bytes, so each such edge voxel is a source of value 1 and a voxel's value
+// is min(distance to the nearest zero or virtual zero, 1 + distance to the
+// nearest edge voxel of its line); the correctors' virtual zeros come from
+// the zeros alone (get_nearest_boundary_index with label 0).
+template <bool kLabels>
 __global__ __launch_bounds__(256) void k_int_pass0(const uint8_t *__restrict__ img, int64_t is0, int64_t is1, int nx,
-                                                   int ny, int nz, int wrap, int plo, int phi,
+                                                   int ny, int nz, int wrap, int plo, int phi, int labels_ok,
                                                    uint16_t *__restrict__ dz, int64_t pitch, uint32_t *__restrict__ ctl)
 {
     __shared__ uint32_t bits[(kIntMaxN / 32) * 256];
+    __shared__ uint32_t ebits[kLabels ? (kIntMaxN / 32) * 256 : 1];
+    if constexpr (kLabels) {
+        if (ctl[kCtlLabels] == 0u) return;  // the image holds no byte above 1
+    }
     const int t = threadIdx.x;
     const int64_t col = (int64_t)blockIdx.x * 256 + t;
     if (col >= (int64_t)ny * nz) return;  // no barriers below: each lane owns its LDS column
@@ -915,6 +929,7 @@ __global__ __launch_bounds__(256) void k_int_pass0(const uint8_t *__restrict__ i
     const uint8_t *p = img + (int64_t)y * is1 + z;
     const int nw = (nx + 31) >> 5;
     uint32_t bad = 0;
+    uint8_t last = 0;  // the byte before the chunk (none before x = 0)
     for (int x0 = 0; x0 < nx; x0 += 32) {
         uint8_t v[32];
 #pragma unroll
@@ -926,58 +941,92 @@ __global__ __launch_bounds__(256) void k_int_pass0(const uint8_t *__restrict__ i
             bad |= v[j] > 1;
         }
         bits[(x0 >> 5) * 256 + t] = word;
+        if constexpr (kLabels) {
+            // bit x: a label change between x - 1 and x (two different non-zero bytes)
+            uint32_t chg = 0;
+#pragma unroll
+            for (int j = 0; j < 32; ++j) {
+                const uint8_t pv = j ? v[j - 1] : last;
+                chg |= (uint32_t)(pv != 0 && v[j] != 0 && pv != v[j]) << j;
+            }
+            last = v[31];
+            const int left = nx - x0;
+            ebits[(x0 >> 5) * 256 + t] = left >= 32 ? chg : chg & ((1u << left) - 1u);
+        }
     }
-    // first zero at or after a / last zero before b, or -1
-    auto first_from = [&](int a) -> int {
+    if constexpr (!kLabels) {
+        if (bad) ctl[labels_ok ? kCtlLabels : kCtlBail] = 1u;
+    } else {
+        // the edge voxels: a change on either side (x, or x + 1)
+        for (int q = 0; q < nw; ++q) {
+            const uint32_t c = ebits[q * 256 + t], cn = q + 1 < nw ? ebits[(q + 1) * 256 + t] : 0u;
+            ebits[q * 256 + t] = c | (c >> 1) | (cn << 31);
+        }
+    }
+    // first set bit at or after a / last set bit before b of a column mask, or -1
+    auto first_from = [&](const uint32_t *m, int a) -> int {
         if (a >= nx) return -1;
         int q = a >> 5;
-        uint32_t wq = bits[q * 256 + t] & (~0u << (a & 31));
+        uint32_t wq = m[q * 256 + t] & (~0u << (a & 31));
         while (true) {
             if (wq) return q * 32 + __builtin_ctz(wq);
             if (++q >= nw) return -1;
-            wq = bits[q * 256 + t];
+            wq = m[q * 256 + t];
         }
     };
-    auto last_before = [&](int b) -> int {
+    auto last_before = [&](const uint32_t *m, int b) -> int {
         if (b <= 0) return -1;
         int q = (b - 1) >> 5;
-        uint32_t wq = bits[q * 256 + t] & (0xffffffffu >> (31 - ((b - 1) & 31)));
+        uint32_t wq = m[q * 256 + t] & (0xffffffffu >> (31 - ((b - 1) & 31)));
         while (true) {
             if (wq) return q * 32 + 31 - __builtin_clz(wq);
             if (--q < 0) return -1;
-            wq = bits[q * 256 + t];
+            wq = m[q * 256 + t];
         }
     };
-    if (bad) ctl[kCtlBail] = 1u;
     uint16_t *o = dz + (int64_t)y * pitch + z;
     const int64_t xs = (int64_t)ny * pitch;
-    const int first = first_from(0);
-    if (first < 0) {
-        // no zero: the reference's infinity (FLT_MAX in its later passes),
-        // which loses to every finite parabola of a line
+    const int first = first_from(bits, 0);
+    const int efirst = kLabels ? first_from(ebits, 0) : -1;
+    if (first < 0 && efirst < 0) {
+        // no zero (and no label edge): the reference's infinity (FLT_MAX in its
+        // later passes), which loses to every finite parabola of a line
         for (int x = 0; x < nx; ++x) o[(int64_t)x * xs] = kNoSolid;
         return;
     }
     constexpr int kFar = 1 << 20;
     int vprev = -kFar, vnext = kFar;
-    if (wrap == 1) {
-        vprev = last_before(nx) - nx;
-        vnext = first + nx;
-    } else if (wrap == 2) {
-        const int f = first_from(2 * plo), l = last_before(nx - 2 * phi);
-        if (l >= 2 * plo) vprev = l + phi + 1 - nx;
-        if (f >= 0) vnext = nx - 1 + f - plo;
+    if (first >= 0) {
+        if (wrap == 1) {
+            vprev = last_before(bits, nx) - nx;
+            vnext = first + nx;
+        } else if (wrap == 2) {
+            const int f = first_from(bits, 2 * plo), l = last_before(bits, nx - 2 * phi);
+            if (l >= 2 * plo) vprev = l + phi + 1 - nx;
+            if (f >= 0) vnext = nx - 1 + f - plo;
+        }
     }
-    int prev = vprev, next = first;
+    int prev = vprev, next = first >= 0 ? first : vnext;
+    int eprev = -kFar, enext = efirst >= 0 ? efirst : kFar;
     for (int x = 0; x < nx; ++x) {
         int d;
         if (x == next) {
             d = 0;
             prev = x;
-            const int nn = first_from(x + 1);
+            const int nn = first_from(bits, x + 1);
             next = nn >= 0 ? nn : vnext;
         } else {
             d = min(x - prev, next - x);
+        }
+        if constexpr (kLabels) {
+            // an edge voxel holds 1 (squared_edt_1d's reset at a label change), and the
+            // distance grows by one per voxel from it
+            if (x == enext) {
+                eprev = x;
+                const int nn = first_from(ebits, x + 1);
+                enext = nn >= 0 ? nn : kFar;
+            }
+            d = min(d, min(x - eprev, enext - x) + 1);
         }
         o[(int64_t)x * xs] = (uint16_t)min(d, 0xfffe);
     }
@@ -1466,9 +1515,14 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
                            dim3(256), 0, s, w.ctl, w.ctl_words);
         {
             ProfScope ps("edt_int_pass0", s);
-            hipLaunchKernelGGL(k_int_pass0, dim3((unsigned)(((int64_t)ny * nz + 255) / 256)), dim3(256), 0, s, img,
-                               is[0], is[1], nx, ny, nz, periodic[0] ? (pads ? 2 : 1) : 0, pad(0, 0), pad(0, 1),
-                               w.dz, pitch, w.ctl);
+            const dim3 g0((unsigned)(((int64_t)ny * nz + 255) / 256));
+            const int wrap0 = periodic[0] ? (pads ? 2 : 1) : 0, labels_ok = pads ? 0 : 1;
+            hipLaunchKernelGGL(k_int_pass0<false>, g0, dim3(256), 0, s, img, is[0], is[1], nx, ny, nz, wrap0,
+                               pad(0, 0), pad(0, 1), labels_ok, w.dz, pitch, w.ctl);
+            // a multi-label image: the label-aware pass over again (returns at once otherwise)
+            if (labels_ok)
+                hipLaunchKernelGGL(k_int_pass0<true>, g0, dim3(256), 0, s, img, is[0], is[1], nx, ny, nz, wrap0,
+                                   pad(0, 0), pad(0, 1), labels_ok, w.dz, pitch, w.ctl);
         }
         if (int e = check_launch("integer pass 0")) return e;
         // the hull masks of the lines along dim, then the lines whose

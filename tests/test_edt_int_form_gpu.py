@@ -114,15 +114,17 @@ def test_periodic_pack_512_forms_agree():
 
 @pytest.mark.parametrize("case", ["label2", "inf_line", "inf_plane", "res_third", "far"])
 def test_cases_at_the_edge_of_the_integer_form(case):
-    """Lines along x with no zero keep the integer form; outside it the float
-    kernels take over, on the device or by the arguments, and the result is
-    still the reference's."""
+    """Lines along x with no zero and multi-label images keep the integer
+    form; outside it the float kernels take over, on the device or by the
+    arguments, and the result is still the reference's."""
     img = rand_binary((30, 26, 34), 0.3, 11)
     res = (2.0 ** -9,) * 3
     periodic = (True, True, True)
     want_form = 1
     if case == "label2":
+        # a multi-label image takes the label-aware integer pass 0
         img[3, 4, 5] = 2
+        want_form = 0
     elif case == "inf_line":
         img[:, 7, 9] = 1
         img[:, 20, 3] = 1
@@ -177,3 +179,70 @@ def test_padded_subdomain_forms_agree_and_match_oracle(kind, own_shape, bts, pad
     np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
     want = np.sqrt(O.ref_corrected_edt_sq(img, periodic, pads, (res,) * 3))
     np.testing.assert_array_equal(a.view(np.uint32), want.view(np.uint32))
+
+
+def labelled(img, seed, n_labels):
+    """The pore voxels of img split into labels 1 .. n_labels by random
+    axis-aligned slabs (runs of one label with changes inside the pore
+    space), solids kept at 0."""
+    rng = np.random.default_rng(seed)
+    lab = np.ones(img.shape, np.uint8)
+    for a in range(3):
+        cuts = np.sort(rng.integers(0, img.shape[a], size=3))
+        idx = np.searchsorted(cuts, np.arange(img.shape[a]), side="right")
+        sh = [1, 1, 1]
+        sh[a] = img.shape[a]
+        lab = ((lab.astype(np.int64) + idx.reshape(sh)) % n_labels + 1).astype(np.uint8)
+    return np.where(img > 0, lab, 0).astype(np.uint8)
+
+
+@pytest.mark.parametrize("periodic", [(True, True, True), (True, False, True), (False, False, False),
+                                      (False, True, True)])
+@pytest.mark.parametrize("kind", ["noise", "pack", "no_zero_lines"])
+def test_multilabel_images_take_the_integer_form(periodic, kind):
+    """Multi-label images (squared_edt_1d's reset at label changes in the
+    first pass) in the integer form: bit-identical to the float kernels and
+    to the oracle, squared and rooted, at a power-of-two and at unit
+    resolution."""
+    if kind == "noise":
+        rng = np.random.default_rng(21)
+        img = rng.choice([0, 1, 2, 3], size=(41, 33, 50), p=[0.1, 0.5, 0.2, 0.2]).astype(np.uint8)
+    elif kind == "pack":
+        img = labelled(pack((48, 40, 56), 0.5, 3, 9, 61, periodic), 62, 3)
+    else:
+        # lines along x with no zero but label changes, and lines with neither
+        rng = np.random.default_rng(23)
+        img = rng.choice([0, 1, 2], size=(30, 26, 34), p=[0.05, 0.6, 0.35]).astype(np.uint8)
+        img[:, 5, 7] = 2
+        img[:, 11, 3] = 1
+        img[:15, 11, 3] = 3
+    print(f"[int form] multi-label {kind} {img.shape} periodic={periodic}", flush=True)
+    for res in (2.0 ** -9, 1.0):
+        for sq in (False, True):
+            a, form, b = both_forms(img, periodic, (res,) * 3, take_sqrt=sq)
+            assert form == 0, "a multi-label image takes the integer form"
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+            want = O.ref_edt_sq(img, periodic, (res,) * 3)
+            if sq:
+                want = np.sqrt(want)
+            np.testing.assert_array_equal(a.view(np.uint32), want.view(np.uint32))
+
+
+def test_multilabel_padded_subdomain_goes_to_the_float_kernels():
+    """corrected_edt's padded mode with labels stays outside the integer
+    form (the float kernels, equal to the oracle)."""
+    own = labelled(pack((30, 31, 29), 0.5, 3, 8, 72, (True,) * 3), 73, 2)
+    pad = (2, 1, 1)
+    img = own
+    pads = []
+    for a in range(3):
+        img = np.pad(img, [(0, 0)] * a + [(pad[a], pad[a])] + [(0, 0)] * (2 - a), mode="wrap")
+        pads += [pad[a], pad[a]]
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    print(f"[int form] multi-label padded {img.shape}", flush=True)
+    a, form, b = both_forms(img, (True,) * 3, (1.0 / 32,) * 3, take_sqrt=True, pads=pads)
+    assert form == 1
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    want = np.sqrt(O.ref_corrected_edt_sq(img, (True,) * 3, pads, (1.0 / 32,) * 3))
+    np.testing.assert_array_equal(a.view(np.uint32), want.view(np.uint32))
+

@@ -68,6 +68,43 @@ def test_initial_envelope_kats(use_ref):
                                   [4.0, 1.0, 0.0, 1.0, 4.0, 9.0, 16.0, 25.0, 16.0])
 
 
+def label_sources_pass0(v, lower, upper):
+    """The first pass as pmx_edt_ref's integer form computes it on a
+    multi-label line: sources are the zeros (value 0), the correctors'
+    virtual zeros at -lower and n - 1 + upper, and the voxels beside a
+    change between two different non-zero labels (value 1); squared."""
+    n = len(v)
+    chg = [x > 0 and v[x - 1] != 0 and v[x] != 0 and v[x - 1] != v[x] for x in range(n)] + [False]
+    zeros = [x for x in range(n) if v[x] == 0]
+    edges = [x for x in range(n) if chg[x] or chg[x + 1]]
+    out = np.empty(n, np.float32)
+    for x in range(n):
+        c = [abs(x - z) for z in zeros] + [1 + abs(x - e) for e in edges]
+        if lower is not None:
+            c.append(x + lower)
+        if upper is not None:
+            c.append(n - 1 + upper - x)
+        out[x] = np.inf if not c else float(min(c) ** 2)
+    return out
+
+
+@pytest.mark.parametrize("use_ref", [False, pytest.param(True, marks=needs_ref)])
+def test_label_pass0_sources_match_the_first_pass(use_ref):
+    """The label-aware integer pass 0 (edge voxels of a label change as
+    distance-1 sources) equals squared_edt_1d on random multi-label lines
+    with and without correctors (_distance.hpp:136-182)."""
+    rng = np.random.default_rng(0)
+    for _ in range(1500):
+        n = int(rng.integers(1, 40))
+        p0 = rng.random() * 0.3
+        v = rng.choice([0, 1, 2, 3], size=n, p=[p0, (1 - p0) / 2, (1 - p0) / 4, (1 - p0) / 4]).astype(np.uint8)
+        lo = None if rng.random() < 0.5 else int(rng.integers(1, 10))
+        up = None if rng.random() < 0.5 else int(rng.integers(1, 10))
+        want = O.sq_edt_1d(v, np.inf if lo is None else float(lo), np.inf if up is None else float(up), 1.0,
+                           use_ref=use_ref)
+        np.testing.assert_array_equal(label_sources_pass0(v, lo, up), want, err_msg=f"{v} {lo} {up}")
+
+
 def test_boundary_hull_kats():
     # test_edt.py:146-218
     img = np.array([4, np.inf, 0, np.inf, np.inf, np.inf, 4, 6, 16], dtype=np.float32)

@@ -881,8 +881,7 @@ __global__ void k_zero_ints(int *p, int n, const unsigned *__restrict__ gate)
 // get_initial_envelope), which every finite parabola of a later line beats
 // in the float envelope too (its intersections come out as +-huge or
 // infinite, so it is popped or never selected).  A case outside the form (a
-// byte above 1 in corrected_edt's padded mode, a squared distance above
-// 65534 after pass 1, which includes
+// squared distance above 65534 after pass 1, which includes
 // a line along y with no finite height, a scan past kIntWrap rows) raises
 // ctl[kCtlBail] or the windowed pass's flags; the float kernels then run in
 // full, gated on ctl[kCtlGate].
@@ -904,17 +903,18 @@ bool g_int_form = true;
 // 2 p_lo and last zero l before n - 2 p_hi, l >= 2 p_lo: l + p_hi + 1 - n and
 // n - 1 + f - p_lo, the virtual zeros of its correctors; :304-447), none
 // (END) with wrap == 0.  A byte above 1 (a multi-label image) raises
-// ctl[kCtlLabels] (ctl[kCtlBail] with pads, labels_ok = 0), and the kLabels
-// instance, which returns at once unless that flag is set, writes the
+// ctl[kCtlLabels], and the kLabels instance, which returns at once unless
+// that flag is set, writes the
 // label-aware pass: squared_edt_1d (_distance.hpp:136-182) resets to one
 // resolution step on both voxels of a change between two different non-zero
 // bytes, so each such edge voxel is a source of value 1 and a voxel's value
 // is min(distance to the nearest zero or virtual zero, 1 + distance to the
 // nearest edge voxel of its line); the correctors' virtual zeros come from
-// the zeros alone (get_nearest_boundary_index with label 0).
+// the zeros alone (get_nearest_boundary_index with label 0), and
+// corrected_edt runs the same first pass over the whole padded line.
 template <bool kLabels>
 __global__ __launch_bounds__(256) void k_int_pass0(const uint8_t *__restrict__ img, int64_t is0, int64_t is1, int nx,
-                                                   int ny, int nz, int wrap, int plo, int phi, int labels_ok,
+                                                   int ny, int nz, int wrap, int plo, int phi,
                                                    uint16_t *__restrict__ dz, int64_t pitch, uint32_t *__restrict__ ctl)
 {
     __shared__ uint32_t bits[(kIntMaxN / 32) * 256];
@@ -955,7 +955,7 @@ __global__ __launch_bounds__(256) void k_int_pass0(const uint8_t *__restrict__ i
         }
     }
     if constexpr (!kLabels) {
-        if (bad) ctl[labels_ok ? kCtlLabels : kCtlBail] = 1u;
+        if (bad) ctl[kCtlLabels] = 1u;
     } else {
         // the edge voxels: a change on either side (x, or x + 1)
         for (int q = 0; q < nw; ++q) {
@@ -1516,13 +1516,12 @@ int pmx_edt_ref(const uint8_t *img, const int64_t *n, const int64_t *in_s, float
         {
             ProfScope ps("edt_int_pass0", s);
             const dim3 g0((unsigned)(((int64_t)ny * nz + 255) / 256));
-            const int wrap0 = periodic[0] ? (pads ? 2 : 1) : 0, labels_ok = pads ? 0 : 1;
+            const int wrap0 = periodic[0] ? (pads ? 2 : 1) : 0;
             hipLaunchKernelGGL(k_int_pass0<false>, g0, dim3(256), 0, s, img, is[0], is[1], nx, ny, nz, wrap0,
-                               pad(0, 0), pad(0, 1), labels_ok, w.dz, pitch, w.ctl);
+                               pad(0, 0), pad(0, 1), w.dz, pitch, w.ctl);
             // a multi-label image: the label-aware pass over again (returns at once otherwise)
-            if (labels_ok)
-                hipLaunchKernelGGL(k_int_pass0<true>, g0, dim3(256), 0, s, img, is[0], is[1], nx, ny, nz, wrap0,
-                                   pad(0, 0), pad(0, 1), labels_ok, w.dz, pitch, w.ctl);
+            hipLaunchKernelGGL(k_int_pass0<true>, g0, dim3(256), 0, s, img, is[0], is[1], nx, ny, nz, wrap0,
+                               pad(0, 0), pad(0, 1), w.dz, pitch, w.ctl);
         }
         if (int e = check_launch("integer pass 0")) return e;
         // the hull masks of the lines along dim, then the lines whose

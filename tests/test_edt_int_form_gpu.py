@@ -228,21 +228,25 @@ def test_multilabel_images_take_the_integer_form(periodic, kind):
             np.testing.assert_array_equal(a.view(np.uint32), want.view(np.uint32))
 
 
-def test_multilabel_padded_subdomain_goes_to_the_float_kernels():
-    """corrected_edt's padded mode with labels stays outside the integer
-    form (the float kernels, equal to the oracle)."""
-    own = labelled(pack((30, 31, 29), 0.5, 3, 8, 72, (True,) * 3), 73, 2)
-    pad = (2, 1, 1)
+@pytest.mark.parametrize("bts,pad,res", [(("periodic",) * 3, (2, 1, 1), 1.0 / 32),
+                                          (("periodic", "wall", "periodic"), (2, 1, 3), 1.0),
+                                          (("wall", "periodic", "end"), (1, 2, 1), 0.5)])
+def test_multilabel_padded_subdomain_takes_the_integer_form(bts, pad, res):
+    """corrected_edt's padded mode with labels (its first pass runs over the
+    whole padded line): the integer form, equal to the float kernels and the
+    oracle."""
+    periodic = [b == "periodic" for b in bts]
+    own = labelled(pack((30, 31, 29), 0.5, 3, 8, 72, tuple(periodic)), 73, 2)
     img = own
     pads = []
     for a in range(3):
-        img = np.pad(img, [(0, 0)] * a + [(pad[a], pad[a])] + [(0, 0)] * (2 - a), mode="wrap")
-        pads += [pad[a], pad[a]]
+        w = (pad[a], pad[a]) if bts[a] != "end" else (0, 0)
+        img = np.pad(img, [(0, 0)] * a + [w] + [(0, 0)] * (2 - a), mode="wrap" if periodic[a] else "constant")
+        pads += list(w)
     img = np.ascontiguousarray(img, dtype=np.uint8)
-    print(f"[int form] multi-label padded {img.shape}", flush=True)
-    a, form, b = both_forms(img, (True,) * 3, (1.0 / 32,) * 3, take_sqrt=True, pads=pads)
-    assert form == 1
+    print(f"[int form] multi-label padded {img.shape} {bts} pads={pads}", flush=True)
+    a, form, b = both_forms(img, periodic, (res,) * 3, take_sqrt=True, pads=pads)
+    assert form == 0
     np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
-    want = np.sqrt(O.ref_corrected_edt_sq(img, (True,) * 3, pads, (1.0 / 32,) * 3))
+    want = np.sqrt(O.ref_corrected_edt_sq(img, periodic, pads, (res,) * 3))
     np.testing.assert_array_equal(a.view(np.uint32), want.view(np.uint32))
-

@@ -86,8 +86,6 @@ __device__ __forceinline__ int block_of(const Dims &d, int x, int y, int z)
     return ((x >> 1) * d.nb1 + (y >> 1)) * d.nb2 + (z >> 1);
 }
 
-__device__ __forceinline__ int64_t n_of(const Dims &d) { return (int64_t)d.n0 * d.n1 * d.n2; }
-
 constexpr int kMaxLevels = 250;
 constexpr int kBucketItems = 16;  // voxels per thread between bucket reservations
 
@@ -464,13 +462,15 @@ __global__ void k_clear_u32(unsigned int *a, int n)
 __global__ __launch_bounds__(256) void k_zero_map(int32_t *__restrict__ map, const uint8_t *__restrict__ img,
                                                   uint8_t *__restrict__ state, Dims d, int64_t s0, int64_t s1)
 {
-    const int64_t n = n_of(d);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t z = i % d.n2, t = i / d.n2, y = t % d.n1, x = t / d.n1;
-        const int64_t j = x * s0 + y * s1 + z;
-        map[j] = 0;
-        state[i] = img[j] == 1 ? 1 : 0;
+    // a row (x, y) per block step: one division per row, none per voxel
+    const int64_t rows = (int64_t)d.n0 * d.n1;
+    for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int64_t x = r / d.n1, y = r - x * d.n1;
+        const int64_t jr = x * s0 + y * s1, ir = r * d.n2;
+        for (int z = threadIdx.x; z < d.n2; z += blockDim.x) {
+            map[jr + z] = 0;
+            state[ir + z] = img[jr + z] == 1 ? 1 : 0;
+        }
     }
 }
 
@@ -604,7 +604,8 @@ int run_psd(const uint8_t *img, const pmx_geom *g, const int64_t *inlet_lo, cons
             hipLaunchKernelGGL(k_block_init, dim3(grid_for(nblk)), dim3(256), 0, s, p.level, d, n_levels, p.blevel,
                                p.bjoin, p.bflag);
         }
-        hipLaunchKernelGGL(k_zero_map, dim3(gr), dim3(256), 0, s, code_map, img, p.state, d, g->out_s[0],
+        const unsigned grow = (unsigned)std::min<int64_t>((int64_t)d.n0 * d.n1, 65536);
+        hipLaunchKernelGGL(k_zero_map, dim3(grow), dim3(256), 0, s, code_map, img, p.state, d, g->out_s[0],
                            g->out_s[1]);
     }
     if (int e = check_launch("psd setup")) return e;
